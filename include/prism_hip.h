@@ -1,0 +1,313 @@
+/*
+ * prism_hip.h -- C ABI of the MI355X (gfx950) hash join / hash aggregation /
+ * fused filter-project library.
+ *
+ * Two layers, both plain C (no torch, no C++ types in any signature):
+ *
+ *  1. Device layer (prism_ctx_*, prism_join_*, prism_groupby_*, prism_datagen_*):
+ *     column buffers already resident in HBM, passed as device pointers.  This
+ *     is what a caller that keeps tables in HBM across requests binds (the
+ *     operator service, the benchmark).
+ *
+ *  2. Arrow layer (prism_hash_join*, prism_hash_aggregate*, prism_filter_*,
+ *     prism_evaluate_*, prism_project_*): the operator API of
+ *     native/prism-executor/src/{hash_join,hash_aggregate,filter_project}.rs
+ *     over the Arrow C Data Interface (ArrowArray/ArrowSchema, host memory),
+ *     so a Rust `-sys` crate binds it with arrow's `ffi` feature
+ *     (native/Cargo.toml:20, prism-jni/src/lib.rs:9).
+ *
+ * Every entry point returns a prism_status; a non-zero status leaves a
+ * thread-local message readable through prism_last_error().  Nothing aborts or
+ * unwinds across this boundary.
+ */
+#ifndef PRISM_HIP_H
+#define PRISM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Arrow C Data Interface (ABI fixed by the Apache Arrow specification).      */
+/* ------------------------------------------------------------------------- */
+#ifndef ARROW_C_DATA_INTERFACE
+#define ARROW_C_DATA_INTERFACE
+#define ARROW_FLAG_DICTIONARY_ORDERED 1
+#define ARROW_FLAG_NULLABLE 2
+#define ARROW_FLAG_MAP_KEYS_SORTED 4
+struct ArrowSchema {
+  const char* format;
+  const char* name;
+  const char* metadata;
+  int64_t flags;
+  int64_t n_children;
+  struct ArrowSchema** children;
+  struct ArrowSchema* dictionary;
+  void (*release)(struct ArrowSchema*);
+  void* private_data;
+};
+struct ArrowArray {
+  int64_t length;
+  int64_t null_count;
+  int64_t offset;
+  int64_t n_buffers;
+  int64_t n_children;
+  const void** buffers;
+  struct ArrowArray** children;
+  struct ArrowArray* dictionary;
+  void (*release)(struct ArrowArray*);
+  void* private_data;
+};
+#endif /* ARROW_C_DATA_INTERFACE */
+
+/* ------------------------------------------------------------------------- */
+/* Status codes: one per PrismError variant (prism-executor/src/error.rs:5-27) */
+/* ------------------------------------------------------------------------- */
+typedef int32_t prism_status;
+#define PRISM_OK 0
+#define PRISM_ERR_ARROW 1                   /* PrismError::Arrow */
+#define PRISM_ERR_SCHEMA_MISMATCH 2         /* PrismError::SchemaMismatch */
+#define PRISM_ERR_INVALID_ARGUMENT 3        /* PrismError::InvalidArgument */
+#define PRISM_ERR_EXECUTION 4               /* PrismError::Execution (HIP errors land here) */
+#define PRISM_ERR_JOIN 5                    /* PrismError::Join */
+#define PRISM_ERR_UNSUPPORTED_AGGREGATION 6 /* PrismError::UnsupportedAggregation */
+#define PRISM_ERR_INTERNAL 7                /* PrismError::Internal */
+#define PRISM_ERR_RESOURCE 8                /* service: HBM admission refused (no PrismError counterpart) */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* prism_last_error(void);
+/* Library version string (replaces Java_io_prism_bridge_PrismNativeExecutor_version,
+ * prism-jni/src/lib.rs:249-259). */
+const char* prism_version(void);
+
+/* ------------------------------------------------------------------------- */
+/* Context: device, HIP stream, scratch memory.                               */
+/* ------------------------------------------------------------------------- */
+typedef struct prism_ctx prism_ctx;
+
+/* Create a context on HIP device `device` with its own non-blocking stream. */
+prism_status prism_ctx_create(int device, prism_ctx** out);
+void prism_ctx_destroy(prism_ctx* ctx);
+/* Launch subsequent work on an existing hipStream_t (NULL = the context's own). */
+prism_status prism_ctx_set_stream(prism_ctx* ctx, void* hip_stream);
+void* prism_ctx_stream(prism_ctx* ctx);
+prism_status prism_ctx_synchronize(prism_ctx* ctx);
+/* Device bytes held by the context and every object created from it. */
+uint64_t prism_ctx_device_bytes(prism_ctx* ctx);
+
+/* ------------------------------------------------------------------------- */
+/* Column descriptors (device layer: every pointer is a device pointer).      */
+/* ------------------------------------------------------------------------- */
+#define PRISM_TYPE_INT32 1
+#define PRISM_TYPE_INT64 2
+#define PRISM_TYPE_FLOAT64 3
+#define PRISM_TYPE_UTF8 4
+#define PRISM_TYPE_BOOL 5
+#define PRISM_TYPE_DATE32 6
+
+typedef struct {
+  int32_t type;            /* PRISM_TYPE_* */
+  int32_t reserved;
+  int64_t offset;          /* Arrow slice offset in elements (bits for BOOL and validity) */
+  const void* data;        /* values; BOOL bit-packed; UTF8 bytes */
+  const int32_t* offsets;  /* UTF8 only: i32 offsets (length + 1 entries after `offset`) */
+  const uint8_t* validity; /* Arrow validity bitmap, or NULL when every row is valid */
+} prism_column;
+
+/* ------------------------------------------------------------------------- */
+/* Row programs: the fused predicate / projection stage.                      */
+/* A program is postfix code over one row; Predicate (filter_project.rs:22-51) */
+/* and ScalarExpr (filter_project.rs:195-213) trees compile to it.            */
+/* ------------------------------------------------------------------------- */
+#define PRISM_OP_LOAD_I 1      /* push column[arg] (INT32/INT64/DATE32/BOOL) as int64 */
+#define PRISM_OP_LOAD_F 2      /* push column[arg] (FLOAT64) */
+#define PRISM_OP_CONST_I 3     /* push i64 */
+#define PRISM_OP_CONST_F 4     /* push f64 */
+#define PRISM_OP_I2F 5         /* top := (double)top */
+#define PRISM_OP_ADD_F 6
+#define PRISM_OP_SUB_F 7
+#define PRISM_OP_MUL_F 8
+#define PRISM_OP_DIV_F 9
+#define PRISM_OP_NEG_F 10
+#define PRISM_OP_ADD_I 11      /* exact int64; overflow raises PRISM_ERR_EXECUTION */
+#define PRISM_OP_SUB_I 12
+#define PRISM_OP_MUL_I 13
+#define PRISM_OP_NEG_I 14
+#define PRISM_OP_CMP_I 15      /* pop b, a; push (a <arg> b) on int64; arg = PRISM_CMP_* */
+#define PRISM_OP_CMP_F 16      /* same on f64 (IEEE: NaN compares false except NE) */
+#define PRISM_OP_CMP_STR 17    /* push (utf8 column[arg] <i64 >> 8 cmp> string[i64 & 0xff]) */
+#define PRISM_OP_LIKE 18       /* push (utf8 column[arg] LIKE pattern string[i64]) */
+#define PRISM_OP_ILIKE 19      /* ASCII case-insensitive LIKE */
+#define PRISM_OP_IS_NULL 20    /* push column[arg] is null (never null) */
+#define PRISM_OP_IS_NOT_NULL 21
+#define PRISM_OP_AND 22        /* null if either side null (arrow `and`) */
+#define PRISM_OP_OR 23         /* null if either side null (arrow `or`) */
+#define PRISM_OP_NOT 24
+#define PRISM_OP_CONST_BOOL 25 /* push i64 != 0 */
+#define PRISM_OP_SELECT 26     /* pop else, then, cond; push cond ? then : else (zip) */
+#define PRISM_OP_CONST_NULL 27 /* push a null */
+
+#define PRISM_CMP_EQ 0
+#define PRISM_CMP_NE 1
+#define PRISM_CMP_LT 2
+#define PRISM_CMP_LE 3
+#define PRISM_CMP_GT 4
+#define PRISM_CMP_GE 5
+
+typedef struct {
+  int32_t op;  /* PRISM_OP_* */
+  int32_t arg; /* column index / PRISM_CMP_* */
+  int64_t i64;
+  double f64;
+} prism_instr;
+
+typedef struct {
+  const char* data; /* host memory; copied at plan creation */
+  uint32_t len;
+} prism_string;
+
+/* ------------------------------------------------------------------------- */
+/* Hash join, device layer (hash_join.rs:87-200, int64 single-key fast path). */
+/* ------------------------------------------------------------------------- */
+typedef struct prism_join_table prism_join_table;
+
+#define PRISM_TABLE_AUTO 0   /* pick DENSE when the key range is compact, else HASHED */
+#define PRISM_TABLE_HASHED 1 /* open addressing on hash_i64_fast(key), linear probing */
+#define PRISM_TABLE_DENSE 2  /* direct-address array over [min_key, max_key] */
+
+typedef struct {
+  uint64_t n_rows;        /* build rows */
+  uint64_t n_distinct;    /* distinct keys */
+  uint64_t max_dup;       /* largest row list of one key */
+  int64_t min_key, max_key;
+  int32_t kind;           /* PRISM_TABLE_HASHED / PRISM_TABLE_DENSE */
+  int32_t unique;         /* 1 when every key occurs once */
+  uint64_t slots;         /* slots (HASHED) or key range (DENSE) */
+  uint64_t device_bytes;  /* HBM held by the table */
+} prism_join_table_info;
+
+/* HashTable::build (hash_join.rs:87-141) on a device-resident int64 key column.
+ * Rows of one key are kept in ascending row order (as the reference's serial
+ * partition pass does, hash_join.rs:116-118).  `kind` = PRISM_TABLE_*. */
+prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_t n_rows,
+                                  int32_t kind, prism_join_table** out);
+prism_status prism_join_table_info_get(const prism_join_table* t, prism_join_table_info* out);
+void prism_join_table_free(prism_join_table* t);
+
+/* Result of one probe, written to device memory by the probe kernels. */
+typedef struct {
+  uint64_t matches;         /* total matched (probe, build) pairs, even beyond capacity */
+  uint64_t probe_sum_lo;    /* Σ probe_payload over matched pairs, int128 two's complement */
+  int64_t probe_sum_hi;
+  uint64_t build_sum_lo;    /* Σ build_payload[build_row] over matched pairs, int128 */
+  int64_t build_sum_hi;
+  uint32_t error;           /* non-zero: a bounded wait gave up (results invalid) */
+  uint32_t reserved;
+} prism_probe_result;
+
+/* HashTable::probe (hash_join.rs:143-200).  Emits pairs in the reference's
+ * order: probe row ascending, build rows ascending within a probe row.
+ * d_probe_idx / d_build_idx may both be NULL (count/sum only, the fused
+ * join->aggregate form).  Indices are written as row + probe_row_base (probe)
+ * and build row.  d_probe_payload (int64) and d_build_payload (int32) are
+ * optional; when given, their sums over the matched pairs are accumulated
+ * exactly in int128.  Asynchronous on the context stream: `d_result` is device
+ * memory and is valid after prism_ctx_synchronize.  When matches > capacity,
+ * only `capacity` pairs are written and the caller re-probes with more room. */
+prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t,
+                                  const int64_t* d_probe_keys, uint64_t n_probe,
+                                  uint32_t probe_row_base,
+                                  const int64_t* d_probe_payload, const int32_t* d_build_payload,
+                                  uint32_t* d_probe_idx, uint32_t* d_build_idx, uint64_t capacity,
+                                  prism_probe_result* d_result);
+
+/* ------------------------------------------------------------------------- */
+/* Hash aggregation, device layer (hash_aggregate.rs:182-339).                */
+/* ------------------------------------------------------------------------- */
+#define PRISM_AGG_SUM 0
+#define PRISM_AGG_COUNT 1
+#define PRISM_AGG_AVG 2
+#define PRISM_AGG_MIN 3
+#define PRISM_AGG_MAX 4
+#define PRISM_AGG_COUNT_DISTINCT 5
+
+typedef struct {
+  int32_t func;  /* PRISM_AGG_* */
+  int32_t value; /* index of the value program that feeds it */
+} prism_groupby_agg;
+
+typedef struct {
+  uint32_t n_columns;
+  const int32_t* column_types;      /* PRISM_TYPE_* per input column */
+  const uint32_t* key_columns;      /* group-by columns (FxHash of their values) */
+  uint32_t n_keys;
+  const prism_instr* filter;        /* predicate program, or NULL */
+  uint32_t n_filter;
+  const prism_instr* values;        /* value programs, concatenated */
+  const uint32_t* value_lengths;    /* length of each value program */
+  uint32_t n_values;
+  const prism_groupby_agg* aggs;
+  uint32_t n_aggs;
+  const prism_string* strings;      /* literals / LIKE patterns used by the programs */
+  uint32_t n_strings;
+} prism_groupby_plan;
+
+typedef struct prism_groupby prism_groupby;
+
+/* Per-group outputs, host arrays of at least n_groups entries (any may be NULL).
+ * Groups come in first-appearance order (hash_aggregate.rs:302-314). */
+typedef struct {
+  uint64_t* first_row;     /* global row (row_base + row) of each group's first row */
+  uint64_t* key_hash;      /* 64-bit FxHash group id (hash_aggregate.rs:483-504) */
+  uint64_t* row_count;     /* rows in the group */
+  double** agg_f64;        /* [n_aggs]: Float64 result as the reference outputs it */
+  int64_t** agg_i64;       /* [n_aggs]: Int64 result for COUNT / COUNT DISTINCT */
+  uint64_t** agg_sum128;   /* [n_aggs]: exact int128 SUM (lo, hi pairs) for integer inputs */
+} prism_groupby_output;
+
+/* expected_groups: sizing hint (0 = unknown); the table grows as needed. */
+prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan,
+                                  uint64_t expected_groups, prism_groupby** out);
+/* Stream one batch of rows into the resident group state (asynchronous). */
+prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns,
+                                  uint64_t n_rows, uint64_t row_base);
+/* Finish: order groups by first appearance; returns the group count (syncs). */
+prism_status prism_groupby_finish(prism_groupby* g, uint64_t* n_groups);
+/* Copy the finished groups into host arrays. */
+prism_status prism_groupby_read(prism_groupby* g, const prism_groupby_output* out);
+/* Clear the state for reuse without freeing device memory. */
+prism_status prism_groupby_reset(prism_groupby* g);
+void prism_groupby_destroy(prism_groupby* g);
+
+/* ------------------------------------------------------------------------- */
+/* Synthetic inputs (seeded, counter-based; bit-identical to the numpy twin    */
+/* in prism_executor.datagen).  Device pointers.                              */
+/* ------------------------------------------------------------------------- */
+/* out[i] = lo + floor(u32(i) * range / 2^32), u32(i) = high word of rnd(seed, stream, i) */
+prism_status prism_datagen_uniform_i64(prism_ctx* ctx, int64_t* d_out, uint64_t n, uint64_t seed,
+                                       uint32_t stream, int64_t lo, uint64_t range);
+prism_status prism_datagen_uniform_i32(prism_ctx* ctx, int32_t* d_out, uint64_t n, uint64_t seed,
+                                       uint32_t stream, int64_t lo, uint64_t range);
+/* out[i] = (int64) rnd(seed, stream, i): full 64-bit range */
+prism_status prism_datagen_bits_i64(prism_ctx* ctx, int64_t* d_out, uint64_t n, uint64_t seed,
+                                    uint32_t stream);
+/* out[i] = perm(i): a seeded bijection of [0, n) (Feistel with cycle walking) */
+prism_status prism_datagen_permutation_i64(prism_ctx* ctx, int64_t* d_out, uint64_t n,
+                                           uint64_t seed, uint32_t stream);
+/* out[i] = perm_V(rank - 1), rank ~ discrete CDF d_cdf[V] (u64 fixed point) */
+prism_status prism_datagen_zipf_i64(prism_ctx* ctx, int64_t* d_out, uint64_t n, uint64_t seed,
+                                    uint32_t stream, const uint64_t* d_cdf, uint64_t n_values,
+                                    uint32_t perm_stream);
+/* Utf8 column of one-byte strings: bytes[i] = alphabet[uniform(0, n_alpha)], offsets[i] = i */
+prism_status prism_datagen_flag_utf8(prism_ctx* ctx, uint8_t* d_bytes, int32_t* d_offsets,
+                                     uint64_t n, uint64_t seed, uint32_t stream,
+                                     const char* alphabet, uint32_t n_alpha);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* PRISM_HIP_H */

@@ -1,0 +1,209 @@
+// Shared host/device helpers for the gfx950 kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/prism_hip.h"
+
+namespace prism {
+
+// ---------------------------------------------------------------------------
+// Errors.  PrismError (native/prism-executor/src/error.rs:5-27) becomes an
+// exception inside the library and a status code at the C boundary.
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  prism_status status;
+  Error(prism_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+[[noreturn]] inline void fail(prism_status s, const std::string& m) { throw Error(s, m); }
+
+void set_last_error(const std::string& msg);
+void clear_last_error();
+
+template <class F>
+prism_status guarded(F&& f) {
+  try {
+    clear_last_error();
+    f();
+    return PRISM_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_last_error("Execution error: host allocation failed");
+    return PRISM_ERR_EXECUTION;
+  } catch (const std::exception& e) {
+    set_last_error(std::string("Internal error: ") + e.what());
+    return PRISM_ERR_INTERNAL;
+  } catch (...) {
+    set_last_error("Internal error: unknown exception");
+    return PRISM_ERR_INTERNAL;
+  }
+}
+
+#define PRISM_HIP(expr)                                                                  \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      ::prism::fail(PRISM_ERR_EXECUTION, std::string("Execution error: HIP ") + #expr +  \
+                                             " failed: " + hipGetErrorString(_e));       \
+  } while (0)
+
+#define PRISM_CHECK_LAUNCH() PRISM_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Device memory owned by a context.
+// ---------------------------------------------------------------------------
+struct Ctx;
+
+struct DeviceBuffer {
+  Ctx* ctx = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(Ctx* c, size_t n) { alloc(c, n); }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept { *this = std::move(o); }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+      release();
+      ctx = o.ctx; ptr = o.ptr; bytes = o.bytes;
+      o.ctx = nullptr; o.ptr = nullptr; o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DeviceBuffer() { release(); }
+  void alloc(Ctx* c, size_t n);
+  // Grow-only: keeps the allocation when it is already large enough.
+  void reserve(Ctx* c, size_t n) {
+    if (n > bytes) { release(); alloc(c, n); }
+  }
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace prism
+
+struct prism_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::atomic<uint64_t> device_bytes{0};
+  int num_cus = 256;
+  int num_xcds = 8;
+  size_t lds_per_block = 64 * 1024;
+  std::mutex mu;
+  // Small per-context device scratch (tile status words, counters, results).
+  prism::DeviceBuffer scratch;
+  void* pinned = nullptr;   // small pinned host staging for results
+  size_t pinned_bytes = 0;
+};
+
+namespace prism {
+struct Ctx : prism_ctx {};
+inline Ctx* as_ctx(prism_ctx* c) { return static_cast<Ctx*>(c); }
+
+// Make the context's device current on this thread.
+void use_device(prism_ctx* ctx);
+
+// Pinned host staging of at least n bytes owned by the context.
+void* pinned_staging(prism_ctx* ctx, size_t n);
+
+}  // namespace prism
+
+// ---------------------------------------------------------------------------
+// Device-side helpers.
+// ---------------------------------------------------------------------------
+namespace prism {
+
+// FxHasher multiplier (hash_aggregate.rs:47,53) and hash_i64_fast
+// (hash_join.rs:371-373).
+constexpr uint64_t kFxK = 0x517cc1b727220a95ULL;
+
+__host__ __device__ __forceinline__ uint64_t fx_step(uint64_t state, uint64_t word) {
+  return state * kFxK + word;
+}
+__host__ __device__ __forceinline__ uint64_t hash_i64_fast(int64_t v) {
+  return static_cast<uint64_t>(v) * kFxK;
+}
+
+// Monotone u64 encoding of an f64 (total order with -0 < +0; NaN is never fed).
+__host__ __device__ __forceinline__ uint64_t f64_order(double x) {
+  uint64_t b;
+  memcpy(&b, &x, 8);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+__host__ __device__ __forceinline__ double f64_unorder(uint64_t o) {
+  uint64_t b = (o >> 63) ? (o & 0x7fffffffffffffffULL) : ~o;
+  double x;
+  memcpy(&x, &b, 8);
+  return x;
+}
+
+// Native 128-bit vector of two int64 (usable with the non-temporal builtins).
+typedef long long v2i64 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+// Number of set bits of `mask` in lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Exact int128 accumulation with 64-bit atomics: the low word is added with
+// its old value returned, so the carry of this very addition is known and goes
+// into the high word together with the addend's high word.  Order independent.
+__device__ __forceinline__ void atomic_add_i128(unsigned long long* lo, unsigned long long* hi,
+                                                uint64_t add_lo, int64_t add_hi) {
+  unsigned long long old = atomicAdd(lo, static_cast<unsigned long long>(add_lo));
+  uint64_t carry = (old + add_lo) < old ? 1u : 0u;
+  uint64_t h = static_cast<uint64_t>(add_hi) + carry;
+  if (h) atomicAdd(hi, static_cast<unsigned long long>(h));
+}
+
+// Bounded spin: the look-back waits end after this many polls and flag an error.
+constexpr uint32_t kSpinLimit = 1u << 26;
+
+// Split accumulator of an int128 sum of int64 addends: low 32-bit halves summed
+// unsigned, high 32-bit halves summed signed.  Neither overflows for < 2^31 addends.
+struct Split128 {
+  uint64_t lo32 = 0;
+  int64_t hi32 = 0;
+  __host__ __device__ __forceinline__ void add(int64_t v) {
+    lo32 += static_cast<uint32_t>(static_cast<uint64_t>(v));
+    hi32 += static_cast<int64_t>(v >> 32);  // arithmetic shift: floor(v / 2^32)
+  }
+  // value = hi32 * 2^32 + lo32 as (lo, hi) int128 words.
+  __host__ __device__ __forceinline__ void to_i128(uint64_t& lo, int64_t& hi) const {
+    uint64_t hl = static_cast<uint64_t>(hi32) << 32;
+    int64_t hh = hi32 >> 32;
+    lo = hl + lo32;
+    hi = hh + ((lo < hl) ? 1 : 0);
+  }
+};
+
+__host__ __device__ __forceinline__ uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+inline uint64_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+inline int log2_pow2(uint64_t p) {
+  int l = 0;
+  while ((1ULL << l) < p) ++l;
+  return l;
+}
+
+}  // namespace prism

@@ -1,0 +1,801 @@
+// Hash aggregation on gfx950 (native/prism-executor/src/hash_aggregate.rs:182-447),
+// with the filter and projection stage fused into the load loop.
+//
+// Group identity is the reference's 64-bit FxHash of the key values
+// (hash_aggregate.rs:483-504), without a key-equality check, exactly as the
+// reference groups (hash_aggregate.rs:215,248-255).  Each workgroup keeps a
+// partial group table in LDS (open addressing on the group hash, bounded
+// probing); rows that do not fit go straight to the global table.  At the end
+// of the launch every LDS entry is merged into the global table in HBM with
+// atomics.  Output order is first appearance (minimum global row per group),
+// as hash_aggregate.rs:302-314 emits it.
+//
+// Accumulators:
+//   SUM of integers   - exact int128.  LDS holds split sums (low 32-bit halves
+//                       unsigned, high halves signed); HBM holds (lo, hi) words
+//                       updated with carry-propagating 64-bit atomics.
+//   SUM of f64        - integral values below 2^62 take the exact int128 path,
+//                       the rest an f64 atomic sum; the two parts add at the end.
+//   MIN / MAX         - f64 order-preserving u64 encoding, NaN never selected
+//                       (GroupAccumulator::accumulate, hash_aggregate.rs:135-140).
+//   COUNT             - row count per group (nulls included, hash_aggregate.rs:257-269);
+//                       the global (no GROUP BY) form counts non-null values
+//                       (hash_aggregate.rs:364-377).
+#include <hipcub/hipcub.hpp>
+
+#include <memory>
+#include <vector>
+
+#include "program.hpp"
+
+namespace prism {
+namespace gb {
+
+constexpr uint64_t kEmptyKey = ~0ULL;
+constexpr uint64_t kNoRow = ~0ULL;
+constexpr uint64_t kMix = 0x9E3779B97F4A7C15ULL;
+constexpr int kW_KEY = 0, kW_FIRST = 1, kW_COUNT = 2, kW_LANES = 3;
+constexpr int kBlock = 256;
+constexpr int kRows = 2;  // rows per lane per iteration
+constexpr int kLdsProbes = 8;
+constexpr size_t kLdsBudget = 64 * 1024;
+
+enum LaneOp : int32_t { L_SUM_I = 1, L_SUM_F = 2, L_MIN_F = 3, L_MAX_F = 4, L_CNT_VALID = 5 };
+
+struct Lane {
+  int32_t op;
+  int32_t value;
+  int32_t word;
+  int32_t is_int;  // value program yields int64 (converted for MIN/MAX)
+};
+
+struct DevPlan {
+  int32_t n_keys;
+  int32_t key_cols[kMaxKeys];
+  int32_t filter_start, filter_len;
+  int32_t n_values;
+  int32_t value_start[kMaxValues];
+  int32_t value_len[kMaxValues];
+  int32_t n_lanes;
+  Lane lanes[kMaxLanes];
+  int32_t stride;
+  int32_t skip_nulls;
+  uint64_t tmpl[kW_LANES + 3 * kMaxLanes];
+  prism_instr code[kMaxCode];
+  StrTable strs;
+};
+
+// Flags written by the kernels.
+constexpr uint32_t kFlagDropped = 1;   // a group could not be placed: redo with a larger table
+constexpr uint32_t kFlagOverflow = 2;  // exact int64 arithmetic overflowed
+constexpr uint32_t kFlagGrow = 4;      // global table above 3/4 load
+
+struct GTable {
+  unsigned long long* w;
+  uint64_t cap;  // power of two; slot `cap` is reserved for the hash value kEmptyKey
+  int shift;
+  int stride;
+  unsigned long long* n_groups;
+};
+
+__device__ __forceinline__ uint64_t hash_key(const prism_column& c, uint64_t row, uint64_t state) {
+  if (!col_valid(c, row)) return fx_step(state, 0);  // 0u8.hash(): write_u8(0)
+  const uint64_t i = c.offset + row;
+  switch (c.type) {
+    case PRISM_TYPE_INT64:
+      return fx_step(state, static_cast<uint64_t>(static_cast<const int64_t*>(c.data)[i]));
+    case PRISM_TYPE_INT32:  // write_i32 -> write_u64(i as u64): sign extension
+      return fx_step(state, static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[i])));
+    case PRISM_TYPE_FLOAT64:  // to_bits().hash()
+      return fx_step(state, static_cast<const uint64_t*>(c.data)[i]);
+    case PRISM_TYPE_DATE32:  // value word kept away from the null word 0
+      return fx_step(state, static_cast<uint64_t>(static_cast<uint32_t>(static_cast<const int32_t*>(c.data)[i])) |
+                                (1ULL << 32));
+    case PRISM_TYPE_BOOL:
+      return fx_step(state, 2 + (bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0));
+    default: {  // UTF8: str::hash = write(bytes) in 8-byte LE words, then write_u8(0xff)
+      const int32_t b = c.offsets[i], e = c.offsets[i + 1];
+      const uint8_t* s = static_cast<const uint8_t*>(c.data) + b;
+      const int32_t len = e - b;
+      for (int32_t p = 0; p < len; p += 8) {
+        uint64_t w = 0;
+        const int32_t m = len - p < 8 ? len - p : 8;
+        for (int32_t k = 0; k < m; ++k) w |= static_cast<uint64_t>(s[p + k]) << (8 * k);
+        state = fx_step(state, w);
+      }
+      return fx_step(state, 0xff);
+    }
+  }
+}
+
+__device__ __forceinline__ int lds_find(unsigned long long* lds, int stride, int slots, int shift, uint64_t h) {
+  uint32_t s = static_cast<uint32_t>((h * kMix) >> shift);
+  for (int p = 0; p < kLdsProbes; ++p) {
+    unsigned long long old = atomicCAS(&lds[static_cast<uint32_t>(s) * stride], kEmptyKey, h);
+    if (old == kEmptyKey || old == h) return static_cast<int>(s);
+    s = (s + 1) & (slots - 1);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int64_t g_find(const GTable& G, uint64_t h, uint32_t* flags) {
+  if (h == kEmptyKey) return static_cast<int64_t>(G.cap);
+  uint64_t s = (h * kMix) >> G.shift;
+  for (uint64_t p = 0; p < G.cap; ++p) {
+    unsigned long long* kp = &G.w[s * G.stride];
+    unsigned long long cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == h) return static_cast<int64_t>(s);
+    if (cur == kEmptyKey) {
+      unsigned long long old = atomicCAS(kp, kEmptyKey, h);
+      if (old == kEmptyKey) {
+        unsigned long long g = atomicAdd(G.n_groups, 1ULL) + 1;
+        if (g * 4 > G.cap * 3) atomicOr(flags, kFlagGrow);
+        return static_cast<int64_t>(s);
+      }
+      if (old == h) return static_cast<int64_t>(s);
+    }
+    s = (s + 1) & (G.cap - 1);
+  }
+  atomicOr(flags, kFlagDropped);
+  return -1;
+}
+
+__device__ __forceinline__ bool integral_small(double d) {
+  return fabs(d) < 4611686018427387904.0 && d == trunc(d);  // |d| < 2^62
+}
+
+__device__ __forceinline__ void lane_lds(unsigned long long* L, const Lane& ln, uint64_t v) {
+  switch (ln.op) {
+    case L_SUM_I: {
+      int64_t x = static_cast<int64_t>(v);
+      atomicAdd(&L[ln.word], static_cast<unsigned long long>(static_cast<uint32_t>(x)));
+      atomicAdd(&L[ln.word + 1], static_cast<unsigned long long>(x >> 32));
+      break;
+    }
+    case L_SUM_F: {
+      double d = as_f(v);
+      if (integral_small(d)) {
+        int64_t x = static_cast<int64_t>(d);
+        atomicAdd(&L[ln.word], static_cast<unsigned long long>(static_cast<uint32_t>(x)));
+        atomicAdd(&L[ln.word + 1], static_cast<unsigned long long>(x >> 32));
+      } else {
+        atomicAdd(reinterpret_cast<double*>(&L[ln.word + 2]), d);
+      }
+      break;
+    }
+    case L_MIN_F:
+    case L_MAX_F: {
+      double d = ln.is_int ? static_cast<double>(static_cast<int64_t>(v)) : as_f(v);
+      if (!isnan(d)) {
+        if (ln.op == L_MIN_F) atomicMin(&L[ln.word], static_cast<unsigned long long>(f64_order(d)));
+        else atomicMax(&L[ln.word], static_cast<unsigned long long>(f64_order(d)));
+      }
+      break;
+    }
+    default:  // L_CNT_VALID
+      atomicAdd(&L[ln.word], 1ULL);
+  }
+}
+
+__device__ __forceinline__ void lane_global(unsigned long long* G, const Lane& ln, uint64_t v) {
+  switch (ln.op) {
+    case L_SUM_I: {
+      int64_t x = static_cast<int64_t>(v);
+      atomic_add_i128(&G[ln.word], &G[ln.word + 1], static_cast<uint64_t>(x), x >> 63);
+      break;
+    }
+    case L_SUM_F: {
+      double d = as_f(v);
+      if (integral_small(d)) {
+        int64_t x = static_cast<int64_t>(d);
+        atomic_add_i128(&G[ln.word], &G[ln.word + 1], static_cast<uint64_t>(x), x >> 63);
+      } else {
+        unsafeAtomicAdd(reinterpret_cast<double*>(&G[ln.word + 2]), d);
+      }
+      break;
+    }
+    case L_MIN_F:
+    case L_MAX_F: {
+      double d = ln.is_int ? static_cast<double>(static_cast<int64_t>(v)) : as_f(v);
+      if (!isnan(d)) {
+        if (ln.op == L_MIN_F) atomicMin(&G[ln.word], static_cast<unsigned long long>(f64_order(d)));
+        else atomicMax(&G[ln.word], static_cast<unsigned long long>(f64_order(d)));
+      }
+      break;
+    }
+    default:
+      atomicAdd(&G[ln.word], 1ULL);
+  }
+}
+
+// Merge one LDS slot (split int128 sums) into a global slot (true int128).
+__device__ __forceinline__ void merge_slot(unsigned long long* G, const unsigned long long* L, const DevPlan* P) {
+  if (L[kW_COUNT]) atomicAdd(&G[kW_COUNT], L[kW_COUNT]);
+  atomicMin(&G[kW_FIRST], L[kW_FIRST]);
+  for (int l = 0; l < P->n_lanes; ++l) {
+    const Lane ln = P->lanes[l];
+    const int w = ln.word;
+    switch (ln.op) {
+      case L_SUM_I:
+      case L_SUM_F: {
+        Split128 sp;
+        sp.lo32 = L[w];
+        sp.hi32 = static_cast<int64_t>(L[w + 1]);
+        uint64_t lo;
+        int64_t hi;
+        sp.to_i128(lo, hi);
+        if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&G[w], &G[w + 1], lo, hi);
+        if (ln.op == L_SUM_F && L[w + 2]) unsafeAtomicAdd(reinterpret_cast<double*>(&G[w + 2]), as_f(L[w + 2]));
+        break;
+      }
+      case L_MIN_F:
+        if (L[w] != P->tmpl[w]) atomicMin(&G[w], L[w]);
+        break;
+      case L_MAX_F:
+        if (L[w] != P->tmpl[w]) atomicMax(&G[w], L[w]);
+        break;
+      default:
+        if (L[w]) atomicAdd(&G[w], L[w]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ P, const prism_column* __restrict__ cols, uint64_t n,
+                                                    uint64_t row_base, GTable G, int lds_slots, int lds_shift,
+                                                    uint32_t* flags) {
+  extern __shared__ unsigned long long lds[];
+  const int stride = P->stride;
+  for (int i = threadIdx.x; i < lds_slots * stride; i += blockDim.x) lds[i] = P->tmpl[i % stride];
+  __syncthreads();
+
+  const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRows;
+  for (uint64_t it = static_cast<uint64_t>(blockIdx.x) * blockDim.x * kRows; it < n; it += step) {
+    uint64_t row[kRows];
+    bool in[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      row[r] = it + static_cast<uint64_t>(r) * blockDim.x + threadIdx.x;
+      in[r] = row[r] < n;
+    }
+    if (P->filter_len) {
+      uint64_t fv[kRows];
+      uint32_t fvalid;
+      run_program<kRows>(P->code + P->filter_start, P->filter_len, cols, &P->strs, row, in, fv, fvalid, flags);
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) in[r] = in[r] && ((fvalid >> r) & 1) && (fv[r] & 1);
+    }
+    uint64_t h[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) h[r] = 0;
+    for (int k = 0; k < P->n_keys; ++k) {
+      const prism_column& c = cols[P->key_cols[k]];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if (in[r]) h[r] = hash_key(c, row[r], h[r]);
+    }
+    // Slot per row: LDS if it fits, else the global table.
+    int32_t lw[kRows];                 // LDS word index, -1 when the row goes to HBM
+    unsigned long long* gp[kRows];     // HBM slot
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      lw[r] = -1;
+      gp[r] = nullptr;
+      if (!in[r]) continue;
+      int ls = (lds_slots && h[r] != kEmptyKey) ? lds_find(lds, stride, lds_slots, lds_shift, h[r]) : -1;
+      if (ls >= 0) {
+        lw[r] = ls * stride;
+        atomicAdd(&lds[lw[r] + kW_COUNT], 1ULL);
+        atomicMin(&lds[lw[r] + kW_FIRST], static_cast<unsigned long long>(row_base + row[r]));
+      } else {
+        int64_t gs = g_find(G, h[r], flags);
+        if (gs < 0) {
+          in[r] = false;
+          continue;
+        }
+        gp[r] = &G.w[static_cast<uint64_t>(gs) * stride];
+        atomicAdd(&gp[r][kW_COUNT], 1ULL);
+        atomicMin(&gp[r][kW_FIRST], static_cast<unsigned long long>(row_base + row[r]));
+      }
+    }
+    for (int v = 0; v < P->n_values; ++v) {
+      uint64_t val[kRows];
+      uint32_t vvalid;
+      run_program<kRows>(P->code + P->value_start[v], P->value_len[v], cols, &P->strs, row, in, val, vvalid, flags);
+      for (int l = 0; l < P->n_lanes; ++l) {
+        const Lane ln = P->lanes[l];
+        if (ln.value != v) continue;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+          if (!in[r]) continue;
+          if (P->skip_nulls && !((vvalid >> r) & 1)) continue;
+          if (lw[r] >= 0) lane_lds(&lds[lw[r]], ln, val[r]);
+          else lane_global(gp[r], ln, val[r]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < lds_slots; s += blockDim.x) {
+    const unsigned long long* L = &lds[static_cast<uint32_t>(s) * stride];
+    if (L[kW_KEY] == kEmptyKey) continue;
+    int64_t gs = g_find(G, L[kW_KEY], flags);
+    if (gs < 0) continue;
+    merge_slot(&G.w[static_cast<uint64_t>(gs) * stride], L, P);
+  }
+}
+
+__global__ void k_fill_tmpl(unsigned long long* w, uint64_t n_words, const DevPlan* __restrict__ P) {
+  const int stride = P->stride;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += (uint64_t)gridDim.x * blockDim.x)
+    w[i] = P->tmpl[i % stride];
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+__global__ void k_clear_counters(unsigned long long* n_groups, uint32_t* flags, unsigned long long* n_found) {
+  *n_groups = 0;
+  *flags = 0;
+  *n_found = 0;
+}
+
+// Re-insert every occupied slot of `old` into `G` (keys are unique there).
+__global__ void k_rehash(const unsigned long long* __restrict__ old, uint64_t old_cap, GTable G, uint32_t* flags) {
+  const int stride = G.stride;
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s <= old_cap; s += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long* src = &old[s * stride];
+    if (src[kW_FIRST] == kNoRow) continue;
+    int64_t d = (s == old_cap) ? static_cast<int64_t>(G.cap) : g_find(G, src[kW_KEY], flags);
+    if (d < 0) continue;
+    unsigned long long* dst = &G.w[static_cast<uint64_t>(d) * stride];
+    for (int j = 1; j < stride; ++j) dst[j] = src[j];
+  }
+}
+
+__global__ void k_collect(const unsigned long long* __restrict__ w, uint64_t cap, int stride,
+                          unsigned long long* n_found, uint64_t* first_out, uint32_t* slot_out) {
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s <= cap; s += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t f = w[s * stride + kW_FIRST];
+    if (f == kNoRow) continue;
+    unsigned long long i = atomicAdd(n_found, 1ULL);
+    first_out[i] = f;
+    slot_out[i] = static_cast<uint32_t>(s);
+  }
+}
+
+__global__ void k_gather_slots(const unsigned long long* __restrict__ w, int stride, const uint32_t* __restrict__ order,
+                               uint64_t n, unsigned long long* out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n * stride; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t g = i / stride;
+    int j = static_cast<int>(i % stride);
+    out[i] = w[static_cast<uint64_t>(order[g]) * stride + j];
+  }
+}
+
+}  // namespace gb
+}  // namespace prism
+
+using namespace prism;
+using namespace prism::gb;
+
+struct prism_groupby {
+  Ctx* ctx = nullptr;
+  DevPlan plan{};
+  std::vector<int32_t> col_types;
+  std::vector<prism_groupby_agg> aggs;
+  std::vector<int32_t> value_types;     // ValType per value program
+  std::vector<int32_t> agg_lane_sum;    // lane index of the SUM lanes per agg (-1)
+  std::vector<int32_t> agg_lane_mm;     // MIN/MAX lane per agg (-1)
+  std::vector<int32_t> agg_lane_cnt;    // CNT_VALID lane per agg (-1)
+  DeviceBuffer d_plan;
+  DeviceBuffer table, backup;
+  uint64_t cap = 0;
+  uint64_t known_groups = 0;
+  DeviceBuffer counters;  // n_groups, flags, n_found
+  DeviceBuffer d_cols;    // column descriptors of the current update
+  DeviceBuffer result;    // finished groups, stride words each
+  uint64_t n_result = 0;
+  uint64_t expected = 0;
+  bool finished = false;
+
+  unsigned long long* n_groups_ptr() { return counters.as<unsigned long long>(); }
+  uint32_t* flags_ptr() { return reinterpret_cast<uint32_t*>(counters.as<char>() + 8); }
+  unsigned long long* n_found_ptr() { return reinterpret_cast<unsigned long long*>(counters.as<char>() + 16); }
+  size_t slot_bytes() const { return static_cast<size_t>(plan.stride) * 8; }
+  GTable view() {
+    GTable G;
+    G.w = table.as<unsigned long long>();
+    G.cap = cap;
+    G.shift = 64 - log2_pow2(cap);
+    G.stride = plan.stride;
+    G.n_groups = n_groups_ptr();
+    return G;
+  }
+};
+
+namespace prism {
+namespace gb {
+
+static dim3 fill_grid(prism_ctx* ctx, uint64_t n) {
+  uint64_t b = ceil_div(n ? n : 1, 256);
+  uint64_t cap = static_cast<uint64_t>(ctx->num_cus) * 8;
+  return dim3(static_cast<unsigned>(b < cap ? b : cap));
+}
+
+static void alloc_table(prism_groupby* g, uint64_t cap) {
+  g->table.alloc(g->ctx, (cap + 1) * g->slot_bytes());
+  g->cap = cap;
+  hipLaunchKernelGGL(k_fill_tmpl, fill_grid(g->ctx, (cap + 1) * g->plan.stride), dim3(256), 0, g->ctx->stream,
+                     g->table.as<unsigned long long>(), (cap + 1) * g->plan.stride, g->d_plan.as<DevPlan>());
+  PRISM_CHECK_LAUNCH();
+}
+
+static void read_counters(prism_groupby* g, uint64_t& groups, uint32_t& flags) {
+  struct {
+    unsigned long long n;
+    uint32_t f;
+    uint32_t pad;
+  } h;
+  PRISM_HIP(hipMemcpyAsync(&h, g->counters.ptr, 16, hipMemcpyDeviceToHost, g->ctx->stream));
+  PRISM_HIP(hipStreamSynchronize(g->ctx->stream));
+  groups = h.n;
+  flags = h.f;
+}
+
+// Move the table into one of `new_cap` slots.
+static void grow(prism_groupby* g, uint64_t new_cap) {
+  DeviceBuffer old = std::move(g->table);
+  const uint64_t old_cap = g->cap;
+  alloc_table(g, new_cap);
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, g->ctx->stream, g->n_groups_ptr(), 0ULL);
+  hipLaunchKernelGGL(k_rehash, fill_grid(g->ctx, old_cap + 1), dim3(256), 0, g->ctx->stream,
+                     old.as<unsigned long long>(), old_cap, g->view(), g->flags_ptr());
+  PRISM_CHECK_LAUNCH();
+  PRISM_HIP(hipStreamSynchronize(g->ctx->stream));
+}
+
+static void launch(prism_groupby* g, const prism_column* cols, uint64_t n_rows, uint64_t row_base) {
+  Ctx* c = g->ctx;
+  const size_t sb = g->slot_bytes();
+  int lds_slots = 0, lds_shift = 0;
+  uint64_t want = g->expected ? g->expected : 512;
+  if (g->known_groups > want) want = g->known_groups;
+  uint64_t s = next_pow2(want * 2 < 64 ? 64 : want * 2);
+  uint64_t fit = 1;
+  while ((fit * 2) * sb <= kLdsBudget) fit *= 2;
+  if (s > fit) s = fit;
+  if (want <= s && s >= 16) {
+    lds_slots = static_cast<int>(s);
+    lds_shift = 64 - log2_pow2(s);
+  }
+  uint64_t blocks = ceil_div(n_rows, static_cast<uint64_t>(kBlock) * kRows);
+  uint64_t max_blocks = static_cast<uint64_t>(c->num_cus) * (lds_slots ? 2 : 8);
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_groupby, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), lds_slots * sb, c->stream,
+                     g->d_plan.as<DevPlan>(), cols, n_rows, row_base, g->view(), lds_slots, lds_shift,
+                     g->flags_ptr());
+  PRISM_CHECK_LAUNCH();
+}
+
+}  // namespace gb
+}  // namespace prism
+
+extern "C" {
+
+prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan, uint64_t expected_groups,
+                                  prism_groupby** out) {
+  return guarded([&] {
+    if (!ctx || !plan || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/plan/output");
+    *out = nullptr;
+    if (plan->n_columns > static_cast<uint32_t>(kMaxCols))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 input columns");
+    if (plan->n_keys > static_cast<uint32_t>(kMaxKeys))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 8 group-by columns");
+    if (plan->n_values > static_cast<uint32_t>(kMaxValues))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 value programs");
+    if (plan->n_columns && !plan->column_types) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL column types");
+    use_device(ctx);
+    auto g = std::make_unique<prism_groupby>();
+    g->ctx = as_ctx(ctx);
+    g->expected = expected_groups;
+    g->col_types.assign(plan->column_types, plan->column_types + plan->n_columns);
+    DevPlan& D = g->plan;
+    memset(&D, 0, sizeof(D));
+    build_str_table(plan->strings, plan->n_strings, D.strs);
+
+    D.n_keys = static_cast<int32_t>(plan->n_keys);
+    for (uint32_t k = 0; k < plan->n_keys; ++k) {
+      uint32_t col = plan->key_columns[k];
+      if (col >= plan->n_columns) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: group-by column out of range");
+      D.key_cols[k] = static_cast<int32_t>(col);
+    }
+    int32_t code_len = 0;
+    auto append = [&](const prism_instr* code, uint32_t len) {
+      if (code_len + static_cast<int64_t>(len) > kMaxCode)
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: row programs exceed 160 instructions");
+      memcpy(&D.code[code_len], code, len * sizeof(prism_instr));
+      int32_t start = code_len;
+      code_len += static_cast<int32_t>(len);
+      return start;
+    };
+    if (plan->filter && plan->n_filter) {
+      ValType t = check_program(plan->filter, plan->n_filter, plan->column_types, plan->n_columns,
+                                plan->n_strings, nullptr);
+      if (t != VT_B && t != VT_NULL) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: filter is not boolean");
+      D.filter_start = append(plan->filter, plan->n_filter);
+      D.filter_len = static_cast<int32_t>(plan->n_filter);
+    }
+    D.n_values = static_cast<int32_t>(plan->n_values);
+    uint32_t off = 0;
+    for (uint32_t v = 0; v < plan->n_values; ++v) {
+      uint32_t len = plan->value_lengths[v];
+      ValType t = check_program(plan->values + off, len, plan->column_types, plan->n_columns, plan->n_strings,
+                                nullptr);
+      g->value_types.push_back(t);
+      D.value_start[v] = append(plan->values + off, len);
+      D.value_len[v] = static_cast<int32_t>(len);
+      off += len;
+    }
+    const bool global = plan->n_keys == 0;
+    D.skip_nulls = global ? 1 : 0;
+    int32_t word = kW_LANES;
+    D.tmpl[kW_KEY] = kEmptyKey;
+    D.tmpl[kW_FIRST] = kNoRow;
+    D.tmpl[kW_COUNT] = 0;
+    auto add_lane = [&](int32_t op, int32_t value, int32_t is_int) {
+      if (D.n_lanes >= kMaxLanes) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: too many aggregates");
+      Lane& L = D.lanes[D.n_lanes];
+      L.op = op;
+      L.value = value;
+      L.word = word;
+      L.is_int = is_int;
+      int words = op == L_SUM_I ? 2 : op == L_SUM_F ? 3 : 1;
+      for (int i = 0; i < words; ++i) D.tmpl[word + i] = 0;
+      if (op == L_MIN_F) D.tmpl[word] = f64_order(INFINITY);
+      if (op == L_MAX_F) D.tmpl[word] = f64_order(-INFINITY);
+      word += words;
+      return D.n_lanes++;
+    };
+    for (uint32_t a = 0; a < plan->n_aggs; ++a) {
+      const prism_groupby_agg& A = plan->aggs[a];
+      g->aggs.push_back(A);
+      int32_t ls = -1, lm = -1, lc = -1;
+      if (A.func == PRISM_AGG_COUNT_DISTINCT)
+        fail(PRISM_ERR_UNSUPPORTED_AGGREGATION,
+             "Aggregation error: unsupported function COUNT DISTINCT in the device aggregation");
+      if (A.func < PRISM_AGG_SUM || A.func > PRISM_AGG_MAX)
+        fail(PRISM_ERR_UNSUPPORTED_AGGREGATION, "Aggregation error: unsupported function " + std::to_string(A.func));
+      if ((A.value < 0 || static_cast<uint32_t>(A.value) >= plan->n_values) && A.func != PRISM_AGG_COUNT)
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: aggregate value index out of range");
+      if (A.value >= 0 && static_cast<uint32_t>(A.value) >= plan->n_values)
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: aggregate value index out of range");
+      int32_t vt = (A.value >= 0 && static_cast<uint32_t>(A.value) < plan->n_values) ? g->value_types[A.value] : VT_I;
+      if (vt == VT_NULL) vt = VT_I;
+      bool is_int = vt != VT_F;
+      switch (A.func) {
+        case PRISM_AGG_SUM:
+        case PRISM_AGG_AVG:
+          ls = add_lane(is_int ? L_SUM_I : L_SUM_F, A.value, is_int);
+          break;
+        case PRISM_AGG_MIN:
+          lm = add_lane(L_MIN_F, A.value, is_int);
+          break;
+        case PRISM_AGG_MAX:
+          lm = add_lane(L_MAX_F, A.value, is_int);
+          break;
+        default:
+          break;
+      }
+      if (global && A.func != PRISM_AGG_SUM && A.func != PRISM_AGG_MIN && A.func != PRISM_AGG_MAX && A.value >= 0)
+        lc = add_lane(L_CNT_VALID, A.value, is_int);
+      g->agg_lane_sum.push_back(ls);
+      g->agg_lane_mm.push_back(lm);
+      g->agg_lane_cnt.push_back(lc);
+    }
+    D.stride = word;
+    // Lanes are visited per value program; keep the order stable.
+    g->d_plan.alloc(g->ctx, sizeof(DevPlan));
+    g->counters.alloc(g->ctx, 256);
+    PRISM_HIP(hipMemcpyAsync(g->d_plan.ptr, &D, sizeof(DevPlan), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, ctx->stream, g->n_groups_ptr(), g->flags_ptr(),
+                       g->n_found_ptr());
+    PRISM_CHECK_LAUNCH();
+    uint64_t cap = next_pow2(expected_groups * 2 < 1024 ? 1024 : expected_groups * 2);
+    alloc_table(g.get(), cap);
+    PRISM_HIP(hipStreamSynchronize(ctx->stream));
+    *out = g.release();
+  });
+}
+
+prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns, uint64_t n_rows, uint64_t row_base) {
+  return guarded([&] {
+    if (!g) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL group-by state");
+    if (n_rows == 0) return;
+    if (!columns && !g->col_types.empty()) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL columns");
+    use_device(g->ctx);
+    std::lock_guard<std::mutex> lock(g->ctx->mu);
+    g->finished = false;
+    ColsArg cols;
+    memset(&cols, 0, sizeof(cols));
+    for (size_t i = 0; i < g->col_types.size(); ++i) {
+      if (columns[i].type != g->col_types[i])
+        fail(PRISM_ERR_SCHEMA_MISMATCH, "Schema mismatch: column " + std::to_string(i) + " type differs from the plan");
+      if (!columns[i].data && n_rows) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL column data");
+      if (columns[i].type == PRISM_TYPE_UTF8 && !columns[i].offsets)
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: Utf8 column without offsets");
+      cols.c[i] = columns[i];
+    }
+    hipStream_t st = g->ctx->stream;
+    g->d_cols.reserve(g->ctx, sizeof(ColsArg));
+    PRISM_HIP(hipMemcpyAsync(g->d_cols.ptr, &cols, sizeof(ColsArg), hipMemcpyHostToDevice, st));
+    const prism_column* dcols = g->d_cols.as<prism_column>();
+    while (true) {
+      // A snapshot is needed only when this batch could overfill the table.
+      const bool risky = (g->known_groups + n_rows) * 4 > g->cap * 3;
+      if (risky) {
+        g->backup.reserve(g->ctx, g->table.bytes);
+        PRISM_HIP(hipMemcpyAsync(g->backup.ptr, g->table.ptr, g->table.bytes, hipMemcpyDeviceToDevice, st));
+      }
+      launch(g, dcols, n_rows, row_base);
+      uint64_t groups;
+      uint32_t flags;
+      read_counters(g, groups, flags);
+      if (flags & kFlagOverflow)
+        fail(PRISM_ERR_EXECUTION, "Execution error: int64 overflow in exact integer arithmetic");
+      if (flags & kFlagDropped) {
+        if (!risky) fail(PRISM_ERR_INTERNAL, "Internal error: group table overflow without snapshot");
+        PRISM_HIP(hipMemcpyAsync(g->table.ptr, g->backup.ptr, g->table.bytes, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, g->flags_ptr(), 0u);
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, g->n_groups_ptr(), static_cast<unsigned long long>(g->known_groups));
+        PRISM_HIP(hipStreamSynchronize(st));
+        uint64_t target = next_pow2((groups > g->known_groups ? groups : g->known_groups) * 4);
+        grow(g, target > g->cap * 4 ? target : g->cap * 4);
+        read_counters(g, g->known_groups, flags);
+        continue;
+      }
+      g->known_groups = groups;
+      if (flags & kFlagGrow) {
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, g->flags_ptr(), 0u);
+        grow(g, g->cap * 4);
+        read_counters(g, g->known_groups, flags);
+      }
+      break;
+    }
+  });
+}
+
+prism_status prism_groupby_finish(prism_groupby* g, uint64_t* n_groups) {
+  return guarded([&] {
+    if (!g || !n_groups) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL group-by state/output");
+    use_device(g->ctx);
+    std::lock_guard<std::mutex> lock(g->ctx->mu);
+    hipStream_t st = g->ctx->stream;
+    Ctx* c = g->ctx;
+    const uint64_t slots = g->cap + 1;
+    DeviceBuffer first(c, slots * 8), first_sorted(c, slots * 8), idx(c, slots * 4), idx_sorted(c, slots * 4);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, g->n_found_ptr(), 0ULL);
+    hipLaunchKernelGGL(k_collect, fill_grid(g->ctx, slots), dim3(256), 0, st, g->table.as<unsigned long long>(),
+                       g->cap, g->plan.stride, g->n_found_ptr(), first.as<uint64_t>(), idx.as<uint32_t>());
+    PRISM_CHECK_LAUNCH();
+    unsigned long long found = 0;
+    PRISM_HIP(hipMemcpyAsync(&found, g->n_found_ptr(), 8, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+    if (found > 1) {
+      size_t tmp_bytes = 0;
+      PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, first.as<uint64_t>(), first_sorted.as<uint64_t>(),
+                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(),
+                                                   static_cast<int>(found), 0, 64, st));
+      DeviceBuffer tmp(c, tmp_bytes);
+      PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, first.as<uint64_t>(), first_sorted.as<uint64_t>(),
+                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(),
+                                                   static_cast<int>(found), 0, 64, st));
+    } else if (found == 1) {
+      PRISM_HIP(hipMemcpyAsync(idx_sorted.ptr, idx.ptr, 4, hipMemcpyDeviceToDevice, st));
+    }
+    g->result.reserve(c, (found ? found : 1) * g->slot_bytes());
+    if (found)
+      hipLaunchKernelGGL(k_gather_slots, fill_grid(g->ctx, found * g->plan.stride), dim3(256), 0, st,
+                         g->table.as<unsigned long long>(), g->plan.stride, idx_sorted.as<uint32_t>(), found,
+                         g->result.as<unsigned long long>());
+    PRISM_CHECK_LAUNCH();
+    PRISM_HIP(hipStreamSynchronize(st));
+    g->n_result = found;
+    g->finished = true;
+    *n_groups = found;
+  });
+}
+
+static double i128_to_f64(uint64_t lo, int64_t hi) {
+  __int128 v = static_cast<__int128>((static_cast<unsigned __int128>(static_cast<uint64_t>(hi)) << 64) | lo);
+  return static_cast<double>(v);  // correctly rounded (round to nearest even)
+}
+
+prism_status prism_groupby_read(prism_groupby* g, const prism_groupby_output* out) {
+  return guarded([&] {
+    if (!g || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL group-by state/output");
+    if (!g->finished) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: call prism_groupby_finish first");
+    use_device(g->ctx);
+    const int stride = g->plan.stride;
+    const uint64_t n = g->n_result;
+    std::vector<uint64_t> h(n * stride);
+    if (n) {
+      PRISM_HIP(hipMemcpyAsync(h.data(), g->result.ptr, n * stride * 8, hipMemcpyDeviceToHost, g->ctx->stream));
+      PRISM_HIP(hipStreamSynchronize(g->ctx->stream));
+    }
+    const DevPlan& D = g->plan;
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint64_t* s = &h[i * stride];
+      if (out->first_row) out->first_row[i] = s[kW_FIRST];
+      if (out->key_hash) out->key_hash[i] = s[kW_KEY];
+      if (out->row_count) out->row_count[i] = s[kW_COUNT];
+      for (size_t a = 0; a < g->aggs.size(); ++a) {
+        const int func = g->aggs[a].func;
+        double sum_f = 0.0;
+        uint64_t slo = 0;
+        int64_t shi = 0;
+        if (g->agg_lane_sum[a] >= 0) {
+          const Lane& L = D.lanes[g->agg_lane_sum[a]];
+          slo = s[L.word];
+          shi = static_cast<int64_t>(s[L.word + 1]);
+          sum_f = i128_to_f64(slo, shi);
+          if (L.op == L_SUM_F) {
+            double fpart;
+            memcpy(&fpart, &s[L.word + 2], 8);
+            sum_f = (slo == 0 && shi == 0) ? fpart : sum_f + fpart;
+          }
+        }
+        const uint64_t count = g->agg_lane_cnt[a] >= 0 ? s[D.lanes[g->agg_lane_cnt[a]].word] : s[kW_COUNT];
+        double f = 0.0;
+        switch (func) {
+          case PRISM_AGG_SUM: f = sum_f; break;
+          case PRISM_AGG_COUNT: f = static_cast<double>(count); break;
+          case PRISM_AGG_AVG: f = count ? sum_f / static_cast<double>(count) : 0.0; break;
+          case PRISM_AGG_MIN:
+          case PRISM_AGG_MAX: {
+            const Lane& L = D.lanes[g->agg_lane_mm[a]];
+            f = f64_unorder(s[L.word]);
+            if (D.skip_nulls) {  // global_aggregate_batches: an empty fold gives 0.0 (hash_aggregate.rs:401-413)
+              if (func == PRISM_AGG_MIN && f == INFINITY) f = 0.0;
+              if (func == PRISM_AGG_MAX && f == -INFINITY) f = 0.0;
+            } else if (s[kW_COUNT] == 0) {
+              f = 0.0;
+            }
+            break;
+          }
+        }
+        if (out->agg_f64 && out->agg_f64[a]) out->agg_f64[a][i] = f;
+        if (out->agg_i64 && out->agg_i64[a]) out->agg_i64[a][i] = static_cast<int64_t>(count);
+        if (out->agg_sum128 && out->agg_sum128[a]) {
+          out->agg_sum128[a][2 * i] = slo;
+          out->agg_sum128[a][2 * i + 1] = static_cast<uint64_t>(shi);
+        }
+      }
+    }
+  });
+}
+
+prism_status prism_groupby_reset(prism_groupby* g) {
+  return guarded([&] {
+    if (!g) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL group-by state");
+    use_device(g->ctx);
+    std::lock_guard<std::mutex> lock(g->ctx->mu);
+    hipLaunchKernelGGL(k_fill_tmpl, fill_grid(g->ctx, (g->cap + 1) * g->plan.stride), dim3(256), 0, g->ctx->stream,
+                       g->table.as<unsigned long long>(), (g->cap + 1) * g->plan.stride, g->d_plan.as<DevPlan>());
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, g->ctx->stream, g->n_groups_ptr(), g->flags_ptr(),
+                       g->n_found_ptr());
+    PRISM_CHECK_LAUNCH();
+    g->known_groups = 0;
+    g->finished = false;
+    g->n_result = 0;
+  });
+}
+
+void prism_groupby_destroy(prism_groupby* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+}
+
+}  // extern "C"

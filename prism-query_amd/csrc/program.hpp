@@ -1,0 +1,351 @@
+// Row programs: the fused predicate / projection stage evaluated inside the
+// aggregation and filter kernels.  Predicate (filter_project.rs:22-51) and
+// ScalarExpr (filter_project.rs:195-213) trees are compiled on the host into
+// postfix code; every lane runs the same instruction at the same time, so the
+// dispatch is scalar and the value stack lives in VGPRs (uniform stack index).
+#pragma once
+
+#include "common.hpp"
+
+namespace prism {
+
+constexpr int kMaxCols = 16;
+constexpr int kMaxKeys = 8;
+constexpr int kMaxValues = 16;
+constexpr int kMaxLanes = 48;
+constexpr int kMaxCode = 160;
+constexpr int kMaxStrings = 16;
+constexpr int kMaxStrBytes = 1024;
+constexpr int kStackDepth = 8;
+
+// Static result type of a program.
+enum ValType : int32_t { VT_I = 0, VT_F = 1, VT_B = 2, VT_NULL = 3 };
+
+struct ColsArg {
+  prism_column c[kMaxCols];
+};
+
+struct StrTable {
+  int32_t n;
+  int32_t off[kMaxStrings + 1];
+  uint8_t bytes[kMaxStrBytes];
+};
+
+// Host-side checks: validates stack discipline and operand types, returns the
+// result type.  Throws Error(InvalidArgument) on a malformed program.
+ValType check_program(const prism_instr* code, uint32_t len, const int32_t* col_types, uint32_t n_cols,
+                      uint32_t n_strings, int* max_depth);
+
+// Build the device string table.
+void build_str_table(const prism_string* s, uint32_t n, StrTable& out);
+
+#if defined(__HIPCC__)
+
+__device__ __forceinline__ bool bit_at(const uint8_t* bm, uint64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
+
+__device__ __forceinline__ bool col_valid(const prism_column& c, uint64_t row) {
+  return c.validity == nullptr || bit_at(c.validity, c.offset + row);
+}
+
+__device__ __forceinline__ int64_t col_load_i(const prism_column& c, uint64_t row) {
+  const uint64_t i = c.offset + row;
+  switch (c.type) {
+    case PRISM_TYPE_INT64:
+      return static_cast<const int64_t*>(c.data)[i];
+    case PRISM_TYPE_BOOL:
+      return bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0;
+    default:  // INT32, DATE32
+      return static_cast<const int32_t*>(c.data)[i];
+  }
+}
+__device__ __forceinline__ double col_load_f(const prism_column& c, uint64_t row) {
+  return static_cast<const double*>(c.data)[c.offset + row];
+}
+
+__device__ __forceinline__ double as_f(uint64_t b) { return __longlong_as_double(static_cast<long long>(b)); }
+__device__ __forceinline__ uint64_t f_bits(double d) { return static_cast<uint64_t>(__double_as_longlong(d)); }
+
+__device__ __forceinline__ bool cmp_i(int op, int64_t a, int64_t b) {
+  switch (op) {
+    case PRISM_CMP_EQ: return a == b;
+    case PRISM_CMP_NE: return a != b;
+    case PRISM_CMP_LT: return a < b;
+    case PRISM_CMP_LE: return a <= b;
+    case PRISM_CMP_GT: return a > b;
+    default: return a >= b;
+  }
+}
+__device__ __forceinline__ bool cmp_f(int op, double a, double b) {
+  switch (op) {
+    case PRISM_CMP_EQ: return a == b;
+    case PRISM_CMP_NE: return a != b;
+    case PRISM_CMP_LT: return a < b;
+    case PRISM_CMP_LE: return a <= b;
+    case PRISM_CMP_GT: return a > b;
+    default: return a >= b;
+  }
+}
+
+// Byte-wise lexicographic compare of a utf8 value with a literal
+// (Rust str ordering = byte ordering).
+__device__ __forceinline__ int str_cmp(const uint8_t* a, int32_t na, const uint8_t* b, int32_t nb) {
+  int32_t n = na < nb ? na : nb;
+  for (int32_t i = 0; i < n; ++i) {
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  }
+  return na == nb ? 0 : (na < nb ? -1 : 1);
+}
+
+__device__ __forceinline__ uint8_t fold_ascii(uint8_t c) { return (c >= 'A' && c <= 'Z') ? c + 32 : c; }
+
+// Length of the UTF-8 sequence starting with byte c (1 for invalid lead bytes).
+__device__ __forceinline__ int utf8_len(uint8_t c) {
+  return c < 0x80 ? 1 : (c >> 5) == 6 ? 2 : (c >> 4) == 14 ? 3 : (c >> 3) == 30 ? 4 : 1;
+}
+
+// SQL LIKE: '%' any sequence, '_' one character, '\' escapes the next pattern
+// character.  Greedy match with backtracking to the last '%' (linear for the
+// common '%lit%' shapes).
+__device__ inline bool like_match(const uint8_t* s, int32_t ns, const uint8_t* p, int32_t np, bool ci) {
+  int32_t si = 0, pi = 0, star_p = -1, star_s = 0;
+  while (si < ns) {
+    if (pi < np) {
+      uint8_t pc = p[pi];
+      if (pc == '%') {
+        star_p = ++pi;
+        star_s = si;
+        continue;
+      }
+      if (pc == '_') {
+        si += utf8_len(s[si]);
+        ++pi;
+        continue;
+      }
+      int32_t pj = pi;
+      if (pc == '\\' && pi + 1 < np) {
+        pj = pi + 1;
+        pc = p[pj];
+      }
+      uint8_t sc = s[si];
+      if (ci ? fold_ascii(sc) == fold_ascii(pc) : sc == pc) {
+        si += 1;
+        pi = pj + 1;
+        continue;
+      }
+    }
+    if (star_p < 0) return false;
+    star_s += utf8_len(s[star_s]);
+    si = star_s;
+    pi = star_p;
+  }
+  while (pi < np && p[pi] == '%') ++pi;
+  return pi == np;
+}
+
+// Evaluate `len` instructions for R rows held by this lane.  Values are raw
+// 64-bit words (int64 or f64 bits); `valid` bit r = row r non-null.
+template <int R>
+__device__ __forceinline__ void run_program(const prism_instr* __restrict__ code, int len,
+                                            const prism_column* __restrict__ cols,
+                                            const StrTable* __restrict__ strs, const uint64_t (&row)[R],
+                                            const bool (&in)[R], uint64_t (&out)[R], uint32_t& out_valid,
+                                            uint32_t* err) {
+  uint64_t st[kStackDepth * R];
+  // Validity bits of stack slot k live in bits [k*R, k*R + R) of one word.
+  uint32_t vbits = 0;
+  constexpr uint32_t kAll = (1u << R) - 1;
+  auto vget = [&](int k) -> uint32_t { return (vbits >> (k * R)) & kAll; };
+  auto vset = [&](int k, uint32_t v) { vbits = (vbits & ~(kAll << (k * R))) | ((v & kAll) << (k * R)); };
+  int sp = 0;
+  for (int pc = 0; pc < len; ++pc) {
+    const int op = code[pc].op;
+    const int arg = code[pc].arg;
+    switch (op) {
+      case PRISM_OP_LOAD_I: {
+        const prism_column& c = cols[arg];
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          st[sp * R + r] = in[r] ? static_cast<uint64_t>(col_load_i(c, row[r])) : 0;
+          v |= (in[r] && col_valid(c, row[r])) ? (1u << r) : 0u;
+        }
+        vset(sp++, v);
+        break;
+      }
+      case PRISM_OP_LOAD_F: {
+        const prism_column& c = cols[arg];
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          st[sp * R + r] = in[r] ? f_bits(col_load_f(c, row[r])) : 0;
+          v |= (in[r] && col_valid(c, row[r])) ? (1u << r) : 0u;
+        }
+        vset(sp++, v);
+        break;
+      }
+      case PRISM_OP_CONST_I:
+      case PRISM_OP_CONST_BOOL: {
+        const uint64_t x = op == PRISM_OP_CONST_I ? static_cast<uint64_t>(code[pc].i64) : (code[pc].i64 != 0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[sp * R + r] = x;
+        vset(sp++, kAll);
+        break;
+      }
+      case PRISM_OP_CONST_F: {
+        const uint64_t x = f_bits(code[pc].f64);
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[sp * R + r] = x;
+        vset(sp++, kAll);
+        break;
+      }
+      case PRISM_OP_CONST_NULL: {
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[sp * R + r] = 0;
+        vset(sp++, 0);
+        break;
+      }
+      case PRISM_OP_I2F: {
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[(sp - 1) * R + r] = f_bits(static_cast<double>(static_cast<int64_t>(st[(sp - 1) * R + r])));
+        break;
+      }
+      case PRISM_OP_NEG_F: {
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[(sp - 1) * R + r] = f_bits(-as_f(st[(sp - 1) * R + r]));
+        break;
+      }
+      case PRISM_OP_NEG_I: {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int64_t a = static_cast<int64_t>(st[(sp - 1) * R + r]), o;
+          if (__builtin_sub_overflow(static_cast<int64_t>(0), a, &o) && ((vget(sp - 1) >> r) & 1)) atomicOr(err, 2u);
+          st[(sp - 1) * R + r] = static_cast<uint64_t>(o);
+        }
+        break;
+      }
+      case PRISM_OP_ADD_F:
+      case PRISM_OP_SUB_F:
+      case PRISM_OP_MUL_F:
+      case PRISM_OP_DIV_F: {
+        --sp;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          double a = as_f(st[(sp - 1) * R + r]), b = as_f(st[sp * R + r]), o;
+          o = op == PRISM_OP_ADD_F ? a + b : op == PRISM_OP_SUB_F ? a - b : op == PRISM_OP_MUL_F ? a * b : a / b;
+          st[(sp - 1) * R + r] = f_bits(o);
+        }
+        vset(sp - 1, vget(sp - 1) & vget(sp));
+        break;
+      }
+      case PRISM_OP_ADD_I:
+      case PRISM_OP_SUB_I:
+      case PRISM_OP_MUL_I: {
+        --sp;
+        const uint32_t both = vget(sp - 1) & vget(sp);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int64_t a = static_cast<int64_t>(st[(sp - 1) * R + r]), b = static_cast<int64_t>(st[sp * R + r]), o;
+          bool ov = op == PRISM_OP_ADD_I ? __builtin_add_overflow(a, b, &o)
+                    : op == PRISM_OP_SUB_I ? __builtin_sub_overflow(a, b, &o)
+                                           : __builtin_mul_overflow(a, b, &o);
+          if (ov && ((both >> r) & 1)) atomicOr(err, 2u);
+          st[(sp - 1) * R + r] = static_cast<uint64_t>(o);
+        }
+        vset(sp - 1, both);
+        break;
+      }
+      case PRISM_OP_CMP_I: {
+        --sp;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          st[(sp - 1) * R + r] = cmp_i(arg, static_cast<int64_t>(st[(sp - 1) * R + r]), static_cast<int64_t>(st[sp * R + r]));
+        vset(sp - 1, vget(sp - 1) & vget(sp));
+        break;
+      }
+      case PRISM_OP_CMP_F: {
+        --sp;
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[(sp - 1) * R + r] = cmp_f(arg, as_f(st[(sp - 1) * R + r]), as_f(st[sp * R + r]));
+        vset(sp - 1, vget(sp - 1) & vget(sp));
+        break;
+      }
+      case PRISM_OP_CMP_STR:
+      case PRISM_OP_LIKE:
+      case PRISM_OP_ILIKE: {
+        const prism_column& c = cols[arg];
+        const int sidx = static_cast<int>(code[pc].i64 & 0xff);
+        const int cop = static_cast<int>(code[pc].i64 >> 8);
+        const uint8_t* lit = strs->bytes + strs->off[sidx];
+        const int32_t nlit = strs->off[sidx + 1] - strs->off[sidx];
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          bool res = false;
+          if (in[r]) {
+            const uint64_t i = c.offset + row[r];
+            const int32_t b = c.offsets[i], e = c.offsets[i + 1];
+            const uint8_t* s = static_cast<const uint8_t*>(c.data) + b;
+            if (op == PRISM_OP_CMP_STR) {
+              int k = str_cmp(s, e - b, lit, nlit);
+              res = cmp_i(cop, k, 0);
+            } else {
+              res = like_match(s, e - b, lit, nlit, op == PRISM_OP_ILIKE);
+            }
+            v |= col_valid(c, row[r]) ? (1u << r) : 0u;
+          }
+          st[sp * R + r] = res;
+        }
+        vset(sp++, v);
+        break;
+      }
+      case PRISM_OP_IS_NULL:
+      case PRISM_OP_IS_NOT_NULL: {
+        const prism_column& c = cols[arg];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          bool nul = in[r] && !col_valid(c, row[r]);
+          st[sp * R + r] = op == PRISM_OP_IS_NULL ? nul : !nul;
+        }
+        vset(sp++, kAll);
+        break;
+      }
+      case PRISM_OP_AND:
+      case PRISM_OP_OR: {
+        --sp;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          st[(sp - 1) * R + r] = op == PRISM_OP_AND ? (st[(sp - 1) * R + r] & st[sp * R + r]) : (st[(sp - 1) * R + r] | st[sp * R + r]);
+        vset(sp - 1, vget(sp - 1) & vget(sp));
+        break;
+      }
+      case PRISM_OP_NOT: {
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[(sp - 1) * R + r] = st[(sp - 1) * R + r] ^ 1;
+        break;
+      }
+      case PRISM_OP_SELECT: {
+        // stack: cond, then, else  ->  zip(cond, then, else): a null cond picks else
+        sp -= 2;
+        const uint32_t cv = vget(sp - 1);
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const bool take = ((cv >> r) & 1) && (st[(sp - 1) * R + r] & 1);
+          st[(sp - 1) * R + r] = take ? st[sp * R + r] : st[(sp + 1) * R + r];
+          v |= ((take ? (vget(sp) >> r) : (vget(sp + 1) >> r)) & 1) ? (1u << r) : 0u;
+        }
+        vset(sp - 1, v);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r] = st[r];
+  out_valid = vget(0);
+}
+
+#endif  // __HIPCC__
+
+}  // namespace prism

@@ -1,0 +1,160 @@
+"""ctypes binding of libprism_hip.so (the C ABI declared in include/prism_hip.h).
+
+The library is required: every entry point here raises when it is missing, and
+there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_int, c_int32, c_int64,
+                    c_uint32, c_uint64, c_void_p)
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_PKG_DIR), "lib", "libprism_hip.so")
+
+# Status codes (one per PrismError variant, native/prism-executor/src/error.rs:5-27).
+PRISM_OK = 0
+STATUS_NAMES = {
+    1: "Arrow",
+    2: "SchemaMismatch",
+    3: "InvalidArgument",
+    4: "Execution",
+    5: "Join",
+    6: "UnsupportedAggregation",
+    7: "Internal",
+    8: "Resource",
+}
+
+TYPE_INT32, TYPE_INT64, TYPE_FLOAT64, TYPE_UTF8, TYPE_BOOL, TYPE_DATE32 = 1, 2, 3, 4, 5, 6
+
+OP = dict(LOAD_I=1, LOAD_F=2, CONST_I=3, CONST_F=4, I2F=5, ADD_F=6, SUB_F=7, MUL_F=8, DIV_F=9,
+          NEG_F=10, ADD_I=11, SUB_I=12, MUL_I=13, NEG_I=14, CMP_I=15, CMP_F=16, CMP_STR=17,
+          LIKE=18, ILIKE=19, IS_NULL=20, IS_NOT_NULL=21, AND=22, OR=23, NOT=24, CONST_BOOL=25,
+          SELECT=26, CONST_NULL=27)
+CMP = dict(EQ=0, NE=1, LT=2, LE=3, GT=4, GE=5)
+AGG = dict(SUM=0, COUNT=1, AVG=2, MIN=3, MAX=4, COUNT_DISTINCT=5)
+TABLE_AUTO, TABLE_HASHED, TABLE_DENSE = 0, 1, 2
+
+
+class PrismError(Exception):
+    """Mirror of PrismError: `.kind` is the variant name, `.status` the C code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+        self.kind = STATUS_NAMES.get(status, "Internal")
+
+
+class Column(Structure):
+    _fields_ = [("type", c_int32), ("reserved", c_int32), ("offset", c_int64), ("data", c_void_p),
+                ("offsets", c_void_p), ("validity", c_void_p)]
+
+
+class Instr(Structure):
+    _fields_ = [("op", c_int32), ("arg", c_int32), ("i64", c_int64), ("f64", c_double)]
+
+
+class PString(Structure):
+    _fields_ = [("data", c_char_p), ("len", c_uint32)]
+
+
+class JoinTableInfo(Structure):
+    _fields_ = [("n_rows", c_uint64), ("n_distinct", c_uint64), ("max_dup", c_uint64),
+                ("min_key", c_int64), ("max_key", c_int64), ("kind", c_int32), ("unique", c_int32),
+                ("slots", c_uint64), ("device_bytes", c_uint64)]
+
+
+class ProbeResult(Structure):
+    _fields_ = [("matches", c_uint64), ("probe_sum_lo", c_uint64), ("probe_sum_hi", c_int64),
+                ("build_sum_lo", c_uint64), ("build_sum_hi", c_int64), ("error", c_uint32),
+                ("reserved", c_uint32)]
+
+    @property
+    def probe_sum(self) -> int:
+        return (self.probe_sum_hi << 64) + self.probe_sum_lo
+
+    @property
+    def build_sum(self) -> int:
+        return (self.build_sum_hi << 64) + self.build_sum_lo
+
+
+class GroupbyAgg(Structure):
+    _fields_ = [("func", c_int32), ("value", c_int32)]
+
+
+class GroupbyPlan(Structure):
+    _fields_ = [("n_columns", c_uint32), ("column_types", POINTER(c_int32)),
+                ("key_columns", POINTER(c_uint32)), ("n_keys", c_uint32),
+                ("filter", POINTER(Instr)), ("n_filter", c_uint32),
+                ("values", POINTER(Instr)), ("value_lengths", POINTER(c_uint32)), ("n_values", c_uint32),
+                ("aggs", POINTER(GroupbyAgg)), ("n_aggs", c_uint32),
+                ("strings", POINTER(PString)), ("n_strings", c_uint32)]
+
+
+class GroupbyOutput(Structure):
+    _fields_ = [("first_row", POINTER(c_uint64)), ("key_hash", POINTER(c_uint64)),
+                ("row_count", POINTER(c_uint64)), ("agg_f64", POINTER(POINTER(c_double))),
+                ("agg_i64", POINTER(POINTER(c_int64))), ("agg_sum128", POINTER(POINTER(c_uint64)))]
+
+
+_lib = None
+
+# (name, restype, argtypes) for every function of include/prism_hip.h.
+_SIGNATURES = [
+    ("prism_last_error", c_char_p, []),
+    ("prism_version", c_char_p, []),
+    ("prism_ctx_create", c_int32, [c_int, POINTER(c_void_p)]),
+    ("prism_ctx_destroy", None, [c_void_p]),
+    ("prism_ctx_set_stream", c_int32, [c_void_p, c_void_p]),
+    ("prism_ctx_stream", c_void_p, [c_void_p]),
+    ("prism_ctx_synchronize", c_int32, [c_void_p]),
+    ("prism_ctx_device_bytes", c_uint64, [c_void_p]),
+    ("prism_join_build_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_int32, POINTER(c_void_p)]),
+    ("prism_join_table_info_get", c_int32, [c_void_p, POINTER(JoinTableInfo)]),
+    ("prism_join_table_free", None, [c_void_p]),
+    ("prism_join_probe_i64", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
+    ("prism_groupby_create", c_int32, [c_void_p, POINTER(GroupbyPlan), c_uint64, POINTER(c_void_p)]),
+    ("prism_groupby_update", c_int32, [c_void_p, POINTER(Column), c_uint64, c_uint64]),
+    ("prism_groupby_finish", c_int32, [c_void_p, POINTER(c_uint64)]),
+    ("prism_groupby_read", c_int32, [c_void_p, POINTER(GroupbyOutput)]),
+    ("prism_groupby_reset", c_int32, [c_void_p]),
+    ("prism_groupby_destroy", None, [c_void_p]),
+    ("prism_datagen_uniform_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_int64,
+                                            c_uint64]),
+    ("prism_datagen_uniform_i32", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_int64,
+                                            c_uint64]),
+    ("prism_datagen_bits_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32]),
+    ("prism_datagen_permutation_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32]),
+    ("prism_datagen_zipf_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_void_p,
+                                         c_uint64, c_uint32]),
+    ("prism_datagen_flag_utf8", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_uint32,
+                                          c_char_p, c_uint32]),
+]
+
+
+def signatures():
+    return list(_SIGNATURES)
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library once.  Raises RuntimeError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libprism_hip.so is not built at {LIB_PATH}; run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in _SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != PRISM_OK:
+        msg = load().prism_last_error().decode(errors="replace")
+        raise PrismError(status, msg)

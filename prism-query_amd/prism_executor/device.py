@@ -1,0 +1,243 @@
+"""Device layer of the C ABI, over torch-allocated HBM buffers.
+
+torch provides device memory and streams only; every computation runs in
+libprism_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_uint64, c_void_p
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def ptr(t) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+class Context:
+    """prism_ctx: a device, a stream, scratch memory."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        self.handle = c_void_p()
+        L.check(self.lib.prism_ctx_create(device, byref(self.handle)))
+        self.device = device
+
+    def set_stream(self, stream_ptr: Optional[int]) -> None:
+        L.check(self.lib.prism_ctx_set_stream(self.handle, stream_ptr))
+
+    @property
+    def stream(self) -> int:
+        return self.lib.prism_ctx_stream(self.handle) or 0
+
+    def sync(self) -> None:
+        L.check(self.lib.prism_ctx_synchronize(self.handle))
+
+    def device_bytes(self) -> int:
+        return int(self.lib.prism_ctx_device_bytes(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.prism_ctx_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Column descriptors
+# ---------------------------------------------------------------------------
+def column(type_code: int, data, offsets=None, validity=None, offset: int = 0) -> L.Column:
+    c = L.Column()
+    c.type = type_code
+    c.offset = offset
+    c.data = ptr(data)
+    c.offsets = ptr(offsets)
+    c.validity = ptr(validity)
+    return c
+
+
+def col_i64(t, validity=None):
+    return column(L.TYPE_INT64, t, validity=validity)
+
+
+def col_i32(t, validity=None):
+    return column(L.TYPE_INT32, t, validity=validity)
+
+
+def col_f64(t, validity=None):
+    return column(L.TYPE_FLOAT64, t, validity=validity)
+
+
+def col_date32(t, validity=None):
+    return column(L.TYPE_DATE32, t, validity=validity)
+
+
+def col_utf8(data, offsets, validity=None):
+    return column(L.TYPE_UTF8, data, offsets=offsets, validity=validity)
+
+
+# ---------------------------------------------------------------------------
+# Join
+# ---------------------------------------------------------------------------
+class JoinTable:
+    """HashTable::build on a device-resident int64 key column (hash_join.rs:87-141)."""
+
+    def __init__(self, ctx: Context, keys, kind: int = L.TABLE_AUTO):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.handle = c_void_p()
+        L.check(self.lib.prism_join_build_i64(ctx.handle, ptr(keys), keys.numel(), kind, byref(self.handle)))
+        self.info = L.JoinTableInfo()
+        L.check(self.lib.prism_join_table_info_get(self.handle, byref(self.info)))
+
+    def probe_async(self, keys, result_buf, probe_payload=None, build_payload=None, out_probe=None,
+                    out_build=None, capacity: int = 0, probe_row_base: int = 0) -> None:
+        L.check(self.lib.prism_join_probe_i64(self.ctx.handle, self.handle, ptr(keys), keys.numel(),
+                                              probe_row_base, ptr(probe_payload), ptr(build_payload),
+                                              ptr(out_probe), ptr(out_build), capacity, ptr(result_buf)))
+
+    def probe(self, keys, probe_payload=None, build_payload=None, want_pairs: bool = True,
+              capacity: Optional[int] = None, probe_row_base: int = 0):
+        """HashTable::probe (hash_join.rs:143-200).  Returns (probe_idx, build_idx, ProbeResult);
+        the index tensors are None when want_pairs is False."""
+        torch = _torch()
+        dev = keys.device
+        res = torch.zeros(ctypes.sizeof(L.ProbeResult), dtype=torch.uint8, device=dev)
+        n = keys.numel()
+        cap = (n if self.info.unique else max(n, 1024)) if capacity is None else capacity
+        while True:
+            op = ob = None
+            if want_pairs:
+                op = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+                ob = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+            self.probe_async(keys, res, probe_payload, build_payload, op, ob, cap if want_pairs else 0,
+                             probe_row_base)
+            self.ctx.sync()
+            r = L.ProbeResult.from_buffer_copy(res.cpu().numpy().tobytes())
+            if r.error:
+                raise L.PrismError(4, "Execution error: probe look-back wait gave up")
+            if want_pairs and r.matches > cap:
+                cap = int(r.matches)
+                continue
+            if want_pairs:
+                return op[: r.matches], ob[: r.matches], r
+            return None, None, r
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.prism_join_table_free(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Group-by
+# ---------------------------------------------------------------------------
+def instrs(prog: Sequence[tuple]) -> list:
+    """[(op_name, arg, i64, f64), ...] -> list of Instr."""
+    out = []
+    for t in prog:
+        op = t[0]
+        ins = L.Instr()
+        ins.op = L.OP[op] if isinstance(op, str) else op
+        ins.arg = t[1] if len(t) > 1 else 0
+        ins.i64 = t[2] if len(t) > 2 else 0
+        ins.f64 = t[3] if len(t) > 3 else 0.0
+        out.append(ins)
+    return out
+
+
+class GroupBy:
+    """prism_groupby: resident group state fed batch by batch."""
+
+    def __init__(self, ctx: Context, column_types: Sequence[int], keys: Sequence[int],
+                 values: Sequence[Sequence[tuple]], aggs: Sequence[tuple], filter_prog=None,
+                 strings: Sequence[bytes] = (), expected_groups: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n_aggs = len(aggs)
+        ct = (ctypes.c_int32 * len(column_types))(*column_types)
+        kc = (ctypes.c_uint32 * max(len(keys), 1))(*keys)
+        flat, lens = [], []
+        for v in values:
+            ins = instrs(v)
+            flat += ins
+            lens.append(len(ins))
+        vals = (L.Instr * max(len(flat), 1))(*flat)
+        vl = (ctypes.c_uint32 * max(len(lens), 1))(*lens)
+        ag = (L.GroupbyAgg * max(len(aggs), 1))(*[L.GroupbyAgg(L.AGG[f] if isinstance(f, str) else f, v)
+                                                  for f, v in aggs])
+        fl = instrs(filter_prog) if filter_prog else []
+        fa = (L.Instr * max(len(fl), 1))(*fl)
+        self._strings = [bytes(s) for s in strings]
+        st = (L.PString * max(len(strings), 1))(*[L.PString(s, len(s)) for s in self._strings])
+        plan = L.GroupbyPlan(len(column_types), ct, kc, len(keys), fa if fl else None, len(fl), vals, vl,
+                             len(lens), ag, len(aggs), st, len(strings))
+        self._keep = (ct, kc, vals, vl, ag, fa, st)
+        self.handle = c_void_p()
+        L.check(self.lib.prism_groupby_create(ctx.handle, byref(plan), expected_groups, byref(self.handle)))
+
+    def update(self, columns: Sequence[L.Column], n_rows: int, row_base: int = 0) -> None:
+        arr = (L.Column * max(len(columns), 1))(*columns)
+        L.check(self.lib.prism_groupby_update(self.handle, arr, n_rows, row_base))
+
+    def reset(self) -> None:
+        L.check(self.lib.prism_groupby_reset(self.handle))
+
+    def finish(self) -> int:
+        n = c_uint64()
+        L.check(self.lib.prism_groupby_finish(self.handle, byref(n)))
+        return int(n.value)
+
+    def read(self, n_groups: int) -> dict:
+        n = max(n_groups, 1)
+        first = np.zeros(n, np.uint64)
+        khash = np.zeros(n, np.uint64)
+        rcount = np.zeros(n, np.uint64)
+        f64 = [np.zeros(n, np.float64) for _ in range(self.n_aggs)]
+        i64 = [np.zeros(n, np.int64) for _ in range(self.n_aggs)]
+        s128 = [np.zeros(2 * n, np.uint64) for _ in range(self.n_aggs)]
+        P = ctypes.POINTER
+        fp = (P(ctypes.c_double) * max(self.n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_double)) for a in f64])
+        ip = (P(ctypes.c_int64) * max(self.n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_int64)) for a in i64])
+        sp = (P(ctypes.c_uint64) * max(self.n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_uint64)) for a in s128])
+        out = L.GroupbyOutput(first.ctypes.data_as(P(c_uint64)), khash.ctypes.data_as(P(c_uint64)),
+                              rcount.ctypes.data_as(P(c_uint64)), fp, ip, sp)
+        L.check(self.lib.prism_groupby_read(self.handle, byref(out)))
+        sums = []
+        for a in s128:
+            lo = a[0::2][:n_groups].astype(object)
+            hi = a[1::2][:n_groups].view(np.int64).astype(object)
+            sums.append([int(h) * (1 << 64) + int(l) for l, h in zip(lo, hi)])
+        return {"first_row": first[:n_groups], "key_hash": khash[:n_groups], "row_count": rcount[:n_groups],
+                "f64": [a[:n_groups] for a in f64], "i64": [a[:n_groups] for a in i64], "sum128": sums}
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.prism_groupby_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
