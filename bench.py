@@ -47,6 +47,87 @@ def cpu_baseline(nb: int, sample: int) -> dict:
                       f"(first {sample} rows of the same seeded probe stream), one thread, {dt:.2f} s"}
 
 
+FILTER_LE_2470 = [("LOAD_I", 0), ("CONST_I", 0, 2470), ("CMP_I", 3)]
+REVENUE_EXACT = [("LOAD_I", 1), ("CONST_I", 0, 100), ("LOAD_I", 2), ("SUB_I",), ("MUL_I",)]
+
+
+def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
+    """Fused filter + group-by: date int32 <= 2470, group by (flag A, flag B)
+    (one-byte Utf8), SUM(price * (100 - discount)) exact int128, COUNT, AVG.
+    Algorithmic bytes: 4 (date) + 8 (price) + 8 (discount) + 2 x (4 offset + 1
+    byte) (flags) = 30 B per row."""
+    import numpy as np
+
+    from prism_executor import _lib as L
+    from prism_executor import datagen as dg
+    from prism_executor import device as D
+
+    lib = ctx.lib
+    date = D.empty(ctx, n, np.int32)
+    price = D.empty(ctx, n, np.int64)
+    disc = D.empty(ctx, n, np.int64)
+    fa, fa_off = D.empty(ctx, n, np.uint8), D.empty(ctx, n + 1, np.int32)
+    fb, fb_off = D.empty(ctx, n, np.uint8), D.empty(ctx, n + 1, np.int32)
+    s = dg.SEED
+    L.check(lib.prism_datagen_uniform_i32(ctx.handle, date.ptr, n, s, 20, 0, 2557))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, price.ptr, n, s, 21, 90_000, 10_500_000 - 90_000))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, disc.ptr, n, s, 22, 0, 11))
+    L.check(lib.prism_datagen_flag_utf8(ctx.handle, fa.ptr, fa_off.ptr, n, s, 25, dg.FLAG_A, len(dg.FLAG_A)))
+    L.check(lib.prism_datagen_flag_utf8(ctx.handle, fb.ptr, fb_off.ptr, n, s, 26, dg.FLAG_B, len(dg.FLAG_B)))
+    ctx.sync()
+    types = [L.TYPE_INT32, L.TYPE_INT64, L.TYPE_INT64, L.TYPE_UTF8, L.TYPE_UTF8]
+    cols = [D.col_i32(date), D.col_i64(price), D.col_i64(disc), D.col_utf8(fa, fa_off), D.col_utf8(fb, fb_off)]
+    g = D.GroupBy(ctx, types, [3, 4], [REVENUE_EXACT], [("SUM", 0), ("COUNT", 0), ("AVG", 0)],
+                  filter_prog=FILTER_LE_2470, expected_groups=8)
+    for _ in range(2):
+        g.reset()
+        g.update(cols, n, 0)
+        g.finish()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.reset()
+        g.update(cols, n, 0)
+        k = g.finish()
+    wall = (time.perf_counter() - t0) / steps
+    out = g.read(k)
+    ctx.timer_begin()
+    for _ in range(steps):
+        L.check(lib.prism_groupby_update(g.handle, (L.Column * len(cols))(*cols), n, 0))
+    upd_ms = ctx.timer_end() / steps
+    rows = int(out["row_count"].sum())
+    res = {"workload": f"fused filter + group-by, {n} rows: date<=2470, group (flagA, flagB), "
+                       "SUM(price*(100-discount)) int128, COUNT, AVG",
+           "rows": n, "groups": k, "rows_passing_filter": rows, "ms_per_step": wall * 1e3,
+           "rows_per_s": n / wall, "update_ms": upd_ms,
+           "roofline": {"bound": "hbm", "achieved": n * 30 / (upd_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": n * 30 / (upd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    if cpu_sample:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyarrow as pa
+
+        import oracle as orc
+        d = dg.filter_groupby(cpu_sample)
+        keep = d["date"] <= 2470
+        t0 = time.perf_counter()
+        rev = d["price"][keep] * (100 - d["discount"][keep])
+        a_b, a_o = d["flag_a"]
+        b_b, b_o = d["flag_b"]
+        a_sel, b_sel = a_b[keep], b_b[keep]
+        m = int(keep.sum())
+        batch = orc.Batch([orc.Col(orc.UTF8, a_sel, offsets=np.arange(m + 1, dtype=np.int32)),
+                           orc.Col(orc.UTF8, b_sel, offsets=np.arange(m + 1, dtype=np.int32)),
+                           orc.Col(orc.INT64, rev)])
+        orc.hash_aggregate([batch], [0, 1], [(2, "SUM"), (2, "COUNT"), (2, "AVG")])
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": cpu_sample / dt, "unit": "rows/s", "cores": 1, "kind": "port",
+                               "sample": f"numpy filter+project then the oracle's hash_aggregate restatement on "
+                                         f"{cpu_sample} rows, one thread, {dt:.2f} s"}
+    g.close()
+    for a in (date, price, disc, fa, fa_off, fb, fb_off):
+        a.free()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,35 +137,35 @@ def main() -> int:
     ap.add_argument("--probe-rows", type=int, default=64_000_000)
     ap.add_argument("--cpu-sample", type=int, default=8_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--groupby-rows", type=int, default=200_000_000)
+    ap.add_argument("--no-groupby", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, help="also run D2D copies of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is a design note, "
               "not a measured path", file=sys.stderr)
         return 2
 
-    import torch
+    import numpy as np
 
     from prism_executor import _lib as L
     from prism_executor import datagen as dg
     from prism_executor import device as D
 
     nb, npr = args.build_rows, args.probe_rows
-    torch.cuda.set_device(0)
-    stream = torch.cuda.current_stream()
     ctx = D.Context(0)
-    ctx.set_stream(stream.cuda_stream)
     lib = ctx.lib
-    bk = torch.empty(nb, dtype=torch.int64, device="cuda")
-    bp = torch.empty(nb, dtype=torch.int32, device="cuda")
-    pk = torch.empty(npr, dtype=torch.int64, device="cuda")
-    pp = torch.empty(npr, dtype=torch.int64, device="cuda")
-    L.check(lib.prism_datagen_permutation_i64(ctx.handle, bk.data_ptr(), nb, dg.SEED, 10))
-    L.check(lib.prism_datagen_uniform_i32(ctx.handle, bp.data_ptr(), nb, dg.SEED, 11, -(1 << 31), 1 << 32))
-    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.data_ptr(), npr, dg.SEED, 12, 0, 2 * nb))
-    L.check(lib.prism_datagen_bits_i64(ctx.handle, pp.data_ptr(), npr, dg.SEED, 13))
-    op = torch.empty(npr, dtype=torch.int32, device="cuda")
-    ob = torch.empty(npr, dtype=torch.int32, device="cuda")
-    res = torch.zeros(48, dtype=torch.uint8, device="cuda")
+    bk = D.empty(ctx, nb, np.int64)
+    bp = D.empty(ctx, nb, np.int32)
+    pk = D.empty(ctx, npr, np.int64)
+    pp = D.empty(ctx, npr, np.int64)
+    L.check(lib.prism_datagen_permutation_i64(ctx.handle, bk.ptr, nb, dg.SEED, 10))
+    L.check(lib.prism_datagen_uniform_i32(ctx.handle, bp.ptr, nb, dg.SEED, 11, -(1 << 31), 1 << 32))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.ptr, npr, dg.SEED, 12, 0, 2 * nb))
+    L.check(lib.prism_datagen_bits_i64(ctx.handle, pp.ptr, npr, dg.SEED, 13))
+    op = D.empty(ctx, npr, np.uint32)
+    ob = D.empty(ctx, npr, np.uint32)
+    res = D.empty(ctx, 48, np.uint8)
     ctx.sync()
 
     def step(prev):
@@ -97,28 +178,25 @@ def main() -> int:
     t = None
     for _ in range(args.warmup):
         t = step(t)
-    torch.cuda.synchronize()
+    ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         t = step(t)
-    torch.cuda.synchronize()
+    ctx.sync()
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
 
-    r = L.ProbeResult.from_buffer_copy(res.cpu().numpy().tobytes())
-    expected = int((pk < nb).sum())
+    r = L.ProbeResult.from_buffer_copy(res.numpy().tobytes())
+    expected = int((pk.numpy() < nb).sum())
     if r.error or r.matches != expected:
         print(f"probe result check failed: matches {r.matches} expected {expected} error {r.error}", file=sys.stderr)
         return 1
 
-    # Probe kernel duration with events on the launch stream.
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    for e0, e1 in evs:
-        e0.record(stream)
+    # Probe kernel duration: HIP events on the launch stream around K launches.
+    ctx.timer_begin()
+    for _ in range(args.steps):
         t.probe_async(pk, res, pp, bp, op, ob, capacity=npr)
-        e1.record(stream)
-    torch.cuda.synchronize()
-    probe_ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / len(evs)
+    probe_ms = ctx.timer_end() / args.steps
     # Build duration (host clock; the build synchronises internally).
     tb = time.perf_counter()
     for _ in range(5):
@@ -155,6 +233,20 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(nb, args.cpu_sample)
+    for a in (op, ob, pk, pp):
+        a.free()
+    if not args.no_groupby:
+        line["groupby_filter"] = bench_groupby(ctx, args.groupby_rows, args.steps,
+                                               0 if args.no_cpu_baseline else 10_000_000)
+    if args.calibrate_copy:
+        src = D.empty(ctx, args.calibrate_copy, np.uint8)
+        dst = D.empty(ctx, args.calibrate_copy, np.uint8)
+        ctx.timer_begin()
+        for _ in range(3):
+            L.check(lib.prism_copy_device(ctx.handle, dst.ptr, src.ptr, args.calibrate_copy))
+        cms = ctx.timer_end() / 3
+        line["copy_calibration"] = {"bytes": args.calibrate_copy, "ms": cms,
+                                    "GBps": 2 * args.calibrate_copy / (cms * 1e-3) / 1e9}
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0

@@ -98,6 +98,20 @@ prism_status prism_ctx_synchronize(prism_ctx* ctx);
 /* Device bytes held by the context and every object created from it. */
 uint64_t prism_ctx_device_bytes(prism_ctx* ctx);
 
+/* Device memory and timing for callers without an allocator of their own
+ * (tests, benchmarks).  No counterpart in the reference. */
+prism_status prism_device_alloc(prism_ctx* ctx, uint64_t bytes, void** out);
+prism_status prism_device_free(prism_ctx* ctx, void* p);
+/* Synchronous copies on the context stream. */
+prism_status prism_copy_to_device(prism_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
+prism_status prism_copy_to_host(prism_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+/* Asynchronous device-to-device copy on the context stream. */
+prism_status prism_copy_device(prism_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);
+/* Elapsed device time between begin and end markers recorded on the context
+ * stream (HIP events); end synchronises. */
+prism_status prism_ctx_timer_begin(prism_ctx* ctx);
+prism_status prism_ctx_timer_end(prism_ctx* ctx, double* ms);
+
 /* ------------------------------------------------------------------------- */
 /* Column descriptors (device layer: every pointer is a device pointer).      */
 /* ------------------------------------------------------------------------- */
@@ -281,6 +295,98 @@ prism_status prism_groupby_read(prism_groupby* g, const prism_groupby_output* ou
 /* Clear the state for reuse without freeing device memory. */
 prism_status prism_groupby_reset(prism_groupby* g);
 void prism_groupby_destroy(prism_groupby* g);
+
+
+/* ========================================================================= */
+/* Arrow layer: prism-executor's operator API over the Arrow C Data          */
+/* Interface.  A RecordBatch travels as a struct array ("+s") + its schema.  */
+/* Inputs are borrowed (host memory); outputs are owned by the caller, who   */
+/* releases them through their `release` callbacks.                          */
+/* ========================================================================= */
+
+/* JoinType (hash_join.rs:48-56) */
+#define PRISM_JOIN_INNER 0
+#define PRISM_JOIN_LEFT 1
+#define PRISM_JOIN_RIGHT 2
+#define PRISM_JOIN_FULL 3
+#define PRISM_JOIN_LEFT_SEMI 4
+#define PRISM_JOIN_LEFT_ANTI 5
+
+/* HashJoinConfig (hash_join.rs:58-64): probe = left, build = right. */
+typedef struct {
+  int32_t join_type;
+  const uint32_t* build_keys;
+  uint32_t n_build_keys;
+  const uint32_t* probe_keys;
+  uint32_t n_probe_keys;
+} prism_hash_join_config;
+
+/* HashTable (hash_join.rs:77-84): device-resident build side. */
+typedef struct prism_hash_table prism_hash_table;
+
+/* HashTable::build (hash_join.rs:87-141) */
+prism_status prism_hash_table_build(prism_ctx* ctx, const struct ArrowArray* batch, const struct ArrowSchema* schema,
+                                    const uint32_t* key_cols, uint32_t n_keys, prism_hash_table** out);
+/* HashTable::probe (hash_join.rs:143-200): two UInt32 arrays, probe rows
+ * ascending and build rows ascending within a probe row. */
+prism_status prism_hash_table_probe(prism_ctx* ctx, prism_hash_table* table, const struct ArrowArray* probe,
+                                    const struct ArrowSchema* probe_schema, const uint32_t* probe_keys,
+                                    uint32_t n_keys, struct ArrowArray* out_probe_idx,
+                                    struct ArrowSchema* out_probe_schema, struct ArrowArray* out_build_idx,
+                                    struct ArrowSchema* out_build_schema);
+void prism_hash_table_free(prism_hash_table* table);
+
+/* hash_join (hash_join.rs:312-365): output = probe columns, then build columns
+ * (a name clash gets "_right", hash_join.rs:438-449). */
+prism_status prism_hash_join(prism_ctx* ctx, const struct ArrowArray* probe, const struct ArrowSchema* probe_schema,
+                             const struct ArrowArray* build, const struct ArrowSchema* build_schema,
+                             const prism_hash_join_config* cfg, struct ArrowArray* out,
+                             struct ArrowSchema* out_schema);
+
+/* probe_matches_chunked (hash_join.rs:286-309): one (probe_idx, build_idx) pair
+ * of UInt32 arrays per probe batch (batch-local probe indices).  out_idx_schema
+ * receives the UInt32 schema once. */
+prism_status prism_probe_matches_chunked(prism_ctx* ctx, const struct ArrowArray* const* probe_batches,
+                                         uint32_t n_batches, const struct ArrowSchema* probe_schema,
+                                         const struct ArrowArray* build, const struct ArrowSchema* build_schema,
+                                         const uint32_t* probe_keys, const uint32_t* build_keys, uint32_t n_keys,
+                                         struct ArrowArray* out_probe_idx, struct ArrowArray* out_build_idx,
+                                         struct ArrowSchema* out_idx_schema);
+
+/* hash_join_probe_chunked (hash_join.rs:206-282): joined batches, empty ones
+ * dropped; Right/Full run as per-batch Inner as in the reference.  out_batches
+ * has room for n_batches arrays; *n_out tells how many were written. */
+prism_status prism_hash_join_probe_chunked(prism_ctx* ctx, const struct ArrowArray* const* probe_batches,
+                                           uint32_t n_batches, const struct ArrowSchema* probe_schema,
+                                           const struct ArrowArray* build, const struct ArrowSchema* build_schema,
+                                           const prism_hash_join_config* cfg, struct ArrowArray* out_batches,
+                                           struct ArrowSchema* out_schema, uint32_t* n_out);
+
+/* AggExpr / HashAggConfig (hash_aggregate.rs:88-101) */
+typedef struct {
+  uint32_t column;
+  int32_t func;            /* PRISM_AGG_* */
+  const char* output_name;
+} prism_agg_expr;
+
+#define PRISM_AGG_FLAG_EXACT_DECIMAL 1 /* SUM of integer columns as Decimal128(38,0), exact */
+
+typedef struct {
+  const uint32_t* group_by;
+  uint32_t n_group_by;
+  const prism_agg_expr* aggregates;
+  uint32_t n_aggregates;
+  uint32_t flags;
+} prism_hash_agg_config;
+
+/* hash_aggregate (hash_aggregate.rs:174-176) */
+prism_status prism_hash_aggregate(prism_ctx* ctx, const struct ArrowArray* batch, const struct ArrowSchema* schema,
+                                  const prism_hash_agg_config* cfg, struct ArrowArray* out,
+                                  struct ArrowSchema* out_schema);
+/* hash_aggregate_batches (hash_aggregate.rs:182-339, global path 343-447) */
+prism_status prism_hash_aggregate_batches(prism_ctx* ctx, const struct ArrowArray* const* batches, uint32_t n_batches,
+                                          const struct ArrowSchema* schema, const prism_hash_agg_config* cfg,
+                                          struct ArrowArray* out, struct ArrowSchema* out_schema);
 
 /* ------------------------------------------------------------------------- */
 /* Synthetic inputs (seeded, counter-based; bit-identical to the numpy twin    */

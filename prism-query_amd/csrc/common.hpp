@@ -107,6 +107,7 @@ struct prism_ctx {
   prism::DeviceBuffer scratch;
   void* pinned = nullptr;   // small pinned host staging for results
   size_t pinned_bytes = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;  // timer markers
 };
 
 namespace prism {

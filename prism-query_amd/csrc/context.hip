@@ -90,6 +90,8 @@ void prism_ctx_destroy(prism_ctx* ctx) {
   (void)hipSetDevice(c->device);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   c->scratch.release();
+  if (c->t0) (void)hipEventDestroy(c->t0);
+  if (c->t1) (void)hipEventDestroy(c->t1);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -113,5 +115,83 @@ prism_status prism_ctx_synchronize(prism_ctx* ctx) {
 }
 
 uint64_t prism_ctx_device_bytes(prism_ctx* ctx) { return ctx ? ctx->device_bytes.load() : 0; }
+
+prism_status prism_device_alloc(prism_ctx* ctx, uint64_t bytes, void** out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/output");
+    use_device(ctx);
+    *out = nullptr;
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      fail(PRISM_ERR_EXECUTION, std::string("Execution error: hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    ctx->device_bytes += bytes ? bytes : 1;
+  });
+}
+
+prism_status prism_device_free(prism_ctx* ctx, void* p) {
+  return guarded([&] {
+    if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
+    if (!p) return;
+    use_device(ctx);
+    size_t sz = 0;
+    void* base = nullptr;
+    if (hipMemGetAddressRange(&base, &sz, p) == hipSuccess && base == p) ctx->device_bytes -= sz;
+    PRISM_HIP(hipFree(p));
+  });
+}
+
+prism_status prism_copy_to_device(prism_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes) {
+  return guarded([&] {
+    if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
+    if (!bytes) return;
+    use_device(ctx);
+    PRISM_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PRISM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+prism_status prism_copy_to_host(prism_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes) {
+  return guarded([&] {
+    if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
+    if (!bytes) return;
+    use_device(ctx);
+    PRISM_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PRISM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+prism_status prism_copy_device(prism_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes) {
+  return guarded([&] {
+    if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
+    if (!bytes) return;
+    use_device(ctx);
+    PRISM_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
+
+prism_status prism_ctx_timer_begin(prism_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
+    use_device(ctx);
+    if (!ctx->t0) PRISM_HIP(hipEventCreate(&ctx->t0));
+    if (!ctx->t1) PRISM_HIP(hipEventCreate(&ctx->t1));
+    PRISM_HIP(hipEventRecord(ctx->t0, ctx->stream));
+  });
+}
+
+prism_status prism_ctx_timer_end(prism_ctx* ctx, double* ms) {
+  return guarded([&] {
+    if (!ctx || !ms || !ctx->t0) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: timer not started");
+    use_device(ctx);
+    PRISM_HIP(hipEventRecord(ctx->t1, ctx->stream));
+    PRISM_HIP(hipEventSynchronize(ctx->t1));
+    float f = 0.0f;
+    PRISM_HIP(hipEventElapsedTime(&f, ctx->t0, ctx->t1));
+    *ms = f;
+  });
+}
 
 }  // extern "C"

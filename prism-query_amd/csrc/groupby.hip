@@ -36,7 +36,7 @@ constexpr uint64_t kNoRow = ~0ULL;
 constexpr uint64_t kMix = 0x9E3779B97F4A7C15ULL;
 constexpr int kW_KEY = 0, kW_FIRST = 1, kW_COUNT = 2, kW_LANES = 3;
 constexpr int kBlock = 256;
-constexpr int kRows = 2;  // rows per lane per iteration
+// Rows per lane per iteration: 4 for programs of stack depth <= 4, else 2.
 constexpr int kLdsProbes = 8;
 constexpr size_t kLdsBudget = 64 * 1024;
 
@@ -78,41 +78,16 @@ struct GTable {
   unsigned long long* n_groups;
 };
 
-__device__ __forceinline__ uint64_t hash_key(const prism_column& c, uint64_t row, uint64_t state) {
-  if (!col_valid(c, row)) return fx_step(state, 0);  // 0u8.hash(): write_u8(0)
-  const uint64_t i = c.offset + row;
-  switch (c.type) {
-    case PRISM_TYPE_INT64:
-      return fx_step(state, static_cast<uint64_t>(static_cast<const int64_t*>(c.data)[i]));
-    case PRISM_TYPE_INT32:  // write_i32 -> write_u64(i as u64): sign extension
-      return fx_step(state, static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[i])));
-    case PRISM_TYPE_FLOAT64:  // to_bits().hash()
-      return fx_step(state, static_cast<const uint64_t*>(c.data)[i]);
-    case PRISM_TYPE_DATE32:  // value word kept away from the null word 0
-      return fx_step(state, static_cast<uint64_t>(static_cast<uint32_t>(static_cast<const int32_t*>(c.data)[i])) |
-                                (1ULL << 32));
-    case PRISM_TYPE_BOOL:
-      return fx_step(state, 2 + (bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0));
-    default: {  // UTF8: str::hash = write(bytes) in 8-byte LE words, then write_u8(0xff)
-      const int32_t b = c.offsets[i], e = c.offsets[i + 1];
-      const uint8_t* s = static_cast<const uint8_t*>(c.data) + b;
-      const int32_t len = e - b;
-      for (int32_t p = 0; p < len; p += 8) {
-        uint64_t w = 0;
-        const int32_t m = len - p < 8 ? len - p : 8;
-        for (int32_t k = 0; k < m; ++k) w |= static_cast<uint64_t>(s[p + k]) << (8 * k);
-        state = fx_step(state, w);
-      }
-      return fx_step(state, 0xff);
-    }
-  }
-}
-
 __device__ __forceinline__ int lds_find(unsigned long long* lds, int stride, int slots, int shift, uint64_t h) {
   uint32_t s = static_cast<uint32_t>((h * kMix) >> shift);
   for (int p = 0; p < kLdsProbes; ++p) {
-    unsigned long long old = atomicCAS(&lds[static_cast<uint32_t>(s) * stride], kEmptyKey, h);
-    if (old == kEmptyKey || old == h) return static_cast<int>(s);
+    unsigned long long* kp = &lds[static_cast<uint32_t>(s) * stride];
+    unsigned long long cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == h) return static_cast<int>(s);
+    if (cur == kEmptyKey) {
+      unsigned long long old = atomicCAS(kp, kEmptyKey, h);
+      if (old == kEmptyKey || old == h) return static_cast<int>(s);
+    }
     s = (s + 1) & (slots - 1);
   }
   return -1;
@@ -147,17 +122,21 @@ __device__ __forceinline__ bool integral_small(double d) {
 __device__ __forceinline__ void lane_lds(unsigned long long* L, const Lane& ln, uint64_t v) {
   switch (ln.op) {
     case L_SUM_I: {
-      int64_t x = static_cast<int64_t>(v);
-      atomicAdd(&L[ln.word], static_cast<unsigned long long>(static_cast<uint32_t>(x)));
-      atomicAdd(&L[ln.word + 1], static_cast<unsigned long long>(x >> 32));
+      const int64_t x = static_cast<int64_t>(v);
+      const uint32_t lo = static_cast<uint32_t>(x);
+      const int64_t hi = x >> 32;
+      if (lo) atomicAdd(&L[ln.word], static_cast<unsigned long long>(lo));
+      if (hi) atomicAdd(&L[ln.word + 1], static_cast<unsigned long long>(hi));
       break;
     }
     case L_SUM_F: {
       double d = as_f(v);
       if (integral_small(d)) {
-        int64_t x = static_cast<int64_t>(d);
-        atomicAdd(&L[ln.word], static_cast<unsigned long long>(static_cast<uint32_t>(x)));
-        atomicAdd(&L[ln.word + 1], static_cast<unsigned long long>(x >> 32));
+        const int64_t x = static_cast<int64_t>(d);
+        const uint32_t lo = static_cast<uint32_t>(x);
+        const int64_t hi = x >> 32;
+        if (lo) atomicAdd(&L[ln.word], static_cast<unsigned long long>(lo));
+        if (hi) atomicAdd(&L[ln.word + 1], static_cast<unsigned long long>(hi));
       } else {
         atomicAdd(reinterpret_cast<double*>(&L[ln.word + 2]), d);
       }
@@ -240,6 +219,7 @@ __device__ __forceinline__ void merge_slot(unsigned long long* G, const unsigned
   }
 }
 
+template <int kRows, int kDepth>
 __global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ P, const prism_column* __restrict__ cols, uint64_t n,
                                                     uint64_t row_base, GTable G, int lds_slots, int lds_shift,
                                                     uint32_t* flags) {
@@ -260,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ 
     if (P->filter_len) {
       uint64_t fv[kRows];
       uint32_t fvalid;
-      run_program<kRows>(P->code + P->filter_start, P->filter_len, cols, &P->strs, row, in, fv, fvalid, flags);
+      run_program<kRows, kDepth>(P->code + P->filter_start, P->filter_len, cols, &P->strs, row, in, fv, fvalid, flags);
 #pragma unroll
       for (int r = 0; r < kRows; ++r) in[r] = in[r] && ((fvalid >> r) & 1) && (fv[r] & 1);
     }
@@ -285,7 +265,10 @@ __global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ 
       if (ls >= 0) {
         lw[r] = ls * stride;
         atomicAdd(&lds[lw[r] + kW_COUNT], 1ULL);
-        atomicMin(&lds[lw[r] + kW_FIRST], static_cast<unsigned long long>(row_base + row[r]));
+        // rows only grow along a lane, so most rows skip the atomic
+        const unsigned long long fr = row_base + row[r];
+        if (fr < __hip_atomic_load(&lds[lw[r] + kW_FIRST], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+          atomicMin(&lds[lw[r] + kW_FIRST], fr);
       } else {
         int64_t gs = g_find(G, h[r], flags);
         if (gs < 0) {
@@ -294,13 +277,16 @@ __global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ 
         }
         gp[r] = &G.w[static_cast<uint64_t>(gs) * stride];
         atomicAdd(&gp[r][kW_COUNT], 1ULL);
-        atomicMin(&gp[r][kW_FIRST], static_cast<unsigned long long>(row_base + row[r]));
+        const unsigned long long fr = row_base + row[r];
+        if (fr < __hip_atomic_load(&gp[r][kW_FIRST], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+          atomicMin(&gp[r][kW_FIRST], fr);
       }
     }
     for (int v = 0; v < P->n_values; ++v) {
       uint64_t val[kRows];
       uint32_t vvalid;
-      run_program<kRows>(P->code + P->value_start[v], P->value_len[v], cols, &P->strs, row, in, val, vvalid, flags);
+      run_program<kRows, kDepth>(P->code + P->value_start[v], P->value_len[v], cols, &P->strs, row, in, val, vvalid,
+                                 flags);
       for (int l = 0; l < P->n_lanes; ++l) {
         const Lane ln = P->lanes[l];
         if (ln.value != v) continue;
@@ -396,6 +382,7 @@ struct prism_groupby {
   DeviceBuffer result;    // finished groups, stride words each
   uint64_t n_result = 0;
   uint64_t expected = 0;
+  int max_depth = 1;
   bool finished = false;
 
   unsigned long long* n_groups_ptr() { return counters.as<unsigned long long>(); }
@@ -468,13 +455,20 @@ static void launch(prism_groupby* g, const prism_column* cols, uint64_t n_rows, 
     lds_slots = static_cast<int>(s);
     lds_shift = 64 - log2_pow2(s);
   }
-  uint64_t blocks = ceil_div(n_rows, static_cast<uint64_t>(kBlock) * kRows);
-  uint64_t max_blocks = static_cast<uint64_t>(c->num_cus) * (lds_slots ? 2 : 8);
+  const bool shallow = g->max_depth <= 4;
+  const uint64_t rows_per_lane = shallow ? 4 : 2;
+  uint64_t blocks = ceil_div(n_rows, static_cast<uint64_t>(kBlock) * rows_per_lane);
+  uint64_t max_blocks = static_cast<uint64_t>(c->num_cus) * (lds_slots ? 4 : 8);
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_groupby, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), lds_slots * sb, c->stream,
-                     g->d_plan.as<DevPlan>(), cols, n_rows, row_base, g->view(), lds_slots, lds_shift,
-                     g->flags_ptr());
+  if (shallow)
+    hipLaunchKernelGGL((k_groupby<4, 4>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), lds_slots * sb,
+                       c->stream, g->d_plan.as<DevPlan>(), cols, n_rows, row_base, g->view(), lds_slots, lds_shift,
+                       g->flags_ptr());
+  else
+    hipLaunchKernelGGL((k_groupby<2, kStackDepth>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
+                       lds_slots * sb, c->stream, g->d_plan.as<DevPlan>(), cols, n_rows, row_base, g->view(),
+                       lds_slots, lds_shift, g->flags_ptr());
   PRISM_CHECK_LAUNCH();
 }
 
@@ -520,8 +514,10 @@ prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan
       return start;
     };
     if (plan->filter && plan->n_filter) {
+      int depth = 1;
       ValType t = check_program(plan->filter, plan->n_filter, plan->column_types, plan->n_columns,
-                                plan->n_strings, nullptr);
+                                plan->n_strings, &depth);
+      g->max_depth = depth > g->max_depth ? depth : g->max_depth;
       if (t != VT_B && t != VT_NULL) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: filter is not boolean");
       D.filter_start = append(plan->filter, plan->n_filter);
       D.filter_len = static_cast<int32_t>(plan->n_filter);
@@ -530,8 +526,10 @@ prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan
     uint32_t off = 0;
     for (uint32_t v = 0; v < plan->n_values; ++v) {
       uint32_t len = plan->value_lengths[v];
+      int depth = 1;
       ValType t = check_program(plan->values + off, len, plan->column_types, plan->n_columns, plan->n_strings,
-                                nullptr);
+                                &depth);
+      g->max_depth = depth > g->max_depth ? depth : g->max_depth;
       g->value_types.push_back(t);
       D.value_start[v] = append(plan->values + off, len);
       D.value_len[v] = static_cast<int32_t>(len);
@@ -544,6 +542,8 @@ prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan
     D.tmpl[kW_FIRST] = kNoRow;
     D.tmpl[kW_COUNT] = 0;
     auto add_lane = [&](int32_t op, int32_t value, int32_t is_int) {
+      for (int32_t i = 0; i < D.n_lanes; ++i)  // SUM and AVG of one value share a lane
+        if (D.lanes[i].op == op && D.lanes[i].value == value) return i;
       if (D.n_lanes >= kMaxLanes) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: too many aggregates");
       Lane& L = D.lanes[D.n_lanes];
       L.op = op;

@@ -11,11 +11,11 @@
 //            stored key), linear probing, load factor <= 1/2, 16 B slots.
 // Duplicate keys: a CSR row list per key built with a stable radix sort.
 //
-// Probe: one 2048-row tile per workgroup, claimed through an atomic ticket so
-// tiles start in order; probe keys stream in with 128-bit non-temporal loads,
-// wave64 ballots + popcounts rank the matches inside the tile, and a
-// decoupled look-back over per-tile status words gives the tile's output
-// offset in one pass.  Pairs therefore come out in the reference's order
+// Probe: one 2048-row tile per workgroup (tile = workgroup index); probe keys
+// stream in with 128-bit non-temporal loads, wave64 ballots + popcounts rank
+// the matches inside the tile, and a decoupled look-back over per-tile status
+// words gives the tile's output offset in one pass.  The look-back never waits
+// unboundedly: a predecessor that has not published is counted by the waiter.  Pairs therefore come out in the reference's order
 // (probe row ascending, then build row ascending).
 #include <hipcub/hipcub.hpp>
 
@@ -69,21 +69,55 @@ namespace join {
 
 constexpr int kBuildBlock = 256;
 
-__global__ void k_minmax(const int64_t* __restrict__ keys, uint64_t n, Stats* st) {
+// Reductions run in two stages (per-block partials, then one block) so that
+// no global address takes more than one atomic per launch.
+constexpr int kRedBlocks = 256;
+
+template <class T, class Op>
+__device__ __forceinline__ T block_reduce(T v, Op op, T* lds) {
+  for (int d = 32; d > 0; d >>= 1) v = op(v, __shfl_xor(v, d, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  T r = lds[0];
+  for (int i = 1; i < static_cast<int>(blockDim.x >> 6); ++i) r = op(r, lds[i]);
+  __syncthreads();
+  return r;
+}
+
+struct MinOp { __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; } };
+struct MaxOp { __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; } };
+struct AddOp { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
+struct UMaxOp { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; } };
+
+__global__ void k_minmax_part(const int64_t* __restrict__ keys, uint64_t n, int64_t* __restrict__ part) {
+  __shared__ int64_t lds[16];
   int64_t mn = INT64_MAX, mx = INT64_MIN;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     int64_t k = keys[i];
     mn = k < mn ? k : mn;
     mx = k > mx ? k : mx;
   }
-  for (int d = 32; d > 0; d >>= 1) {
-    int64_t a = __shfl_xor(mn, d, 64), b = __shfl_xor(mx, d, 64);
-    mn = a < mn ? a : mn;
-    mx = b > mx ? b : mx;
+  mn = block_reduce(mn, MinOp(), lds);
+  mx = block_reduce(mx, MaxOp(), lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = mn;
+    part[2 * blockIdx.x + 1] = mx;
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(reinterpret_cast<long long*>(&st->min_key), static_cast<long long>(mn));
-    atomicMax(reinterpret_cast<long long*>(&st->max_key), static_cast<long long>(mx));
+}
+
+__global__ void k_minmax_final(const int64_t* __restrict__ part, int n_part, Stats* st) {
+  __shared__ int64_t lds[16];
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  for (int i = threadIdx.x; i < n_part; i += blockDim.x) {
+    mn = part[2 * i] < mn ? part[2 * i] : mn;
+    mx = part[2 * i + 1] > mx ? part[2 * i + 1] : mx;
+  }
+  mn = block_reduce(mn, MinOp(), lds);
+  mx = block_reduce(mx, MaxOp(), lds);
+  if (threadIdx.x == 0) {
+    st->min_key = mn;
+    st->max_key = mx;
   }
 }
 
@@ -139,21 +173,34 @@ __global__ void k_hashed_insert(const int64_t* __restrict__ keys, uint64_t n, Sl
   }
 }
 
-__global__ void k_count_stats_u32(const uint32_t* __restrict__ cnt, uint64_t n, Stats* st) {
-  unsigned long long nd = 0, mx = 0;
+__global__ void k_count_part(const uint32_t* __restrict__ cnt, uint64_t n, uint64_t* __restrict__ part) {
+  __shared__ uint64_t lds[16];
+  uint64_t nd = 0, mx = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t c = cnt[i];
     nd += c ? 1 : 0;
     mx = c > mx ? c : mx;
   }
-  for (int d = 32; d > 0; d >>= 1) {
-    nd += __shfl_xor(nd, d, 64);
-    unsigned long long o = __shfl_xor(mx, d, 64);
-    mx = o > mx ? o : mx;
+  nd = block_reduce(nd, AddOp(), lds);
+  mx = block_reduce(mx, UMaxOp(), lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = nd;
+    part[2 * blockIdx.x + 1] = mx;
   }
-  if ((threadIdx.x & 63) == 0) {
-    if (nd) atomicAdd(&st->n_distinct, nd);
-    atomicMax(&st->max_dup, mx);
+}
+
+__global__ void k_count_final(const uint64_t* __restrict__ part, int n_part, Stats* st) {
+  __shared__ uint64_t lds[16];
+  uint64_t nd = 0, mx = 0;
+  for (int i = threadIdx.x; i < n_part; i += blockDim.x) {
+    nd += part[2 * i];
+    mx = part[2 * i + 1] > mx ? part[2 * i + 1] : mx;
+  }
+  nd = block_reduce(nd, AddOp(), lds);
+  mx = block_reduce(mx, UMaxOp(), lds);
+  if (threadIdx.x == 0) {
+    st->n_distinct = nd;
+    st->max_dup = mx;
   }
 }
 
@@ -194,6 +241,22 @@ static dim3 grid_for(prism_ctx* ctx, uint64_t n) {
   return dim3(static_cast<unsigned>(blocks < cap ? blocks : cap));
 }
 
+static void launch_minmax(prism_ctx* ctx, const int64_t* keys, uint64_t n, int64_t* part, Stats* st) {
+  uint64_t b = ceil_div(n ? n : 1, kBuildBlock);
+  int nb = static_cast<int>(b < kRedBlocks ? b : kRedBlocks);
+  hipLaunchKernelGGL(k_minmax_part, dim3(nb), dim3(kBuildBlock), 0, ctx->stream, keys, n, part);
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBuildBlock), 0, ctx->stream, part, nb, st);
+  PRISM_CHECK_LAUNCH();
+}
+
+static void launch_count_stats(prism_ctx* ctx, const uint32_t* cnt, uint64_t n, uint64_t* part, Stats* st) {
+  uint64_t b = ceil_div(n ? n : 1, kBuildBlock);
+  int nb = static_cast<int>(b < kRedBlocks ? b : kRedBlocks);
+  hipLaunchKernelGGL(k_count_part, dim3(nb), dim3(kBuildBlock), 0, ctx->stream, cnt, n, part);
+  hipLaunchKernelGGL(k_count_final, dim3(1), dim3(kBuildBlock), 0, ctx->stream, part, nb, st);
+  PRISM_CHECK_LAUNCH();
+}
+
 // Stable CSR: sort (slot, row) pairs by slot (LSD radix sort keeps rows
 // ascending within a slot), then starts = exclusive scan of counts.
 static void build_csr(prism_ctx* ctx, uint32_t* slot_of_row, uint64_t n, const uint32_t* cnt,
@@ -222,10 +285,10 @@ static void build_csr(prism_ctx* ctx, uint32_t* slot_of_row, uint64_t n, const u
 // ---------------------------------------------------------------------------
 // Probe.
 // ---------------------------------------------------------------------------
-constexpr int kPB = 256;                  // threads per workgroup (4 waves)
+constexpr int kPB = 512;                  // threads per workgroup (8 waves)
 constexpr int kPairs = 4;                 // 16-byte key pairs per thread
 constexpr int kChunk = kPB * 2;           // rows per chunk (one pair per thread)
-constexpr int kTileRows = kChunk * kPairs;  // 2048 rows per tile
+constexpr int kTileRows = kChunk * kPairs;  // 4096 rows per tile
 constexpr int kWaves = kPB / 64;
 
 constexpr uint64_t kStAgg = 1ULL << 62;
@@ -270,9 +333,34 @@ __device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& 
   }
 }
 
+// Matches of one tile counted by a whole wave (used when a predecessor tile
+// has not published its count yet, so no tile ever waits on another's progress).
+template <int KIND, bool UNIQUE>
+__device__ uint64_t tile_count(const TableView& T, const int64_t* __restrict__ keys, uint64_t n, uint64_t tile) {
+  const uint32_t lane = lane_id();
+  uint64_t c = 0;
+  const uint64_t base = tile * kTileRows;
+  for (uint32_t i = lane; i < static_cast<uint32_t>(kTileRows); i += 64) {
+    const uint64_t r = base + i;
+    if (r < n) {
+      uint32_t v, cn;
+      if (lookup<KIND, UNIQUE>(T, keys[r], v, cn)) c += cn;
+    }
+  }
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+  return c;
+}
+
 // Wave-parallel decoupled look-back (executed by one whole wave).  Returns the
-// exclusive prefix of `tile`; publishes the tile's inclusive prefix.
-__device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t total, uint32_t* err) {
+// exclusive prefix of `tile`; publishes the tile's inclusive prefix.  A
+// predecessor that stays unpublished for kStallPolls polls is counted here
+// instead of waited for (and its count published), so the scan finishes
+// whatever order the workgroups are scheduled in.
+constexpr uint32_t kStallPolls = 256;
+
+template <int KIND, bool UNIQUE>
+__device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t total, const TableView& T,
+                             const int64_t* __restrict__ keys, uint64_t n) {
   const uint32_t lane = lane_id();
   if (tile == 0) {
     if (lane == 0) __hip_atomic_store(&status[0], kStPrefix | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -281,7 +369,7 @@ __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t total, ui
   if (lane == 0) __hip_atomic_store(&status[tile], kStAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint64_t excl = 0;
   int64_t base = static_cast<int64_t>(tile) - 1;
-  uint32_t spins = 0;
+  uint32_t polls = 0;
   while (true) {
     int64_t idx = base - static_cast<int64_t>(lane);
     uint64_t v = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
@@ -292,11 +380,21 @@ __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t total, ui
     int first_p = pm ? __builtin_ctzll(pm) : 64;
     uint64_t before = first_p == 64 ? ~0ULL : ((1ULL << first_p) - 1);
     if (zm & before) {
-      if (++spins > kSpinLimit) {
-        if (lane == 0) atomicOr(err, 1u);
-        break;
+      if (++polls < kStallPolls) {
+        __builtin_amdgcn_s_sleep(2);
+        continue;
       }
-      __builtin_amdgcn_s_sleep(1);
+      // count the nearest unpublished predecessor ourselves
+      const int j = __builtin_ctzll(zm & before);
+      const int64_t q = base - j;
+      const uint64_t c = tile_count<KIND, UNIQUE>(T, keys, n, static_cast<uint64_t>(q));
+      if (lane == 0) {
+        unsigned long long expect = 0;
+        __hip_atomic_compare_exchange_strong(reinterpret_cast<unsigned long long*>(&status[q]), &expect,
+                                             static_cast<unsigned long long>(kStAgg | c), __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      polls = 0;
       continue;
     }
     uint64_t mine = (static_cast<int>(lane) <= first_p) ? (v & kStMask) : 0;
@@ -319,11 +417,10 @@ __device__ __forceinline__ void wave_reduce_i128(__int128& v) {
   }
 }
 
-__global__ void k_probe_clear(uint64_t* status, uint64_t n_tiles, uint32_t* ticket, prism_probe_result* res) {
+__global__ void k_probe_clear(uint64_t* status, uint64_t n_tiles, prism_probe_result* res) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   for (; i < n_tiles; i += (uint64_t)gridDim.x * blockDim.x) status[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *ticket = 0;
     res->matches = 0;
     res->probe_sum_lo = 0;
     res->probe_sum_hi = 0;
@@ -339,21 +436,20 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
                                                uint32_t n_tiles, uint32_t probe_base,
                                                const int64_t* __restrict__ ppay, const int32_t* __restrict__ bpay,
                                                uint32_t* __restrict__ out_p, uint32_t* __restrict__ out_b,
-                                               uint64_t capacity, uint64_t* status, uint32_t* ticket,
+                                               uint64_t capacity, uint64_t* status, __int128* tile_sums,
                                                prism_probe_result* res) {
-  __shared__ uint32_t s_tile;
   __shared__ uint64_t s_wave_tot[kPairs][kWaves];
   __shared__ uint64_t s_excl;
   __shared__ __int128 s_red[2][kWaves];
 
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  if (t == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = blockIdx.x;
   const uint64_t base = static_cast<uint64_t>(tile) * kTileRows;
 
-  int64_t k[kPairs][2];
+  // Keys (and probe payloads) for 8 rows: pairs of consecutive rows, one
+  // 128-bit load per pair, consecutive lanes on consecutive pairs.
+  int64_t k[kPairs][2], pv[kPairs][2];
   bool ok[kPairs][2];
 #pragma unroll
   for (int j = 0; j < kPairs; ++j) {
@@ -363,11 +459,20 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
       k[j][0] = v.x;
       k[j][1] = v.y;
       ok[j][0] = ok[j][1] = true;
+      if (ppay) {
+        const v2i64 p = __builtin_nontemporal_load(reinterpret_cast<const v2i64*>(ppay + r0));
+        pv[j][0] = p.x;
+        pv[j][1] = p.y;
+      } else {
+        pv[j][0] = pv[j][1] = 0;
+      }
     } else {
       ok[j][0] = r0 < n;
       ok[j][1] = r0 + 1 < n;
       k[j][0] = ok[j][0] ? keys[r0] : 0;
       k[j][1] = ok[j][1] ? keys[r0 + 1] : 0;
+      pv[j][0] = (ppay && ok[j][0]) ? ppay[r0] : 0;
+      pv[j][1] = (ppay && ok[j][1]) ? ppay[r0 + 1] : 0;
     }
   }
 
@@ -381,6 +486,21 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
       val[j][e] = v;
       cnt[j][e] = m ? c : 0;
     }
+
+  // Payload sums over the matches (gathers issued before the scan so their
+  // latency overlaps it).
+  __int128 psum = 0, bsum = 0;
+  if constexpr (UNIQUE) {
+#pragma unroll
+    for (int j = 0; j < kPairs; ++j)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        if (cnt[j][e]) {
+          psum += pv[j][e];
+          if (bpay) bsum += bpay[val[j][e]];
+        }
+      }
+  }
 
   // Rank matches inside the tile: row order is (chunk, thread, element).
   uint64_t in_wave[kPairs];
@@ -417,14 +537,13 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
     wave_base[j] = wb;
   }
   if (wave == 0) {
-    uint64_t ex = lookback(status, tile, tile_total, &res->error);
+    uint64_t ex = lookback<KIND, UNIQUE>(status, tile, tile_total, T, keys, n);
     if (lane == 0) s_excl = ex;
   }
   __syncthreads();
   const uint64_t gbase = s_excl;
 
   const bool want_pairs = out_p != nullptr;
-  __int128 psum = 0, bsum = 0;
 #pragma unroll
   for (int j = 0; j < kPairs; ++j)
 #pragma unroll
@@ -437,8 +556,6 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
           __builtin_nontemporal_store(static_cast<uint32_t>(row) + probe_base, out_p + pos);
           __builtin_nontemporal_store(val[j][e], out_b + pos);
         }
-        if (ppay) psum += ppay[row];
-        if (bpay) bsum += bpay[val[j][e]];
       } else {
         const uint32_t c = cnt[j][e];
         const uint32_t start = val[j][e];
@@ -450,11 +567,11 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
           }
           if (bpay) bsum += bpay[brow];
         }
-        if (ppay) psum += static_cast<__int128>(ppay[row]) * static_cast<__int128>(c);
+        psum += static_cast<__int128>(pv[j][e]) * static_cast<__int128>(c);
       }
     }
 
-  if (ppay || bpay) {
+  if (ppay || bpay) {  // per-tile partials, summed by k_probe_sums (no contended atomics)
     wave_reduce_i128(psum);
     wave_reduce_i128(bsum);
     if (lane == 0) {
@@ -468,30 +585,54 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
         p += s_red[0][w];
         b += s_red[1][w];
       }
-      if (p != 0)
-        atomic_add_i128(reinterpret_cast<unsigned long long*>(&res->probe_sum_lo),
-                        reinterpret_cast<unsigned long long*>(&res->probe_sum_hi), static_cast<uint64_t>(p),
-                        static_cast<int64_t>(p >> 64));
-      if (b != 0)
-        atomic_add_i128(reinterpret_cast<unsigned long long*>(&res->build_sum_lo),
-                        reinterpret_cast<unsigned long long*>(&res->build_sum_hi), static_cast<uint64_t>(b),
-                        static_cast<int64_t>(b >> 64));
+      tile_sums[2 * tile] = p;
+      tile_sums[2 * tile + 1] = b;
     }
   }
   if (tile == n_tiles - 1 && t == 0) res->matches = gbase + tile_total;
+}
+
+// Sum of the per-tile payload partials (one workgroup).
+__global__ void k_probe_sums(const __int128* __restrict__ tile_sums, uint64_t n_tiles, prism_probe_result* res) {
+  __shared__ __int128 s_red[2][16];
+  __int128 p = 0, b = 0;
+  for (uint64_t i = threadIdx.x; i < n_tiles; i += blockDim.x) {
+    p += tile_sums[2 * i];
+    b += tile_sums[2 * i + 1];
+  }
+  wave_reduce_i128(p);
+  wave_reduce_i128(b);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_red[0][w] = p;
+    s_red[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    p = 0;
+    b = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) {
+      p += s_red[0][i];
+      b += s_red[1][i];
+    }
+    res->probe_sum_lo = static_cast<uint64_t>(p);
+    res->probe_sum_hi = static_cast<int64_t>(p >> 64);
+    res->build_sum_lo = static_cast<uint64_t>(b);
+    res->build_sum_hi = static_cast<int64_t>(b >> 64);
+  }
 }
 
 template <int KIND, bool UNIQUE>
 static void launch_probe_vec(prism_ctx* ctx, bool vec, dim3 g, const TableView& T, const int64_t* keys,
                              uint64_t n, uint32_t n_tiles, uint32_t probe_base, const int64_t* ppay,
                              const int32_t* bpay, uint32_t* op, uint32_t* ob, uint64_t cap, uint64_t* status,
-                             uint32_t* ticket, prism_probe_result* res) {
+                             __int128* tile_sums, prism_probe_result* res) {
   if (vec)
     hipLaunchKernelGGL((k_probe<KIND, UNIQUE, true>), g, dim3(kPB), 0, ctx->stream, T, keys, n, n_tiles,
-                       probe_base, ppay, bpay, op, ob, cap, status, ticket, res);
+                       probe_base, ppay, bpay, op, ob, cap, status, tile_sums, res);
   else
     hipLaunchKernelGGL((k_probe<KIND, UNIQUE, false>), g, dim3(kPB), 0, ctx->stream, T, keys, n, n_tiles,
-                       probe_base, ppay, bpay, op, ob, cap, status, ticket, res);
+                       probe_base, ppay, bpay, op, ob, cap, status, tile_sums, res);
 }
 
 }  // namespace join
@@ -521,10 +662,11 @@ prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_
     T->ctx = c;
     T->info.n_rows = n_rows;
 
-    DeviceBuffer sbuf(c, sizeof(Stats));
+    DeviceBuffer sbuf(c, sizeof(Stats) + 16 * kRedBlocks);
     Stats* dst = sbuf.as<Stats>();
+    void* part = sbuf.as<char>() + sizeof(Stats);
     hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(1), 0, st, dst);
-    if (n_rows) hipLaunchKernelGGL(k_minmax, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, dst);
+    if (n_rows) launch_minmax(ctx, d_keys, n_rows, static_cast<int64_t*>(part), dst);
     PRISM_CHECK_LAUNCH();
     Stats hs;
     PRISM_HIP(hipMemcpyAsync(&hs, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
@@ -553,9 +695,7 @@ prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_
       if (n_rows)
         hipLaunchKernelGGL(k_dense_count, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows,
                            hs.min_key, cnt.as<uint32_t>(), slot_of_row.as<uint32_t>());
-      hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(1), 0, st, dst);
-      hipLaunchKernelGGL(k_count_stats_u32, grid_for(ctx, range), dim3(kBuildBlock), 0, st, cnt.as<uint32_t>(),
-                         range, dst);
+      launch_count_stats(ctx, cnt.as<uint32_t>(), range, static_cast<uint64_t*>(part), dst);
       PRISM_CHECK_LAUNCH();
       PRISM_HIP(hipMemcpyAsync(&hs, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
       PRISM_HIP(hipStreamSynchronize(st));
@@ -582,7 +722,7 @@ prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_
         T->view.dense_nu = T->dense.as<uint2>();
         T->view.rows = T->rows.as<uint32_t>();
       }
-      T->view.min_key = hs.min_key;
+      T->view.min_key = T->info.min_key;  // hs was reused for the count statistics
       T->view.range = range;
     } else {
       T->info.kind = PRISM_TABLE_HASHED;
@@ -598,9 +738,7 @@ prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_
       DeviceBuffer cnt(c, (S + 1) * 4);
       hipLaunchKernelGGL(k_slot_counts, grid_for(ctx, S + 1), dim3(kBuildBlock), 0, st, slots, S + 1,
                          cnt.as<uint32_t>());
-      hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(1), 0, st, dst);
-      hipLaunchKernelGGL(k_count_stats_u32, grid_for(ctx, S + 1), dim3(kBuildBlock), 0, st, cnt.as<uint32_t>(),
-                         S + 1, dst);
+      launch_count_stats(ctx, cnt.as<uint32_t>(), S + 1, static_cast<uint64_t*>(part), dst);
       PRISM_CHECK_LAUNCH();
       PRISM_HIP(hipMemcpyAsync(&hs, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
       PRISM_HIP(hipStreamSynchronize(st));
@@ -659,11 +797,13 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
     Ctx* c = as_ctx(ctx);
     std::lock_guard<std::mutex> lock(ctx->mu);
     const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
-    c->scratch.reserve(c, 256 + (n_tiles + 1) * 8);
-    uint32_t* ticket = c->scratch.as<uint32_t>();
-    uint64_t* status = reinterpret_cast<uint64_t*>(c->scratch.as<char>() + 256);
+    // scratch: tile status words, then per-tile int128 payload partials
+    const uint64_t status_bytes = ((n_tiles + 1) * 8 + 255) & ~255ULL;
+    c->scratch.reserve(c, status_bytes + n_tiles * 32 + 32);
+    uint64_t* status = c->scratch.as<uint64_t>();
+    __int128* tile_sums = reinterpret_cast<__int128*>(c->scratch.as<char>() + status_bytes);
     hipLaunchKernelGGL(k_probe_clear, grid_for(ctx, n_tiles + 1), dim3(kBuildBlock), 0, ctx->stream, status,
-                       n_tiles, ticket, d_result);
+                       n_tiles, d_result);
     PRISM_CHECK_LAUNCH();
     if (n_probe == 0) return;
     const bool vec = (reinterpret_cast<uintptr_t>(d_probe_keys) & 15) == 0;
@@ -674,22 +814,26 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
       if (t->info.unique)
         launch_probe_vec<PRISM_TABLE_DENSE, true>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
                                                   d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                  capacity, status, ticket, d_result);
+                                                  capacity, status, tile_sums, d_result);
       else
         launch_probe_vec<PRISM_TABLE_DENSE, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
                                                    d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                   capacity, status, ticket, d_result);
+                                                   capacity, status, tile_sums, d_result);
     } else {
       if (t->info.unique)
         launch_probe_vec<PRISM_TABLE_HASHED, true>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
                                                    d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                   capacity, status, ticket, d_result);
+                                                   capacity, status, tile_sums, d_result);
       else
         launch_probe_vec<PRISM_TABLE_HASHED, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
                                                     d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                    capacity, status, ticket, d_result);
+                                                    capacity, status, tile_sums, d_result);
     }
     PRISM_CHECK_LAUNCH();
+    if (d_probe_payload || d_build_payload) {
+      hipLaunchKernelGGL(k_probe_sums, dim3(1), dim3(1024), 0, ctx->stream, tile_sums, n_tiles, d_result);
+      PRISM_CHECK_LAUNCH();
+    }
   });
 }
 

@@ -1,0 +1,398 @@
+// Arrow-level hash aggregation: hash_aggregate / hash_aggregate_batches
+// (native/prism-executor/src/hash_aggregate.rs:174-527) over the Arrow C Data
+// Interface, running the device group-by (groupby.hip) and, for COUNT
+// DISTINCT, a sort + reduce-by-key over (group id, value) pairs in HBM.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "gather.hpp"
+#include "program.hpp"
+
+namespace prism {
+namespace aagg {
+
+constexpr int kB = 256;
+
+static dim3 agrid(prism_ctx* ctx, uint64_t n) {
+  uint64_t b = ceil_div(n ? n : 1, kB);
+  uint64_t cap = static_cast<uint64_t>(ctx->num_cus) * 16;
+  return dim3(static_cast<unsigned>(b < cap ? b : cap));
+}
+
+struct KeyArg {
+  prism_column c[kMaxKeys];
+  int n;
+};
+
+__global__ void k_fill_u64_zero(unsigned long long* p) { *p = 0; }
+
+__device__ __forceinline__ uint64_t smix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// (group id, DistinctValue word) for every non-null value (hash_aggregate.rs:457-481):
+// Float64 folds -0.0 into 0.0 and every NaN into one NaN; Utf8 values are
+// represented by a 64-bit fingerprint of their bytes.
+__global__ void k_distinct_pairs(KeyArg K, prism_column v, uint64_t n, unsigned long long* __restrict__ cursor,
+                                 uint64_t* __restrict__ g_out, uint64_t* __restrict__ v_out) {
+  for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    if (!col_valid(v, r)) continue;
+    uint64_t h = 0;
+    for (int k = 0; k < K.n; ++k) h = hash_key(K.c[k], r, h);
+    const uint64_t i = v.offset + r;
+    uint64_t w;
+    switch (v.type) {
+      case PRISM_TYPE_INT64: w = static_cast<uint64_t>(static_cast<const int64_t*>(v.data)[i]); break;
+      case PRISM_TYPE_FLOAT64: {
+        double d = static_cast<const double*>(v.data)[i];
+        if (d == 0.0) d = 0.0;
+        w = isnan(d) ? 0x7ff8000000000000ULL : static_cast<uint64_t>(__double_as_longlong(d));
+        break;
+      }
+      case PRISM_TYPE_BOOL: w = bit_at(static_cast<const uint8_t*>(v.data), i); break;
+      case PRISM_TYPE_UTF8: {
+        const int32_t b = v.offsets[i], e = v.offsets[i + 1];
+        const uint8_t* s = static_cast<const uint8_t*>(v.data) + b;
+        uint64_t x = smix(static_cast<uint64_t>(e - b) + 0x51ED27ULL);
+        for (int32_t p = 0; p < e - b; ++p) x = smix(x ^ (static_cast<uint64_t>(s[p]) << ((p & 7) * 8)) ^ p);
+        w = x;
+        break;
+      }
+      default: w = static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(v.data)[i]));
+    }
+    unsigned long long p = atomicAdd(cursor, 1ULL);
+    g_out[p] = h;
+    v_out[p] = w;
+  }
+}
+
+__global__ void k_first_of_run(const uint64_t* __restrict__ g, const uint64_t* __restrict__ v, uint64_t n,
+                               uint32_t* __restrict__ flag) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    flag[i] = (i == 0 || g[i] != g[i - 1] || v[i] != v[i - 1]) ? 1u : 0u;
+}
+
+// Distinct counts per group id for one aggregate column across all batches.
+static std::unordered_map<uint64_t, int64_t> count_distinct(prism_ctx* ctx, const std::vector<HostBatch>& batches,
+                                                            const std::vector<uint32_t>& group_by, uint32_t column) {
+  Ctx* c = as_ctx(ctx);
+  hipStream_t st = ctx->stream;
+  uint64_t total = 0;
+  for (const HostBatch& b : batches) total += static_cast<uint64_t>(b.n_rows);
+  DeviceBuffer g(c, (total ? total : 1) * 8), v(c, (total ? total : 1) * 8), cur(c, 8);
+  hipLaunchKernelGGL(k_fill_u64_zero, dim3(1), dim3(1), 0, st, cur.as<unsigned long long>());
+  for (const HostBatch& b : batches) {
+    if (b.n_rows == 0) continue;
+    std::vector<DevColumn> keys;
+    KeyArg K;
+    memset(&K, 0, sizeof(K));
+    K.n = static_cast<int>(group_by.size());
+    for (size_t k = 0; k < group_by.size(); ++k) {
+      keys.push_back(upload(ctx, b.cols[group_by[k]]));
+      K.c[k] = keys.back().desc();
+    }
+    DevColumn vc = upload(ctx, b.cols[column]);
+    hipLaunchKernelGGL(k_distinct_pairs, agrid(ctx, b.n_rows), dim3(kB), 0, st, K, vc.desc(),
+                       static_cast<uint64_t>(b.n_rows), cur.as<unsigned long long>(), g.as<uint64_t>(),
+                       v.as<uint64_t>());
+    PRISM_CHECK_LAUNCH();
+    PRISM_HIP(hipStreamSynchronize(st));
+  }
+  unsigned long long n = 0;
+  PRISM_HIP(hipMemcpyAsync(&n, cur.ptr, 8, hipMemcpyDeviceToHost, st));
+  PRISM_HIP(hipStreamSynchronize(st));
+  std::unordered_map<uint64_t, int64_t> out;
+  if (n == 0) return out;
+  // LSD order: by value, then stably by group -> sorted by (group, value).
+  DeviceBuffer g2(c, n * 8), v2(c, n * 8), flag(c, n * 4), ug(c, n * 8), uc(c, n * 8), nr(c, 8);
+  size_t t1 = 0, t2 = 0, t3 = 0;
+  const int ni = static_cast<int>(n);
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, v.as<uint64_t>(), v2.as<uint64_t>(), g.as<uint64_t>(),
+                                               g2.as<uint64_t>(), ni, 0, 64, st));
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, g2.as<uint64_t>(), g.as<uint64_t>(), v2.as<uint64_t>(),
+                                               v.as<uint64_t>(), ni, 0, 64, st));
+  PRISM_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, t3, g.as<uint64_t>(), ug.as<uint64_t>(), flag.as<uint32_t>(),
+                                              uc.as<uint64_t>(), nr.as<uint64_t>(), hipcub::Sum(), ni, st));
+  size_t tmax = t1 > t2 ? t1 : t2;
+  tmax = tmax > t3 ? tmax : t3;
+  DeviceBuffer tmp(c, tmax);
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, t1, v.as<uint64_t>(), v2.as<uint64_t>(), g.as<uint64_t>(),
+                                               g2.as<uint64_t>(), ni, 0, 64, st));
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, t2, g2.as<uint64_t>(), g.as<uint64_t>(), v2.as<uint64_t>(),
+                                               v.as<uint64_t>(), ni, 0, 64, st));
+  hipLaunchKernelGGL(k_first_of_run, agrid(ctx, n), dim3(kB), 0, st, g.as<uint64_t>(), v.as<uint64_t>(), n,
+                     flag.as<uint32_t>());
+  PRISM_CHECK_LAUNCH();
+  PRISM_HIP(hipcub::DeviceReduce::ReduceByKey(tmp.ptr, t3, g.as<uint64_t>(), ug.as<uint64_t>(), flag.as<uint32_t>(),
+                                              uc.as<uint64_t>(), nr.as<uint64_t>(), hipcub::Sum(), ni, st));
+  uint64_t runs = 0;
+  PRISM_HIP(hipMemcpyAsync(&runs, nr.ptr, 8, hipMemcpyDeviceToHost, st));
+  PRISM_HIP(hipStreamSynchronize(st));
+  std::vector<uint64_t> hg(runs), hc(runs);
+  PRISM_HIP(hipMemcpyAsync(hg.data(), ug.ptr, runs * 8, hipMemcpyDeviceToHost, st));
+  PRISM_HIP(hipMemcpyAsync(hc.data(), uc.ptr, runs * 8, hipMemcpyDeviceToHost, st));
+  PRISM_HIP(hipStreamSynchronize(st));
+  for (uint64_t i = 0; i < runs; ++i) out[hg[i]] = static_cast<int64_t>(hc[i]);
+  return out;
+}
+
+// Group key value of one row copied into an output column (slice + concat,
+// hash_aggregate.rs:302-314).
+static void append_value(OutColumn& o, const HostColumn& c, int64_t row, int64_t out_row) {
+  const bool valid = host_valid(c, row);
+  if (!valid) {
+    if (o.validity.empty()) o.validity.assign(static_cast<size_t>((o.length + 7) / 8), 0xff);
+    o.validity[out_row >> 3] &= static_cast<uint8_t>(~(1u << (out_row & 7)));
+    o.null_count++;
+  }
+  const int64_t i = c.offset + row;
+  switch (c.type) {
+    case DType::Utf8: {
+      const int32_t b = c.offsets[i], e = c.offsets[i + 1];
+      const uint8_t* s = static_cast<const uint8_t*>(c.data) + b;
+      o.data.insert(o.data.end(), s, s + (e - b));
+      o.offsets.push_back(static_cast<int32_t>(o.data.size()));
+      break;
+    }
+    case DType::Bool: {
+      const bool bit = (static_cast<const uint8_t*>(c.data)[i >> 3] >> (i & 7)) & 1;
+      if (bit) o.data[out_row >> 3] |= static_cast<uint8_t>(1u << (out_row & 7));
+      break;
+    }
+    default: {
+      const int w = dtype_width(c.type);
+      memcpy(o.data.data() + out_row * w, static_cast<const uint8_t*>(c.data) + i * w, w);
+    }
+  }
+}
+
+static bool is_int_type(DType t) { return t == DType::Int32 || t == DType::Int64 || t == DType::Bool; }
+
+static void aggregate(prism_ctx* ctx, const std::vector<HostBatch>& batches, const std::vector<HostColumn>& schema,
+                      const prism_hash_agg_config* cfg, ArrowArray* out, ArrowSchema* out_schema) {
+  if (!cfg) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL aggregation config");
+  std::vector<uint32_t> group_by(cfg->group_by, cfg->group_by + cfg->n_group_by);
+  std::vector<prism_agg_expr> aggs(cfg->aggregates, cfg->aggregates + cfg->n_aggregates);
+  const bool exact = (cfg->flags & PRISM_AGG_FLAG_EXACT_DECIMAL) != 0;
+  const size_t n_cols = schema.size();
+  for (uint32_t k : group_by)
+    if (k >= n_cols) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: group-by column out of range");
+  for (const prism_agg_expr& a : aggs) {
+    if (a.func < PRISM_AGG_SUM || a.func > PRISM_AGG_COUNT_DISTINCT)
+      fail(PRISM_ERR_UNSUPPORTED_AGGREGATION, "Aggregation error: unsupported function " + std::to_string(a.func));
+    const bool star = !group_by.empty() ? false : (a.func == PRISM_AGG_COUNT && a.column >= n_cols);
+    if (a.column >= n_cols && !star && !(a.func == PRISM_AGG_COUNT && !group_by.empty() && a.column >= n_cols))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: aggregate column out of range");
+    if (a.func != PRISM_AGG_COUNT && a.func != PRISM_AGG_COUNT_DISTINCT && a.column < n_cols) {
+      DType t = schema[a.column].type;
+      if (!(t == DType::Int32 || t == DType::Int64 || t == DType::Float64 || t == DType::Bool))
+        fail(PRISM_ERR_ARROW, "Arrow error: cannot cast " + dtype_name(t) + " to Float64 for aggregation");
+    }
+  }
+  bool any_rows = false;
+  for (const HostBatch& b : batches) any_rows |= b.n_rows > 0;
+  std::vector<OutColumn> cols;
+  auto agg_type = [&](const prism_agg_expr& a) {
+    if (a.func == PRISM_AGG_COUNT || a.func == PRISM_AGG_COUNT_DISTINCT) return DType::Int64;
+    if (exact && a.func == PRISM_AGG_SUM && a.column < n_cols && is_int_type(schema[a.column].type))
+      return DType::Decimal128;
+    return DType::Float64;
+  };
+  if (!any_rows) {  // build_empty_result (hash_aggregate.rs:506-527)
+    for (uint32_t k : group_by) {
+      OutColumn o;
+      o.type = schema[k].type;
+      o.name = schema[k].name;
+      o.nullable = schema[k].nullable;
+      cols.push_back(std::move(o));
+    }
+    for (const prism_agg_expr& a : aggs) {
+      OutColumn o;
+      o.type = agg_type(a);
+      o.name = a.output_name ? a.output_name : "";
+      cols.push_back(std::move(o));
+    }
+    export_batch(std::move(cols), 0, out, out_schema);
+    return;
+  }
+  const bool global = group_by.empty();
+  // Plan over the columns the aggregation reads: keys first, then values.
+  std::vector<uint32_t> used;
+  std::map<uint32_t, uint32_t> slot;
+  auto use = [&](uint32_t col) {
+    auto it = slot.find(col);
+    if (it != slot.end()) return it->second;
+    uint32_t s = static_cast<uint32_t>(used.size());
+    slot[col] = s;
+    used.push_back(col);
+    return s;
+  };
+  std::vector<uint32_t> key_slots;
+  for (uint32_t k : group_by) key_slots.push_back(use(k));
+  std::vector<prism_instr> code;
+  std::vector<uint32_t> lens;
+  std::map<uint32_t, int32_t> value_of_col;
+  std::vector<prism_groupby_agg> gaggs;
+  std::vector<int> distinct_idx;
+  for (size_t i = 0; i < aggs.size(); ++i) {
+    const prism_agg_expr& a = aggs[i];
+    prism_groupby_agg g;
+    g.func = a.func == PRISM_AGG_COUNT_DISTINCT ? PRISM_AGG_COUNT : a.func;
+    g.value = -1;
+    if (a.func == PRISM_AGG_COUNT_DISTINCT) distinct_idx.push_back(static_cast<int>(i));
+    const bool needs_value = a.column < n_cols && (a.func != PRISM_AGG_COUNT || global) &&
+                             a.func != PRISM_AGG_COUNT_DISTINCT;
+    if (needs_value) {
+      auto it = value_of_col.find(a.column);
+      if (it == value_of_col.end()) {
+        const uint32_t s = use(a.column);
+        prism_instr ins;
+        memset(&ins, 0, sizeof(ins));
+        const DType t = schema[a.column].type;
+        ins.op = t == DType::Float64 ? PRISM_OP_LOAD_F : PRISM_OP_LOAD_I;
+        ins.arg = static_cast<int32_t>(s);
+        code.push_back(ins);
+        lens.push_back(1);
+        it = value_of_col.emplace(a.column, static_cast<int32_t>(lens.size() - 1)).first;
+      }
+      g.value = it->second;
+    }
+    gaggs.push_back(g);
+  }
+  std::vector<int32_t> types;
+  for (uint32_t col : used) types.push_back(static_cast<int32_t>(schema[col].type));
+  prism_groupby_plan plan;
+  memset(&plan, 0, sizeof(plan));
+  plan.n_columns = static_cast<uint32_t>(used.size());
+  plan.column_types = types.data();
+  plan.key_columns = key_slots.data();
+  plan.n_keys = static_cast<uint32_t>(key_slots.size());
+  plan.values = code.data();
+  plan.value_lengths = lens.data();
+  plan.n_values = static_cast<uint32_t>(lens.size());
+  plan.aggs = gaggs.data();
+  plan.n_aggs = static_cast<uint32_t>(gaggs.size());
+  prism_groupby* g = nullptr;
+  prism_status s = prism_groupby_create(ctx, &plan, 0, &g);
+  if (s != PRISM_OK) fail(s, prism_last_error());
+  std::unique_ptr<prism_groupby, void (*)(prism_groupby*)> guard(g, prism_groupby_destroy);
+  std::vector<int64_t> batch_start;
+  int64_t base = 0;
+  for (const HostBatch& b : batches) {
+    batch_start.push_back(base);
+    if (b.n_rows > 0) {
+      std::vector<DevColumn> dc;
+      std::vector<prism_column> desc;
+      for (uint32_t col : used) {
+        dc.push_back(upload(ctx, b.cols[col]));
+        desc.push_back(dc.back().desc());
+      }
+      PRISM_HIP(hipStreamSynchronize(ctx->stream));
+      s = prism_groupby_update(g, desc.data(), static_cast<uint64_t>(b.n_rows), static_cast<uint64_t>(base));
+      if (s != PRISM_OK) fail(s, prism_last_error());
+    }
+    base += b.n_rows;
+  }
+  uint64_t n_groups = 0;
+  s = prism_groupby_finish(g, &n_groups);
+  if (s != PRISM_OK) fail(s, prism_last_error());
+  const size_t na = aggs.size();
+  std::vector<uint64_t> first(n_groups), khash(n_groups), rcount(n_groups);
+  std::vector<std::vector<double>> f64(na, std::vector<double>(n_groups));
+  std::vector<std::vector<int64_t>> i64(na, std::vector<int64_t>(n_groups));
+  std::vector<std::vector<uint64_t>> s128(na, std::vector<uint64_t>(2 * n_groups));
+  std::vector<double*> fp(na);
+  std::vector<int64_t*> ip(na);
+  std::vector<uint64_t*> sp(na);
+  for (size_t a = 0; a < na; ++a) {
+    fp[a] = f64[a].data();
+    ip[a] = i64[a].data();
+    sp[a] = s128[a].data();
+  }
+  prism_groupby_output go{first.data(), khash.data(), rcount.data(), fp.data(), ip.data(), sp.data()};
+  s = prism_groupby_read(g, &go);
+  if (s != PRISM_OK) fail(s, prism_last_error());
+  // COUNT DISTINCT per group id
+  std::vector<std::unordered_map<uint64_t, int64_t>> distinct(na);
+  for (int a : distinct_idx) distinct[a] = count_distinct(ctx, batches, group_by, aggs[a].column);
+  // Output: keys from each group's first row, then aggregates.
+  for (uint32_t k : group_by) {
+    const HostColumn& sc = schema[k];
+    OutColumn o;
+    o.type = sc.type;
+    o.name = sc.name;
+    o.nullable = sc.nullable;
+    o.length = static_cast<int64_t>(n_groups);
+    if (sc.type == DType::Utf8) o.offsets.push_back(0);
+    else if (sc.type == DType::Bool) o.data.assign(static_cast<size_t>((n_groups + 7) / 8), 0);
+    else o.data.resize(static_cast<size_t>(n_groups) * dtype_width(sc.type));
+    for (uint64_t i = 0; i < n_groups; ++i) {
+      size_t bi = std::upper_bound(batch_start.begin(), batch_start.end(), static_cast<int64_t>(first[i])) -
+                  batch_start.begin() - 1;
+      while (batches[bi].n_rows == 0) --bi;
+      append_value(o, batches[bi].cols[k], static_cast<int64_t>(first[i]) - batch_start[bi], static_cast<int64_t>(i));
+    }
+    cols.push_back(std::move(o));
+  }
+  for (size_t a = 0; a < na; ++a) {
+    OutColumn o;
+    o.type = agg_type(aggs[a]);
+    o.name = aggs[a].output_name ? aggs[a].output_name : "";
+    o.length = static_cast<int64_t>(n_groups);
+    if (o.type == DType::Int64) {
+      o.data.resize(n_groups * 8);
+      for (uint64_t i = 0; i < n_groups; ++i) {
+        int64_t v = i64[a][i];
+        if (aggs[a].func == PRISM_AGG_COUNT_DISTINCT) {
+          auto it = distinct[a].find(khash[i]);
+          v = it == distinct[a].end() ? 0 : it->second;
+        }
+        memcpy(o.data.data() + i * 8, &v, 8);
+      }
+    } else if (o.type == DType::Decimal128) {
+      o.data.resize(n_groups * 16);
+      memcpy(o.data.data(), s128[a].data(), n_groups * 16);
+    } else {
+      o.data.resize(n_groups * 8);
+      memcpy(o.data.data(), f64[a].data(), n_groups * 8);
+    }
+    cols.push_back(std::move(o));
+  }
+  export_batch(std::move(cols), static_cast<int64_t>(n_groups), out, out_schema);
+}
+
+}  // namespace aagg
+}  // namespace prism
+
+using namespace prism;
+using namespace prism::aagg;
+
+extern "C" {
+
+prism_status prism_hash_aggregate_batches(prism_ctx* ctx, const struct ArrowArray* const* batches, uint32_t n_batches,
+                                          const struct ArrowSchema* schema, const prism_hash_agg_config* cfg,
+                                          struct ArrowArray* out, struct ArrowSchema* out_schema) {
+  return guarded([&] {
+    if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
+    use_device(ctx);
+    std::vector<HostBatch> bs;
+    for (uint32_t i = 0; i < n_batches; ++i) bs.push_back(import_batch(batches[i], schema));
+    std::vector<HostColumn> sch = import_schema(schema);
+    aggregate(ctx, bs, sch, cfg, out, out_schema);
+  });
+}
+
+prism_status prism_hash_aggregate(prism_ctx* ctx, const struct ArrowArray* batch, const struct ArrowSchema* schema,
+                                  const prism_hash_agg_config* cfg, struct ArrowArray* out,
+                                  struct ArrowSchema* out_schema) {
+  const struct ArrowArray* one[1] = {batch};
+  return prism_hash_aggregate_batches(ctx, one, 1, schema, cfg, out, out_schema);
+}
+
+}  // extern "C"

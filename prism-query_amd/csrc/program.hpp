@@ -142,15 +142,51 @@ __device__ inline bool like_match(const uint8_t* s, int32_t ns, const uint8_t* p
   return pi == np;
 }
 
+// FxHash word(s) of one key value (hash_aggregate.rs:24-75,492-504): null
+// writes u8 0; Int32 sign-extends; Float64 hashes its bits; Utf8 writes its
+// bytes as zero-padded 8-byte LE words then 0xff.  Date32 / Boolean use an
+// injective word away from 0 (the reference hashes a Debug string there; only
+// the partition into groups is observable).
+__device__ __forceinline__ uint64_t hash_key(const prism_column& c, uint64_t row, uint64_t state) {
+  if (!col_valid(c, row)) return fx_step(state, 0);  // 0u8.hash(): write_u8(0)
+  const uint64_t i = c.offset + row;
+  switch (c.type) {
+    case PRISM_TYPE_INT64:
+      return fx_step(state, static_cast<uint64_t>(static_cast<const int64_t*>(c.data)[i]));
+    case PRISM_TYPE_INT32:  // write_i32 -> write_u64(i as u64): sign extension
+      return fx_step(state, static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[i])));
+    case PRISM_TYPE_FLOAT64:  // to_bits().hash()
+      return fx_step(state, static_cast<const uint64_t*>(c.data)[i]);
+    case PRISM_TYPE_DATE32:  // value word kept away from the null word 0
+      return fx_step(state, static_cast<uint64_t>(static_cast<uint32_t>(static_cast<const int32_t*>(c.data)[i])) |
+                                (1ULL << 32));
+    case PRISM_TYPE_BOOL:
+      return fx_step(state, 2 + (bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0));
+    default: {  // UTF8: str::hash = write(bytes) in 8-byte LE words, then write_u8(0xff)
+      const int32_t b = c.offsets[i], e = c.offsets[i + 1];
+      const uint8_t* s = static_cast<const uint8_t*>(c.data) + b;
+      const int32_t len = e - b;
+      for (int32_t p = 0; p < len; p += 8) {
+        uint64_t w = 0;
+        const int32_t m = len - p < 8 ? len - p : 8;
+        for (int32_t k = 0; k < m; ++k) w |= static_cast<uint64_t>(s[p + k]) << (8 * k);
+        state = fx_step(state, w);
+      }
+      return fx_step(state, 0xff);
+    }
+  }
+}
+
 // Evaluate `len` instructions for R rows held by this lane.  Values are raw
 // 64-bit words (int64 or f64 bits); `valid` bit r = row r non-null.
-template <int R>
+template <int R, int DEPTH = kStackDepth>
 __device__ __forceinline__ void run_program(const prism_instr* __restrict__ code, int len,
                                             const prism_column* __restrict__ cols,
                                             const StrTable* __restrict__ strs, const uint64_t (&row)[R],
                                             const bool (&in)[R], uint64_t (&out)[R], uint32_t& out_valid,
                                             uint32_t* err) {
-  uint64_t st[kStackDepth * R];
+  static_assert(DEPTH * R <= 32, "validity bits fit one word");
+  uint64_t st[DEPTH * R];
   // Validity bits of stack slot k live in bits [k*R, k*R + R) of one word.
   uint32_t vbits = 0;
   constexpr uint32_t kAll = (1u << R) - 1;

@@ -98,6 +98,42 @@ class GroupbyOutput(Structure):
                 ("agg_i64", POINTER(POINTER(c_int64))), ("agg_sum128", POINTER(POINTER(c_uint64)))]
 
 
+class ArrowSchema(Structure):
+    pass
+
+
+ArrowSchema._fields_ = [("format", c_char_p), ("name", c_char_p), ("metadata", c_char_p), ("flags", c_int64),
+                        ("n_children", c_int64), ("children", POINTER(POINTER(ArrowSchema))),
+                        ("dictionary", POINTER(ArrowSchema)), ("release", c_void_p), ("private_data", c_void_p)]
+
+
+class ArrowArray(Structure):
+    pass
+
+
+ArrowArray._fields_ = [("length", c_int64), ("null_count", c_int64), ("offset", c_int64), ("n_buffers", c_int64),
+                       ("n_children", c_int64), ("buffers", POINTER(c_void_p)),
+                       ("children", POINTER(POINTER(ArrowArray))), ("dictionary", POINTER(ArrowArray)),
+                       ("release", c_void_p), ("private_data", c_void_p)]
+
+
+class HashJoinConfig(Structure):
+    _fields_ = [("join_type", c_int32), ("build_keys", POINTER(c_uint32)), ("n_build_keys", c_uint32),
+                ("probe_keys", POINTER(c_uint32)), ("n_probe_keys", c_uint32)]
+
+
+class AggExprC(Structure):
+    _fields_ = [("column", c_uint32), ("func", c_int32), ("output_name", c_char_p)]
+
+
+class HashAggConfigC(Structure):
+    _fields_ = [("group_by", POINTER(c_uint32)), ("n_group_by", c_uint32), ("aggregates", POINTER(AggExprC)),
+                ("n_aggregates", c_uint32), ("flags", c_uint32)]
+
+
+JOIN = dict(INNER=0, LEFT=1, RIGHT=2, FULL=3, LEFT_SEMI=4, LEFT_ANTI=5)
+AGG_FLAG_EXACT_DECIMAL = 1
+
 _lib = None
 
 # (name, restype, argtypes) for every function of include/prism_hip.h.
@@ -110,6 +146,13 @@ _SIGNATURES = [
     ("prism_ctx_stream", c_void_p, [c_void_p]),
     ("prism_ctx_synchronize", c_int32, [c_void_p]),
     ("prism_ctx_device_bytes", c_uint64, [c_void_p]),
+    ("prism_device_alloc", c_int32, [c_void_p, c_uint64, POINTER(c_void_p)]),
+    ("prism_device_free", c_int32, [c_void_p, c_void_p]),
+    ("prism_copy_to_device", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64]),
+    ("prism_copy_to_host", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64]),
+    ("prism_copy_device", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64]),
+    ("prism_ctx_timer_begin", c_int32, [c_void_p]),
+    ("prism_ctx_timer_end", c_int32, [c_void_p, POINTER(c_double)]),
     ("prism_join_build_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_int32, POINTER(c_void_p)]),
     ("prism_join_table_info_get", c_int32, [c_void_p, POINTER(JoinTableInfo)]),
     ("prism_join_table_free", None, [c_void_p]),
@@ -121,6 +164,22 @@ _SIGNATURES = [
     ("prism_groupby_read", c_int32, [c_void_p, POINTER(GroupbyOutput)]),
     ("prism_groupby_reset", c_int32, [c_void_p]),
     ("prism_groupby_destroy", None, [c_void_p]),
+    ("prism_hash_table_build", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_uint32), c_uint32,
+                                         POINTER(c_void_p)]),
+    ("prism_hash_table_probe", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_uint32), c_uint32,
+                                         c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("prism_hash_table_free", None, [c_void_p]),
+    ("prism_hash_join", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(HashJoinConfig),
+                                  c_void_p, c_void_p]),
+    ("prism_probe_matches_chunked", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p, c_void_p, c_void_p,
+                                              POINTER(c_uint32), POINTER(c_uint32), c_uint32, c_void_p, c_void_p,
+                                              c_void_p]),
+    ("prism_hash_join_probe_chunked", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p, c_void_p,
+                                                c_void_p, POINTER(HashJoinConfig), c_void_p, c_void_p,
+                                                POINTER(c_uint32)]),
+    ("prism_hash_aggregate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HashAggConfigC), c_void_p, c_void_p]),
+    ("prism_hash_aggregate_batches", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p,
+                                               POINTER(HashAggConfigC), c_void_p, c_void_p]),
     ("prism_datagen_uniform_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_int64,
                                             c_uint64]),
     ("prism_datagen_uniform_i32", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_int64,

@@ -1,7 +1,7 @@
-"""Device layer of the C ABI, over torch-allocated HBM buffers.
+"""Device layer of the C ABI over HBM buffers allocated through the library.
 
-torch provides device memory and streams only; every computation runs in
-libprism_hip.so.
+No other GPU runtime is loaded in the process: torch ships its own HIP/HSA
+runtime, and two runtimes in one process do not share a device.
 """
 from __future__ import annotations
 
@@ -14,13 +14,8 @@ import numpy as np
 from . import _lib as L
 
 
-def _torch():
-    import torch
-    return torch
-
-
 def ptr(t) -> Optional[int]:
-    return None if t is None else t.data_ptr()
+    return None if t is None else t.ptr
 
 
 class Context:
@@ -45,6 +40,14 @@ class Context:
     def device_bytes(self) -> int:
         return int(self.lib.prism_ctx_device_bytes(self.handle))
 
+    def timer_begin(self) -> None:
+        L.check(self.lib.prism_ctx_timer_begin(self.handle))
+
+    def timer_end(self) -> float:
+        ms = ctypes.c_double()
+        L.check(self.lib.prism_ctx_timer_end(self.handle, byref(ms)))
+        return ms.value
+
     def close(self) -> None:
         if self.handle:
             self.lib.prism_ctx_destroy(self.handle)
@@ -55,6 +58,68 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class DeviceArray:
+    """A 1-D typed buffer in HBM (allocated by prism_device_alloc)."""
+
+    def __init__(self, ctx: Context, n: int, dtype, _base: "DeviceArray" = None, _start: int = 0):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.size = int(n)
+        self._base = _base
+        if _base is None:
+            p = c_void_p()
+            L.check(ctx.lib.prism_device_alloc(ctx.handle, max(self.size * self.dtype.itemsize, 1), byref(p)))
+            self._alloc = p.value
+            self.ptr = p.value
+        else:
+            self._alloc = None
+            self.ptr = _base.ptr + _start * self.dtype.itemsize
+
+    @classmethod
+    def from_numpy(cls, ctx: Context, a) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.size, a.dtype)
+        if a.size:
+            L.check(ctx.lib.prism_copy_to_device(ctx.handle, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.size, self.dtype)
+        if self.size:
+            L.check(self.ctx.lib.prism_copy_to_host(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def numel(self) -> int:
+        return self.size
+
+    def __len__(self) -> int:
+        return self.size
+
+    def __getitem__(self, sl) -> "DeviceArray":
+        start, stop, step = sl.indices(self.size)
+        assert step == 1, "only contiguous views"
+        return DeviceArray(self.ctx, max(stop - start, 0), self.dtype, _base=self, _start=start)
+
+    def free(self) -> None:
+        if self._alloc:
+            self.ctx.lib.prism_device_free(self.ctx.handle, self._alloc)
+            self._alloc = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def empty(ctx: Context, n: int, dtype) -> DeviceArray:
+    return DeviceArray(ctx, n, dtype)
+
+
+def to_device(ctx: Context, a) -> DeviceArray:
+    return DeviceArray.from_numpy(ctx, a)
 
 
 # ---------------------------------------------------------------------------
@@ -114,27 +179,26 @@ class JoinTable:
               capacity: Optional[int] = None, probe_row_base: int = 0):
         """HashTable::probe (hash_join.rs:143-200).  Returns (probe_idx, build_idx, ProbeResult);
         the index tensors are None when want_pairs is False."""
-        torch = _torch()
-        dev = keys.device
-        res = torch.zeros(ctypes.sizeof(L.ProbeResult), dtype=torch.uint8, device=dev)
+        ctx = self.ctx
+        res = DeviceArray(ctx, ctypes.sizeof(L.ProbeResult), np.uint8)
         n = keys.numel()
         cap = (n if self.info.unique else max(n, 1024)) if capacity is None else capacity
         while True:
             op = ob = None
             if want_pairs:
-                op = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
-                ob = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+                op = DeviceArray(ctx, max(cap, 1), np.uint32)
+                ob = DeviceArray(ctx, max(cap, 1), np.uint32)
             self.probe_async(keys, res, probe_payload, build_payload, op, ob, cap if want_pairs else 0,
                              probe_row_base)
             self.ctx.sync()
-            r = L.ProbeResult.from_buffer_copy(res.cpu().numpy().tobytes())
+            r = L.ProbeResult.from_buffer_copy(res.numpy().tobytes())
             if r.error:
                 raise L.PrismError(4, "Execution error: probe look-back wait gave up")
             if want_pairs and r.matches > cap:
                 cap = int(r.matches)
                 continue
             if want_pairs:
-                return op[: r.matches], ob[: r.matches], r
+                return op[: int(r.matches)], ob[: int(r.matches)], r
             return None, None, r
 
     def close(self) -> None:

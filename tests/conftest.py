@@ -15,7 +15,5 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
-    from prism_executor.device import Context
-    c = Context(0)
-    yield c
-    c.close()
+    from prism_executor.context import context
+    return context()

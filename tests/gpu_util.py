@@ -5,8 +5,9 @@ import oracle as orc
 
 
 def to_dev(a):
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    from prism_executor.context import context
+    from prism_executor.device import DeviceArray
+    return DeviceArray.from_numpy(context(), np.ascontiguousarray(a))
 
 
 def exact_sum(a) -> int:

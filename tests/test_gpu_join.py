@@ -19,8 +19,8 @@ def check_pairs(ctx, bk, pk, kind, pp=None, bp=None, **kw):
     t = D.JoinTable(ctx, to_dev(bk), kind)
     pi, bi, r = t.probe(to_dev(pk), None if pp is None else to_dev(pp), None if bp is None else to_dev(bp), **kw)
     opi, obi = orc.join_probe(i64_batch(bk), [0], i64_batch(pk), [0])
-    gpi = pi.cpu().numpy().view(np.uint32)
-    gbi = bi.cpu().numpy().view(np.uint32)
+    gpi = pi.numpy().view(np.uint32)
+    gbi = bi.numpy().view(np.uint32)
     assert r.matches == len(opi)
     assert np.array_equal(gpi, opi.astype(np.uint32) + np.uint32(kw.get("probe_row_base", 0)))
     assert np.array_equal(gbi, obi)
@@ -84,58 +84,52 @@ def test_unaligned_probe_keys(ctx):
     t = D.JoinTable(ctx, bk)
     pi, bi, r = t.probe(pk)
     opi, obi = orc.join_probe(i64_batch(d["build_key"]), [0], i64_batch(d["probe_key"][1:]), [0])
-    assert np.array_equal(pi.cpu().numpy().view(np.uint32), opi)
-    assert np.array_equal(bi.cpu().numpy().view(np.uint32), obi)
+    assert np.array_equal(pi.numpy().view(np.uint32), opi)
+    assert np.array_equal(bi.numpy().view(np.uint32), obi)
 
 
 def test_device_datagen_matches_numpy_twin(ctx):
-    import torch
     say("device datagen")
     n = 1 << 20
-    x = torch.empty(n, dtype=torch.int64, device="cuda")
-    L.check(ctx.lib.prism_datagen_permutation_i64(ctx.handle, x.data_ptr(), n, dg.SEED, 10))
+    x = D.empty(ctx, n, np.int64)
+    L.check(ctx.lib.prism_datagen_permutation_i64(ctx.handle, x.ptr, n, dg.SEED, 10))
     ctx.sync()
-    assert np.array_equal(x.cpu().numpy(), dg.permutation_i64(n, dg.SEED, 10))
-    L.check(ctx.lib.prism_datagen_uniform_i64(ctx.handle, x.data_ptr(), n, dg.SEED, 12, 0, 2_000_000))
+    assert np.array_equal(x.numpy(), dg.permutation_i64(n, dg.SEED, 10))
+    L.check(ctx.lib.prism_datagen_uniform_i64(ctx.handle, x.ptr, n, dg.SEED, 12, 0, 2_000_000))
     ctx.sync()
-    assert np.array_equal(x.cpu().numpy(), dg.uniform_i64(n, dg.SEED, 12, 0, 2_000_000))
-    L.check(ctx.lib.prism_datagen_bits_i64(ctx.handle, x.data_ptr(), n, dg.SEED, 13))
+    assert np.array_equal(x.numpy(), dg.uniform_i64(n, dg.SEED, 12, 0, 2_000_000))
+    L.check(ctx.lib.prism_datagen_bits_i64(ctx.handle, x.ptr, n, dg.SEED, 13))
     ctx.sync()
-    assert np.array_equal(x.cpu().numpy(), dg.bits_i64(n, dg.SEED, 13))
+    assert np.array_equal(x.numpy(), dg.bits_i64(n, dg.SEED, 13))
     cdf = dg.zipf_cdf(1 << 16, 1.1)
-    dcdf = torch.from_numpy(cdf.view(np.int64)).cuda()
-    L.check(ctx.lib.prism_datagen_zipf_i64(ctx.handle, x.data_ptr(), n, dg.SEED, 30, dcdf.data_ptr(), len(cdf), 31))
+    dcdf = to_dev(cdf)
+    L.check(ctx.lib.prism_datagen_zipf_i64(ctx.handle, x.ptr, n, dg.SEED, 30, dcdf.ptr, len(cdf), 31))
     ctx.sync()
-    assert np.array_equal(x.cpu().numpy(), dg.zipf_i64(n, dg.SEED, 30, cdf, 31))
+    assert np.array_equal(x.numpy(), dg.zipf_i64(n, dg.SEED, 30, cdf, 31))
 
 
 def test_full_size_probe_properties(ctx):
     """1M-row build, 64M-row probe: size-independent checks (pair validity,
     order, closed-form count and exact payload sums)."""
-    import torch
     say("full size 1M x 64M")
     nb, npr = 1_000_000, 64_000_000
-    bk = torch.empty(nb, dtype=torch.int64, device="cuda")
-    bp = torch.empty(nb, dtype=torch.int32, device="cuda")
-    pk = torch.empty(npr, dtype=torch.int64, device="cuda")
-    pp = torch.empty(npr, dtype=torch.int64, device="cuda")
+    bk = D.empty(ctx, nb, np.int64)
+    bp = D.empty(ctx, nb, np.int32)
+    pk = D.empty(ctx, npr, np.int64)
+    pp = D.empty(ctx, npr, np.int64)
     lib = ctx.lib
-    L.check(lib.prism_datagen_permutation_i64(ctx.handle, bk.data_ptr(), nb, dg.SEED, 10))
-    L.check(lib.prism_datagen_uniform_i32(ctx.handle, bp.data_ptr(), nb, dg.SEED, 11, -(1 << 31), 1 << 32))
-    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.data_ptr(), npr, dg.SEED, 12, 0, 2 * nb))
-    L.check(lib.prism_datagen_bits_i64(ctx.handle, pp.data_ptr(), npr, dg.SEED, 13))
+    L.check(lib.prism_datagen_permutation_i64(ctx.handle, bk.ptr, nb, dg.SEED, 10))
+    L.check(lib.prism_datagen_uniform_i32(ctx.handle, bp.ptr, nb, dg.SEED, 11, -(1 << 31), 1 << 32))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.ptr, npr, dg.SEED, 12, 0, 2 * nb))
+    L.check(lib.prism_datagen_bits_i64(ctx.handle, pp.ptr, npr, dg.SEED, 13))
     ctx.sync()
     t = D.JoinTable(ctx, bk)
     pi, bi, r = t.probe(pk, pp, bp)
-    mask = pk < nb
+    hk, hpk, hpp, hbp = bk.numpy(), pk.numpy(), pp.numpy(), bp.numpy()
+    gpi, gbi = pi.numpy().astype(np.int64), bi.numpy().astype(np.int64)
+    mask = hpk < nb
     assert r.matches == int(mask.sum())
-    pil = pi.long()
-    bil = bi.long()
-    assert bool((pil[1:] > pil[:-1]).all())
-    assert torch.equal(bk[bil], pk[pil])
-    assert torch.equal(pil, torch.nonzero(mask).flatten())
-    sel = pp[mask]
-    exp = int((sel >> 32).sum()) * (1 << 32) + int((sel & 0xFFFFFFFF).sum())
-    assert r.probe_sum == exp
-    bsel = bp[bil].long()
-    assert r.build_sum == int(bsel.sum())
+    assert np.array_equal(gpi, np.nonzero(mask)[0])
+    assert np.array_equal(hk[gbi], hpk[gpi])
+    assert r.probe_sum == exact_sum(hpp[mask])
+    assert r.build_sum == int(hbp[gbi].astype(np.int64).sum())
