@@ -171,8 +171,8 @@ def main() -> int:
     def step(prev):
         if prev is not None:
             prev.close()
-        t = D.JoinTable(ctx, bk)
-        t.probe_async(pk, res, pp, bp, op, ob, capacity=npr)
+        t = D.JoinTable(ctx, bk, payload=bp)  # the build payload rides in the table entries
+        t.probe_async(pk, res, pp, None, op, ob, capacity=npr)
         return t
 
     t = None
@@ -187,20 +187,27 @@ def main() -> int:
     ms_per_step = wall / args.steps * 1e3
 
     r = L.ProbeResult.from_buffer_copy(res.numpy().tobytes())
-    expected = int((pk.numpy() < nb).sum())
-    if r.error or r.matches != expected:
+    hpk = pk.numpy()
+    sel = hpk < nb
+    expected = int(sel.sum())
+    hbk, hbp, hpp = bk.numpy(), bp.numpy(), pp.numpy()[sel]
+    inv = np.empty(nb, np.int64)
+    inv[hbk] = np.arange(nb)
+    exp_build = int(hbp[inv[hpk[sel]]].astype(np.int64).sum())
+    exp_probe = int((hpp >> 32).sum()) * (1 << 32) + int((hpp & 0xFFFFFFFF).sum())
+    if r.error or r.matches != expected or r.build_sum != exp_build or r.probe_sum != exp_probe:
         print(f"probe result check failed: matches {r.matches} expected {expected} error {r.error}", file=sys.stderr)
         return 1
 
     # Probe kernel duration: HIP events on the launch stream around K launches.
     ctx.timer_begin()
     for _ in range(args.steps):
-        t.probe_async(pk, res, pp, bp, op, ob, capacity=npr)
+        t.probe_async(pk, res, pp, None, op, ob, capacity=npr)
     probe_ms = ctx.timer_end() / args.steps
     # Build duration (host clock; the build synchronises internally).
     tb = time.perf_counter()
     for _ in range(5):
-        t2 = D.JoinTable(ctx, bk)
+        t2 = D.JoinTable(ctx, bk, payload=bp)
         t2.close()
     build_ms = (time.perf_counter() - tb) / 5 * 1e3
     info = t.info

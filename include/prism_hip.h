@@ -208,6 +208,12 @@ typedef struct {
  * partition pass does, hash_join.rs:116-118).  `kind` = PRISM_TABLE_*. */
 prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_t n_rows,
                                   int32_t kind, prism_join_table** out);
+/* The same build with one int32 build column carried in the table entries
+ * (the reference's HashTable keeps the build batch, hash_join.rs:135-140): a
+ * probe given no d_build_payload sums this column over its matches, reading it
+ * with the row id instead of gathering it. */
+prism_status prism_join_build_i64_payload(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload,
+                                          uint64_t n_rows, int32_t kind, prism_join_table** out);
 prism_status prism_join_table_info_get(const prism_join_table* t, prism_join_table_info* out);
 void prism_join_table_free(prism_join_table* t);
 

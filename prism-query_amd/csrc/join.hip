@@ -19,6 +19,8 @@
 // (probe row ascending, then build row ascending).
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace prism {
@@ -50,6 +52,10 @@ struct TableView {
   int64_t min_key;
   uint64_t range;
   const uint32_t* rows;     // CSR rows (duplicates)
+  const uint2* dense_up;    // DENSE unique with payload: {row, payload}
+  const int32_t* row_payload;  // duplicates with payload: payload of rows[i]
+  int has_payload;          // entries carry an int32 build payload
+  int sentinel_present;     // HASHED: a build key equals INT64_MIN
 };
 
 }  // namespace join
@@ -61,6 +67,7 @@ struct prism_join_table {
   prism::DeviceBuffer slots;  // HASHED
   prism::DeviceBuffer dense;  // DENSE
   prism::DeviceBuffer rows;   // CSR rows
+  prism::DeviceBuffer payload;  // DENSE unique {row, payload} entries / CSR row payloads
   prism::join::TableView view{};
 };
 
@@ -230,6 +237,30 @@ __global__ void k_dense_pack(uint2* dense, const uint32_t* __restrict__ start, c
     dense[i] = make_uint2(start[i], cnt[i]);
 }
 
+__global__ void k_dense_pack_payload(uint2* out, const uint32_t* __restrict__ rows, const int32_t* __restrict__ pay,
+                                     uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rows[i];
+    out[i] = make_uint2(r, r != kNone ? static_cast<uint32_t>(pay[r]) : 0u);
+  }
+}
+
+__global__ void k_slot_payload(Slot* slots, uint64_t n, const int32_t* __restrict__ pay) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (slots[i].cnt) slots[i].cnt = static_cast<uint32_t>(pay[slots[i].val]);
+}
+
+__global__ void k_gather_i32(int32_t* out, const uint32_t* __restrict__ idx, const int32_t* __restrict__ src,
+                             uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = src[idx[i]];
+}
+
+__global__ void k_widen_u32(uint64_t* out, const uint32_t* __restrict__ in, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
 __global__ void k_iota_u32(uint32_t* p, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = static_cast<uint32_t>(i);
@@ -296,11 +327,18 @@ constexpr uint64_t kStPrefix = 2ULL << 62;
 constexpr uint64_t kStMask = (1ULL << 62) - 1;
 
 template <int KIND, bool UNIQUE>
-__device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& val, uint32_t& cnt) {
+__device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& val, uint32_t& cnt, int32_t& pay) {
   if constexpr (KIND == PRISM_TABLE_DENSE) {
     uint64_t d = static_cast<uint64_t>(k) - static_cast<uint64_t>(T.min_key);
     if (d >= T.range) return false;
     if constexpr (UNIQUE) {
+      if (T.has_payload) {
+        const uint2 e = T.dense_up[d];
+        val = e.x;
+        pay = static_cast<int32_t>(e.y);
+        cnt = 1;
+        return e.x != kNone;
+      }
       uint32_t v = T.dense_u[d];
       val = v;
       cnt = 1;
@@ -314,23 +352,32 @@ __device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& 
   } else {
     uint64_t s;
     if (k == kEmpty) {
+      if (!T.sentinel_present) return false;
       const Slot e = T.slots[T.mask + 1];
       val = e.val;
-      cnt = e.cnt;
-      return e.cnt != 0;
+      cnt = (UNIQUE && T.has_payload) ? 1u : e.cnt;
+      pay = static_cast<int32_t>(e.cnt);
+      return true;
     }
     s = hash_i64_fast(k) >> T.shift;
     while (true) {  // load factor <= 1/2: an empty slot always ends the run
       const Slot e = T.slots[s];
       if (e.key == k) {
         val = e.val;
-        cnt = e.cnt;
+        cnt = (UNIQUE && T.has_payload) ? 1u : e.cnt;
+        pay = static_cast<int32_t>(e.cnt);
         return true;
       }
       if (e.key == kEmpty) return false;
       s = (s + 1) & T.mask;
     }
   }
+}
+
+template <int KIND, bool UNIQUE>
+__device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& val, uint32_t& cnt) {
+  int32_t pay;
+  return lookup<KIND, UNIQUE>(T, k, val, cnt, pay);
 }
 
 // Matches of one tile counted by a whole wave (used when a predecessor tile
@@ -351,59 +398,87 @@ __device__ uint64_t tile_count(const TableView& T, const int64_t* __restrict__ k
   return c;
 }
 
-// Wave-parallel decoupled look-back (executed by one whole wave).  Returns the
-// exclusive prefix of `tile`; publishes the tile's inclusive prefix.  A
-// predecessor that stays unpublished for kStallPolls polls is counted here
-// instead of waited for (and its count published), so the scan finishes
-// whatever order the workgroups are scheduled in.
+// Block-wide decoupled look-back (every thread of the workgroup takes part:
+// one status word per thread, so one memory round trip covers kPB
+// predecessors).  Returns the exclusive prefix of `tile` and publishes its
+// inclusive prefix.  A predecessor that stays unpublished for kStallPolls
+// polls is counted here instead of waited for (and its count published), so
+// the scan finishes whatever order the workgroups are scheduled in.
 constexpr uint32_t kStallPolls = 256;
+
+struct LookbackLds {
+  uint64_t zero[kWaves];
+  uint64_t sum[kWaves];
+  int first[kWaves];
+};
 
 template <int KIND, bool UNIQUE>
 __device__ uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t total, const TableView& T,
-                             const int64_t* __restrict__ keys, uint64_t n) {
-  const uint32_t lane = lane_id();
+                             const int64_t* __restrict__ keys, uint64_t n, LookbackLds& L) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   if (tile == 0) {
-    if (lane == 0) __hip_atomic_store(&status[0], kStPrefix | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == 0) __hip_atomic_store(&status[0], kStPrefix | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
-  if (lane == 0) __hip_atomic_store(&status[tile], kStAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t == 0) __hip_atomic_store(&status[tile], kStAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint64_t excl = 0;
   int64_t base = static_cast<int64_t>(tile) - 1;
   uint32_t polls = 0;
   while (true) {
-    int64_t idx = base - static_cast<int64_t>(lane);
-    uint64_t v = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                          : kStPrefix;
-    uint64_t st = v >> 62;
-    uint64_t pm = ballot64(st == 2);
-    uint64_t zm = ballot64(st == 0);
-    int first_p = pm ? __builtin_ctzll(pm) : 64;
-    uint64_t before = first_p == 64 ? ~0ULL : ((1ULL << first_p) - 1);
-    if (zm & before) {
+    const int64_t idx = base - t;
+    const uint64_t v = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : kStPrefix;
+    const uint64_t st = v >> 62;
+    const uint64_t pm = ballot64(st == 2), zm = ballot64(st == 0);
+    if (lane == 0) {
+      L.first[wave] = pm ? __builtin_ctzll(pm) : 64;
+      L.zero[wave] = zm;
+    }
+    __syncthreads();
+    int fp = kPB;  // thread holding the nearest inclusive prefix
+    for (int w = 0; w < kWaves; ++w)
+      if (L.first[w] < 64) {
+        fp = w * 64 + L.first[w];
+        break;
+      }
+    int zfirst = -1;  // nearest unpublished predecessor before it
+    for (int w = 0; w < kWaves && w * 64 < fp && zfirst < 0; ++w) {
+      const int lim = fp - w * 64 < 64 ? fp - w * 64 : 64;
+      const uint64_t mask = lim == 64 ? ~0ULL : ((1ULL << lim) - 1);
+      const uint64_t z = L.zero[w] & mask;
+      if (z) zfirst = w * 64 + __builtin_ctzll(z);
+    }
+    if (zfirst >= 0) {
+      __syncthreads();
       if (++polls < kStallPolls) {
         __builtin_amdgcn_s_sleep(2);
         continue;
       }
-      // count the nearest unpublished predecessor ourselves
-      const int j = __builtin_ctzll(zm & before);
-      const int64_t q = base - j;
-      const uint64_t c = tile_count<KIND, UNIQUE>(T, keys, n, static_cast<uint64_t>(q));
-      if (lane == 0) {
-        unsigned long long expect = 0;
-        __hip_atomic_compare_exchange_strong(reinterpret_cast<unsigned long long*>(&status[q]), &expect,
-                                             static_cast<unsigned long long>(kStAgg | c), __ATOMIC_RELAXED,
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (wave == 0) {
+        const int64_t q = base - zfirst;
+        const uint64_t c = tile_count<KIND, UNIQUE>(T, keys, n, static_cast<uint64_t>(q));
+        if (lane == 0) {
+          unsigned long long expect = 0;
+          __hip_atomic_compare_exchange_strong(reinterpret_cast<unsigned long long*>(&status[q]), &expect,
+                                               static_cast<unsigned long long>(kStAgg | c), __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
       polls = 0;
       continue;
     }
-    uint64_t mine = (static_cast<int>(lane) <= first_p) ? (v & kStMask) : 0;
+    uint64_t mine = (t <= fp) ? (v & kStMask) : 0;
     for (int d = 32; d > 0; d >>= 1) mine += __shfl_xor(mine, d, 64);
-    excl += mine;
-    if (first_p < 64) break;
-    base -= 64;
+    if (lane == 0) L.sum[wave] = mine;
+    __syncthreads();
+    uint64_t sum = 0;
+    for (int w = 0; w < kWaves; ++w) sum += L.sum[w];
+    excl += sum;
+    __syncthreads();
+    if (fp < kPB) break;
+    base -= kPB;
   }
-  if (lane == 0)
+  if (t == 0)
     __hip_atomic_store(&status[tile], kStPrefix | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return excl;
 }
@@ -431,15 +506,168 @@ __global__ void k_probe_clear(uint64_t* status, uint64_t n_tiles, prism_probe_re
   }
 }
 
+// Unique-key probe, phase 1: each tile looks its keys up, ranks its matches in
+// LDS (row order), writes them contiguously at tile * kTileRows of a staging
+// buffer and records its match count and payload partials.  No tile waits on
+// another.  Phase 2 scans the counts; phase 3 moves each tile's pairs to their
+// final offset with coalesced copies.
+template <int KIND, bool VEC>
+__global__ __launch_bounds__(kPB) void k_probe_unique(TableView T, const int64_t* __restrict__ keys, uint64_t n,
+                                                      uint32_t probe_base, const int64_t* __restrict__ ppay,
+                                                      const int32_t* __restrict__ bpay, uint2* __restrict__ staged,
+                                                      uint32_t* __restrict__ tile_count,
+                                                      __int128* __restrict__ tile_sums) {
+  __shared__ uint2 s_pair[kTileRows];
+  __shared__ uint32_t s_wave_tot[kPairs][kWaves];
+  __shared__ __int128 s_red[2][kWaves];
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t base = static_cast<uint64_t>(tile) * kTileRows;
+  const bool table_pay = T.has_payload && bpay == nullptr;
+  const bool sums = ppay || bpay || table_pay;
+
+  uint32_t val[kPairs][2];
+  bool hit[kPairs][2];
+  __int128 psum = 0, bsum = 0;
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    const uint64_t r0 = base + static_cast<uint64_t>(j) * kChunk + 2 * t;
+    int64_t k0, k1, p0 = 0, p1 = 0;
+    bool ok0, ok1;
+    if (VEC && r0 + 1 < n) {
+      const v2i64 v = __builtin_nontemporal_load(reinterpret_cast<const v2i64*>(keys + r0));
+      k0 = v.x;
+      k1 = v.y;
+      ok0 = ok1 = true;
+      if (ppay) {
+        const v2i64 p = __builtin_nontemporal_load(reinterpret_cast<const v2i64*>(ppay + r0));
+        p0 = p.x;
+        p1 = p.y;
+      }
+    } else {
+      ok0 = r0 < n;
+      ok1 = r0 + 1 < n;
+      k0 = ok0 ? keys[r0] : 0;
+      k1 = ok1 ? keys[r0 + 1] : 0;
+      if (ppay) {
+        p0 = ok0 ? ppay[r0] : 0;
+        p1 = ok1 ? ppay[r0 + 1] : 0;
+      }
+    }
+    uint32_t v0 = 0, v1 = 0, c;
+    int32_t y0 = 0, y1 = 0;
+    hit[j][0] = ok0 && lookup<KIND, true>(T, k0, v0, c, y0);
+    hit[j][1] = ok1 && lookup<KIND, true>(T, k1, v1, c, y1);
+    val[j][0] = v0;
+    val[j][1] = v1;
+    if (hit[j][0]) {
+      psum += p0;
+      bsum += bpay ? bpay[v0] : (table_pay ? y0 : 0);
+    }
+    if (hit[j][1]) {
+      psum += p1;
+      bsum += bpay ? bpay[v1] : (table_pay ? y1 : 0);
+    }
+  }
+  if (sums) {
+    wave_reduce_i128(psum);
+    wave_reduce_i128(bsum);
+    if (lane == 0) {
+      s_red[0][wave] = psum;
+      s_red[1][wave] = bsum;
+    }
+  }
+  // Rank matches inside the tile: row order is (chunk, thread, element).
+  uint32_t local[kPairs];
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    const uint64_t b0 = ballot64(hit[j][0]), b1 = ballot64(hit[j][1]);
+    local[j] = lanes_below(b0) + lanes_below(b1);
+    if (lane == 0) s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
+  }
+  __syncthreads();
+  uint32_t tile_total = 0;
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    uint32_t before = tile_total;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t v = s_wave_tot[j][w];
+      before += (w < wave) ? v : 0;
+      tile_total += v;
+    }
+    local[j] += before;
+  }
+  if (t == 0) {
+    tile_count[tile] = tile_total;
+    if (sums) {
+      __int128 p = 0, b = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        p += s_red[0][w];
+        b += s_red[1][w];
+      }
+      tile_sums[2 * tile] = p;
+      tile_sums[2 * tile + 1] = b;
+    }
+  }
+  if (!staged) return;
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    const uint32_t row0 = static_cast<uint32_t>(base + static_cast<uint64_t>(j) * kChunk + 2 * t) + probe_base;
+    if (hit[j][0]) s_pair[local[j]] = make_uint2(row0, val[j][0]);
+    if (hit[j][1]) s_pair[local[j] + (hit[j][0] ? 1 : 0)] = make_uint2(row0 + 1, val[j][1]);
+  }
+  __syncthreads();
+  uint2* dst = staged + base;
+  for (uint32_t i = t; i < tile_total; i += kPB) dst[i] = s_pair[i];
+}
+
+// Phase 3: move each tile's staged pairs to their final offset (one workgroup
+// per tile; consecutive threads copy consecutive pairs).
+__global__ __launch_bounds__(kPB) void k_probe_move(const uint2* __restrict__ staged,
+                                                    const uint32_t* __restrict__ tile_count,
+                                                    const uint64_t* __restrict__ tile_off, uint32_t* __restrict__ out_p,
+                                                    uint32_t* __restrict__ out_b, uint64_t capacity) {
+  const uint32_t tile = blockIdx.x;
+  const uint32_t cnt = tile_count[tile];
+  const uint64_t off = tile_off[tile];
+  const uint2* src = staged + static_cast<uint64_t>(tile) * kTileRows;
+  for (uint32_t i = threadIdx.x; i < cnt; i += kPB) {
+    const uint64_t pos = off + i;
+    if (pos < capacity) {
+      const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(src) + i);
+      __builtin_nontemporal_store(static_cast<uint32_t>(v), out_p + pos);
+      __builtin_nontemporal_store(static_cast<uint32_t>(v >> 32), out_b + pos);
+    }
+  }
+}
+
+// Sum of tile match counts when no pairs are written (count / sum-only probes).
+__global__ void k_probe_count_sum(const uint32_t* __restrict__ tile_count, uint64_t n_tiles, prism_probe_result* res) {
+  __shared__ uint64_t s[16];
+  uint64_t c = 0;
+  for (uint64_t i = threadIdx.x; i < n_tiles; i += blockDim.x) c += tile_count[i];
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t tot = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) tot += s[i];
+    res->matches = tot;
+  }
+}
+
 template <int KIND, bool UNIQUE, bool VEC>
-__global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __restrict__ keys, uint64_t n,
+__global__ __launch_bounds__(kPB, 2) void k_probe(TableView T, const int64_t* __restrict__ keys, uint64_t n,
                                                uint32_t n_tiles, uint32_t probe_base,
                                                const int64_t* __restrict__ ppay, const int32_t* __restrict__ bpay,
                                                uint32_t* __restrict__ out_p, uint32_t* __restrict__ out_b,
                                                uint64_t capacity, uint64_t* status, __int128* tile_sums,
                                                prism_probe_result* res) {
   __shared__ uint64_t s_wave_tot[kPairs][kWaves];
-  __shared__ uint64_t s_excl;
+  __shared__ LookbackLds s_lb;
   __shared__ __int128 s_red[2][kWaves];
 
   const int t = threadIdx.x;
@@ -477,15 +705,19 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
   }
 
   uint32_t val[kPairs][2], cnt[kPairs][2];
+  int32_t tpay[kPairs][2];
 #pragma unroll
   for (int j = 0; j < kPairs; ++j)
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       uint32_t v = 0, c = 0;
-      bool m = ok[j][e] && lookup<KIND, UNIQUE>(T, k[j][e], v, c);
+      int32_t py = 0;
+      bool m = ok[j][e] && lookup<KIND, UNIQUE>(T, k[j][e], v, c, py);
       val[j][e] = v;
       cnt[j][e] = m ? c : 0;
+      tpay[j][e] = py;
     }
+  const bool table_pay = T.has_payload && bpay == nullptr;
 
   // Payload sums over the matches (gathers issued before the scan so their
   // latency overlaps it).
@@ -498,17 +730,20 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
         if (cnt[j][e]) {
           psum += pv[j][e];
           if (bpay) bsum += bpay[val[j][e]];
+          else if (table_pay) bsum += tpay[j][e];
         }
       }
   }
 
   // Rank matches inside the tile: row order is (chunk, thread, element).
-  uint64_t in_wave[kPairs];
+  // UNIQUE tiles hold at most kTileRows matches, so positions fit in 32 bits.
+  using Pos = typename std::conditional<UNIQUE, uint32_t, uint64_t>::type;
+  Pos local[kPairs];
 #pragma unroll
   for (int j = 0; j < kPairs; ++j) {
     if constexpr (UNIQUE) {
       uint64_t b0 = ballot64(cnt[j][0] != 0), b1 = ballot64(cnt[j][1] != 0);
-      in_wave[j] = lanes_below(b0) + lanes_below(b1);
+      local[j] = lanes_below(b0) + lanes_below(b1);
       if (lane == 0) s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
     } else {
       uint64_t c = static_cast<uint64_t>(cnt[j][0]) + cnt[j][1];
@@ -517,31 +752,24 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
         uint64_t y = __shfl_up(x, d, 64);
         if (lane >= d) x += y;
       }
-      in_wave[j] = x - c;
+      local[j] = x - c;
       if (lane == 63) s_wave_tot[j][wave] = x;
     }
   }
   __syncthreads();
-  uint64_t chunk_base[kPairs], wave_base[kPairs];
   uint64_t tile_total = 0;
 #pragma unroll
   for (int j = 0; j < kPairs; ++j) {
-    chunk_base[j] = tile_total;
-    uint64_t wb = 0;
+    Pos before = static_cast<Pos>(tile_total);
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
       uint64_t v = s_wave_tot[j][w];
-      wb += (w < wave) ? v : 0;
+      before += (w < wave) ? static_cast<Pos>(v) : 0;
       tile_total += v;
     }
-    wave_base[j] = wb;
+    local[j] += before;
   }
-  if (wave == 0) {
-    uint64_t ex = lookback<KIND, UNIQUE>(status, tile, tile_total, T, keys, n);
-    if (lane == 0) s_excl = ex;
-  }
-  __syncthreads();
-  const uint64_t gbase = s_excl;
+  const uint64_t gbase = lookback<KIND, UNIQUE>(status, tile, tile_total, T, keys, n, s_lb);
 
   const bool want_pairs = out_p != nullptr;
 #pragma unroll
@@ -550,7 +778,7 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
     for (int e = 0; e < 2; ++e) {
       if (cnt[j][e] == 0) continue;
       const uint64_t row = base + static_cast<uint64_t>(j) * kChunk + 2 * t + e;
-      uint64_t pos = gbase + chunk_base[j] + wave_base[j] + in_wave[j] + (e ? cnt[j][0] : 0);
+      uint64_t pos = gbase + local[j] + (e ? cnt[j][0] : 0);
       if constexpr (UNIQUE) {
         if (want_pairs && pos < capacity) {
           __builtin_nontemporal_store(static_cast<uint32_t>(row) + probe_base, out_p + pos);
@@ -566,12 +794,13 @@ __global__ __launch_bounds__(kPB) void k_probe(TableView T, const int64_t* __res
             out_b[pos + r] = brow;
           }
           if (bpay) bsum += bpay[brow];
+          else if (table_pay) bsum += T.row_payload[start + r];
         }
         psum += static_cast<__int128>(pv[j][e]) * static_cast<__int128>(c);
       }
     }
 
-  if (ppay || bpay) {  // per-tile partials, summed by k_probe_sums (no contended atomics)
+  if (ppay || bpay || table_pay) {  // per-tile partials, summed by k_probe_sums (no contended atomics)
     wave_reduce_i128(psum);
     wave_reduce_i128(bsum);
     if (lane == 0) {
@@ -622,6 +851,38 @@ __global__ void k_probe_sums(const __int128* __restrict__ tile_sums, uint64_t n_
   }
 }
 
+// Phase 2: exclusive scan of the tile match counts (one workgroup of 1024
+// threads; each thread owns a contiguous run of tiles), plus the total.
+__global__ __launch_bounds__(1024) void k_probe_scan(const uint32_t* __restrict__ tile_count,
+                                                     uint64_t* __restrict__ tile_off, uint64_t n_tiles,
+                                                     prism_probe_result* res) {
+  __shared__ uint64_t s_wave[16];
+  const uint64_t per = (n_tiles + blockDim.x - 1) / blockDim.x;
+  const uint64_t lo = threadIdx.x * per, hi = lo + per < n_tiles ? lo + per : n_tiles;
+  uint64_t sum = 0;
+  for (uint64_t i = lo; i < hi; ++i) sum += tile_count[i];
+  // inclusive scan of per-thread sums inside the wave, then across waves
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t inc = sum;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  uint64_t before = 0, total = 0;
+  for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+    before += (w < wave) ? s_wave[w] : 0;
+    total += s_wave[w];
+  }
+  uint64_t run = before + inc - sum;
+  for (uint64_t i = lo; i < hi; ++i) {
+    tile_off[i] = run;
+    run += tile_count[i];
+  }
+  if (threadIdx.x == 0) res->matches = total;
+}
+
 template <int KIND, bool UNIQUE>
 static void launch_probe_vec(prism_ctx* ctx, bool vec, dim3 g, const TableView& T, const int64_t* keys,
                              uint64_t n, uint32_t n_tiles, uint32_t probe_base, const int64_t* ppay,
@@ -643,8 +904,8 @@ using namespace prism::join;
 
 extern "C" {
 
-prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_t n_rows, int32_t kind,
-                                  prism_join_table** out) {
+static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload, uint64_t n_rows,
+                               int32_t kind, prism_join_table** out) {
   return guarded([&] {
     if (!ctx || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or output");
     *out = nullptr;
@@ -762,11 +1023,45 @@ prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_
       T->view.slots = slots;
       T->view.mask = S - 1;
       T->view.shift = shift;
+      uint32_t sentinel_cnt = 0;
+      PRISM_HIP(hipMemcpyAsync(&sentinel_cnt, cnt.as<uint32_t>() + S, 4, hipMemcpyDeviceToHost, st));
+      PRISM_HIP(hipStreamSynchronize(st));
+      T->view.sentinel_present = sentinel_cnt != 0;
+    }
+    if (d_payload && n_rows) {  // carry the build payload in the table entries
+      T->view.has_payload = 1;
+      if (!T->info.unique) {
+        T->payload.alloc(c, n_rows * 4);
+        hipLaunchKernelGGL(k_gather_i32, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, T->payload.as<int32_t>(),
+                           T->rows.as<uint32_t>(), d_payload, n_rows);
+        T->view.row_payload = T->payload.as<int32_t>();
+      } else if (T->info.kind == PRISM_TABLE_DENSE) {
+        T->payload.alloc(c, T->info.slots * 8);
+        hipLaunchKernelGGL(k_dense_pack_payload, grid_for(ctx, T->info.slots), dim3(kBuildBlock), 0, st,
+                           T->payload.as<uint2>(), T->dense.as<uint32_t>(), d_payload, T->info.slots);
+        T->view.dense_up = T->payload.as<uint2>();
+        T->dense.release();  // the packed entries replace the row array
+        T->view.dense_u = nullptr;
+      } else {
+        hipLaunchKernelGGL(k_slot_payload, grid_for(ctx, T->info.slots + 1), dim3(kBuildBlock), 0, st,
+                           T->slots.as<Slot>(), T->info.slots + 1, d_payload);
+      }
+      PRISM_CHECK_LAUNCH();
     }
     PRISM_HIP(hipStreamSynchronize(st));
-    T->info.device_bytes = T->slots.bytes + T->dense.bytes + T->rows.bytes;
+    T->info.device_bytes = T->slots.bytes + T->dense.bytes + T->rows.bytes + T->payload.bytes;
     *out = guard.release();
   });
+}
+
+prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_t n_rows, int32_t kind,
+                                  prism_join_table** out) {
+  return build_impl(ctx, d_keys, nullptr, n_rows, kind, out);
+}
+
+prism_status prism_join_build_i64_payload(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload,
+                                          uint64_t n_rows, int32_t kind, prism_join_table** out) {
+  return build_impl(ctx, d_keys, d_payload, n_rows, kind, out);
 }
 
 prism_status prism_join_table_info_get(const prism_join_table* t, prism_join_table_info* out) {
@@ -797,31 +1092,60 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
     Ctx* c = as_ctx(ctx);
     std::lock_guard<std::mutex> lock(ctx->mu);
     const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
-    // scratch: tile status words, then per-tile int128 payload partials
-    const uint64_t status_bytes = ((n_tiles + 1) * 8 + 255) & ~255ULL;
-    c->scratch.reserve(c, status_bytes + n_tiles * 32 + 32);
-    uint64_t* status = c->scratch.as<uint64_t>();
-    __int128* tile_sums = reinterpret_cast<__int128*>(c->scratch.as<char>() + status_bytes);
-    hipLaunchKernelGGL(k_probe_clear, grid_for(ctx, n_tiles + 1), dim3(kBuildBlock), 0, ctx->stream, status,
-                       n_tiles, d_result);
-    PRISM_CHECK_LAUNCH();
-    if (n_probe == 0) return;
     const bool vec = (reinterpret_cast<uintptr_t>(d_probe_keys) & 15) == 0;
     const TableView& T = t->view;
-    dim3 g(static_cast<unsigned>(n_tiles));
+    dim3 g(static_cast<unsigned>(n_tiles ? n_tiles : 1));
     const uint32_t nt = static_cast<uint32_t>(n_tiles);
-    if (t->info.kind == PRISM_TABLE_DENSE) {
-      if (t->info.unique)
-        launch_probe_vec<PRISM_TABLE_DENSE, true>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
-                                                  d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                  capacity, status, tile_sums, d_result);
-      else
-        launch_probe_vec<PRISM_TABLE_DENSE, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
-                                                   d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                   capacity, status, tile_sums, d_result);
+    const bool pairs = d_probe_idx != nullptr;
+    auto align = [](uint64_t b) { return (b + 255) & ~255ULL; };
+    // scratch: per-tile int128 payload partials, then either the unique
+    // path's counts / offsets / staged pairs or the duplicate path's
+    // look-back status words
+    const uint64_t sums_bytes = align(n_tiles * 32 + 32);
+    __int128* tile_sums = c->scratch.as<__int128>();
+    if (t->info.unique) {
+      const uint64_t cnt_bytes = align(n_tiles * 4 + 4), off_bytes = align(n_tiles * 8 + 8);
+      const uint64_t stage_bytes = pairs ? n_tiles * kTileRows * sizeof(uint2) : 0;
+      c->scratch.reserve(c, sums_bytes + cnt_bytes + off_bytes + stage_bytes);
+      tile_sums = c->scratch.as<__int128>();
+      char* p = c->scratch.as<char>() + sums_bytes;
+      uint32_t* tile_count = reinterpret_cast<uint32_t*>(p);
+      uint64_t* tile_off = reinterpret_cast<uint64_t*>(p + cnt_bytes);
+      uint2* staged = pairs ? reinterpret_cast<uint2*>(p + cnt_bytes + off_bytes) : nullptr;
+      hipLaunchKernelGGL(k_probe_clear, dim3(1), dim3(64), 0, ctx->stream, nullptr, 0, d_result);
+      PRISM_CHECK_LAUNCH();
+      if (n_probe == 0) return;
+#define PRISM_PROBE1(KIND_, VEC_)                                                                          \
+  hipLaunchKernelGGL((k_probe_unique<KIND_, VEC_>), g, dim3(kPB), 0, ctx->stream, T, d_probe_keys, n_probe, \
+                     probe_row_base, d_probe_payload, d_build_payload, staged, tile_count, tile_sums)
+      if (t->info.kind == PRISM_TABLE_DENSE) {
+        if (vec) PRISM_PROBE1(PRISM_TABLE_DENSE, true);
+        else PRISM_PROBE1(PRISM_TABLE_DENSE, false);
+      } else {
+        if (vec) PRISM_PROBE1(PRISM_TABLE_HASHED, true);
+        else PRISM_PROBE1(PRISM_TABLE_HASHED, false);
+      }
+#undef PRISM_PROBE1
+      PRISM_CHECK_LAUNCH();
+      if (pairs) {
+        hipLaunchKernelGGL(k_probe_scan, dim3(1), dim3(1024), 0, ctx->stream, tile_count, tile_off, n_tiles, d_result);
+        PRISM_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_probe_move, g, dim3(kPB), 0, ctx->stream, staged, tile_count, tile_off, d_probe_idx,
+                           d_build_idx, capacity);
+      } else {
+        hipLaunchKernelGGL(k_probe_count_sum, dim3(1), dim3(1024), 0, ctx->stream, tile_count, n_tiles, d_result);
+      }
     } else {
-      if (t->info.unique)
-        launch_probe_vec<PRISM_TABLE_HASHED, true>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
+      const uint64_t status_bytes = align((n_tiles + 1) * 8);
+      c->scratch.reserve(c, sums_bytes + status_bytes);
+      tile_sums = c->scratch.as<__int128>();
+      uint64_t* status = reinterpret_cast<uint64_t*>(c->scratch.as<char>() + sums_bytes);
+      hipLaunchKernelGGL(k_probe_clear, grid_for(ctx, n_tiles + 1), dim3(kBuildBlock), 0, ctx->stream, status,
+                         n_tiles, d_result);
+      PRISM_CHECK_LAUNCH();
+      if (n_probe == 0) return;
+      if (t->info.kind == PRISM_TABLE_DENSE)
+        launch_probe_vec<PRISM_TABLE_DENSE, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
                                                    d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
                                                    capacity, status, tile_sums, d_result);
       else
@@ -830,7 +1154,7 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
                                                     capacity, status, tile_sums, d_result);
     }
     PRISM_CHECK_LAUNCH();
-    if (d_probe_payload || d_build_payload) {
+    if (d_probe_payload || d_build_payload || t->view.has_payload) {
       hipLaunchKernelGGL(k_probe_sums, dim3(1), dim3(1024), 0, ctx->stream, tile_sums, n_tiles, d_result);
       PRISM_CHECK_LAUNCH();
     }

@@ -154,6 +154,8 @@ _SIGNATURES = [
     ("prism_ctx_timer_begin", c_int32, [c_void_p]),
     ("prism_ctx_timer_end", c_int32, [c_void_p, POINTER(c_double)]),
     ("prism_join_build_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_int32, POINTER(c_void_p)]),
+    ("prism_join_build_i64_payload", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_int32,
+                                               POINTER(c_void_p)]),
     ("prism_join_table_info_get", c_int32, [c_void_p, POINTER(JoinTableInfo)]),
     ("prism_join_table_free", None, [c_void_p]),
     ("prism_join_probe_i64", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_void_p,

@@ -161,11 +161,15 @@ def col_utf8(data, offsets, validity=None):
 class JoinTable:
     """HashTable::build on a device-resident int64 key column (hash_join.rs:87-141)."""
 
-    def __init__(self, ctx: Context, keys, kind: int = L.TABLE_AUTO):
+    def __init__(self, ctx: Context, keys, kind: int = L.TABLE_AUTO, payload=None):
         self.ctx = ctx
         self.lib = ctx.lib
         self.handle = c_void_p()
-        L.check(self.lib.prism_join_build_i64(ctx.handle, ptr(keys), keys.numel(), kind, byref(self.handle)))
+        if payload is None:
+            L.check(self.lib.prism_join_build_i64(ctx.handle, ptr(keys), keys.numel(), kind, byref(self.handle)))
+        else:
+            L.check(self.lib.prism_join_build_i64_payload(ctx.handle, ptr(keys), ptr(payload), keys.numel(), kind,
+                                                          byref(self.handle)))
         self.info = L.JoinTableInfo()
         L.check(self.lib.prism_join_table_info_get(self.handle, byref(self.info)))
 

@@ -53,6 +53,26 @@ def test_duplicate_keys_zipf(ctx, kind):
     assert t.info.unique == 0 and t.info.max_dup > 100
 
 
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("dups", [False, True])
+def test_payload_in_table_matches_gather(ctx, kind, dups):
+    say("payload in table", kind, dups)
+    if dups:
+        cdf = dg.zipf_cdf(3_000, 1.1)
+        bk = dg.zipf_i64(10_000, 12, 40, cdf, 41)
+        pk = dg.zipf_i64(40_000, 12, 43, cdf, 41)
+    else:
+        d = dg.join_inputs(30_000, 200_000)
+        bk, pk = d["build_key"], d["probe_key"]
+    bp = dg.uniform_i32(len(bk), 12, 45, -(1 << 31), 1 << 32)
+    t = D.JoinTable(ctx, to_dev(bk), kind, payload=to_dev(bp))
+    pi, bi, r = t.probe(to_dev(pk))
+    opi, obi = orc.join_probe(i64_batch(bk), [0], i64_batch(pk), [0])
+    assert np.array_equal(pi.numpy(), opi) and np.array_equal(bi.numpy(), obi)
+    assert r.build_sum == exact_sum(bp.astype(np.int64)[obi])
+    assert r.probe_sum == 0
+
+
 def test_sparse_extreme_keys_hashed(ctx):
     say("extreme keys")
     rng = np.random.default_rng(3)
