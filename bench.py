@@ -8,10 +8,11 @@ seeded permutation of 0..1M-1, int32 payload), then probe it with 64M rows
 int128 sums of both payloads over the matches.  Inputs are generated on the
 device and stay resident in HBM; the timed region holds build + probe.
 
-Prints one JSON line.  The roofline block covers the probe kernel (the
-dominant one): algorithmic bytes = 16 B per probe row (key + payload read) +
-8 B per matched pair (two u32 indices written), divided by the probe's
-average duration measured with HIP events on the launch stream.
+Prints one JSON line.  The roofline block covers the dominant kernel, the
+probe's lookup/compaction pass ("join_probe"): algorithmic bytes per launch =
+16 B per probe row (key + payload read) + 8 B per matched pair (two u32
+indices written), divided by that kernel's average duration, measured with
+HIP events that the library records on the launch stream around each launch.
 The CPU baseline times the oracle's restatement of the reference join on a
 bounded sample (1M build rows, 8M probe rows) on one host core.
 """
@@ -25,6 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "prism-query_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each dominant kernel on the default workloads, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by
+# the counter's reading for a plain device copy of known size (profiles/).
+TRAFFIC = {}
 METRIC = "rows/s & %HBM peak (join probe, group-by); p99 latency batched vs plain"
 
 
@@ -90,17 +95,20 @@ def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
         k = g.finish()
     wall = (time.perf_counter() - t0) / steps
     out = g.read(k)
-    ctx.timer_begin()
+    ctx.kernel_timing(True)
     for _ in range(steps):
         L.check(lib.prism_groupby_update(g.handle, (L.Column * len(cols))(*cols), n, 0))
-    upd_ms = ctx.timer_end() / steps
+    tot, cnt = ctx.kernel_time("groupby_update")
+    ctx.kernel_timing(False)
+    upd_ms = tot / max(cnt, 1)
     rows = int(out["row_count"].sum())
     res = {"workload": f"fused filter + group-by, {n} rows: date<=2470, group (flagA, flagB), "
                        "SUM(price*(100-discount)) int128, COUNT, AVG",
            "rows": n, "groups": k, "rows_passing_filter": rows, "ms_per_step": wall * 1e3,
            "rows_per_s": n / wall, "update_ms": upd_ms,
-           "roofline": {"bound": "hbm", "achieved": n * 30 / (upd_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": n * 30 / (upd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+           "roofline": {"bound": "hbm", "kernel": "groupby_update", "achieved": n * 30 / (upd_ms * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": n * 30 / (upd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": TRAFFIC.get("groupby_update")}}
     if cpu_sample:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyarrow as pa
@@ -199,11 +207,20 @@ def main() -> int:
         print(f"probe result check failed: matches {r.matches} expected {expected} error {r.error}", file=sys.stderr)
         return 1
 
-    # Probe kernel duration: HIP events on the launch stream around K launches.
+    # Whole-probe duration (events around K probes), then per-kernel
+    # durations (events around each launch of the probe's kernels).
     ctx.timer_begin()
     for _ in range(args.steps):
         t.probe_async(pk, res, pp, None, op, ob, capacity=npr)
     probe_ms = ctx.timer_end() / args.steps
+    ctx.kernel_timing(True)
+    for _ in range(args.steps):
+        t.probe_async(pk, res, pp, None, op, ob, capacity=npr)
+    kernels = {}
+    for name in ("join_probe", "join_probe_move"):
+        tot, cnt = ctx.kernel_time(name)
+        kernels[name] = {"avg_us": tot / max(cnt, 1) * 1e3, "launches": cnt}
+    ctx.kernel_timing(False)
     # Build duration (host clock; the build synchronises internally).
     tb = time.perf_counter()
     for _ in range(5):
@@ -214,7 +231,11 @@ def main() -> int:
     t.close()
 
     alg_bytes = npr * 16 + r.matches * 8
-    achieved = alg_bytes / (probe_ms * 1e-3) / 1e9
+    kern_ms = kernels["join_probe"]["avg_us"] * 1e-3
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    kernels["join_probe"]["alg_bytes"] = alg_bytes
+    kernels["join_probe_move"]["alg_bytes"] = r.matches * 16
+    kernels["join_probe_move"]["GBps"] = r.matches * 16 / (kernels["join_probe_move"]["avg_us"] * 1e-6) / 1e9
     line = {
         "metric": METRIC,
         "value": npr / (ms_per_step * 1e-3),
@@ -231,9 +252,11 @@ def main() -> int:
         "config": {"workload": f"inner equi-join int64: build {nb} rows (unique keys, int32 payload) + probe {npr} "
                                f"rows (int64 payload) -> ordered row-index pairs + int128 payload sums",
                    "build_rows": nb, "probe_rows": npr, "table": "dense" if info.kind == L.TABLE_DENSE else "hashed"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "join_probe", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get("join_probe")},
+        "kernels": kernels,
         "probe_ms": probe_ms,
+        "probe_GBps": alg_bytes / (probe_ms * 1e-3) / 1e9,
         "probe_rows_per_s": npr / (probe_ms * 1e-3),
         "build_ms": build_ms,
         "matches": int(r.matches),

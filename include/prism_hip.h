@@ -111,6 +111,15 @@ prism_status prism_copy_device(prism_ctx* ctx, void* d_dst, const void* d_src, u
  * stream (HIP events); end synchronises. */
 prism_status prism_ctx_timer_begin(prism_ctx* ctx);
 prism_status prism_ctx_timer_end(prism_ctx* ctx, double* ms);
+/* Per-kernel timing: while enabled, the library brackets each launch of its
+ * main kernels with HIP events on the context stream, under these names:
+ * "join_probe" (lookup + in-tile compaction), "join_probe_move" (pairs to
+ * their final offsets), "groupby_update" (fused filter/aggregate pass).
+ * _read synchronises, returns the summed device time and launch count of
+ * one name since the last read, and clears that record. */
+prism_status prism_ctx_kernel_timing(prism_ctx* ctx, int32_t enable);
+prism_status prism_ctx_kernel_timing_read(prism_ctx* ctx, const char* kernel, double* total_ms,
+                                          uint64_t* launches);
 
 /* ------------------------------------------------------------------------- */
 /* Column descriptors (device layer: every pointer is a device pointer).      */

@@ -6,10 +6,12 @@
 #include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/prism_hip.h"
 
@@ -108,6 +110,16 @@ struct prism_ctx {
   void* pinned = nullptr;   // small pinned host staging for results
   size_t pinned_bytes = 0;
   hipEvent_t t0 = nullptr, t1 = nullptr;  // timer markers
+  // Stream-ordered cache of device blocks released by library-owned buffers:
+  // a block is reused only by work queued later on the context's stream, so
+  // no synchronisation is needed; hipFree happens at trim/destroy.
+  std::mutex pool_mu;
+  std::multimap<size_t, void*> pool;
+  size_t pooled_bytes = 0;
+  size_t pool_limit = size_t(16) << 30;
+  // Per-kernel timing (event pairs per kernel name) while enabled.
+  bool ktime_on = false;
+  std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> ktime;
 };
 
 namespace prism {
@@ -119,6 +131,19 @@ void use_device(prism_ctx* ctx);
 
 // Pinned host staging of at least n bytes owned by the context.
 void* pinned_staging(prism_ctx* ctx, size_t n);
+
+// Return every cached block to the device (hipFree).
+void pool_trim(prism_ctx* ctx);
+
+// Brackets the launches made during its lifetime with events on the context
+// stream when per-kernel timing is enabled.
+struct KernelTimer {
+  prism_ctx* ctx;
+  const char* name;
+  hipEvent_t a = nullptr;
+  KernelTimer(prism_ctx* c, const char* n);
+  ~KernelTimer();
+};
 
 }  // namespace prism
 
@@ -153,6 +178,7 @@ __host__ __device__ __forceinline__ double f64_unorder(uint64_t o) {
 
 // Native 128-bit vector of two int64 (usable with the non-temporal builtins).
 typedef long long v2i64 __attribute__((ext_vector_type(2)));
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 // Number of set bits of `mask` in lanes below this one.

@@ -8,29 +8,109 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 void clear_last_error() { g_last_error.clear(); }
 
+// Block sizes: powers of two up to 1 MiB, then multiples of 2 MiB, so that
+// repeated operations of similar size hit the cache.
+static size_t block_size(size_t n) {
+  if (n <= (size_t(1) << 20)) return next_pow2(n < 256 ? 256 : n);
+  const size_t g = size_t(2) << 20;
+  return (n + g - 1) / g * g;
+}
+
 void DeviceBuffer::alloc(Ctx* c, size_t n) {
   ctx = c;
   bytes = 0;
   ptr = nullptr;
   if (n == 0) return;
-  hipError_t e = hipMalloc(&ptr, n);
+  const size_t cap = block_size(n);
+  if (c) {
+    std::lock_guard<std::mutex> lock(c->pool_mu);
+    auto it = c->pool.lower_bound(cap);
+    if (it != c->pool.end() && it->first <= cap + cap / 4) {
+      ptr = it->second;
+      bytes = it->first;
+      c->pooled_bytes -= it->first;
+      c->pool.erase(it);
+      c->device_bytes += bytes;
+      return;
+    }
+  }
+  hipError_t e = hipMalloc(&ptr, cap);
+  if (e != hipSuccess && c) {  // give the cached blocks back and try once more
+    (void)hipGetLastError();
+    pool_trim(c);
+    e = hipMalloc(&ptr, cap);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     ptr = nullptr;
     fail(PRISM_ERR_EXECUTION, "Execution error: hipMalloc of " + std::to_string(n) +
                                   " bytes failed: " + hipGetErrorString(e));
   }
-  bytes = n;
-  if (ctx) ctx->device_bytes += n;
+  bytes = cap;
+  if (ctx) ctx->device_bytes += cap;
 }
 
 void DeviceBuffer::release() {
   if (ptr) {
-    (void)hipFree(ptr);
-    if (ctx) ctx->device_bytes -= bytes;
+    bool cached = false;
+    if (ctx) {
+      ctx->device_bytes -= bytes;
+      std::lock_guard<std::mutex> lock(ctx->pool_mu);
+      if (ctx->pooled_bytes + bytes <= ctx->pool_limit) {
+        ctx->pool.emplace(bytes, ptr);
+        ctx->pooled_bytes += bytes;
+        cached = true;
+      }
+    }
+    if (!cached) (void)hipFree(ptr);
   }
   ptr = nullptr;
   bytes = 0;
+}
+
+__global__ __launch_bounds__(256) void k_copy16(v4u32* __restrict__ dst, const v4u32* __restrict__ src, uint64_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+KernelTimer::KernelTimer(prism_ctx* c, const char* n) : ctx(c), name(n) {
+  if (!ctx->ktime_on) return;
+  if (hipEventCreate(&a) != hipSuccess || hipEventRecord(a, ctx->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    if (a) (void)hipEventDestroy(a);
+    a = nullptr;
+  }
+}
+
+KernelTimer::~KernelTimer() {
+  if (!a) return;
+  hipEvent_t b = nullptr;
+  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, ctx->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return;
+  }
+  ctx->ktime[name].emplace_back(a, b);
+}
+
+static void drop_kernel_times(prism_ctx* ctx) {
+  for (auto& kv : ctx->ktime)
+    for (auto& ev : kv.second) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+  ctx->ktime.clear();
+}
+
+void pool_trim(prism_ctx* ctx) {
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
+  if (ctx->pool.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->pool) (void)hipFree(kv.second);
+  ctx->pool.clear();
+  ctx->pooled_bytes = 0;
 }
 
 void use_device(prism_ctx* ctx) { PRISM_HIP(hipSetDevice(ctx->device)); }
@@ -90,6 +170,8 @@ void prism_ctx_destroy(prism_ctx* ctx) {
   (void)hipSetDevice(c->device);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   c->scratch.release();
+  pool_trim(c);
+  drop_kernel_times(c);
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
   if (c->pinned) (void)hipHostFree(c->pinned);
@@ -100,6 +182,9 @@ void prism_ctx_destroy(prism_ctx* ctx) {
 prism_status prism_ctx_set_stream(prism_ctx* ctx, void* hip_stream) {
   return guarded([&] {
     if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: ctx is NULL");
+    use_device(ctx);
+    // cached blocks are ordered on the old stream: drain it before switching
+    PRISM_HIP(hipStreamSynchronize(ctx->stream));
     ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
   });
 }
@@ -167,8 +252,22 @@ prism_status prism_copy_device(prism_ctx* ctx, void* d_dst, const void* d_src, u
   return guarded([&] {
     if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
     if (!bytes) return;
+    if (!d_dst || !d_src) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL pointer");
     use_device(ctx);
-    PRISM_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool aligned = ((reinterpret_cast<uintptr_t>(d_dst) | reinterpret_cast<uintptr_t>(d_src)) & 15) == 0;
+    if (aligned && bytes % 16 == 0) {
+      // a plain streaming copy kernel: also the reference point for HBM
+      // traffic counters (it reads and writes exactly `bytes`)
+      const uint64_t n = bytes / 16;
+      uint64_t blocks = ceil_div(n, uint64_t(256) * 4);
+      const uint64_t cap = static_cast<uint64_t>(ctx->num_cus) * 16;
+      if (blocks > cap) blocks = cap;
+      hipLaunchKernelGGL(k_copy16, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream,
+                         static_cast<v4u32*>(d_dst), static_cast<const v4u32*>(d_src), n);
+      PRISM_CHECK_LAUNCH();
+    } else {
+      PRISM_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
   });
 }
 
@@ -191,6 +290,43 @@ prism_status prism_ctx_timer_end(prism_ctx* ctx, double* ms) {
     float f = 0.0f;
     PRISM_HIP(hipEventElapsedTime(&f, ctx->t0, ctx->t1));
     *ms = f;
+  });
+}
+
+prism_status prism_ctx_kernel_timing(prism_ctx* ctx, int32_t enable) {
+  return guarded([&] {
+    if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    use_device(ctx);
+    PRISM_HIP(hipStreamSynchronize(ctx->stream));
+    drop_kernel_times(ctx);
+    ctx->ktime_on = enable != 0;
+  });
+}
+
+prism_status prism_ctx_kernel_timing_read(prism_ctx* ctx, const char* kernel, double* total_ms,
+                                          uint64_t* launches) {
+  return guarded([&] {
+    if (!ctx || !kernel || !total_ms || !launches) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    use_device(ctx);
+    PRISM_HIP(hipStreamSynchronize(ctx->stream));
+    double ms = 0;
+    uint64_t cnt = 0;
+    auto it = ctx->ktime.find(kernel);
+    if (it != ctx->ktime.end()) {
+      for (auto& ev : it->second) {
+        float f = 0.0f;
+        PRISM_HIP(hipEventElapsedTime(&f, ev.first, ev.second));
+        ms += f;
+        ++cnt;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+      }
+      ctx->ktime.erase(it);
+    }
+    *total_ms = ms;
+    *launches = cnt;
   });
 }
 

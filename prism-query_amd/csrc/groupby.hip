@@ -3,12 +3,19 @@
 //
 // Group identity is the reference's 64-bit FxHash of the key values
 // (hash_aggregate.rs:483-504), without a key-equality check, exactly as the
-// reference groups (hash_aggregate.rs:215,248-255).  Each workgroup keeps a
-// partial group table in LDS (open addressing on the group hash, bounded
-// probing); rows that do not fit go straight to the global table.  At the end
-// of the launch every LDS entry is merged into the global table in HBM with
-// atomics.  Output order is first appearance (minimum global row per group),
-// as hash_aggregate.rs:302-314 emits it.
+// reference groups (hash_aggregate.rs:215,248-255).  Output order is first
+// appearance (minimum global row per group), as hash_aggregate.rs:302-314
+// emits it.
+//
+// Every fixed-width column the plan reads is loaded once per row batch, all
+// loads in flight together, before the row programs run on registers.  Two
+// workgroup-local forms then hold partial groups in LDS:
+//   private - few groups: a small shared map (group hash -> slot) and, per
+//             thread, its own accumulators for every slot ([slot][word][thread]
+//             so a wave touches consecutive words); no atomics on hot groups.
+//   shared  - many groups: one LDS table of slots updated with LDS atomics.
+// Rows whose group does not fit go straight to the global table in HBM.  At
+// the end of the launch the LDS partials merge into the global table.
 //
 // Accumulators:
 //   SUM of integers   - exact int128.  LDS holds split sums (low 32-bit halves
@@ -39,6 +46,8 @@ constexpr int kBlock = 256;
 // Rows per lane per iteration: 4 for programs of stack depth <= 4, else 2.
 constexpr int kLdsProbes = 8;
 constexpr size_t kLdsBudget = 64 * 1024;
+constexpr size_t kPrivBudget = 56 * 1024;
+constexpr uint64_t kSampleRows = 1 << 20;
 
 enum LaneOp : int32_t { L_SUM_I = 1, L_SUM_F = 2, L_MIN_F = 3, L_MAX_F = 4, L_CNT_VALID = 5 };
 
@@ -52,6 +61,9 @@ struct Lane {
 struct DevPlan {
   int32_t n_keys;
   int32_t key_cols[kMaxKeys];
+  int32_t key_pref[kMaxKeys];  // prefetch slot of a fixed-width key, -1 for Utf8
+  int32_t n_pref;              // fixed-width columns loaded once per row batch
+  int32_t pref_col[kMaxPref];
   int32_t filter_start, filter_len;
   int32_t n_values;
   int32_t value_start[kMaxValues];
@@ -187,6 +199,71 @@ __device__ __forceinline__ void lane_global(unsigned long long* G, const Lane& l
   }
 }
 
+// Thread-private accumulators: word w of this thread's slot lives at
+// base[(w - kW_COUNT) * kBlock].  Only the owning thread touches it.
+__device__ __forceinline__ void add_i128_priv(unsigned long long* lo_p, unsigned long long* hi_p, int64_t x) {
+  const unsigned long long lo = *lo_p, nlo = lo + static_cast<unsigned long long>(x);
+  *lo_p = nlo;
+  *hi_p = *hi_p + static_cast<unsigned long long>(x >> 63) + (nlo < lo ? 1ULL : 0ULL);
+}
+
+__device__ __forceinline__ void lane_priv(unsigned long long* base, const Lane& ln, uint64_t v) {
+  unsigned long long* w = base + (ln.word - kW_COUNT) * kBlock;
+  switch (ln.op) {
+    case L_SUM_I:
+      add_i128_priv(w, w + kBlock, static_cast<int64_t>(v));
+      break;
+    case L_SUM_F: {
+      const double d = as_f(v);
+      if (integral_small(d)) add_i128_priv(w, w + kBlock, static_cast<int64_t>(d));
+      else w[2 * kBlock] = f_bits(as_f(w[2 * kBlock]) + d);
+      break;
+    }
+    case L_MIN_F:
+    case L_MAX_F: {
+      const double d = ln.is_int ? static_cast<double>(static_cast<int64_t>(v)) : as_f(v);
+      if (!isnan(d)) {
+        const unsigned long long u = f64_order(d);
+        if (ln.op == L_MIN_F ? u < *w : u > *w) *w = u;
+      }
+      break;
+    }
+    default:
+      *w += 1;
+  }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ void wave_sum_i128(uint64_t& lo, int64_t& hi) {
+  __int128 v = static_cast<__int128>((static_cast<unsigned __int128>(static_cast<uint64_t>(hi)) << 64) | lo);
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint64_t a = static_cast<uint64_t>(v), b = static_cast<uint64_t>(v >> 64);
+    const uint64_t oa = __shfl_xor(a, d, 64), ob = __shfl_xor(b, d, 64);
+    v += static_cast<__int128>((static_cast<unsigned __int128>(ob) << 64) | oa);
+  }
+  lo = static_cast<uint64_t>(v);
+  hi = static_cast<int64_t>(v >> 64);
+}
+
+// Map of the private form: group hash -> slot, linear probing over all slots.
+__device__ __forceinline__ int pmap_find(unsigned long long* keys, int slots, int shift, uint64_t h) {
+  uint32_t s = static_cast<uint32_t>((h * kMix) >> shift);
+  for (int p = 0; p < slots; ++p) {
+    const unsigned long long cur = __hip_atomic_load(&keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == h) return static_cast<int>(s);
+    if (cur == kEmptyKey) {
+      const unsigned long long old = atomicCAS(&keys[s], kEmptyKey, h);
+      if (old == kEmptyKey || old == h) return static_cast<int>(s);
+    }
+    s = (s + 1) & (slots - 1);
+  }
+  return -1;
+}
+
 // Merge one LDS slot (split int128 sums) into a global slot (true int128).
 __device__ __forceinline__ void merge_slot(unsigned long long* G, const unsigned long long* L, const DevPlan* P) {
   if (L[kW_COUNT]) atomicAdd(&G[kW_COUNT], L[kW_COUNT]);
@@ -219,28 +296,61 @@ __device__ __forceinline__ void merge_slot(unsigned long long* G, const unsigned
   }
 }
 
-template <int kRows, int kDepth>
-__global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ P, const prism_column* __restrict__ cols, uint64_t n,
-                                                    uint64_t row_base, GTable G, int lds_slots, int lds_shift,
-                                                    uint32_t* flags) {
+template <int kRows, int kDepth, int kPref, bool kPriv>
+__global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ P, const prism_column* __restrict__ cols,
+                                                    uint64_t first, uint64_t n, uint64_t row_base, GTable G,
+                                                    int lds_slots, int lds_shift, uint32_t* flags) {
   extern __shared__ unsigned long long lds[];
   const int stride = P->stride;
-  for (int i = threadIdx.x; i < lds_slots * stride; i += blockDim.x) lds[i] = P->tmpl[i % stride];
+  const int tid = threadIdx.x;
+  // private form: [map keys][map first rows][accumulators: slot, word, thread]
+  const int PW = stride - kW_COUNT;
+  unsigned long long* mapk = lds;
+  unsigned long long* mapf = lds + lds_slots;
+  unsigned long long* priv = lds + 2 * lds_slots;
+  if (kPriv) {
+    for (int i = tid; i < lds_slots; i += kBlock) {
+      mapk[i] = kEmptyKey;
+      mapf[i] = kNoRow;
+    }
+    for (int i = 0; i < lds_slots * PW; ++i) priv[i * kBlock + tid] = P->tmpl[kW_COUNT + i % PW];
+  } else {
+    for (int i = tid; i < lds_slots * stride; i += kBlock) lds[i] = P->tmpl[i % stride];
+  }
   __syncthreads();
 
-  const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x * kRows;
-  for (uint64_t it = static_cast<uint64_t>(blockIdx.x) * blockDim.x * kRows; it < n; it += step) {
+  const int n_pref = P->n_pref;
+  const uint64_t step = static_cast<uint64_t>(gridDim.x) * kBlock * kRows;
+  for (uint64_t it = first + static_cast<uint64_t>(blockIdx.x) * kBlock * kRows; it < n; it += step) {
     uint64_t row[kRows];
     bool in[kRows];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-      row[r] = it + static_cast<uint64_t>(r) * blockDim.x + threadIdx.x;
+      row[r] = it + static_cast<uint64_t>(r) * kBlock + tid;
       in[r] = row[r] < n;
+    }
+    // One round trip for every fixed-width column of the plan.
+    uint64_t pre[kPref][kRows];
+    uint32_t pre_v = 0;
+#pragma unroll
+    for (int j = 0; j < kPref; ++j) {
+      if (j < n_pref) {
+        const prism_column& c = cols[P->pref_col[j]];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+          pre[j][r] = in[r] ? col_load_raw(c, row[r]) : 0;
+          pre_v |= (in[r] && col_valid(c, row[r])) ? (1u << (j * kRows + r)) : 0u;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) pre[j][r] = 0;
+      }
     }
     if (P->filter_len) {
       uint64_t fv[kRows];
       uint32_t fvalid;
-      run_program<kRows, kDepth>(P->code + P->filter_start, P->filter_len, cols, &P->strs, row, in, fv, fvalid, flags);
+      run_program<kRows, kDepth, kPref>(P->code + P->filter_start, P->filter_len, cols, &P->strs, row, in, pre, pre_v,
+                                        fv, fvalid, flags);
 #pragma unroll
       for (int r = 0; r < kRows; ++r) in[r] = in[r] && ((fvalid >> r) & 1) && (fv[r] & 1);
     }
@@ -249,44 +359,65 @@ __global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ 
     for (int r = 0; r < kRows; ++r) h[r] = 0;
     for (int k = 0; k < P->n_keys; ++k) {
       const prism_column& c = cols[P->key_cols[k]];
+      const int kp = P->key_pref[k];
+      if (kp >= 0) {
 #pragma unroll
-      for (int r = 0; r < kRows; ++r)
-        if (in[r]) h[r] = hash_key(c, row[r], h[r]);
+        for (int j = 0; j < kPref; ++j)
+          if (kp == j) {
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+              h[r] = fx_step(h[r], ((pre_v >> (j * kRows + r)) & 1) ? key_word(c.type, pre[j][r]) : 0);
+          }
+      } else {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+          if (in[r]) h[r] = hash_key(c, row[r], h[r]);
+      }
     }
-    // Slot per row: LDS if it fits, else the global table.
-    int32_t lw[kRows];                 // LDS word index, -1 when the row goes to HBM
-    unsigned long long* gp[kRows];     // HBM slot
+    // Slot per row: LDS (private or shared form) if it fits, else HBM.
+    int32_t lw[kRows];              // private: slot; shared: LDS word index; -1 = HBM
+    unsigned long long* gp[kRows];  // HBM slot
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
       lw[r] = -1;
       gp[r] = nullptr;
       if (!in[r]) continue;
-      int ls = (lds_slots && h[r] != kEmptyKey) ? lds_find(lds, stride, lds_slots, lds_shift, h[r]) : -1;
-      if (ls >= 0) {
-        lw[r] = ls * stride;
-        atomicAdd(&lds[lw[r] + kW_COUNT], 1ULL);
-        // rows only grow along a lane, so most rows skip the atomic
-        const unsigned long long fr = row_base + row[r];
-        if (fr < __hip_atomic_load(&lds[lw[r] + kW_FIRST], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-          atomicMin(&lds[lw[r] + kW_FIRST], fr);
-      } else {
-        int64_t gs = g_find(G, h[r], flags);
-        if (gs < 0) {
-          in[r] = false;
+      const unsigned long long fr = row_base + row[r];
+      if (kPriv) {
+        const int ps = (lds_slots && h[r] != kEmptyKey) ? pmap_find(mapk, lds_slots, lds_shift, h[r]) : -1;
+        if (ps >= 0) {
+          lw[r] = ps;
+          unsigned long long* cnt = &priv[(ps * PW) * kBlock + tid];
+          // this thread's rows only grow, so its first row of a slot is its minimum
+          if (*cnt == 0) atomicMin(&mapf[ps], fr);
+          *cnt += 1;
           continue;
         }
-        gp[r] = &G.w[static_cast<uint64_t>(gs) * stride];
-        atomicAdd(&gp[r][kW_COUNT], 1ULL);
-        const unsigned long long fr = row_base + row[r];
-        if (fr < __hip_atomic_load(&gp[r][kW_FIRST], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-          atomicMin(&gp[r][kW_FIRST], fr);
+      } else {
+        const int ls = (lds_slots && h[r] != kEmptyKey) ? lds_find(lds, stride, lds_slots, lds_shift, h[r]) : -1;
+        if (ls >= 0) {
+          lw[r] = ls * stride;
+          atomicAdd(&lds[lw[r] + kW_COUNT], 1ULL);
+          if (fr < __hip_atomic_load(&lds[lw[r] + kW_FIRST], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+            atomicMin(&lds[lw[r] + kW_FIRST], fr);
+          continue;
+        }
       }
+      const int64_t gs = g_find(G, h[r], flags);
+      if (gs < 0) {
+        in[r] = false;
+        continue;
+      }
+      gp[r] = &G.w[static_cast<uint64_t>(gs) * stride];
+      atomicAdd(&gp[r][kW_COUNT], 1ULL);
+      if (fr < __hip_atomic_load(&gp[r][kW_FIRST], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(&gp[r][kW_FIRST], fr);
     }
     for (int v = 0; v < P->n_values; ++v) {
       uint64_t val[kRows];
       uint32_t vvalid;
-      run_program<kRows, kDepth>(P->code + P->value_start[v], P->value_len[v], cols, &P->strs, row, in, val, vvalid,
-                                 flags);
+      run_program<kRows, kDepth, kPref>(P->code + P->value_start[v], P->value_len[v], cols, &P->strs, row, in, pre,
+                                        pre_v, val, vvalid, flags);
       for (int l = 0; l < P->n_lanes; ++l) {
         const Lane ln = P->lanes[l];
         if (ln.value != v) continue;
@@ -294,19 +425,92 @@ __global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ 
         for (int r = 0; r < kRows; ++r) {
           if (!in[r]) continue;
           if (P->skip_nulls && !((vvalid >> r) & 1)) continue;
-          if (lw[r] >= 0) lane_lds(&lds[lw[r]], ln, val[r]);
-          else lane_global(gp[r], ln, val[r]);
+          if (lw[r] >= 0) {
+            if (kPriv) lane_priv(&priv[(lw[r] * PW) * kBlock + tid], ln, val[r]);
+            else lane_lds(&lds[lw[r]], ln, val[r]);
+          } else {
+            lane_global(gp[r], ln, val[r]);
+          }
         }
       }
     }
   }
   __syncthreads();
-  for (int s = threadIdx.x; s < lds_slots; s += blockDim.x) {
-    const unsigned long long* L = &lds[static_cast<uint32_t>(s) * stride];
-    if (L[kW_KEY] == kEmptyKey) continue;
-    int64_t gs = g_find(G, L[kW_KEY], flags);
-    if (gs < 0) continue;
-    merge_slot(&G.w[static_cast<uint64_t>(gs) * stride], L, P);
+  if (!kPriv) {
+    for (int s = tid; s < lds_slots; s += kBlock) {
+      const unsigned long long* L = &lds[static_cast<uint32_t>(s) * stride];
+      if (L[kW_KEY] == kEmptyKey) continue;
+      int64_t gs = g_find(G, L[kW_KEY], flags);
+      if (gs < 0) continue;
+      merge_slot(&G.w[static_cast<uint64_t>(gs) * stride], L, P);
+    }
+    return;
+  }
+  // Private form: one wave per slot reduces the threads' accumulators and
+  // merges the totals into HBM.
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int s = wave; s < lds_slots; s += kBlock / 64) {
+    const unsigned long long key = mapk[s];
+    if (key == kEmptyKey) continue;
+    const unsigned long long* A = &priv[(s * PW) * kBlock];
+    unsigned long long cnt = 0;
+    for (int t = lane; t < kBlock; t += 64) cnt += A[t];
+    cnt = wave_sum_u64(cnt);
+    if (cnt == 0) continue;
+    long long gsl = -1;
+    if (lane == 0) gsl = g_find(G, key, flags);
+    gsl = __shfl(gsl, 0, 64);
+    if (gsl < 0) continue;
+    unsigned long long* gq = &G.w[static_cast<uint64_t>(gsl) * stride];
+    if (lane == 0) {
+      atomicAdd(&gq[kW_COUNT], cnt);
+      atomicMin(&gq[kW_FIRST], mapf[s]);
+    }
+    for (int l = 0; l < P->n_lanes; ++l) {
+      const Lane ln = P->lanes[l];
+      const int w = ln.word;
+      const unsigned long long* W = A + (w - kW_COUNT) * kBlock;
+      switch (ln.op) {
+        case L_SUM_I:
+        case L_SUM_F: {
+          __int128 acc = 0;
+          for (int t = lane; t < kBlock; t += 64)
+            acc += static_cast<__int128>((static_cast<unsigned __int128>(W[kBlock + t]) << 64) | W[t]);
+          uint64_t lo = static_cast<uint64_t>(acc);
+          int64_t hi = static_cast<int64_t>(acc >> 64);
+          wave_sum_i128(lo, hi);
+          if (lane == 0 && (lo | static_cast<uint64_t>(hi))) atomic_add_i128(&gq[w], &gq[w + 1], lo, hi);
+          if (ln.op == L_SUM_F) {
+            double f = 0;
+            for (int t = lane; t < kBlock; t += 64) f += as_f(W[2 * kBlock + t]);
+            for (int d = 32; d > 0; d >>= 1) f += __shfl_xor(f, d, 64);
+            if (lane == 0 && f != 0) unsafeAtomicAdd(reinterpret_cast<double*>(&gq[w + 2]), f);
+          }
+          break;
+        }
+        case L_MIN_F:
+        case L_MAX_F: {
+          const bool mn = ln.op == L_MIN_F;
+          unsigned long long m = P->tmpl[w];
+          for (int t = lane; t < kBlock; t += 64) m = mn ? (W[t] < m ? W[t] : m) : (W[t] > m ? W[t] : m);
+          for (int d = 32; d > 0; d >>= 1) {
+            const unsigned long long o = __shfl_xor(m, d, 64);
+            m = mn ? (o < m ? o : m) : (o > m ? o : m);
+          }
+          if (lane == 0 && m != P->tmpl[w]) {
+            if (mn) atomicMin(&gq[w], m);
+            else atomicMax(&gq[w], m);
+          }
+          break;
+        }
+        default: {
+          unsigned long long c = 0;
+          for (int t = lane; t < kBlock; t += 64) c += W[t];
+          c = wave_sum_u64(c);
+          if (lane == 0 && c) atomicAdd(&gq[w], c);
+        }
+      }
+    }
   }
 }
 
@@ -377,6 +581,7 @@ struct prism_groupby {
   DeviceBuffer table, backup;
   uint64_t cap = 0;
   uint64_t known_groups = 0;
+  uint64_t seen_groups = 0;  // largest group count of any earlier update, kept across resets
   DeviceBuffer counters;  // n_groups, flags, n_found
   DeviceBuffer d_cols;    // column descriptors of the current update
   DeviceBuffer result;    // finished groups, stride words each
@@ -441,34 +646,72 @@ static void grow(prism_groupby* g, uint64_t new_cap) {
   PRISM_HIP(hipStreamSynchronize(g->ctx->stream));
 }
 
-static void launch(prism_groupby* g, const prism_column* cols, uint64_t n_rows, uint64_t row_base) {
+template <int kRows, int kDepth, int kPref>
+static void launch_form(prism_groupby* g, bool priv, dim3 grid, size_t lds_bytes, const prism_column* cols,
+                        uint64_t first, uint64_t n_rows, uint64_t row_base, int lds_slots, int lds_shift) {
+  Ctx* c = g->ctx;
+  if (priv)
+    hipLaunchKernelGGL((k_groupby<kRows, kDepth, kPref, true>), grid, dim3(kBlock), lds_bytes, c->stream,
+                       g->d_plan.as<DevPlan>(), cols, first, n_rows, row_base, g->view(), lds_slots, lds_shift,
+                       g->flags_ptr());
+  else
+    hipLaunchKernelGGL((k_groupby<kRows, kDepth, kPref, false>), grid, dim3(kBlock), lds_bytes, c->stream,
+                       g->d_plan.as<DevPlan>(), cols, first, n_rows, row_base, g->view(), lds_slots, lds_shift,
+                       g->flags_ptr());
+}
+
+// Group-count estimate behind the choice of form: what this state has seen
+// (also before a reset), else the caller's expectation.
+static uint64_t group_estimate(const prism_groupby* g) {
+  const uint64_t seen = g->known_groups > g->seen_groups ? g->known_groups : g->seen_groups;
+  return seen ? seen : g->expected;
+}
+
+// Rows [first, n_rows) of the update.
+static void launch(prism_groupby* g, const prism_column* cols, uint64_t first, uint64_t n_rows, uint64_t row_base) {
   Ctx* c = g->ctx;
   const size_t sb = g->slot_bytes();
+  const uint64_t known = group_estimate(g);
+  // Private form when the group count is known to be small.
+  const uint64_t pw = static_cast<uint64_t>(g->plan.stride - kW_COUNT);
+  const uint64_t ps = next_pow2(known > 2 ? known : 2);
+  const size_t priv_bytes = (2 * ps + ps * pw * kBlock) * 8;
+  const bool priv = known > 0 && ps <= 16 && priv_bytes <= kPrivBudget;
   int lds_slots = 0, lds_shift = 0;
-  uint64_t want = g->expected ? g->expected : 512;
-  if (g->known_groups > want) want = g->known_groups;
-  uint64_t s = next_pow2(want * 2 < 64 ? 64 : want * 2);
-  uint64_t fit = 1;
-  while ((fit * 2) * sb <= kLdsBudget) fit *= 2;
-  if (s > fit) s = fit;
-  if (want <= s && s >= 16) {
-    lds_slots = static_cast<int>(s);
-    lds_shift = 64 - log2_pow2(s);
+  size_t lds_bytes = 0;
+  if (priv) {
+    lds_slots = static_cast<int>(ps);
+    lds_shift = 64 - log2_pow2(ps);
+    lds_bytes = priv_bytes;
+  } else {
+    uint64_t want = known ? known : 512;
+    uint64_t s = next_pow2(want * 2 < 64 ? 64 : want * 2);
+    uint64_t fit = 1;
+    while ((fit * 2) * sb <= kLdsBudget) fit *= 2;
+    if (s > fit) s = fit;
+    if (want <= s && s >= 16) {
+      lds_slots = static_cast<int>(s);
+      lds_shift = 64 - log2_pow2(s);
+    }
+    lds_bytes = lds_slots * sb;
   }
   const bool shallow = g->max_depth <= 4;
   const uint64_t rows_per_lane = shallow ? 4 : 2;
-  uint64_t blocks = ceil_div(n_rows, static_cast<uint64_t>(kBlock) * rows_per_lane);
-  uint64_t max_blocks = static_cast<uint64_t>(c->num_cus) * (lds_slots ? 4 : 8);
+  uint64_t blocks = ceil_div(n_rows - first, static_cast<uint64_t>(kBlock) * rows_per_lane);
+  const uint64_t per_cu = lds_bytes ? (160 * 1024 / lds_bytes < 8 ? 160 * 1024 / lds_bytes : 8) : 8;
+  uint64_t max_blocks = static_cast<uint64_t>(c->num_cus) * (per_cu ? per_cu : 1);
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks == 0) blocks = 1;
-  if (shallow)
-    hipLaunchKernelGGL((k_groupby<4, 4>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), lds_slots * sb,
-                       c->stream, g->d_plan.as<DevPlan>(), cols, n_rows, row_base, g->view(), lds_slots, lds_shift,
-                       g->flags_ptr());
-  else
-    hipLaunchKernelGGL((k_groupby<2, kStackDepth>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
-                       lds_slots * sb, c->stream, g->d_plan.as<DevPlan>(), cols, n_rows, row_base, g->view(),
-                       lds_slots, lds_shift, g->flags_ptr());
+  const dim3 grid(static_cast<unsigned>(blocks));
+  KernelTimer kt(c, "groupby_update");
+  if (shallow) {
+    if (g->plan.n_pref <= 4)
+      launch_form<4, 4, 4>(g, priv, grid, lds_bytes, cols, first, n_rows, row_base, lds_slots, lds_shift);
+    else
+      launch_form<4, 4, 8>(g, priv, grid, lds_bytes, cols, first, n_rows, row_base, lds_slots, lds_shift);
+  } else {
+    launch_form<2, kStackDepth, 8>(g, priv, grid, lds_bytes, cols, first, n_rows, row_base, lds_slots, lds_shift);
+  }
   PRISM_CHECK_LAUNCH();
 }
 
@@ -535,6 +778,28 @@ prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan
       D.value_len[v] = static_cast<int32_t>(len);
       off += len;
     }
+    // Prefetch slots: fixed-width columns read by the programs or hashed as
+    // keys; their loads become kOpLoadPre of the slot.
+    auto fixed = [&](uint32_t col) { return plan->column_types[col] != PRISM_TYPE_UTF8; };
+    auto pref_slot = [&](uint32_t col) -> int32_t {
+      for (int32_t j = 0; j < D.n_pref; ++j)
+        if (D.pref_col[j] == static_cast<int32_t>(col)) return j;
+      if (D.n_pref >= kMaxPref) return -1;
+      D.pref_col[D.n_pref] = static_cast<int32_t>(col);
+      return D.n_pref++;
+    };
+    for (int32_t pc = 0; pc < code_len; ++pc) {
+      prism_instr& ins = D.code[pc];
+      if ((ins.op == PRISM_OP_LOAD_I || ins.op == PRISM_OP_LOAD_F) && fixed(static_cast<uint32_t>(ins.arg))) {
+        const int32_t j = pref_slot(static_cast<uint32_t>(ins.arg));
+        if (j >= 0) {
+          ins.op = kOpLoadPre;
+          ins.arg = j;
+        }
+      }
+    }
+    for (int32_t k = 0; k < D.n_keys; ++k)
+      D.key_pref[k] = fixed(static_cast<uint32_t>(D.key_cols[k])) ? pref_slot(static_cast<uint32_t>(D.key_cols[k])) : -1;
     const bool global = plan->n_keys == 0;
     D.skip_nulls = global ? 1 : 0;
     int32_t word = kW_LANES;
@@ -630,14 +895,18 @@ prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns,
     g->d_cols.reserve(g->ctx, sizeof(ColsArg));
     PRISM_HIP(hipMemcpyAsync(g->d_cols.ptr, &cols, sizeof(ColsArg), hipMemcpyHostToDevice, st));
     const prism_column* dcols = g->d_cols.as<prism_column>();
-    while (true) {
-      // A snapshot is needed only when this batch could overfill the table.
-      const bool risky = (g->known_groups + n_rows) * 4 > g->cap * 3;
+    uint64_t done = 0;
+    while (done < n_rows) {
+      // With no group count to go by, a prefix of the rows measures it first.
+      uint64_t end = n_rows;
+      if (group_estimate(g) == 0 && n_rows - done > 2 * kSampleRows) end = done + kSampleRows;
+      // A snapshot is needed only when this segment could overfill the table.
+      const bool risky = (g->known_groups + (end - done)) * 4 > g->cap * 3;
       if (risky) {
         g->backup.reserve(g->ctx, g->table.bytes);
         PRISM_HIP(hipMemcpyAsync(g->backup.ptr, g->table.ptr, g->table.bytes, hipMemcpyDeviceToDevice, st));
       }
-      launch(g, dcols, n_rows, row_base);
+      launch(g, dcols, done, end, row_base);
       uint64_t groups;
       uint32_t flags;
       read_counters(g, groups, flags);
@@ -660,7 +929,8 @@ prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns,
         grow(g, g->cap * 4);
         read_counters(g, g->known_groups, flags);
       }
-      break;
+      if (g->known_groups > g->seen_groups) g->seen_groups = g->known_groups;
+      done = end;
     }
   });
 }

@@ -41,6 +41,7 @@ struct Stats {
   int64_t max_key;
   unsigned long long n_distinct;
   unsigned long long max_dup;
+  unsigned long long sentinel;  // HASHED: build rows holding INT64_MIN
 };
 
 struct TableView {
@@ -133,7 +134,10 @@ __global__ void k_init_stats(Stats* st) {
   st->max_key = INT64_MIN;
   st->n_distinct = 0;
   st->max_dup = 0;
+  st->sentinel = 0;
 }
+
+__global__ void k_copy_sentinel(const uint32_t* cnt_of_sentinel, Stats* st) { st->sentinel = *cnt_of_sentinel; }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -628,11 +632,13 @@ __global__ __launch_bounds__(kPB) void k_probe_unique(TableView T, const int64_t
 // per tile; consecutive threads copy consecutive pairs).
 __global__ __launch_bounds__(kPB) void k_probe_move(const uint2* __restrict__ staged,
                                                     const uint32_t* __restrict__ tile_count,
-                                                    const uint64_t* __restrict__ tile_off, uint32_t* __restrict__ out_p,
-                                                    uint32_t* __restrict__ out_b, uint64_t capacity) {
+                                                    const uint64_t* __restrict__ tile_off,
+                                                    const uint64_t* __restrict__ slice_off, int slice_shift,
+                                                    uint32_t* __restrict__ out_p, uint32_t* __restrict__ out_b,
+                                                    uint64_t capacity) {
   const uint32_t tile = blockIdx.x;
   const uint32_t cnt = tile_count[tile];
-  const uint64_t off = tile_off[tile];
+  const uint64_t off = slice_off[tile >> slice_shift] + tile_off[tile];
   const uint2* src = staged + static_cast<uint64_t>(tile) * kTileRows;
   for (uint32_t i = threadIdx.x; i < cnt; i += kPB) {
     const uint64_t pos = off + i;
@@ -644,20 +650,6 @@ __global__ __launch_bounds__(kPB) void k_probe_move(const uint2* __restrict__ st
   }
 }
 
-// Sum of tile match counts when no pairs are written (count / sum-only probes).
-__global__ void k_probe_count_sum(const uint32_t* __restrict__ tile_count, uint64_t n_tiles, prism_probe_result* res) {
-  __shared__ uint64_t s[16];
-  uint64_t c = 0;
-  for (uint64_t i = threadIdx.x; i < n_tiles; i += blockDim.x) c += tile_count[i];
-  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
-  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t tot = 0;
-    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) tot += s[i];
-    res->matches = tot;
-  }
-}
 
 template <int KIND, bool UNIQUE, bool VEC>
 __global__ __launch_bounds__(kPB, 2) void k_probe(TableView T, const int64_t* __restrict__ keys, uint64_t n,
@@ -851,19 +843,28 @@ __global__ void k_probe_sums(const __int128* __restrict__ tile_sums, uint64_t n_
   }
 }
 
-// Phase 2: exclusive scan of the tile match counts (one workgroup of 1024
-// threads; each thread owns a contiguous run of tiles), plus the total.
-__global__ __launch_bounds__(1024) void k_probe_scan(const uint32_t* __restrict__ tile_count,
-                                                     uint64_t* __restrict__ tile_off, uint64_t n_tiles,
-                                                     prism_probe_result* res) {
-  __shared__ uint64_t s_wave[16];
-  const uint64_t per = (n_tiles + blockDim.x - 1) / blockDim.x;
-  const uint64_t lo = threadIdx.x * per, hi = lo + per < n_tiles ? lo + per : n_tiles;
-  uint64_t sum = 0;
-  for (uint64_t i = lo; i < hi; ++i) sum += tile_count[i];
-  // inclusive scan of per-thread sums inside the wave, then across waves
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint64_t inc = sum;
+// Phase 2a: each workgroup takes a slice of kSliceTiles consecutive tiles,
+// writes the exclusive scan of their match counts (offsets local to the
+// slice) and the slice's match total and payload sums.
+constexpr int kSliceThreads = 256, kSlicePer = 4, kSliceTiles = kSliceThreads * kSlicePer;
+
+__global__ __launch_bounds__(kSliceThreads) void k_probe_slices(const uint32_t* __restrict__ tile_count,
+                                                                uint64_t* __restrict__ tile_off,
+                                                                const __int128* __restrict__ tile_sums,
+                                                                uint64_t n_tiles, uint64_t* __restrict__ slice_total,
+                                                                __int128* __restrict__ slice_sums) {
+  __shared__ uint64_t s_wave[kSliceThreads / 64];
+  __shared__ __int128 s_red[2][kSliceThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSliceTiles + static_cast<uint64_t>(t) * kSlicePer;
+  uint32_t c[kSlicePer];
+  uint64_t run = 0;
+#pragma unroll
+  for (int j = 0; j < kSlicePer; ++j) {
+    c[j] = base + j < n_tiles ? tile_count[base + j] : 0;
+    run += c[j];
+  }
+  uint64_t inc = run;
   for (int d = 1; d < 64; d <<= 1) {
     const uint64_t o = __shfl_up(inc, d, 64);
     if (lane >= d) inc += o;
@@ -871,16 +872,108 @@ __global__ __launch_bounds__(1024) void k_probe_scan(const uint32_t* __restrict_
   if (lane == 63) s_wave[wave] = inc;
   __syncthreads();
   uint64_t before = 0, total = 0;
-  for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+#pragma unroll
+  for (int w = 0; w < kSliceThreads / 64; ++w) {
     before += (w < wave) ? s_wave[w] : 0;
     total += s_wave[w];
   }
-  uint64_t run = before + inc - sum;
-  for (uint64_t i = lo; i < hi; ++i) {
-    tile_off[i] = run;
-    run += tile_count[i];
+  if (tile_off) {
+    uint64_t x = before + inc - run;
+#pragma unroll
+    for (int j = 0; j < kSlicePer; ++j) {
+      if (base + j < n_tiles) tile_off[base + j] = x;
+      x += c[j];
+    }
   }
-  if (threadIdx.x == 0) res->matches = total;
+  if (t == 0) slice_total[blockIdx.x] = total;
+  if (tile_sums) {
+    __int128 p = 0, b = 0;
+#pragma unroll
+    for (int j = 0; j < kSlicePer; ++j)
+      if (base + j < n_tiles) {
+        p += tile_sums[2 * (base + j)];
+        b += tile_sums[2 * (base + j) + 1];
+      }
+    wave_reduce_i128(p);
+    wave_reduce_i128(b);
+    if (lane == 0) {
+      s_red[0][wave] = p;
+      s_red[1][wave] = b;
+    }
+    __syncthreads();
+    if (t == 0) {
+      p = 0;
+      b = 0;
+      for (int w = 0; w < kSliceThreads / 64; ++w) {
+        p += s_red[0][w];
+        b += s_red[1][w];
+      }
+      slice_sums[2 * blockIdx.x] = p;
+      slice_sums[2 * blockIdx.x + 1] = b;
+    }
+  }
+}
+
+// Phase 2b (one workgroup): exclusive scan of the slice totals in place,
+// total matches and payload sums into the result.
+constexpr int kFinishThreads = 1024;
+
+__global__ __launch_bounds__(kFinishThreads) void k_probe_finish(uint64_t* __restrict__ slice_total,
+                                                                 const __int128* __restrict__ slice_sums,
+                                                                 uint64_t n_slices, prism_probe_result* res) {
+  __shared__ uint64_t s_wave[kFinishThreads / 64];
+  __shared__ __int128 s_red[2][kFinishThreads / 64];
+  __shared__ uint64_t s_carry;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t == 0) s_carry = 0;
+  __syncthreads();
+  for (uint64_t base = 0; base < n_slices; base += kFinishThreads) {
+    const uint64_t i = base + t;
+    const uint64_t v = i < n_slices ? slice_total[i] : 0;
+    uint64_t inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += o;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    uint64_t before = s_carry, total = 0;
+    for (int w = 0; w < kFinishThreads / 64; ++w) {
+      before += (w < wave) ? s_wave[w] : 0;
+      total += s_wave[w];
+    }
+    if (i < n_slices) slice_total[i] = before + inc - v;
+    __syncthreads();
+    if (t == 0) s_carry += total;
+    __syncthreads();
+  }
+  if (slice_sums) {
+    __int128 p = 0, b = 0;
+    for (uint64_t i = t; i < n_slices; i += kFinishThreads) {
+      p += slice_sums[2 * i];
+      b += slice_sums[2 * i + 1];
+    }
+    wave_reduce_i128(p);
+    wave_reduce_i128(b);
+    if (lane == 0) {
+      s_red[0][wave] = p;
+      s_red[1][wave] = b;
+    }
+    __syncthreads();
+    if (t == 0) {
+      p = 0;
+      b = 0;
+      for (int w = 0; w < kFinishThreads / 64; ++w) {
+        p += s_red[0][w];
+        b += s_red[1][w];
+      }
+      res->probe_sum_lo = static_cast<uint64_t>(p);
+      res->probe_sum_hi = static_cast<int64_t>(p >> 64);
+      res->build_sum_lo = static_cast<uint64_t>(b);
+      res->build_sum_hi = static_cast<int64_t>(b >> 64);
+    }
+  }
+  if (t == 0) res->matches = s_carry;
 }
 
 template <int KIND, bool UNIQUE>
@@ -1000,6 +1093,7 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       hipLaunchKernelGGL(k_slot_counts, grid_for(ctx, S + 1), dim3(kBuildBlock), 0, st, slots, S + 1,
                          cnt.as<uint32_t>());
       launch_count_stats(ctx, cnt.as<uint32_t>(), S + 1, static_cast<uint64_t*>(part), dst);
+      hipLaunchKernelGGL(k_copy_sentinel, dim3(1), dim3(1), 0, st, cnt.as<uint32_t>() + S, dst);
       PRISM_CHECK_LAUNCH();
       PRISM_HIP(hipMemcpyAsync(&hs, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
       PRISM_HIP(hipStreamSynchronize(st));
@@ -1023,10 +1117,7 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       T->view.slots = slots;
       T->view.mask = S - 1;
       T->view.shift = shift;
-      uint32_t sentinel_cnt = 0;
-      PRISM_HIP(hipMemcpyAsync(&sentinel_cnt, cnt.as<uint32_t>() + S, 4, hipMemcpyDeviceToHost, st));
-      PRISM_HIP(hipStreamSynchronize(st));
-      T->view.sentinel_present = sentinel_cnt != 0;
+      T->view.sentinel_present = hs.sentinel != 0;
     }
     if (d_payload && n_rows) {  // carry the build payload in the table entries
       T->view.has_payload = 1;
@@ -1104,37 +1195,52 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
     const uint64_t sums_bytes = align(n_tiles * 32 + 32);
     __int128* tile_sums = c->scratch.as<__int128>();
     if (t->info.unique) {
+      const uint64_t n_slices_max = ceil_div(n_tiles, kSliceTiles) + 1;
       const uint64_t cnt_bytes = align(n_tiles * 4 + 4), off_bytes = align(n_tiles * 8 + 8);
+      const uint64_t slice_bytes = align(n_slices_max * 8) + align(n_slices_max * 32);
       const uint64_t stage_bytes = pairs ? n_tiles * kTileRows * sizeof(uint2) : 0;
-      c->scratch.reserve(c, sums_bytes + cnt_bytes + off_bytes + stage_bytes);
+      c->scratch.reserve(c, sums_bytes + cnt_bytes + off_bytes + slice_bytes + stage_bytes);
       tile_sums = c->scratch.as<__int128>();
       char* p = c->scratch.as<char>() + sums_bytes;
       uint32_t* tile_count = reinterpret_cast<uint32_t*>(p);
       uint64_t* tile_off = reinterpret_cast<uint64_t*>(p + cnt_bytes);
-      uint2* staged = pairs ? reinterpret_cast<uint2*>(p + cnt_bytes + off_bytes) : nullptr;
+      uint64_t* slice_total = reinterpret_cast<uint64_t*>(p + cnt_bytes + off_bytes);
+      __int128* slice_sums = reinterpret_cast<__int128*>(p + cnt_bytes + off_bytes + align(n_slices_max * 8));
+      uint2* staged = pairs ? reinterpret_cast<uint2*>(p + cnt_bytes + off_bytes + slice_bytes) : nullptr;
       hipLaunchKernelGGL(k_probe_clear, dim3(1), dim3(64), 0, ctx->stream, nullptr, 0, d_result);
       PRISM_CHECK_LAUNCH();
       if (n_probe == 0) return;
 #define PRISM_PROBE1(KIND_, VEC_)                                                                          \
   hipLaunchKernelGGL((k_probe_unique<KIND_, VEC_>), g, dim3(kPB), 0, ctx->stream, T, d_probe_keys, n_probe, \
                      probe_row_base, d_probe_payload, d_build_payload, staged, tile_count, tile_sums)
-      if (t->info.kind == PRISM_TABLE_DENSE) {
-        if (vec) PRISM_PROBE1(PRISM_TABLE_DENSE, true);
-        else PRISM_PROBE1(PRISM_TABLE_DENSE, false);
-      } else {
-        if (vec) PRISM_PROBE1(PRISM_TABLE_HASHED, true);
-        else PRISM_PROBE1(PRISM_TABLE_HASHED, false);
+      {
+        KernelTimer kt(ctx, "join_probe");
+        if (t->info.kind == PRISM_TABLE_DENSE) {
+          if (vec) PRISM_PROBE1(PRISM_TABLE_DENSE, true);
+          else PRISM_PROBE1(PRISM_TABLE_DENSE, false);
+        } else {
+          if (vec) PRISM_PROBE1(PRISM_TABLE_HASHED, true);
+          else PRISM_PROBE1(PRISM_TABLE_HASHED, false);
+        }
       }
 #undef PRISM_PROBE1
       PRISM_CHECK_LAUNCH();
+      const bool want_sums = d_probe_payload || d_build_payload || t->view.has_payload;
+      const uint64_t n_slices = ceil_div(n_tiles, kSliceTiles);
+      hipLaunchKernelGGL(k_probe_slices, dim3(static_cast<unsigned>(n_slices)), dim3(kSliceThreads), 0, ctx->stream,
+                         tile_count, pairs ? tile_off : nullptr, want_sums ? tile_sums : nullptr, n_tiles,
+                         slice_total, slice_sums);
+      PRISM_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_probe_finish, dim3(1), dim3(kFinishThreads), 0, ctx->stream, slice_total,
+                         want_sums ? slice_sums : nullptr, n_slices, d_result);
+      PRISM_CHECK_LAUNCH();
       if (pairs) {
-        hipLaunchKernelGGL(k_probe_scan, dim3(1), dim3(1024), 0, ctx->stream, tile_count, tile_off, n_tiles, d_result);
+        KernelTimer kt(ctx, "join_probe_move");
+        hipLaunchKernelGGL(k_probe_move, g, dim3(kPB), 0, ctx->stream, staged, tile_count, tile_off, slice_total,
+                           log2_pow2(kSliceTiles), d_probe_idx, d_build_idx, capacity);
         PRISM_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_probe_move, g, dim3(kPB), 0, ctx->stream, staged, tile_count, tile_off, d_probe_idx,
-                           d_build_idx, capacity);
-      } else {
-        hipLaunchKernelGGL(k_probe_count_sum, dim3(1), dim3(1024), 0, ctx->stream, tile_count, n_tiles, d_result);
       }
+      return;
     } else {
       const uint64_t status_bytes = align((n_tiles + 1) * 8);
       c->scratch.reserve(c, sums_bytes + status_bytes);
@@ -1144,6 +1250,7 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
                          n_tiles, d_result);
       PRISM_CHECK_LAUNCH();
       if (n_probe == 0) return;
+      KernelTimer kt(ctx, "join_probe");
       if (t->info.kind == PRISM_TABLE_DENSE)
         launch_probe_vec<PRISM_TABLE_DENSE, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
                                                    d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,

@@ -17,6 +17,9 @@ constexpr int kMaxCode = 160;
 constexpr int kMaxStrings = 16;
 constexpr int kMaxStrBytes = 1024;
 constexpr int kStackDepth = 8;
+constexpr int kMaxPref = 8;
+// Internal opcode (never in user code): push prefetched column slot `arg`.
+constexpr int32_t kOpLoadPre = 64;
 
 // Static result type of a program.
 enum ValType : int32_t { VT_I = 0, VT_F = 1, VT_B = 2, VT_NULL = 3 };
@@ -60,6 +63,32 @@ __device__ __forceinline__ int64_t col_load_i(const prism_column& c, uint64_t ro
 }
 __device__ __forceinline__ double col_load_f(const prism_column& c, uint64_t row) {
   return static_cast<const double*>(c.data)[c.offset + row];
+}
+// Raw 64-bit word of a fixed-width column: int64 / f64 bits as stored, int32 /
+// date32 sign-extended, bool as 0 / 1.
+__device__ __forceinline__ uint64_t col_load_raw(const prism_column& c, uint64_t row) {
+  const uint64_t i = c.offset + row;
+  switch (c.type) {
+    case PRISM_TYPE_INT64:
+    case PRISM_TYPE_FLOAT64:
+      return static_cast<const uint64_t*>(c.data)[i];
+    case PRISM_TYPE_BOOL:
+      return bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0;
+    default:
+      return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[i]));
+  }
+}
+// FxHash word of a non-null fixed-width key from its raw word (the same words
+// hash_key writes).
+__device__ __forceinline__ uint64_t key_word(int32_t type, uint64_t raw) {
+  switch (type) {
+    case PRISM_TYPE_DATE32:
+      return static_cast<uint64_t>(static_cast<uint32_t>(raw)) | (1ULL << 32);
+    case PRISM_TYPE_BOOL:
+      return 2 + raw;
+    default:
+      return raw;
+  }
 }
 
 __device__ __forceinline__ double as_f(uint64_t b) { return __longlong_as_double(static_cast<long long>(b)); }
@@ -179,12 +208,15 @@ __device__ __forceinline__ uint64_t hash_key(const prism_column& c, uint64_t row
 
 // Evaluate `len` instructions for R rows held by this lane.  Values are raw
 // 64-bit words (int64 or f64 bits); `valid` bit r = row r non-null.
-template <int R, int DEPTH = kStackDepth>
+// `pre` holds NP prefetched fixed-width columns (kOpLoadPre reads them);
+// bit j*R + r of pre_v = row r of slot j is non-null.
+template <int R, int DEPTH, int NP>
 __device__ __forceinline__ void run_program(const prism_instr* __restrict__ code, int len,
                                             const prism_column* __restrict__ cols,
                                             const StrTable* __restrict__ strs, const uint64_t (&row)[R],
-                                            const bool (&in)[R], uint64_t (&out)[R], uint32_t& out_valid,
-                                            uint32_t* err) {
+                                            const bool (&in)[R], const uint64_t (&pre)[NP][R], uint32_t pre_v,
+                                            uint64_t (&out)[R], uint32_t& out_valid, uint32_t* err) {
+  static_assert(NP * R <= 32, "prefetch validity bits fit one word");
   static_assert(DEPTH * R <= 32, "validity bits fit one word");
   uint64_t st[DEPTH * R];
   // Validity bits of stack slot k live in bits [k*R, k*R + R) of one word.
@@ -197,6 +229,18 @@ __device__ __forceinline__ void run_program(const prism_instr* __restrict__ code
     const int op = code[pc].op;
     const int arg = code[pc].arg;
     switch (op) {
+      case kOpLoadPre: {
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+          if (arg == j) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) st[sp * R + r] = pre[j][r];
+            v = (pre_v >> (j * R)) & kAll;
+          }
+        vset(sp++, v);
+        break;
+      }
       case PRISM_OP_LOAD_I: {
         const prism_column& c = cols[arg];
         uint32_t v = 0;

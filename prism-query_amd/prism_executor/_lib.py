@@ -153,6 +153,8 @@ _SIGNATURES = [
     ("prism_copy_device", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64]),
     ("prism_ctx_timer_begin", c_int32, [c_void_p]),
     ("prism_ctx_timer_end", c_int32, [c_void_p, POINTER(c_double)]),
+    ("prism_ctx_kernel_timing", c_int32, [c_void_p, c_int32]),
+    ("prism_ctx_kernel_timing_read", c_int32, [c_void_p, c_char_p, POINTER(c_double), POINTER(c_uint64)]),
     ("prism_join_build_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_int32, POINTER(c_void_p)]),
     ("prism_join_build_i64_payload", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_int32,
                                                POINTER(c_void_p)]),

@@ -48,6 +48,17 @@ class Context:
         L.check(self.lib.prism_ctx_timer_end(self.handle, byref(ms)))
         return ms.value
 
+    def kernel_timing(self, enable: bool) -> None:
+        """Bracket the library's main kernels with HIP events (see the
+        prism_ctx_kernel_timing names) until disabled."""
+        L.check(self.lib.prism_ctx_kernel_timing(self.handle, 1 if enable else 0))
+
+    def kernel_time(self, name: str) -> tuple:
+        """(summed ms, launches) of one timed kernel since the last read."""
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        L.check(self.lib.prism_ctx_kernel_timing_read(self.handle, name.encode(), byref(ms), byref(n)))
+        return ms.value, n.value
+
     def close(self) -> None:
         if self.handle:
             self.lib.prism_ctx_destroy(self.handle)

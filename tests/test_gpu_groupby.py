@@ -210,3 +210,71 @@ def test_exact_overflow_is_an_error(ctx):
         gpu_groupby(ctx, [I64], [D.col_i64(x)], [], [[("LOAD_I", 0), ("CONST_I", 0, 4), ("MUL_I",)]],
                     [("SUM", 0)], 2)
     assert e.value.kind == "Execution"
+
+
+# The forms below pick the thread-private LDS accumulators (a small expected
+# group count and few accumulator words), its overflow into the global table
+# (expected count below the real one), and the prefix that measures the group
+# count when none is given (more than two sample prefixes of rows).
+@pytest.mark.parametrize("n,expected", [(2_000_000, 8), (2_000_000, 4), (3_000_000, 0), (2_000_000, 2)])
+def test_config2_shape_all_forms(ctx, n, expected):
+    say(f"config2 shape n={n} expected={expected}")
+    d, dev, cols = _config2_cols(n, seed=5 + expected)
+    out = gpu_groupby(ctx, TYPES2, cols, [4, 5], [EXACT2], [("SUM", 0), ("COUNT", 0), ("AVG", 0)], n,
+                      filt=FILTER2, expected=expected)
+    keep = d["date"] <= 2470
+    rev = d["price"] * (100 - d["discount"])
+    a, b = d["flag_a"][0], d["flag_b"][0]
+    assert out["row_count"].sum() == keep.sum() and len(out["first_row"]) == 6
+    firsts = []
+    for i, fr in enumerate(out["first_row"]):
+        sel = keep & (a == a[fr]) & (b == b[fr])
+        firsts.append(int(np.nonzero(sel)[0][0]))
+        s = exact_sum(rev[sel])
+        assert out["sum128"][0][i] == s
+        assert out["i64"][1][i] == sel.sum()
+        assert out["f64"][2][i] == float(s) / float(sel.sum())
+    assert firsts == list(out["first_row"]) and firsts == sorted(firsts)
+
+
+def test_private_form_nullable_keys(ctx):
+    say("private form, nullable utf8 + int32 keys")
+    rng = np.random.default_rng(11)
+    n = 300_000
+    words = ["yy", None]
+    s = pa.array([words[i] for i in rng.integers(0, 2, n)])
+    k32 = pa.array([None if i == 2 else int(i) for i in rng.integers(0, 3, n)], pa.int32())
+    val = rng.integers(-(1 << 40), 1 << 40, n)
+    rb = pa.record_batch([s, k32, pa.array(val)], names=["s", "k", "v"])
+    sc = orc.from_arrow(s)
+    sv = np.packbits(np.asarray(s.is_valid()), bitorder="little")
+    kv = np.packbits(np.asarray(k32.is_valid()), bitorder="little")
+    kvals = np.asarray(k32.fill_null(0)).astype(np.int32)
+    data_d, off_d, sv_d = to_dev(sc.data), to_dev(sc.offsets), to_dev(sv)
+    kd, kv_d, vd = to_dev(kvals), to_dev(kv), to_dev(val)
+    cols = [D.col_utf8(data_d, off_d, sv_d), D.col_i32(kd, kv_d), D.col_i64(vd)]
+    out = gpu_groupby(ctx, [UTF8, I32, I64], cols, [0, 1], [[("LOAD_I", 2)]], [("SUM", 0), ("COUNT", 0)], n,
+                      expected=6)
+    ref = orc.hash_aggregate([rb], [0, 1], [(2, "SUM"), (2, "COUNT")])
+    assert out["first_row"].tolist() == ref["first_row"].tolist()
+    assert out["key_hash"].tolist() == ref["key_hash"].tolist()
+    assert np.array_equal(out["i64"][1], ref["i64"][1])
+    assert list(out["sum128"][0]) == list(ref["sum128"][0])
+
+
+def test_private_form_global_aggregates(ctx):
+    say("private form, global aggregates with nulls")
+    rng = np.random.default_rng(12)
+    n = 500_000
+    vals = rng.integers(-10**12, 10**12, n)
+    valid = rng.random(n) > 0.1
+    v = pa.array(vals, mask=~valid)
+    rb = pa.record_batch([v], names=["v"])
+    vd, md = to_dev(vals), to_dev(np.packbits(valid, bitorder="little"))
+    out = gpu_groupby(ctx, [I64], [D.col_i64(vd, md)], [], [[("LOAD_I", 0)]],
+                      [("COUNT", 0), ("SUM", 0), ("AVG", 0), ("MIN", 0), ("MAX", 0)], n, expected=1)
+    ref = orc.hash_aggregate([rb], [], [(0, "COUNT"), (0, "SUM"), (0, "AVG"), (0, "MIN"), (0, "MAX")])
+    assert out["i64"][0][0] == ref["i64"][0][0] == int(valid.sum())
+    assert out["sum128"][1][0] == exact_sum(vals[valid])
+    for a in (3, 4):
+        assert out["f64"][a][0] == ref["f64"][a][0]
