@@ -160,7 +160,8 @@ typedef struct {
 #define PRISM_OP_MUL_I 13
 #define PRISM_OP_NEG_I 14
 #define PRISM_OP_CMP_I 15      /* pop b, a; push (a <arg> b) on int64; arg = PRISM_CMP_* */
-#define PRISM_OP_CMP_F 16      /* same on f64 (IEEE: NaN compares false except NE) */
+#define PRISM_OP_CMP_F 16      /* same on f64 in IEEE total order (f64::total_cmp, as arrow-ord's
+                                  cmp kernels compare floats): NaN == NaN, -0.0 < +0.0 */
 #define PRISM_OP_CMP_STR 17    /* push (utf8 column[arg] <i64 >> 8 cmp> string[i64 & 0xff]) */
 #define PRISM_OP_LIKE 18       /* push (utf8 column[arg] LIKE pattern string[i64]) */
 #define PRISM_OP_ILIKE 19      /* ASCII case-insensitive LIKE */
@@ -172,6 +173,7 @@ typedef struct {
 #define PRISM_OP_CONST_BOOL 25 /* push i64 != 0 */
 #define PRISM_OP_SELECT 26     /* pop else, then, cond; push cond ? then : else (zip) */
 #define PRISM_OP_CONST_NULL 27 /* push a null */
+#define PRISM_OP_ZIP 28        /* pop then, cond, else; push cond ? then : else (else pushed first) */
 
 #define PRISM_CMP_EQ 0
 #define PRISM_CMP_NE 1
@@ -402,6 +404,114 @@ prism_status prism_hash_aggregate(prism_ctx* ctx, const struct ArrowArray* batch
 prism_status prism_hash_aggregate_batches(prism_ctx* ctx, const struct ArrowArray* const* batches, uint32_t n_batches,
                                           const struct ArrowSchema* schema, const prism_hash_agg_config* cfg,
                                           struct ArrowArray* out, struct ArrowSchema* out_schema);
+
+/* ------------------------------------------------------------------------- */
+/* Arrow layer: filter / project (filter_project.rs:20-333) and LIKE          */
+/* (string_ops.rs:16-25).  Predicate and ScalarExpr trees travel as node     */
+/* arrays; a node refers to its children by index into the same array.      */
+/* ------------------------------------------------------------------------- */
+/* ScalarValue (filter_project.rs:54-62) */
+#define PRISM_SCALAR_INT32 0
+#define PRISM_SCALAR_INT64 1
+#define PRISM_SCALAR_FLOAT64 2
+#define PRISM_SCALAR_UTF8 3
+#define PRISM_SCALAR_BOOLEAN 4
+#define PRISM_SCALAR_DATE32 5
+typedef struct {
+  int32_t type;     /* PRISM_SCALAR_* */
+  int32_t reserved;
+  int64_t i64;      /* Int32 / Int64 / Boolean / Date32 */
+  double f64;       /* Float64 */
+  const char* str;  /* Utf8 bytes */
+  uint32_t str_len;
+  uint32_t reserved2;
+} prism_scalar;
+
+/* Predicate (filter_project.rs:21-51) */
+#define PRISM_PRED_LITERAL 0
+#define PRISM_PRED_EQ 1
+#define PRISM_PRED_NE 2
+#define PRISM_PRED_LT 3
+#define PRISM_PRED_LE 4
+#define PRISM_PRED_GT 5
+#define PRISM_PRED_GE 6
+#define PRISM_PRED_IS_NULL 7
+#define PRISM_PRED_IS_NOT_NULL 8
+#define PRISM_PRED_AND 9
+#define PRISM_PRED_OR 10
+#define PRISM_PRED_NOT 11
+#define PRISM_PRED_LIKE 12
+#define PRISM_PRED_ILIKE 13
+typedef struct {
+  int32_t kind;          /* PRISM_PRED_* */
+  uint32_t column;       /* comparisons, IS [NOT] NULL, LIKE */
+  int32_t left, right;   /* AND / OR children; NOT uses left */
+  prism_scalar value;    /* comparisons: the literal; LITERAL: i64 != 0 */
+  const char* pattern;   /* LIKE / ILIKE */
+  uint32_t pattern_len;
+  uint32_t reserved;
+} prism_pred_node;
+
+/* ScalarExpr (filter_project.rs:185-213) */
+#define PRISM_EXPR_COLUMN 0
+#define PRISM_EXPR_LITERAL 1
+#define PRISM_EXPR_BINARY 2
+#define PRISM_EXPR_NEGATE 3
+#define PRISM_EXPR_IF_THEN 4
+/* ArithmeticOp (filter_project.rs:185-192) */
+#define PRISM_ARITH_ADD 0
+#define PRISM_ARITH_SUB 1
+#define PRISM_ARITH_MUL 2
+#define PRISM_ARITH_DIV 3
+typedef struct {
+  int32_t kind;              /* PRISM_EXPR_* */
+  uint32_t column;           /* COLUMN */
+  prism_scalar value;        /* LITERAL */
+  int32_t op;                /* BINARY: PRISM_ARITH_* */
+  int32_t left, right;       /* BINARY children; NEGATE uses left (expression indices) */
+  const int32_t* clause_preds; /* IF_THEN: predicate roots (indices into the predicate nodes) */
+  const int32_t* clause_exprs; /* IF_THEN: THEN expressions (indices into the expression nodes) */
+  uint32_t n_clauses;
+  int32_t else_expr;         /* IF_THEN: ELSE expression index, or -1 */
+} prism_expr_node;
+
+/* evaluate_predicate (filter_project.rs:64-123): BooleanArray mask.  nodes[0]
+ * is the root. */
+prism_status prism_evaluate_predicate(prism_ctx* ctx, const struct ArrowArray* batch, const struct ArrowSchema* schema,
+                                      const prism_pred_node* nodes, uint32_t n_nodes, struct ArrowArray* out_mask,
+                                      struct ArrowSchema* out_schema);
+/* filter_batch (filter_project.rs:125-135): rows whose mask entry is true
+ * (null = false), every column. */
+prism_status prism_filter_batch(prism_ctx* ctx, const struct ArrowArray* batch, const struct ArrowSchema* schema,
+                                const struct ArrowArray* mask, const struct ArrowSchema* mask_schema,
+                                struct ArrowArray* out, struct ArrowSchema* out_schema);
+/* project_batch (filter_project.rs:137-160): the selected columns in order;
+ * n_indices = 0 gives a batch with no columns and the input row count. */
+prism_status prism_project_batch(prism_ctx* ctx, const struct ArrowArray* batch, const struct ArrowSchema* schema,
+                                 const uint32_t* indices, uint32_t n_indices, struct ArrowArray* out,
+                                 struct ArrowSchema* out_schema);
+/* filter_and_project (filter_project.rs:162-180): nodes may be NULL (no
+ * predicate); n_indices = 0 keeps every column.  Only the projected columns
+ * are compacted on the device. */
+prism_status prism_filter_and_project(prism_ctx* ctx, const struct ArrowArray* batch, const struct ArrowSchema* schema,
+                                      const prism_pred_node* nodes, uint32_t n_nodes, const uint32_t* indices,
+                                      uint32_t n_indices, struct ArrowArray* out, struct ArrowSchema* out_schema);
+/* evaluate_scalar_expr (filter_project.rs:215-294): expression root `root`. */
+prism_status prism_evaluate_scalar_expr(prism_ctx* ctx, const struct ArrowArray* batch,
+                                        const struct ArrowSchema* schema, const prism_expr_node* exprs,
+                                        uint32_t n_exprs, const prism_pred_node* preds, uint32_t n_preds,
+                                        int32_t root, struct ArrowArray* out, struct ArrowSchema* out_schema);
+/* project_batch_with_exprs (filter_project.rs:304-333): passthrough columns,
+ * then one column "expr_{i}" per expression root. */
+prism_status prism_project_batch_with_exprs(prism_ctx* ctx, const struct ArrowArray* batch,
+                                            const struct ArrowSchema* schema, const uint32_t* indices,
+                                            uint32_t n_indices, const prism_expr_node* exprs, uint32_t n_exprs,
+                                            const int32_t* roots, uint32_t n_roots, const prism_pred_node* preds,
+                                            uint32_t n_preds, struct ArrowArray* out, struct ArrowSchema* out_schema);
+/* string_like / string_ilike (string_ops.rs:16-25) over a Utf8 array. */
+prism_status prism_string_like(prism_ctx* ctx, const struct ArrowArray* array, const struct ArrowSchema* schema,
+                               const char* pattern, uint32_t pattern_len, int32_t case_insensitive,
+                               struct ArrowArray* out, struct ArrowSchema* out_schema);
 
 /* ------------------------------------------------------------------------- */
 /* Synthetic inputs (seeded, counter-based; bit-identical to the numpy twin    */

@@ -32,6 +32,7 @@ static const char* op_name(int op) {
     case PRISM_OP_CONST_BOOL: return "CONST_BOOL";
     case PRISM_OP_SELECT: return "SELECT";
     case PRISM_OP_CONST_NULL: return "CONST_NULL";
+    case PRISM_OP_ZIP: return "ZIP";
     default: return "?";
   }
 }
@@ -139,6 +140,15 @@ ValType check_program(const prism_instr* code, uint32_t len, const int32_t* col_
         if (st[sp - 1] != VT_B && st[sp - 1] != VT_NULL) bad(pc, "operand is not boolean");
         st[sp - 1] = VT_B;
         break;
+      case PRISM_OP_ZIP: {
+        need(pc, 3);
+        ValType e = st[sp - 3], c = st[sp - 2], a = st[sp - 1];
+        if (c != VT_B && c != VT_NULL) bad(pc, "condition is not boolean");
+        if (!(a == e || a == VT_NULL || e == VT_NULL)) bad(pc, "branches differ in type");
+        sp -= 2;
+        st[sp - 1] = a == VT_NULL ? e : a;
+        break;
+      }
       case PRISM_OP_SELECT: {
         need(pc, 3);
         ValType c = st[sp - 3], a = st[sp - 2], b = st[sp - 1];

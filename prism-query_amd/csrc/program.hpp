@@ -104,19 +104,14 @@ __device__ __forceinline__ bool cmp_i(int op, int64_t a, int64_t b) {
     default: return a >= b;
   }
 }
-__device__ __forceinline__ bool cmp_f(int op, double a, double b) {
-  switch (op) {
-    case PRISM_CMP_EQ: return a == b;
-    case PRISM_CMP_NE: return a != b;
-    case PRISM_CMP_LT: return a < b;
-    case PRISM_CMP_LE: return a <= b;
-    case PRISM_CMP_GT: return a > b;
-    default: return a >= b;
-  }
+// IEEE total order (f64::total_cmp): negative values have every bit but the
+// sign flipped, then the words compare as signed integers.
+__device__ __forceinline__ int64_t total_key(double d) {
+  const int64_t b = static_cast<int64_t>(__double_as_longlong(d));
+  return b ^ static_cast<int64_t>(static_cast<uint64_t>(b >> 63) >> 1);
 }
+__device__ __forceinline__ bool cmp_f(int op, double a, double b) { return cmp_i(op, total_key(a), total_key(b)); }
 
-// Byte-wise lexicographic compare of a utf8 value with a literal
-// (Rust str ordering = byte ordering).
 __device__ __forceinline__ int str_cmp(const uint8_t* a, int32_t na, const uint8_t* b, int32_t nb) {
   int32_t n = na < nb ? na : nb;
   for (int32_t i = 0; i < n; ++i) {
@@ -401,6 +396,20 @@ __device__ __forceinline__ void run_program(const prism_instr* __restrict__ code
       case PRISM_OP_NOT: {
 #pragma unroll
         for (int r = 0; r < R; ++r) st[(sp - 1) * R + r] = st[(sp - 1) * R + r] ^ 1;
+        break;
+      }
+      case PRISM_OP_ZIP: {
+        // stack: else, cond, then  ->  cond ? then : else (a null cond picks else)
+        sp -= 2;
+        const uint32_t cv = vget(sp);
+        uint32_t v = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const bool take = ((cv >> r) & 1) && (st[sp * R + r] & 1);
+          st[(sp - 1) * R + r] = take ? st[(sp + 1) * R + r] : st[(sp - 1) * R + r];
+          v |= ((take ? (vget(sp + 1) >> r) : (vget(sp - 1) >> r)) & 1) ? (1u << r) : 0u;
+        }
+        vset(sp - 1, v);
         break;
       }
       case PRISM_OP_SELECT: {

@@ -31,7 +31,7 @@ TYPE_INT32, TYPE_INT64, TYPE_FLOAT64, TYPE_UTF8, TYPE_BOOL, TYPE_DATE32 = 1, 2, 
 OP = dict(LOAD_I=1, LOAD_F=2, CONST_I=3, CONST_F=4, I2F=5, ADD_F=6, SUB_F=7, MUL_F=8, DIV_F=9,
           NEG_F=10, ADD_I=11, SUB_I=12, MUL_I=13, NEG_I=14, CMP_I=15, CMP_F=16, CMP_STR=17,
           LIKE=18, ILIKE=19, IS_NULL=20, IS_NOT_NULL=21, AND=22, OR=23, NOT=24, CONST_BOOL=25,
-          SELECT=26, CONST_NULL=27)
+          SELECT=26, CONST_NULL=27, ZIP=28)
 CMP = dict(EQ=0, NE=1, LT=2, LE=3, GT=4, GE=5)
 AGG = dict(SUM=0, COUNT=1, AVG=2, MIN=3, MAX=4, COUNT_DISTINCT=5)
 TABLE_AUTO, TABLE_HASHED, TABLE_DENSE = 0, 1, 2
@@ -131,6 +131,26 @@ class HashAggConfigC(Structure):
                 ("n_aggregates", c_uint32), ("flags", c_uint32)]
 
 
+class Scalar(Structure):
+    _fields_ = [("type", c_int32), ("reserved", c_int32), ("i64", c_int64), ("f64", c_double),
+                ("str", c_char_p), ("str_len", c_uint32), ("reserved2", c_uint32)]
+
+
+class PredNode(Structure):
+    _fields_ = [("kind", c_int32), ("column", c_uint32), ("left", c_int32), ("right", c_int32),
+                ("value", Scalar), ("pattern", c_char_p), ("pattern_len", c_uint32), ("reserved", c_uint32)]
+
+
+class ExprNode(Structure):
+    _fields_ = [("kind", c_int32), ("column", c_uint32), ("value", Scalar), ("op", c_int32), ("left", c_int32),
+                ("right", c_int32), ("clause_preds", POINTER(c_int32)), ("clause_exprs", POINTER(c_int32)),
+                ("n_clauses", c_uint32), ("else_expr", c_int32)]
+
+
+SCALAR = dict(INT32=0, INT64=1, FLOAT64=2, UTF8=3, BOOLEAN=4, DATE32=5)
+PRED = dict(LITERAL=0, EQ=1, NE=2, LT=3, LE=4, GT=5, GE=6, IS_NULL=7, IS_NOT_NULL=8, AND=9, OR=10, NOT=11,
+            LIKE=12, ILIKE=13)
+EXPR = dict(COLUMN=0, LITERAL=1, BINARY=2, NEGATE=3, IF_THEN=4)
 JOIN = dict(INNER=0, LEFT=1, RIGHT=2, FULL=3, LEFT_SEMI=4, LEFT_ANTI=5)
 AGG_FLAG_EXACT_DECIMAL = 1
 
@@ -184,6 +204,20 @@ _SIGNATURES = [
     ("prism_hash_aggregate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HashAggConfigC), c_void_p, c_void_p]),
     ("prism_hash_aggregate_batches", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p,
                                                POINTER(HashAggConfigC), c_void_p, c_void_p]),
+    ("prism_evaluate_predicate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(PredNode), c_uint32, c_void_p,
+                                           c_void_p]),
+    ("prism_filter_batch", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("prism_project_batch", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_uint32), c_uint32, c_void_p,
+                                      c_void_p]),
+    ("prism_filter_and_project", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(PredNode), c_uint32,
+                                           POINTER(c_uint32), c_uint32, c_void_p, c_void_p]),
+    ("prism_evaluate_scalar_expr", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(ExprNode), c_uint32,
+                                             POINTER(PredNode), c_uint32, c_int32, c_void_p, c_void_p]),
+    ("prism_project_batch_with_exprs", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_uint32), c_uint32,
+                                                 POINTER(ExprNode), c_uint32, POINTER(c_int32), c_uint32,
+                                                 POINTER(PredNode), c_uint32, c_void_p, c_void_p]),
+    ("prism_string_like", c_int32, [c_void_p, c_void_p, c_void_p, c_char_p, c_uint32, c_int32, c_void_p,
+                                    c_void_p]),
     ("prism_datagen_uniform_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_int64,
                                             c_uint64]),
     ("prism_datagen_uniform_i32", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_int64,
