@@ -406,6 +406,32 @@ prism_status prism_hash_aggregate_batches(prism_ctx* ctx, const struct ArrowArra
                                           struct ArrowArray* out, struct ArrowSchema* out_schema);
 
 /* ------------------------------------------------------------------------- */
+/* Join -> aggregate by a build-side attribute without materialising pairs:  */
+/* the fused join-aggregate of the prism-substrait executor                  */
+/* (executor.rs:333-458) for                                                 */
+/*   SELECT b.attr, COUNT(*), SUM(p.payload) FROM probe p JOIN build b        */
+/*   ON p.key = b.key GROUP BY b.attr                                         */
+/* Exact (int128 sums, u64 counts).  Groups come out in first appearance     */
+/* over the join output (probe row ascending, then build row ascending).     */
+/* Device pointers; keys and attributes carry no nulls.  The build arrays    */
+/* must stay valid until finish.                                             */
+/* ------------------------------------------------------------------------- */
+typedef struct prism_join_agg prism_join_agg;
+prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, const int32_t* d_build_attr,
+                                   uint64_t n_build, prism_join_agg** out);
+/* Accumulate one probe batch; probe_row_base = global row of its first row.
+ * d_probe_payload may be NULL (sums stay 0). */
+prism_status prism_join_agg_probe(prism_join_agg* j, const int64_t* d_probe_keys, const int64_t* d_probe_payload,
+                                  uint64_t n_probe, uint64_t probe_row_base);
+prism_status prism_join_agg_finish(prism_join_agg* j, uint64_t* n_groups);
+/* Host arrays of n_groups entries (any may be NULL).  first_pair = (probe row
+ * << 32) | build row of the group's first join output row. */
+prism_status prism_join_agg_read(prism_join_agg* j, int32_t* attr, uint64_t* count, uint64_t* sum_lo,
+                                 int64_t* sum_hi, uint64_t* first_pair);
+prism_status prism_join_agg_reset(prism_join_agg* j);
+void prism_join_agg_destroy(prism_join_agg* j);
+
+/* ------------------------------------------------------------------------- */
 /* Arrow layer: filter / project (filter_project.rs:20-333) and LIKE          */
 /* (string_ops.rs:16-25).  Predicate and ScalarExpr trees travel as node     */
 /* arrays; a node refers to its children by index into the same array.      */

@@ -989,6 +989,268 @@ static void launch_probe_vec(prism_ctx* ctx, bool vec, dim3 g, const TableView& 
                        probe_base, ppay, bpay, op, ob, cap, status, tile_sums, res);
 }
 
+
+// ---------------------------------------------------------------------------
+// Join -> GROUP BY build attribute, COUNT(*), SUM(probe payload), without
+// materialising pairs (the fused join-aggregate of the prism-substrait
+// executor, executor.rs:333-458).  Eager aggregation:
+//   1. every probe row adds (1, payload, row) to its join key's slot
+//      (count, exact int128 sum, first probe row); hot keys are merged in an
+//      LDS cache first, so a Zipf head does not serialise on global atomics;
+//   2. every build row adds its key's (count, sum) to its attribute's group,
+//      and offers (first probe row, build row) as the group's first pair.
+// Groups come out in first appearance over the join output (probe row
+// ascending, build row ascending within a probe row), the order in which
+// hash_aggregate meets them in the reference.
+// ---------------------------------------------------------------------------
+constexpr int kJaBlock = 256;
+constexpr int kJaCache = 1024;   // LDS cache entries per workgroup
+constexpr int kJaProbes = 8;
+constexpr uint32_t kJaEmpty = 0xFFFFFFFFu;
+
+// Slot of a key in the table (DENSE: key offset; HASHED: slot index, the
+// INT64_MIN key at mask + 1), or -1 when no build row has the key.
+template <int KIND>
+__device__ __forceinline__ int64_t slot_of(const TableView& T, int64_t k, bool unique) {
+  if constexpr (KIND == PRISM_TABLE_DENSE) {
+    const uint64_t d = static_cast<uint64_t>(k) - static_cast<uint64_t>(T.min_key);
+    if (d >= T.range) return -1;
+    if (unique) return T.dense_u[d] != kNone ? static_cast<int64_t>(d) : -1;
+    return T.dense_nu[d].y != 0 ? static_cast<int64_t>(d) : -1;
+  } else {
+    if (k == kEmpty) return T.sentinel_present ? static_cast<int64_t>(T.mask + 1) : -1;
+    uint64_t s = hash_i64_fast(k) >> T.shift;
+    while (true) {
+      const int64_t key = T.slots[s].key;
+      if (key == k) return static_cast<int64_t>(s);
+      if (key == kEmpty) return -1;
+      s = (s + 1) & T.mask;
+    }
+  }
+}
+
+struct JaAcc {
+  unsigned long long* cnt;    // per key slot
+  unsigned long long* sum;    // per key slot: lo, hi words
+  unsigned long long* first;  // per key slot: first probe row
+};
+
+template <int KIND>
+__global__ __launch_bounds__(kJaBlock) void k_ja_probe(TableView T, bool unique, const int64_t* __restrict__ keys,
+                                                       const int64_t* __restrict__ pay, uint64_t n, uint64_t row_base,
+                                                       JaAcc A) {
+  __shared__ uint32_t ck[kJaCache];
+  __shared__ unsigned long long cc[kJaCache], clo[kJaCache], cf[kJaCache];
+  __shared__ long long chi[kJaCache];
+  for (int i = threadIdx.x; i < kJaCache; i += kJaBlock) {
+    ck[i] = kJaEmpty;
+    cc[i] = 0;
+    clo[i] = 0;
+    chi[i] = 0;
+    cf[i] = ~0ULL;
+  }
+  __syncthreads();
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kJaBlock;
+  for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kJaBlock) + threadIdx.x; r < n; r += stride) {
+    const int64_t s = slot_of<KIND>(T, keys[r], unique);
+    if (s < 0) continue;
+    const int64_t x = pay ? pay[r] : 0;
+    const unsigned long long row = row_base + r;
+    uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(s) * 0x9E3779B97F4A7C15ULL) >> 54);  // 10 bits
+    int c = -1;
+    for (int p = 0; p < kJaProbes; ++p) {
+      const uint32_t cur = __hip_atomic_load(&ck[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (cur == static_cast<uint32_t>(s)) {
+        c = static_cast<int>(h);
+        break;
+      }
+      if (cur == kJaEmpty) {
+        const uint32_t old = atomicCAS(&ck[h], kJaEmpty, static_cast<uint32_t>(s));
+        if (old == kJaEmpty || old == static_cast<uint32_t>(s)) {
+          c = static_cast<int>(h);
+          break;
+        }
+      }
+      h = (h + 1) & (kJaCache - 1);
+    }
+    if (c >= 0) {
+      atomicAdd(&cc[c], 1ULL);
+      const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(x));
+      const long long hi = x >> 32;
+      if (lo) atomicAdd(&clo[c], static_cast<unsigned long long>(lo));
+      if (hi) atomicAdd(reinterpret_cast<unsigned long long*>(&chi[c]), static_cast<unsigned long long>(hi));
+      if (row < __hip_atomic_load(&cf[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(&cf[c], row);
+    } else {
+      atomicAdd(&A.cnt[s], 1ULL);
+      atomic_add_i128(&A.sum[2 * s], &A.sum[2 * s + 1], static_cast<uint64_t>(x), x >> 63);
+      atomicMin(&A.first[s], row);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < kJaCache; c += kJaBlock) {
+    if (ck[c] == kJaEmpty) continue;
+    const uint64_t s = ck[c];
+    atomicAdd(&A.cnt[s], cc[c]);
+    Split128 sp;
+    sp.lo32 = clo[c];
+    sp.hi32 = chi[c];
+    uint64_t lo;
+    int64_t hi;
+    sp.to_i128(lo, hi);
+    if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&A.sum[2 * s], &A.sum[2 * s + 1], lo, hi);
+    atomicMin(&A.first[s], cf[c]);
+  }
+}
+
+struct JaGroups {
+  unsigned long long* cnt;    // per group
+  unsigned long long* sum;    // per group: lo, hi
+  unsigned long long* first;  // per group: (probe row << 32) | build row
+  unsigned long long* key;    // HASHED groups: attribute + 2^32 (0 = empty); null for DENSE
+  int64_t attr_min;
+  uint64_t n;                 // DENSE: attribute range; HASHED: slots (power of two)
+  uint32_t* flags;
+};
+
+__device__ __forceinline__ int64_t ja_group(const JaGroups& G, int32_t attr) {
+  if (!G.key) return static_cast<int64_t>(static_cast<int64_t>(attr) - G.attr_min);
+  const unsigned long long want = static_cast<unsigned long long>(static_cast<uint32_t>(attr)) + (1ULL << 32);
+  uint64_t s = (static_cast<uint64_t>(want) * 0x9E3779B97F4A7C15ULL) & (G.n - 1);
+  for (uint64_t p = 0; p < G.n; ++p) {
+    const unsigned long long cur = __hip_atomic_load(&G.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == want) return static_cast<int64_t>(s);
+    if (cur == 0) {
+      const unsigned long long old = atomicCAS(&G.key[s], 0ULL, want);
+      if (old == 0 || old == want) return static_cast<int64_t>(s);
+    }
+    s = (s + 1) & (G.n - 1);
+  }
+  atomicOr(G.flags, 1u);
+  return -1;
+}
+
+template <int KIND>
+__global__ void k_ja_build(TableView T, bool unique, const int64_t* __restrict__ bkeys,
+                           const int32_t* __restrict__ battr, uint64_t n_build, JaAcc A, JaGroups G) {
+  for (uint64_t b = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; b < n_build;
+       b += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const int64_t s = slot_of<KIND>(T, bkeys[b], unique);
+    if (s < 0) continue;
+    const unsigned long long c = A.cnt[s];
+    if (c == 0) continue;
+    const int64_t g = ja_group(G, battr[b]);
+    if (g < 0) continue;
+    atomicAdd(&G.cnt[g], c);
+    const uint64_t lo = A.sum[2 * s];
+    const int64_t hi = static_cast<int64_t>(A.sum[2 * s + 1]);
+    if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&G.sum[2 * g], &G.sum[2 * g + 1], lo, hi);
+    atomicMin(&G.first[g], (A.first[s] << 32) | b);
+  }
+}
+
+__global__ void k_ja_collect(JaGroups G, unsigned long long* n_found, uint64_t* first_out, uint32_t* idx_out) {
+  for (uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; g < G.n;
+       g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (G.cnt[g] == 0) continue;
+    const unsigned long long i = atomicAdd(n_found, 1ULL);
+    first_out[i] = G.first[g];
+    idx_out[i] = static_cast<uint32_t>(g);
+  }
+}
+
+// Output row i: attribute, count, sum lo, sum hi, first pair.
+__global__ void k_ja_gather(JaGroups G, const uint32_t* __restrict__ order, uint64_t n, int32_t* attr,
+                            unsigned long long* cnt, unsigned long long* sum_lo, long long* sum_hi,
+                            unsigned long long* first) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t g = order[i];
+    attr[i] = G.key ? static_cast<int32_t>(static_cast<uint32_t>(G.key[g])) : static_cast<int32_t>(G.attr_min + g);
+    cnt[i] = G.cnt[g];
+    sum_lo[i] = G.sum[2 * g];
+    sum_hi[i] = static_cast<long long>(G.sum[2 * g + 1]);
+    first[i] = G.first[g];
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// min / max of an int32 column (two stages, no contended atomics).
+__global__ void k_minmax_i32_part(const int32_t* __restrict__ v, uint64_t n, int64_t* __restrict__ part) {
+  __shared__ int64_t lds[16];
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const int64_t x = v[i];
+    mn = x < mn ? x : mn;
+    mx = x > mx ? x : mx;
+  }
+  mn = block_reduce(mn, MinOp(), lds);
+  mx = block_reduce(mx, MaxOp(), lds);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = mn;
+    part[2 * blockIdx.x + 1] = mx;
+  }
+}
+
+}  // namespace join
+}  // namespace prism
+
+struct prism_join_agg {
+  prism::Ctx* ctx = nullptr;
+  prism_join_table* table = nullptr;
+  const int64_t* build_keys = nullptr;
+  const int32_t* build_attr = nullptr;
+  uint64_t n_build = 0;
+  uint64_t n_slots = 0;
+  prism::DeviceBuffer acc;     // cnt | sum (2 words) | first, per key slot
+  prism::DeviceBuffer groups;  // cnt | sum | first | key, per group
+  prism::DeviceBuffer counters;
+  int64_t attr_min = 0;
+  uint64_t n_groups_cap = 0;
+  bool dense_groups = true;
+  prism::DeviceBuffer result;  // finished rows: attr(i32) cnt sum_lo sum_hi first
+  uint64_t n_result = 0;
+  bool finished = false;
+
+  prism::join::JaAcc accs() {
+    prism::join::JaAcc A;
+    A.cnt = acc.as<unsigned long long>();
+    A.sum = A.cnt + n_slots;
+    A.first = A.sum + 2 * n_slots;
+    return A;
+  }
+  prism::join::JaGroups groups_view() {
+    prism::join::JaGroups G;
+    G.cnt = groups.as<unsigned long long>();
+    G.sum = G.cnt + n_groups_cap;
+    G.first = G.sum + 2 * n_groups_cap;
+    G.key = dense_groups ? nullptr : G.first + n_groups_cap;
+    G.attr_min = attr_min;
+    G.n = n_groups_cap;
+    G.flags = counters.as<uint32_t>() + 2;
+    return G;
+  }
+};
+
+namespace prism {
+namespace join {
+
+static void ja_clear(prism_join_agg* J) {
+  hipStream_t st = J->ctx->stream;
+  const uint64_t S = J->n_slots;
+  unsigned long long* a = J->acc.as<unsigned long long>();
+  hipLaunchKernelGGL(k_fill_u64, grid_for(J->ctx, 3 * S), dim3(kBuildBlock), 0, st, a, 3 * S, 0ULL);
+  hipLaunchKernelGGL(k_fill_u64, grid_for(J->ctx, S), dim3(kBuildBlock), 0, st, a + 3 * S, S, ~0ULL);
+  PRISM_CHECK_LAUNCH();
+  J->finished = false;
+  J->n_result = 0;
+}
+
 }  // namespace join
 }  // namespace prism
 
@@ -1266,6 +1528,194 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
       PRISM_CHECK_LAUNCH();
     }
   });
+}
+
+prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, const int32_t* d_build_attr,
+                                   uint64_t n_build, prism_join_agg** out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or output");
+    *out = nullptr;
+    if (n_build && (!d_build_keys || !d_build_attr))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL build keys or attribute");
+    auto J = std::make_unique<prism_join_agg>();
+    prism_join_table* t = nullptr;
+    prism_status st0 = build_impl(ctx, d_build_keys, nullptr, n_build, PRISM_TABLE_AUTO, &t);
+    if (st0 != PRISM_OK) fail(st0, prism_last_error());
+    J->table = t;
+    use_device(ctx);
+    Ctx* c = as_ctx(ctx);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    J->ctx = c;
+    J->build_keys = d_build_keys;
+    J->build_attr = d_build_attr;
+    J->n_build = n_build;
+    J->n_slots = t->info.kind == PRISM_TABLE_DENSE ? (t->info.slots ? t->info.slots : 1) : t->info.slots + 2;
+    if (J->n_slots >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: build side too large for the join aggregate");
+    J->acc.alloc(c, J->n_slots * 4 * 8);
+    J->counters.alloc(c, 64);
+    // attribute range: dense groups when it is small, else a hashed group table
+    hipStream_t st = ctx->stream;
+    int64_t lo = 0, hi = -1;
+    if (n_build) {
+      DeviceBuffer part(c, kRedBlocks * 16);
+      hipLaunchKernelGGL(k_minmax_i32_part, dim3(kRedBlocks), dim3(kBuildBlock), 0, st, d_build_attr, n_build,
+                         part.as<int64_t>());
+      PRISM_CHECK_LAUNCH();
+      std::vector<int64_t> h(kRedBlocks * 2);
+      PRISM_HIP(hipMemcpyAsync(h.data(), part.ptr, kRedBlocks * 16, hipMemcpyDeviceToHost, st));
+      PRISM_HIP(hipStreamSynchronize(st));
+      lo = INT64_MAX;
+      hi = INT64_MIN;
+      for (int i = 0; i < kRedBlocks; ++i) {
+        lo = h[2 * i] < lo ? h[2 * i] : lo;
+        hi = h[2 * i + 1] > hi ? h[2 * i + 1] : hi;
+      }
+    }
+    const uint64_t range = hi >= lo ? static_cast<uint64_t>(hi - lo) + 1 : 1;
+    J->attr_min = lo;
+    if (range <= (64ULL << 20) && range <= 4 * n_build + 1024) {
+      J->dense_groups = true;
+      J->n_groups_cap = range;
+      J->groups.alloc(c, range * 4 * 8);
+    } else {
+      J->dense_groups = false;
+      J->n_groups_cap = next_pow2(2 * n_build > 1024 ? 2 * n_build : 1024);
+      J->groups.alloc(c, J->n_groups_cap * 5 * 8);
+    }
+    ja_clear(J.get());
+    PRISM_HIP(hipStreamSynchronize(st));
+    *out = J.release();
+  });
+}
+
+prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys, const int64_t* d_probe_payload,
+                                  uint64_t n_probe, uint64_t probe_row_base) {
+  return guarded([&] {
+    if (!J) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL join aggregate");
+    if (n_probe && !d_probe_keys) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL probe keys");
+    if (n_probe + probe_row_base > 0xFFFFFFFFULL)
+      fail(PRISM_ERR_JOIN, "Join error: probe rows exceed u32 row indices (UInt32Array, hash_join.rs:68-69)");
+    use_device(J->ctx);
+    std::lock_guard<std::mutex> lock(J->ctx->mu);
+    J->finished = false;
+    if (n_probe == 0 || J->n_build == 0) return;
+    const TableView& T = J->table->view;
+    const bool unique = J->table->info.unique;
+    uint64_t blocks = ceil_div(n_probe, kJaBlock * 8);
+    const uint64_t cap = static_cast<uint64_t>(J->ctx->num_cus) * 4;
+    if (blocks > cap) blocks = cap;
+    KernelTimer kt(J->ctx, "join_agg_probe");
+    if (J->table->info.kind == PRISM_TABLE_DENSE)
+      hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_DENSE>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0,
+                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs());
+    else
+      hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0,
+                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs());
+    PRISM_CHECK_LAUNCH();
+  });
+}
+
+prism_status prism_join_agg_finish(prism_join_agg* J, uint64_t* n_groups) {
+  return guarded([&] {
+    if (!J || !n_groups) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL join aggregate/output");
+    use_device(J->ctx);
+    std::lock_guard<std::mutex> lock(J->ctx->mu);
+    Ctx* c = J->ctx;
+    hipStream_t st = c->stream;
+    const uint64_t G = J->n_groups_cap;
+    unsigned long long* g = J->groups.as<unsigned long long>();
+    hipLaunchKernelGGL(k_fill_u64, grid_for(c, 3 * G), dim3(kBuildBlock), 0, st, g, 3 * G, 0ULL);
+    hipLaunchKernelGGL(k_fill_u64, grid_for(c, G), dim3(kBuildBlock), 0, st, g + 3 * G, G, ~0ULL);
+    if (!J->dense_groups) hipLaunchKernelGGL(k_fill_u64, grid_for(c, G), dim3(kBuildBlock), 0, st, g + 4 * G, G, 0ULL);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, J->counters.as<unsigned long long>(), 8, 0ULL);
+    PRISM_CHECK_LAUNCH();
+    const TableView& T = J->table->view;
+    const bool unique = J->table->info.unique;
+    JaGroups GV = J->groups_view();
+    if (J->n_build) {
+      KernelTimer kt(c, "join_agg_build");
+      if (J->table->info.kind == PRISM_TABLE_DENSE)
+        hipLaunchKernelGGL(k_ja_build<PRISM_TABLE_DENSE>, grid_for(c, J->n_build), dim3(kBuildBlock), 0, st, T, unique,
+                           J->build_keys, J->build_attr, J->n_build, J->accs(), GV);
+      else
+        hipLaunchKernelGGL(k_ja_build<PRISM_TABLE_HASHED>, grid_for(c, J->n_build), dim3(kBuildBlock), 0, st, T,
+                           unique, J->build_keys, J->build_attr, J->n_build, J->accs(), GV);
+      PRISM_CHECK_LAUNCH();
+    }
+    DeviceBuffer first(c, G * 8), first_sorted(c, G * 8), idx(c, G * 4), idx_sorted(c, G * 4);
+    unsigned long long* found_p = J->counters.as<unsigned long long>();
+    hipLaunchKernelGGL(k_ja_collect, grid_for(c, G), dim3(kBuildBlock), 0, st, GV, found_p, first.as<uint64_t>(),
+                       idx.as<uint32_t>());
+    PRISM_CHECK_LAUNCH();
+    unsigned long long h[2];
+    PRISM_HIP(hipMemcpyAsync(h, J->counters.ptr, 16, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+    const uint64_t found = h[0];
+    if (static_cast<uint32_t>(h[1]) & 1u) fail(PRISM_ERR_INTERNAL, "Internal error: join aggregate group table full");
+    if (found > 1) {
+      size_t tmp = 0;
+      PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, first.as<uint64_t>(), first_sorted.as<uint64_t>(),
+                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(), static_cast<int>(found),
+                                                   0, 64, st));
+      DeviceBuffer t(c, tmp);
+      PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(t.ptr, tmp, first.as<uint64_t>(), first_sorted.as<uint64_t>(),
+                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(), static_cast<int>(found),
+                                                   0, 64, st));
+    } else if (found == 1) {
+      PRISM_HIP(hipMemcpyAsync(idx_sorted.ptr, idx.ptr, 4, hipMemcpyDeviceToDevice, st));
+    }
+    const uint64_t nr = found ? found : 1;
+    J->result.reserve(c, nr * (4 + 8 * 4) + 64);
+    char* r = J->result.as<char>();
+    int32_t* attr = reinterpret_cast<int32_t*>(r);
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(r + ((nr * 4 + 7) & ~7ULL));
+    if (found)
+      hipLaunchKernelGGL(k_ja_gather, grid_for(c, found), dim3(kBuildBlock), 0, st, GV, idx_sorted.as<uint32_t>(), found,
+                         attr, cnt, cnt + nr, reinterpret_cast<long long*>(cnt + 2 * nr), cnt + 3 * nr);
+    PRISM_CHECK_LAUNCH();
+    PRISM_HIP(hipStreamSynchronize(st));
+    J->n_result = found;
+    J->finished = true;
+    *n_groups = found;
+  });
+}
+
+prism_status prism_join_agg_read(prism_join_agg* J, int32_t* attr, uint64_t* count, uint64_t* sum_lo, int64_t* sum_hi,
+                                 uint64_t* first_pair) {
+  return guarded([&] {
+    if (!J) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL join aggregate");
+    if (!J->finished) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: call prism_join_agg_finish first");
+    use_device(J->ctx);
+    const uint64_t n = J->n_result;
+    if (!n) return;
+    const uint64_t nr = n;
+    hipStream_t st = J->ctx->stream;
+    const char* r = J->result.as<char>();
+    const unsigned long long* cnt = reinterpret_cast<const unsigned long long*>(r + ((nr * 4 + 7) & ~7ULL));
+    if (attr) PRISM_HIP(hipMemcpyAsync(attr, r, n * 4, hipMemcpyDeviceToHost, st));
+    if (count) PRISM_HIP(hipMemcpyAsync(count, cnt, n * 8, hipMemcpyDeviceToHost, st));
+    if (sum_lo) PRISM_HIP(hipMemcpyAsync(sum_lo, cnt + nr, n * 8, hipMemcpyDeviceToHost, st));
+    if (sum_hi) PRISM_HIP(hipMemcpyAsync(sum_hi, cnt + 2 * nr, n * 8, hipMemcpyDeviceToHost, st));
+    if (first_pair) PRISM_HIP(hipMemcpyAsync(first_pair, cnt + 3 * nr, n * 8, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+  });
+}
+
+prism_status prism_join_agg_reset(prism_join_agg* J) {
+  return guarded([&] {
+    if (!J) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL join aggregate");
+    use_device(J->ctx);
+    std::lock_guard<std::mutex> lock(J->ctx->mu);
+    ja_clear(J);
+  });
+}
+
+void prism_join_agg_destroy(prism_join_agg* J) {
+  if (!J) return;
+  (void)hipSetDevice(J->ctx ? J->ctx->device : 0);
+  if (J->ctx) (void)hipStreamSynchronize(J->ctx->stream);
+  if (J->table) prism_join_table_free(J->table);
+  delete J;
 }
 
 }  // extern "C"

@@ -204,6 +204,12 @@ _SIGNATURES = [
     ("prism_hash_aggregate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HashAggConfigC), c_void_p, c_void_p]),
     ("prism_hash_aggregate_batches", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p,
                                                POINTER(HashAggConfigC), c_void_p, c_void_p]),
+    ("prism_join_agg_create", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(c_void_p)]),
+    ("prism_join_agg_probe", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64]),
+    ("prism_join_agg_finish", c_int32, [c_void_p, POINTER(c_uint64)]),
+    ("prism_join_agg_read", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("prism_join_agg_reset", c_int32, [c_void_p]),
+    ("prism_join_agg_destroy", None, [c_void_p]),
     ("prism_evaluate_predicate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(PredNode), c_uint32, c_void_p,
                                            c_void_p]),
     ("prism_filter_batch", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -171,3 +171,19 @@ def filter_groupby(n: int, seed: int = SEED, start: int = 0):
         "flag_a": (a_bytes, a_off),
         "flag_b": (b_bytes, b_off),
     }
+
+
+def skewed_join(n_build: int, n_probe: int, n_values: int, n_attr: int, seed: int = SEED, s: float = 1.1,
+                start: int = 0, count: int | None = None):
+    """Build: keys Zipf(s) over n_values (seeded rank -> key permutation), int32
+    attribute U[0, n_attr).  Probe: keys Zipf(s) over the same values, int64
+    payload U[0, 10^6)."""
+    count = n_probe - start if count is None else count
+    cdf = zipf_cdf(n_values, s)
+    return {
+        "build_key": zipf_i64(n_build, seed, 30, cdf, 31),
+        "build_attr": uniform_i32(n_build, seed, 32, 0, n_attr),
+        "probe_key": zipf_i64(count, seed, 33, cdf, 31, start=start),
+        "probe_payload": uniform_i64(count, seed, 34, 0, 1_000_000, start=start),
+        "cdf": cdf,
+    }

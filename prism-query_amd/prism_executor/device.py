@@ -228,6 +228,56 @@ class JoinTable:
             pass
 
 
+class JoinAggregate:
+    """Join -> GROUP BY build attribute, COUNT(*), SUM(probe payload) without
+    materialising pairs (the fused join-aggregate path, prism-substrait
+    executor.rs:333-458).  Build arrays must stay alive until finish()."""
+
+    def __init__(self, ctx: Context, build_keys, build_attr):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.handle = c_void_p()
+        self._keep = (build_keys, build_attr)
+        L.check(self.lib.prism_join_agg_create(ctx.handle, ptr(build_keys), ptr(build_attr), build_keys.numel(),
+                                               byref(self.handle)))
+
+    def probe(self, keys, payload=None, probe_row_base: int = 0) -> None:
+        L.check(self.lib.prism_join_agg_probe(self.handle, ptr(keys), ptr(payload), keys.numel(), probe_row_base))
+
+    def finish(self) -> int:
+        n = c_uint64()
+        L.check(self.lib.prism_join_agg_finish(self.handle, byref(n)))
+        return n.value
+
+    def read(self, n_groups: int) -> dict:
+        k = max(n_groups, 1)
+        attr = np.zeros(k, np.int32)
+        cnt = np.zeros(k, np.uint64)
+        lo = np.zeros(k, np.uint64)
+        hi = np.zeros(k, np.int64)
+        first = np.zeros(k, np.uint64)
+        L.check(self.lib.prism_join_agg_read(self.handle, attr.ctypes.data, cnt.ctypes.data, lo.ctypes.data,
+                                             hi.ctypes.data, first.ctypes.data))
+        sums = [int(h) * (1 << 64) + int(l) for l, h in zip(lo[:n_groups].tolist(), hi[:n_groups].tolist())]
+        return {"attr": attr[:n_groups], "count": cnt[:n_groups], "sum": sums,
+                "first_probe_row": (first[:n_groups] >> np.uint64(32)).astype(np.int64),
+                "first_build_row": (first[:n_groups] & np.uint64(0xFFFFFFFF)).astype(np.int64)}
+
+    def reset(self) -> None:
+        L.check(self.lib.prism_join_agg_reset(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.prism_join_agg_destroy(self.handle)
+            self.handle = c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 # Group-by
 # ---------------------------------------------------------------------------
