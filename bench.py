@@ -136,6 +136,89 @@ def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
     return res
 
 
+def bench_join_agg(ctx, n_build: int, n_probe: int, n_values: int, n_attr: int, steps: int, cpu_sample: int) -> dict:
+    """Skewed join then aggregation by a build attribute: build keys Zipf(1.1)
+    over n_values with an int32 attribute U[0, n_attr), probe keys Zipf(1.1)
+    over the same values with an int64 payload U[0, 10^6); GROUP BY attribute,
+    COUNT(*), SUM(payload) exact.  Algorithmic bytes: 12 B per build row + 16 B
+    per probe row + 28 B per output group."""
+    import numpy as np
+
+    from prism_executor import _lib as L
+    from prism_executor import datagen as dg
+    from prism_executor import device as D
+
+    lib = ctx.lib
+    s = dg.SEED
+    cdf = D.to_device(ctx, dg.zipf_cdf(n_values, 1.1))
+    bk, ba = D.empty(ctx, n_build, np.int64), D.empty(ctx, n_build, np.int32)
+    pk, pp = D.empty(ctx, n_probe, np.int64), D.empty(ctx, n_probe, np.int64)
+    L.check(lib.prism_datagen_zipf_i64(ctx.handle, bk.ptr, n_build, s, 30, cdf.ptr, n_values, 31))
+    L.check(lib.prism_datagen_uniform_i32(ctx.handle, ba.ptr, n_build, s, 32, 0, n_attr))
+    L.check(lib.prism_datagen_zipf_i64(ctx.handle, pk.ptr, n_probe, s, 33, cdf.ptr, n_values, 31))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pp.ptr, n_probe, s, 34, 0, 1_000_000))
+    ctx.sync()
+
+    def step():
+        j = D.JoinAggregate(ctx, bk, ba)
+        j.probe(pk, pp, 0)
+        n = j.finish()
+        return j, n
+
+    j, n = step()
+    out = j.read(n)
+    j.close()
+    # full-size checksum: total pairs and payload over pairs = per-key probe
+    # totals summed over build rows
+    hk, hb = pk.numpy(), bk.numpy()
+    cnt_p = np.bincount(hk, minlength=n_values)
+    sum_p = np.bincount(hk, weights=pp.numpy().astype(np.float64), minlength=n_values)  # < 2^53: exact
+    exp_pairs = int(cnt_p[hb].astype(np.uint64).sum())
+    per_row = sum_p[hb].astype(np.int64)  # each < 2^53; total may pass 2^63, so sum in halves
+    exp_sum = int((per_row >> 32).sum()) * (1 << 32) + int((per_row & 0xFFFFFFFF).sum())
+    got_pairs = int(out["count"].astype(np.uint64).sum())
+    got_sum = sum(out["sum"])
+    ok = exp_pairs == got_pairs and exp_sum == got_sum and np.all(np.diff(out["first_probe_row"]) >= 0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        j, n = step()
+        j.close()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / steps
+    ctx.kernel_timing(True)
+    for _ in range(steps):
+        j, n = step()
+        j.close()
+    kern = {}
+    for name in ("join_agg_probe", "join_agg_build"):
+        tot, cnt = ctx.kernel_time(name)
+        kern[name] = {"avg_us": tot / max(cnt, 1) * 1e3, "launches": cnt}
+    ctx.kernel_timing(False)
+    alg = n_build * 12 + n_probe * 16 + n * 28
+    res = {"workload": f"skewed join then aggregation: build {n_build} rows (Zipf(1.1) keys over {n_values}, int32 "
+                       f"attribute U[0,{n_attr})), probe {n_probe} rows (Zipf(1.1) keys, int64 payload); GROUP BY "
+                       "attribute COUNT(*), SUM(payload) exact, eager (no pair materialisation)",
+           "groups": n, "join_pairs": got_pairs, "checksum_ok": bool(ok), "ms_per_step": wall * 1e3,
+           "probe_rows_per_s": n_probe / wall, "kernels": kern,
+           "roofline": {"bound": "hbm", "kernel": "whole query", "achieved": alg / wall / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": alg / wall / 1e9 / HBM_PEAK_GBS, "alg_bytes": alg}}
+    if cpu_sample:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        m = min(cpu_sample, n_probe)
+        hp = pp.numpy()[:m]
+        t0 = time.perf_counter()
+        orc.join_agg_eager(hb, ba.numpy(), hk[:m], hp, n_attr)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "probe rows/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle eager join-aggregate restatement, all {n_build} build rows + first "
+                                         f"{m} probe rows, one thread, {dt:.2f} s"}
+    for a in (cdf, bk, ba, pk, pp):
+        a.free()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +230,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--groupby-rows", type=int, default=200_000_000)
     ap.add_argument("--no-groupby", action="store_true")
+    ap.add_argument("--no-join-agg", action="store_true")
+    ap.add_argument("--join-agg-probe-rows", type=int, default=256_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="also run D2D copies of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -268,6 +353,10 @@ def main() -> int:
     if not args.no_groupby:
         line["groupby_filter"] = bench_groupby(ctx, args.groupby_rows, args.steps,
                                                0 if args.no_cpu_baseline else 10_000_000)
+    if not args.no_join_agg:
+        line["join_aggregate"] = bench_join_agg(ctx, 16_000_000, args.join_agg_probe_rows, 8_000_000, 1_000_000,
+                                                max(2, min(args.steps, 5)),
+                                                0 if args.no_cpu_baseline else 8_000_000)
     if args.calibrate_copy:
         src = D.empty(ctx, args.calibrate_copy, np.uint8)
         dst = D.empty(ctx, args.calibrate_copy, np.uint8)

@@ -57,6 +57,7 @@ struct TableView {
   const int32_t* row_payload;  // duplicates with payload: payload of rows[i]
   int has_payload;          // entries carry an int32 build payload
   int sentinel_present;     // HASHED: a build key equals INT64_MIN
+  int presence_only;        // DENSE: dense_u holds the key's build row count (0 = absent), no rows
 };
 
 }  // namespace join
@@ -1015,6 +1016,7 @@ __device__ __forceinline__ int64_t slot_of(const TableView& T, int64_t k, bool u
   if constexpr (KIND == PRISM_TABLE_DENSE) {
     const uint64_t d = static_cast<uint64_t>(k) - static_cast<uint64_t>(T.min_key);
     if (d >= T.range) return -1;
+    if (T.presence_only) return T.dense_u[d] != 0 ? static_cast<int64_t>(d) : -1;
     if (unique) return T.dense_u[d] != kNone ? static_cast<int64_t>(d) : -1;
     return T.dense_nu[d].y != 0 ? static_cast<int64_t>(d) : -1;
   } else {
@@ -1082,8 +1084,8 @@ __global__ __launch_bounds__(kJaBlock) void k_ja_probe(TableView T, bool unique,
       if (row < __hip_atomic_load(&cf[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(&cf[c], row);
     } else {
       atomicAdd(&A.cnt[s], 1ULL);
-      atomic_add_i128(&A.sum[2 * s], &A.sum[2 * s + 1], static_cast<uint64_t>(x), x >> 63);
-      atomicMin(&A.first[s], row);
+      if (x) atomic_add_i128(&A.sum[2 * s], &A.sum[2 * s + 1], static_cast<uint64_t>(x), x >> 63);
+      if (row < __hip_atomic_load(&A.first[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&A.first[s], row);
     }
   }
   __syncthreads();
@@ -1260,7 +1262,7 @@ using namespace prism::join;
 extern "C" {
 
 static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload, uint64_t n_rows,
-                               int32_t kind, prism_join_table** out) {
+                               int32_t kind, prism_join_table** out, bool keys_only = false) {
   return guarded([&] {
     if (!ctx || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or output");
     *out = nullptr;
@@ -1318,7 +1320,11 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       T->info.n_distinct = hs.n_distinct;
       T->info.max_dup = hs.max_dup;
       T->info.unique = hs.max_dup <= 1;
-      if (T->info.unique) {
+      if (keys_only) {  // key presence (and multiplicity) only: no row lists
+        T->dense = std::move(cnt);
+        T->view.dense_u = T->dense.as<uint32_t>();
+        T->view.presence_only = 1;
+      } else if (T->info.unique) {
         T->dense = std::move(cnt);  // reuse as the row array
         uint32_t* d = T->dense.as<uint32_t>();
         hipLaunchKernelGGL(k_fill_u32, grid_for(ctx, range), dim3(kBuildBlock), 0, st, d, range, kNone);
@@ -1362,7 +1368,9 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       T->info.n_distinct = hs.n_distinct;
       T->info.max_dup = hs.max_dup;
       T->info.unique = hs.max_dup <= 1;
-      if (T->info.unique) {
+      if (keys_only) {
+        // slots keep key and count; no row lists
+      } else if (T->info.unique) {
         if (n_rows)
           hipLaunchKernelGGL(k_hashed_unique_rows, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, slots,
                              slot_of_row.as<uint32_t>(), n_rows);
@@ -1539,7 +1547,7 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL build keys or attribute");
     auto J = std::make_unique<prism_join_agg>();
     prism_join_table* t = nullptr;
-    prism_status st0 = build_impl(ctx, d_build_keys, nullptr, n_build, PRISM_TABLE_AUTO, &t);
+    prism_status st0 = build_impl(ctx, d_build_keys, nullptr, n_build, PRISM_TABLE_AUTO, &t, true);
     if (st0 != PRISM_OK) fail(st0, prism_last_error());
     J->table = t;
     use_device(ctx);
