@@ -432,6 +432,85 @@ prism_status prism_join_agg_reset(prism_join_agg* j);
 void prism_join_agg_destroy(prism_join_agg* j);
 
 /* ------------------------------------------------------------------------- */
+/* Operator service (no counterpart in the reference; the north star's       */
+/* device-resident service for prism-jni): continuous batching of group-by,  */
+/* probe and filter requests into persistent tile-queue kernels, resident    */
+/* build tables with HBM admission and LRU eviction, graph replay of         */
+/* recurring batch shapes, an online tile/occupancy tuner, p50/p99 latency.  */
+/* Request inputs are device-resident; they must stay valid until the        */
+/* request completes.                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct prism_service prism_service;
+typedef struct {
+  uint64_t hbm_budget;          /* bytes for resident tables + working arenas (0: 64 GiB) */
+  uint32_t max_inflight;        /* batches on the device at once, one stream each (0: 3, max 8) */
+  uint32_t max_batch_requests;  /* requests per batch (0 or > 256: 256) */
+  uint64_t max_batch_rows;      /* rows per batch (0: 256 Mi) */
+  uint64_t arena_bytes;         /* working HBM per in-flight batch (0: 256 MiB) */
+  int32_t use_graphs;           /* replay captured launch sequences for recurring batch shapes */
+  int32_t autotune;             /* pick tile rows / workgroups per CU per kind from the first batches */
+} prism_service_config;
+
+#define PRISM_REQ_GROUPBY 1     /* target = group-by plan id */
+#define PRISM_REQ_PROBE 2       /* target = table id: matches + int128 payload sums */
+#define PRISM_REQ_FILTER 3      /* target = filter plan id: selected rows (+ optional bitmap) */
+typedef struct {
+  int32_t kind;
+  uint32_t target;
+  const prism_column* columns;    /* GROUPBY / FILTER: device columns, as many as the plan declares */
+  const int64_t* probe_keys;      /* PROBE */
+  const int64_t* probe_payload;   /* PROBE, may be NULL */
+  uint64_t n_rows;
+  unsigned long long* out_mask;   /* FILTER: ceil(n_rows / 64) device words, may be NULL */
+} prism_request;
+
+typedef struct {
+  int32_t status;       /* prism_status of the request */
+  int32_t kind;
+  uint64_t n_rows;
+  uint64_t count;       /* GROUPBY: groups; PROBE: matches; FILTER: selected rows */
+  uint64_t probe_sum_lo;
+  int64_t probe_sum_hi;
+  uint64_t build_sum_lo;
+  int64_t build_sum_hi;
+  double latency_ms;    /* submit -> result on the host */
+  double queue_ms;      /* submit -> admitted into a batch */
+  uint64_t batch_id;
+} prism_request_result;
+
+typedef struct {
+  uint64_t submitted, completed, batches, tables_built, tables_evicted, resident_table_bytes, graph_launches;
+  double p50_ms, p99_ms, mean_batch_requests;
+  uint32_t tile_rows[3];      /* per kind: group-by, probe, filter */
+  uint32_t blocks_per_cu[3];
+  int32_t tuned[3];
+} prism_service_stats;
+
+prism_status prism_service_create(prism_ctx* ctx, const prism_service_config* cfg, prism_service** out);
+void prism_service_destroy(prism_service* s);
+/* Resident build table with unique keys (host arrays are copied; the table is
+ * rebuilt from them after an eviction). */
+prism_status prism_service_register_table(prism_service* s, uint32_t table_id, const int64_t* h_keys,
+                                          const int32_t* h_payload, uint64_t n);
+/* Group-by plan (same plan as prism_groupby_create) holding at most max_groups
+ * (1..1024) groups per request. */
+prism_status prism_service_register_groupby(prism_service* s, uint32_t plan_id, const prism_groupby_plan* plan,
+                                            uint64_t max_groups);
+/* Filter plan: a boolean row program (the compiled form of a Predicate). */
+prism_status prism_service_register_filter(prism_service* s, uint32_t plan_id, const prism_instr* code, uint32_t n_code,
+                                           const prism_string* strings, uint32_t n_strings, const int32_t* column_types,
+                                           uint32_t n_columns);
+prism_status prism_service_submit(prism_service* s, const prism_request* req, uint64_t* ticket);
+prism_status prism_service_poll(prism_service* s, uint64_t ticket, int32_t* done);
+prism_status prism_service_wait(prism_service* s, uint64_t ticket, prism_request_result* result);
+/* Groups of a finished GROUPBY request, as prism_groupby_read writes them. */
+prism_status prism_service_read_groups(prism_service* s, uint64_t ticket, const prism_groupby_output* out);
+prism_status prism_service_release(prism_service* s, uint64_t ticket);
+prism_status prism_service_drain(prism_service* s);
+prism_status prism_service_stats_get(prism_service* s, prism_service_stats* out);
+prism_status prism_service_reset_stats(prism_service* s);
+
+/* ------------------------------------------------------------------------- */
 /* Arrow layer: filter / project (filter_project.rs:20-333) and LIKE          */
 /* (string_ops.rs:16-25).  Predicate and ScalarExpr trees travel as node     */
 /* arrays; a node refers to its children by index into the same array.      */
@@ -534,6 +613,13 @@ prism_status prism_project_batch_with_exprs(prism_ctx* ctx, const struct ArrowAr
                                             uint32_t n_indices, const prism_expr_node* exprs, uint32_t n_exprs,
                                             const int32_t* roots, uint32_t n_roots, const prism_pred_node* preds,
                                             uint32_t n_preds, struct ArrowArray* out, struct ArrowSchema* out_schema);
+/* Device-level predicate: a boolean row program over device columns (the
+ * compiled form of a Predicate).  Writes ceil(n_rows / 64) mask words (bit set =
+ * row selected: value true and not null) when d_mask is not NULL, and the
+ * number of selected rows.  Synchronous. */
+prism_status prism_filter_mask(prism_ctx* ctx, const prism_instr* code, uint32_t n_code, const prism_string* strings,
+                               uint32_t n_strings, const int32_t* column_types, const prism_column* columns,
+                               uint32_t n_columns, uint64_t n_rows, unsigned long long* d_mask, uint64_t* selected);
 /* string_like / string_ilike (string_ops.rs:16-25) over a Utf8 array. */
 prism_status prism_string_like(prism_ctx* ctx, const struct ArrowArray* array, const struct ArrowSchema* schema,
                                const char* pattern, uint32_t pattern_len, int32_t case_insensitive,
@@ -562,6 +648,13 @@ prism_status prism_datagen_zipf_i64(prism_ctx* ctx, int64_t* d_out, uint64_t n, 
 prism_status prism_datagen_flag_utf8(prism_ctx* ctx, uint8_t* d_bytes, int32_t* d_offsets,
                                      uint64_t n, uint64_t seed, uint32_t stream,
                                      const char* alphabet, uint32_t n_alpha);
+
+/* Random lowercase Utf8 rows: length_i = min_len + uniform(max_len - min_len + 1)
+ * (stream), byte p of the data = 'a' + uniform(26) (stream + 1).  Writes n + 1
+ * offsets and *total_bytes bytes (at most bytes_capacity). */
+prism_status prism_datagen_lower_utf8(prism_ctx* ctx, uint8_t* d_bytes, uint64_t bytes_capacity, int32_t* d_offsets,
+                                      uint64_t n, uint64_t seed, uint32_t stream, uint32_t min_len, uint32_t max_len,
+                                      uint64_t* total_bytes);
 
 #ifdef __cplusplus
 } /* extern "C" */

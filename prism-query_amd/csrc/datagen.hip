@@ -1,3 +1,4 @@
+#include <hipcub/hipcub.hpp>
 // Seeded synthetic columns for tests and benchmarks.  Counter-based and
 // integer-only, so the device output is bit-identical to the numpy twin
 // (prism_executor/datagen.py) at any size.
@@ -102,6 +103,18 @@ __global__ void k_flag_utf8(uint8_t* bytes, int32_t* offsets, uint64_t n, uint64
   }
 }
 
+// Random lowercase strings: row lengths first (plus a trailing 0, so that an
+// exclusive scan gives the n + 1 offsets), then byte p of the data buffer.
+__global__ void k_lower_lengths(int32_t* len, uint64_t n, uint64_t base, uint32_t min_len, uint32_t span) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x)
+    len[i] = i < n ? static_cast<int32_t>(min_len + gen::uniform(gen::rnd(base, i), span)) : 0;
+}
+
+__global__ void k_lower_bytes(uint8_t* bytes, uint64_t total, uint64_t base) {
+  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < total; p += (uint64_t)gridDim.x * blockDim.x)
+    bytes[p] = static_cast<uint8_t>('a' + gen::uniform(gen::rnd(base, p), 26));
+}
+
 static dim3 gen_grid(prism_ctx* ctx, uint64_t n) {
   uint64_t blocks = ceil_div(n ? n : 1, 256);
   uint64_t cap = static_cast<uint64_t>(ctx->num_cus) * 16;
@@ -188,6 +201,39 @@ prism_status prism_datagen_flag_utf8(prism_ctx* ctx, uint8_t* d_bytes, int32_t* 
     hipLaunchKernelGGL(k_flag_utf8, gen_grid(ctx, n + 1), dim3(256), 0, ctx->stream, d_bytes,
                        d_offsets, n, gen::stream_base(seed, stream), a, n_alpha);
     PRISM_CHECK_LAUNCH();
+  });
+}
+
+prism_status prism_datagen_lower_utf8(prism_ctx* ctx, uint8_t* d_bytes, uint64_t bytes_capacity, int32_t* d_offsets,
+                                      uint64_t n, uint64_t seed, uint32_t stream, uint32_t min_len, uint32_t max_len,
+                                      uint64_t* total_bytes) {
+  return guarded([&] {
+    if (!ctx || !d_offsets || !total_bytes) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL argument");
+    if (min_len > max_len) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: min_len > max_len");
+    if (static_cast<double>(n) * max_len >= 2147483647.0)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: utf8 data would exceed i32 offsets");
+    use_device(ctx);
+    Ctx* c = as_ctx(ctx);
+    hipStream_t st = ctx->stream;
+    DeviceBuffer len(c, (n + 1) * 4);
+    hipLaunchKernelGGL(k_lower_lengths, gen_grid(ctx, n + 1), dim3(256), 0, st, len.as<int32_t>(), n,
+                       gen::stream_base(seed, stream), min_len, max_len - min_len + 1);
+    PRISM_CHECK_LAUNCH();
+    size_t tmp = 0;
+    PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, len.as<int32_t>(), d_offsets, static_cast<int>(n + 1), st));
+    DeviceBuffer t(c, tmp ? tmp : 1);
+    PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(t.ptr, tmp, len.as<int32_t>(), d_offsets, static_cast<int>(n + 1), st));
+    int32_t total = 0;
+    PRISM_HIP(hipMemcpyAsync(&total, d_offsets + n, 4, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+    *total_bytes = static_cast<uint64_t>(total);
+    if (static_cast<uint64_t>(total) > bytes_capacity)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: byte buffer too small (" + std::to_string(total) + " needed)");
+    if (total) {
+      hipLaunchKernelGGL(k_lower_bytes, gen_grid(ctx, static_cast<uint64_t>(total)), dim3(256), 0, st, d_bytes,
+                         static_cast<uint64_t>(total), gen::stream_base(seed, stream + 1));
+      PRISM_CHECK_LAUNCH();
+    }
   });
 }
 

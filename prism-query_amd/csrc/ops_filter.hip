@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kEB) void k_eval(const DevProgram* __restrict__ P, 
   const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(kEB) + threadIdx.x) >> 6;
   const uint64_t n_waves = (static_cast<uint64_t>(gridDim.x) * kEB) >> 6;
   const uint64_t pre[1][R] = {};
-  unsigned long long nulls = 0;
+  unsigned long long nulls = 0, selected = 0;
   for (uint64_t w0 = wave * R; w0 < n_words; w0 += n_waves * R) {
     uint64_t row[R];
     bool in[R];
@@ -79,6 +79,7 @@ __global__ __launch_bounds__(kEB) void k_eval(const DevProgram* __restrict__ P, 
       nulls += __popcll(inw & ~vw);
       if (out_kind == OUT_BITS) {
         const unsigned long long bw = __ballot(v && (val[r] & 1));
+        selected += __popcll(bw);
         if (lane == 0) static_cast<unsigned long long*>(out)[w0 + r] = bw;
       } else if (in[r]) {
         if (out_kind == OUT_W8) static_cast<uint64_t*>(out)[row[r]] = v ? val[r] : 0;
@@ -89,6 +90,7 @@ __global__ __launch_bounds__(kEB) void k_eval(const DevProgram* __restrict__ P, 
   }
   // one atomic per wave (wave-uniform value)
   if (lane == 0 && nulls) atomicAdd(null_count, nulls);
+  if (lane == 0 && selected) atomicAdd(null_count + 2, selected);
 }
 
 // Mask words from an Arrow BooleanArray (values AND validity), bit offset < 8.
@@ -534,35 +536,38 @@ static void upload_cols(prism_ctx* ctx, const HostBatch& hb, const std::vector<b
 }
 
 struct EvalOut {
-  DeviceBuffer values, valid, counters;  // counters: null count, error flags
+  DeviceBuffer values, valid, counters;  // counters: null count, error flags, selected rows (bit output)
   uint64_t nulls = 0;
+  uint64_t selected = 0;
 };
 
 static void run_eval(prism_ctx* ctx, const DevProgram* P, int depth, const prism_column* cols, uint64_t n,
-                     OutKind kind, EvalOut& o) {
+                     OutKind kind, EvalOut& o, void* out_values = nullptr) {
   Ctx* c = as_ctx(ctx);
   hipStream_t st = ctx->stream;
   const uint64_t nw = ceil_div(n, 64);
-  o.values.alloc(c, kind == OUT_BITS ? (nw ? nw : 1) * 8 : (n ? n : 1) * (kind == OUT_W8 ? 8 : 4));
+  if (!out_values) o.values.alloc(c, kind == OUT_BITS ? (nw ? nw : 1) * 8 : (n ? n : 1) * (kind == OUT_W8 ? 8 : 4));
+  void* vals = out_values ? out_values : o.values.ptr;
   o.valid.alloc(c, (nw ? nw : 1) * 8);
-  o.counters.alloc(c, 16);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, o.counters.as<unsigned long long>(), 2);
+  o.counters.alloc(c, 32);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, o.counters.as<unsigned long long>(), 3);
   if (n) {
     KernelTimer kt(ctx, "filter_eval");
     if (depth <= 4)
       hipLaunchKernelGGL((k_eval<4, 4>), egrid(ctx, nw, 4), dim3(kEB), 0, st, P, cols, n, static_cast<int>(kind),
-                         o.values.ptr, o.valid.as<unsigned long long>(), o.counters.as<unsigned long long>(),
+                         vals, o.valid.as<unsigned long long>(), o.counters.as<unsigned long long>(),
                          reinterpret_cast<uint32_t*>(o.counters.as<char>() + 8));
     else
       hipLaunchKernelGGL((k_eval<2, kStackDepth>), egrid(ctx, nw, 2), dim3(kEB), 0, st, P, cols, n,
-                         static_cast<int>(kind), o.values.ptr, o.valid.as<unsigned long long>(),
+                         static_cast<int>(kind), vals, o.valid.as<unsigned long long>(),
                          o.counters.as<unsigned long long>(), reinterpret_cast<uint32_t*>(o.counters.as<char>() + 8));
   }
   PRISM_CHECK_LAUNCH();
-  unsigned long long h[2];
-  PRISM_HIP(hipMemcpyAsync(h, o.counters.ptr, 16, hipMemcpyDeviceToHost, st));
+  unsigned long long h[3];
+  PRISM_HIP(hipMemcpyAsync(h, o.counters.ptr, 24, hipMemcpyDeviceToHost, st));
   PRISM_HIP(hipStreamSynchronize(st));
   o.nulls = h[0];
+  o.selected = h[2];
 }
 
 // Mask words (value AND valid) of a predicate over the batch.
@@ -975,6 +980,33 @@ prism_status prism_project_batch_with_exprs(prism_ctx* ctx, const struct ArrowAr
       cols.push_back(std::move(c));
     }
     export_batch(std::move(cols), hb.n_rows, out, out_schema);
+  });
+}
+
+prism_status prism_filter_mask(prism_ctx* ctx, const prism_instr* code, uint32_t n_code, const prism_string* strings,
+                               uint32_t n_strings, const int32_t* column_types, const prism_column* columns,
+                               uint32_t n_columns, uint64_t n_rows, unsigned long long* d_mask, uint64_t* selected) {
+  return guarded([&] {
+    if (!ctx || !code || !selected || (n_columns && (!column_types || !columns)))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/program/columns/output");
+    if (n_columns > static_cast<uint32_t>(kMaxCols)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 columns");
+    use_device(ctx);
+    int depth = 1;
+    const ValType vt = check_program(code, n_code, column_types, n_columns, n_strings, &depth);
+    if (vt != VT_B && vt != VT_NULL) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: filter program is not boolean");
+    auto P = std::make_unique<DevProgram>();
+    memset(P.get(), 0, sizeof(DevProgram));
+    P->len = static_cast<int32_t>(n_code);
+    memcpy(P->code, code, n_code * sizeof(prism_instr));
+    build_str_table(strings, n_strings, P->strs);
+    Ctx* c = as_ctx(ctx);
+    DeviceBuffer prog(c, sizeof(DevProgram)), desc(c, (n_columns ? n_columns : 1) * sizeof(prism_column));
+    PRISM_HIP(hipMemcpyAsync(prog.ptr, P.get(), sizeof(DevProgram), hipMemcpyHostToDevice, ctx->stream));
+    if (n_columns)
+      PRISM_HIP(hipMemcpyAsync(desc.ptr, columns, n_columns * sizeof(prism_column), hipMemcpyHostToDevice, ctx->stream));
+    EvalOut o;
+    run_eval(ctx, prog.as<DevProgram>(), depth, desc.as<prism_column>(), n_rows, OUT_BITS, o, d_mask);
+    *selected = o.selected;
   });
 }
 

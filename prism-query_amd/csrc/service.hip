@@ -1,0 +1,1192 @@
+// Device-resident operator service with continuous batching.
+//
+// Requests (group-by over a registered plan, probe of a registered resident
+// build table, filter by a registered predicate) arrive from any thread.  A
+// dispatcher thread admits them in arrival order into batches, as in-flight
+// slots and HBM free up, and launches each batch on its slot's stream while
+// other batches are still running, so the work of requests that arrive while
+// others are on the GPU overlaps on the device.
+//
+// One batch = one H2D copy of its control block (request descriptors and the
+// (request, row-range) tiles), then per request kind one persistent kernel
+// whose workgroups pull tiles from a queue until it is empty, one kernel that
+// finishes the group-by requests (collect + order groups by first row), one
+// D2H copy of the results, one event.  Launch sequences recur for a plan
+// shape, so they are captured once into a hipGraph and replayed.
+//
+// Memory: resident build tables are rebuilt from their host copy on demand and
+// evicted least-recently-used when the HBM budget needs room (never while a
+// batch in flight uses them); every slot owns a fixed working arena, and a
+// request joins a batch only while its footprint fits the arena.
+//
+// Results are identical to the one-request-per-launch operators: group-by
+// accumulates in the same exact int128 / f64-order-preserving words, probes
+// sum payloads exactly, filters evaluate the same row programs.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "groupby.hpp"
+#include "join.hpp"
+#include "program.hpp"
+
+namespace prism {
+namespace svc {
+
+using clk = std::chrono::steady_clock;
+
+constexpr int kSB = 256;               // threads per workgroup (group-by kernels assume kBlock)
+static_assert(kSB == gb::kBlock, "service group-by tiles use the group-by workgroup size");
+constexpr int kMaxReq = 256;           // requests per batch
+constexpr int kMaxTiles = 32768;       // tiles per batch
+constexpr int kFinishCap = 4096;       // group-table slots a finish workgroup orders in LDS
+constexpr int kVariants = 10;          // persistent-kernel variants (see Variant)
+constexpr size_t kResultsBytes = 48u << 20;
+
+enum Variant : int {
+  V_GB_444P = 0, V_GB_444S, V_GB_448P, V_GB_448S, V_GB_288P, V_GB_288S,
+  V_PROBE_DENSE, V_PROBE_HASHED, V_FILT_4, V_FILT_2
+};
+enum Kind : int { K_GB = 0, K_PROBE = 1, K_FILT = 2 };
+static Kind kind_of(int v) { return v <= V_GB_288S ? K_GB : v <= V_PROBE_HASHED ? K_PROBE : K_FILT; }
+
+struct FilterProg {
+  int32_t len;
+  int32_t pad;
+  prism_instr code[kMaxCode];
+  StrTable strs;
+};
+
+struct GbDesc {
+  const gb::DevPlan* P;
+  const prism_column* cols;
+  uint64_t n;
+  gb::GTable G;
+  int32_t lds_slots, lds_shift;
+  unsigned long long* out;  // n groups, flags, then rows (stride words each)
+  uint32_t* flags;
+  uint32_t pad;
+};
+
+struct ProbeDesc {
+  join::TableView T;
+  const int64_t* keys;
+  const int64_t* pay;
+  uint64_t n;
+  unsigned long long* acc;  // matches, probe sum lo/hi, build sum lo/hi
+};
+
+struct FiltDesc {
+  const FilterProg* P;
+  const prism_column* cols;
+  uint64_t n;
+  unsigned long long* mask;   // optional caller bitmap
+  unsigned long long* count;  // selected rows, then error flags
+};
+
+struct Tile {
+  uint32_t req;
+  uint32_t pad;
+  uint64_t start, end;
+};
+
+struct Queue {
+  uint32_t n_tiles;
+  uint32_t next;
+  uint32_t offset;  // first tile of this variant in the tile array
+  uint32_t pad;
+};
+
+struct InitDesc {
+  unsigned long long* dst;
+  uint64_t n_words;
+  const unsigned long long* pattern;  // dst[i] = pattern[i % period], or 0 when NULL
+  uint32_t period;
+  uint32_t pad;
+};
+
+// Control block (device mirror of a pinned host staging block).
+struct Control {
+  Queue q[kVariants];
+  uint32_t n_init, n_gb, pad0, pad1;
+  InitDesc init[2 * kMaxReq];
+  GbDesc gb[kMaxReq];
+  ProbeDesc pr[kMaxReq];
+  FiltDesc fd[kMaxReq];
+  uint32_t gb_idx[kMaxReq];
+  prism_column cols[kMaxReq][kMaxCols];
+  Tile tiles[kMaxTiles];
+};
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ uint32_t next_tile(Queue* q, uint32_t* s_t) {
+  if (threadIdx.x == 0) *s_t = atomicAdd(&q->next, 1u);
+  __syncthreads();
+  const uint32_t t = *s_t;
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kSB) void k_svc_init(Control* C) {
+  const uint32_t n = C->n_init;
+  for (uint32_t d = blockIdx.x; d < n; d += gridDim.x) {
+    const InitDesc D = C->init[d];
+    for (uint64_t i = threadIdx.x; i < D.n_words; i += kSB) D.dst[i] = D.pattern ? D.pattern[i % D.period] : 0ULL;
+  }
+}
+
+template <int kRows, int kDepth, int kPref, bool kPriv>
+__global__ __launch_bounds__(kSB) void k_svc_groupby(Control* C, int v) {
+  extern __shared__ unsigned long long lds[];
+  __shared__ uint32_t s_t;
+  Queue* q = &C->q[v];
+  const uint32_t n_tiles = q->n_tiles;
+  while (true) {
+    const uint32_t t = next_tile(q, &s_t);
+    if (t >= n_tiles) break;
+    const Tile tl = C->tiles[q->offset + t];
+    const GbDesc& d = C->gb[tl.req];
+    gb::gb_init<kPriv>(d.P, lds, d.lds_slots);
+    gb::gb_rows<kRows, kDepth, kPref, kPriv>(d.P, d.cols, tl.start, static_cast<uint64_t>(kSB) * kRows, tl.end, 0,
+                                             d.G, lds, d.lds_slots, d.lds_shift, d.flags);
+    gb::gb_flush<kPriv>(d.P, d.G, lds, d.lds_slots, d.flags);
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void wave_i128(__int128& v) {
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint64_t lo = static_cast<uint64_t>(v), hi = static_cast<uint64_t>(v >> 64);
+    const uint64_t olo = __shfl_xor(lo, d, 64), ohi = __shfl_xor(hi, d, 64);
+    v += static_cast<__int128>((static_cast<unsigned __int128>(ohi) << 64) | olo);
+  }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
+  __shared__ uint32_t s_t;
+  __shared__ unsigned long long s_m[kSB / 64];
+  __shared__ __int128 s_p[kSB / 64], s_b[kSB / 64];
+  Queue* q = &C->q[v];
+  const uint32_t n_tiles = q->n_tiles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  while (true) {
+    const uint32_t t = next_tile(q, &s_t);
+    if (t >= n_tiles) break;
+    const Tile tl = C->tiles[q->offset + t];
+    const ProbeDesc& d = C->pr[tl.req];
+    unsigned long long m = 0;
+    __int128 ps = 0, bs = 0;
+    for (uint64_t r = tl.start + threadIdx.x; r < tl.end; r += kSB) {
+      uint32_t val, cnt;
+      int32_t pay = 0;
+      if (join::lookup<KIND, true>(d.T, d.keys[r], val, cnt, pay)) {
+        ++m;
+        if (d.pay) ps += d.pay[r];
+        if (d.T.has_payload) bs += pay;
+      }
+    }
+    for (int k = 32; k > 0; k >>= 1) m += __shfl_xor(m, k, 64);
+    wave_i128(ps);
+    wave_i128(bs);
+    if (lane == 0) {
+      s_m[wave] = m;
+      s_p[wave] = ps;
+      s_b[wave] = bs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      m = 0;
+      ps = 0;
+      bs = 0;
+      for (int w = 0; w < kSB / 64; ++w) {
+        m += s_m[w];
+        ps += s_p[w];
+        bs += s_b[w];
+      }
+      if (m) atomicAdd(&d.acc[0], m);
+      if (ps) atomic_add_i128(&d.acc[1], &d.acc[2], static_cast<uint64_t>(ps), static_cast<int64_t>(ps >> 64));
+      if (bs) atomic_add_i128(&d.acc[3], &d.acc[4], static_cast<uint64_t>(bs), static_cast<int64_t>(bs >> 64));
+    }
+    __syncthreads();
+  }
+}
+
+// Filter tiles start on 64-row boundaries: every wave step covers whole words.
+template <int R, int DEPTH>
+__global__ __launch_bounds__(kSB) void k_svc_filter(Control* C, int v) {
+  __shared__ uint32_t s_t;
+  __shared__ unsigned long long s_c[kSB / 64];
+  Queue* q = &C->q[v];
+  const uint32_t n_tiles = q->n_tiles;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t pre[1][R] = {};
+  while (true) {
+    const uint32_t t = next_tile(q, &s_t);
+    if (t >= n_tiles) break;
+    const Tile tl = C->tiles[q->offset + t];
+    const FiltDesc& d = C->fd[tl.req];
+    unsigned long long sel = 0;
+    const uint64_t w_end = ceil_div(tl.end, 64);
+    for (uint64_t w0 = tl.start / 64 + static_cast<uint64_t>(wave) * R; w0 < w_end; w0 += (kSB / 64) * R) {
+      uint64_t row[R];
+      bool in[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        row[r] = (w0 + r) * 64 + lane;
+        in[r] = row[r] < tl.end;
+      }
+      uint64_t val[R];
+      uint32_t valid;
+      run_program<R, DEPTH, 1>(d.P->code, d.P->len, d.cols, &d.P->strs, row, in, pre, 0u, val, valid,
+                               reinterpret_cast<uint32_t*>(d.count + 1));
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (w0 + r >= w_end) break;
+        const unsigned long long bw = __ballot(in[r] && ((valid >> r) & 1) && (val[r] & 1));
+        sel += __popcll(bw);
+        if (lane == 0 && d.mask) d.mask[w0 + r] = bw;
+      }
+    }
+    if (lane == 0) s_c[wave] = sel;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long s = 0;
+      for (int w = 0; w < kSB / 64; ++w) s += s_c[w];
+      if (s) atomicAdd(d.count, s);
+    }
+    __syncthreads();
+  }
+}
+
+// Group-by finish: one workgroup per request collects the occupied slots,
+// orders them by first row (bitonic sort in LDS) and writes the rows.
+__global__ __launch_bounds__(kSB) void k_svc_gb_finish(Control* C) {
+  __shared__ unsigned long long s_first[kFinishCap];
+  __shared__ uint32_t s_slot[kFinishCap];
+  __shared__ uint32_t s_n;
+  const uint32_t n_gb = C->n_gb;
+  for (uint32_t k = blockIdx.x; k < n_gb; k += gridDim.x) {
+    const GbDesc& d = C->gb[C->gb_idx[k]];
+    const int stride = d.G.stride;
+    const uint64_t slots = d.G.cap + 1;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (uint64_t s = threadIdx.x; s < slots; s += kSB) {
+      const unsigned long long f = d.G.w[s * stride + gb::kW_FIRST];
+      if (f != gb::kNoRow) {
+        const uint32_t i = atomicAdd(&s_n, 1u);
+        s_first[i] = f;
+        s_slot[i] = static_cast<uint32_t>(s);
+      }
+    }
+    __syncthreads();
+    const uint32_t n = s_n;
+    uint32_t p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    for (uint32_t i = n + threadIdx.x; i < p2; i += kSB) {
+      s_first[i] = ~0ULL;
+      s_slot[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= p2; size <<= 1)
+      for (uint32_t stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
+        for (uint32_t i = threadIdx.x; i < p2; i += kSB) {
+          const uint32_t j = i ^ stride2;
+          if (j > i) {
+            const bool up = (i & size) == 0;
+            const unsigned long long a = s_first[i], b = s_first[j];
+            if ((a > b) == up) {
+              s_first[i] = b;
+              s_first[j] = a;
+              const uint32_t t = s_slot[i];
+              s_slot[i] = s_slot[j];
+              s_slot[j] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    unsigned long long* rows = d.out + 2;
+    for (uint64_t e = threadIdx.x; e < static_cast<uint64_t>(n) * stride; e += kSB) {
+      const uint64_t i = e / stride, j = e % stride;
+      rows[e] = d.G.w[static_cast<uint64_t>(s_slot[i]) * stride + j];
+    }
+    if (threadIdx.x == 0) {
+      d.out[0] = n;
+      d.out[1] = *d.flags;
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------- host side
+struct Table {
+  uint32_t id = 0;
+  std::vector<int64_t> keys;
+  std::vector<int32_t> payload;
+  prism_join_table* t = nullptr;
+  uint64_t bytes = 0;
+  uint64_t last_use = 0;
+  int pins = 0;
+};
+
+struct GbPlan {
+  prism_groupby* g = nullptr;
+  int variant = 0;
+  bool priv = false;
+  int lds_slots = 0, lds_shift = 0;
+  size_t lds_bytes = 0;
+  uint64_t cap = 0;
+};
+
+struct FiltPlan {
+  DeviceBuffer prog;
+  int variant = V_FILT_4;
+  std::vector<int32_t> col_types;
+};
+
+struct Req {
+  uint64_t ticket = 0;
+  prism_request r{};
+  std::vector<prism_column> cols;
+  clk::time_point t_submit, t_launch;
+  prism_request_result res{};
+  std::vector<unsigned long long> rows;  // group-by result rows
+  GbPlan* gbp = nullptr;
+  Table* table = nullptr;
+  unsigned long long* d_out = nullptr;   // device results of this request (in the slot's results area)
+  size_t out_words = 0;
+  size_t res_off = 0;                    // word offset in the slot's results area
+  bool done = false;
+};
+
+struct Cand {
+  uint32_t tile_rows;
+  uint32_t bpc;  // workgroups per CU
+};
+
+struct Tuner {
+  std::vector<Cand> cands;
+  std::vector<double> rows_per_us;
+  std::vector<int> samples;
+  int best = -1;
+  int next = 0;
+};
+
+struct Slot {
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  DeviceBuffer arena;      // results area, then group tables
+  DeviceBuffer control;    // Control
+  Control* h_control = nullptr;
+  unsigned long long* h_results = nullptr;
+  size_t results_words = 0, table_words = 0;
+  std::vector<Req*> reqs;
+  bool busy = false;
+  uint64_t batch_id = 0;
+  std::map<std::string, hipGraphExec_t> graphs;
+  // tuning: (variant, events, rows, candidate)
+  struct Timed {
+    int variant;
+    hipEvent_t a, b;
+    uint64_t rows;
+    int cand;
+  };
+  std::vector<Timed> timed;
+};
+
+}  // namespace svc
+}  // namespace prism
+
+struct prism_service {
+  prism::Ctx* ctx = nullptr;           // owned context: table builds, allocations
+  prism_service_config cfg{};
+  std::vector<prism::svc::Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<prism::svc::Req*> pending;
+  std::unordered_map<uint64_t, prism::svc::Req*> reqs;
+  std::map<uint32_t, prism::svc::Table> tables;
+  std::map<uint32_t, prism::svc::GbPlan> gb_plans;
+  std::map<uint32_t, prism::svc::FiltPlan> filt_plans;
+  prism::svc::Tuner tuner[3];
+  uint64_t next_ticket = 1, use_clock = 0, batches = 0, built = 0, evicted = 0, graph_launches = 0;
+  uint64_t completed = 0, submitted = 0, batch_reqs = 0;
+  uint64_t resident_bytes = 0;
+  std::vector<double> latencies;
+  bool stop = false;
+  std::thread worker;
+  std::string fatal;
+};
+
+namespace prism {
+namespace svc {
+
+static double ms_between(clk::time_point a, clk::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+static int gb_variant(const prism_groupby* g, bool priv) {
+  const bool shallow = g->max_depth <= 4;
+  if (shallow && g->plan.n_pref <= 4) return priv ? V_GB_444P : V_GB_444S;
+  if (shallow) return priv ? V_GB_448P : V_GB_448S;
+  return priv ? V_GB_288P : V_GB_288S;
+}
+
+static Cand cand_of(prism_service* S, Kind k) {
+  Tuner& T = S->tuner[k];
+  if (T.best >= 0) return T.cands[T.best];
+  if (!S->cfg.autotune) return T.cands[0];
+  return T.cands[T.next % T.cands.size()];
+}
+
+static int cand_index(prism_service* S, Kind k) {
+  Tuner& T = S->tuner[k];
+  if (T.best >= 0) return T.best;
+  if (!S->cfg.autotune) return 0;
+  return static_cast<int>(T.next % T.cands.size());
+}
+
+static bool tuning(prism_service* S) {
+  if (!S->cfg.autotune) return false;
+  for (int k = 0; k < 3; ++k)
+    if (S->tuner[k].best < 0) return true;
+  return false;
+}
+
+// Make a table resident: evict least-recently-used unpinned tables until it fits.
+static bool make_resident(prism_service* S, Table& T) {
+  T.last_use = ++S->use_clock;
+  if (T.t) return true;
+  const uint64_t need = T.keys.size() * 16 + (64u << 20);  // generous bound of the device table
+  const uint64_t arenas = S->slots.size() * (S->cfg.arena_bytes + sizeof(Control));
+  while (S->resident_bytes + need + arenas > S->cfg.hbm_budget) {
+    Table* victim = nullptr;
+    for (auto& kv : S->tables)
+      if (kv.second.t && kv.second.pins == 0 && (!victim || kv.second.last_use < victim->last_use)) victim = &kv.second;
+    if (!victim) return false;  // wait until a batch releases its tables
+    S->resident_bytes -= victim->bytes;
+    prism_join_table_free(victim->t);
+    victim->t = nullptr;
+    victim->bytes = 0;
+    ++S->evicted;
+  }
+  Ctx* c = S->ctx;
+  DeviceBuffer k(c, T.keys.size() * 8 + 8), p;
+  PRISM_HIP(hipMemcpyAsync(k.ptr, T.keys.data(), T.keys.size() * 8, hipMemcpyHostToDevice, c->stream));
+  if (!T.payload.empty()) {
+    p.alloc(c, T.payload.size() * 4 + 4);
+    PRISM_HIP(hipMemcpyAsync(p.ptr, T.payload.data(), T.payload.size() * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  prism_join_table* t = nullptr;
+  prism_status st = T.payload.empty()
+                        ? prism_join_build_i64(c, k.as<int64_t>(), T.keys.size(), PRISM_TABLE_AUTO, &t)
+                        : prism_join_build_i64_payload(c, k.as<int64_t>(), p.as<int32_t>(), T.keys.size(),
+                                                       PRISM_TABLE_AUTO, &t);
+  if (st != PRISM_OK) fail(st, prism_last_error());
+  PRISM_HIP(hipStreamSynchronize(c->stream));
+  if (!t->info.unique) {
+    prism_join_table_free(t);
+    fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: service build tables need unique keys");
+  }
+  T.t = t;
+  T.bytes = t->info.device_bytes;
+  S->resident_bytes += T.bytes;
+  ++S->built;
+  return true;
+}
+
+// Words a request needs in the slot's results area and table area.
+static void footprint(const Req* R, size_t& res_words, size_t& table_words) {
+  res_words = 0;
+  table_words = 0;
+  switch (R->r.kind) {
+    case PRISM_REQ_GROUPBY: {
+      const uint64_t stride = static_cast<uint64_t>(R->gbp->g->plan.stride);
+      res_words = 2 + (R->gbp->cap + 1) * stride;
+      table_words = (R->gbp->cap + 1) * stride + 2;  // + n_groups counter, flags
+      break;
+    }
+    case PRISM_REQ_PROBE:
+      res_words = 5;
+      break;
+    default:
+      res_words = 2;
+  }
+}
+
+static std::string shape_key(const int (&grid)[kVariants], const size_t (&lds)[kVariants], int n_init_grid,
+                             int n_fin_grid) {
+  std::string k;
+  for (int v = 0; v < kVariants; ++v) k += std::to_string(grid[v]) + ":" + std::to_string(lds[v]) + ",";
+  return k + std::to_string(n_init_grid) + "," + std::to_string(n_fin_grid);
+}
+
+static void launch_variant(int v, dim3 grid, size_t lds, hipStream_t st, Control* dC) {
+  switch (v) {
+    case V_GB_444P: hipLaunchKernelGGL((k_svc_groupby<4, 4, 4, true>), grid, dim3(kSB), lds, st, dC, v); break;
+    case V_GB_444S: hipLaunchKernelGGL((k_svc_groupby<4, 4, 4, false>), grid, dim3(kSB), lds, st, dC, v); break;
+    case V_GB_448P: hipLaunchKernelGGL((k_svc_groupby<4, 4, 8, true>), grid, dim3(kSB), lds, st, dC, v); break;
+    case V_GB_448S: hipLaunchKernelGGL((k_svc_groupby<4, 4, 8, false>), grid, dim3(kSB), lds, st, dC, v); break;
+    case V_GB_288P: hipLaunchKernelGGL((k_svc_groupby<2, kStackDepth, 8, true>), grid, dim3(kSB), lds, st, dC, v); break;
+    case V_GB_288S: hipLaunchKernelGGL((k_svc_groupby<2, kStackDepth, 8, false>), grid, dim3(kSB), lds, st, dC, v); break;
+    case V_PROBE_DENSE: hipLaunchKernelGGL(k_svc_probe<PRISM_TABLE_DENSE>, grid, dim3(kSB), 0, st, dC, v); break;
+    case V_PROBE_HASHED: hipLaunchKernelGGL(k_svc_probe<PRISM_TABLE_HASHED>, grid, dim3(kSB), 0, st, dC, v); break;
+    case V_FILT_4: hipLaunchKernelGGL((k_svc_filter<4, 4>), grid, dim3(kSB), 0, st, dC, v); break;
+    default: hipLaunchKernelGGL((k_svc_filter<2, kStackDepth>), grid, dim3(kSB), 0, st, dC, v); break;
+  }
+}
+
+// Build the batch's control block, launch it on the slot's stream.
+static void launch_batch(prism_service* S, Slot& sl) {
+  Control* H = sl.h_control;
+  memset(H, 0, offsetof(Control, tiles));
+  const int n_cu = S->ctx->num_cus;
+  std::vector<Tile> vt[kVariants];
+  uint64_t vrows[kVariants] = {};
+  size_t lds[kVariants] = {};
+  unsigned long long* res_base = sl.arena.as<unsigned long long>();
+  unsigned long long* tab_base = res_base + kResultsBytes / 8;
+  size_t res_used = 0, tab_used = 0;
+  uint32_t n_gb = 0, n_pr = 0, n_fd = 0;
+  for (size_t i = 0; i < sl.reqs.size(); ++i) {
+    Req* R = sl.reqs[i];
+    size_t rw, tw;
+    footprint(R, rw, tw);
+    R->res_off = res_used;
+    R->d_out = res_base + res_used;
+    R->out_words = rw;
+    res_used += rw;
+    switch (R->r.kind) {
+      case PRISM_REQ_GROUPBY: {
+        const GbPlan& gp = *R->gbp;
+        const uint32_t k = n_gb++;
+        GbDesc& d = H->gb[k];
+        const int stride = gp.g->plan.stride;
+        unsigned long long* table = tab_base + tab_used;
+        tab_used += tw;
+        memcpy(H->cols[k], R->cols.data(), R->cols.size() * sizeof(prism_column));
+        d.P = gp.g->d_plan.as<gb::DevPlan>();
+        d.cols = &reinterpret_cast<Control*>(sl.control.ptr)->cols[k][0];
+        d.n = R->r.n_rows;
+        d.G.w = table;
+        d.G.cap = gp.cap;
+        d.G.shift = 64 - log2_pow2(gp.cap);
+        d.G.stride = stride;
+        d.G.n_groups = table + (gp.cap + 1) * stride;
+        d.flags = reinterpret_cast<uint32_t*>(table + (gp.cap + 1) * stride + 1);
+        d.lds_slots = gp.lds_slots;
+        d.lds_shift = gp.lds_shift;
+        d.out = R->d_out;
+        H->gb_idx[H->n_gb++] = k;
+        InitDesc& a = H->init[H->n_init++];
+        a.dst = table;
+        a.n_words = (gp.cap + 1) * stride;
+        a.pattern = reinterpret_cast<const unsigned long long*>(gp.g->d_plan.as<char>() + offsetof(gb::DevPlan, tmpl));
+        a.period = static_cast<uint32_t>(stride);
+        InitDesc& b = H->init[H->n_init++];
+        b.dst = table + (gp.cap + 1) * stride;
+        b.n_words = 2;
+        const Cand cd = cand_of(S, K_GB);
+        const uint64_t step = static_cast<uint64_t>(kSB) * (gp.variant >= V_GB_288P ? 2 : 4);
+        const uint64_t tr = std::max<uint64_t>(step, cd.tile_rows / step * step);
+        for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
+          vt[gp.variant].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
+        vrows[gp.variant] += R->r.n_rows;
+        lds[gp.variant] = std::max(lds[gp.variant], gp.lds_bytes);
+        break;
+      }
+      case PRISM_REQ_PROBE: {
+        const uint32_t k = n_pr++;
+        ProbeDesc& d = H->pr[k];
+        d.T = R->table->t->view;
+        d.keys = R->r.probe_keys;
+        d.pay = R->r.probe_payload;
+        d.n = R->r.n_rows;
+        d.acc = R->d_out;
+        InitDesc& a = H->init[H->n_init++];
+        a.dst = R->d_out;
+        a.n_words = 5;
+        const int v = R->table->t->info.kind == PRISM_TABLE_DENSE ? V_PROBE_DENSE : V_PROBE_HASHED;
+        const uint64_t tr = std::max<uint64_t>(kSB, cand_of(S, K_PROBE).tile_rows);
+        for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
+          vt[v].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
+        vrows[v] += R->r.n_rows;
+        break;
+      }
+      default: {
+        const FiltPlan& fp = S->filt_plans[R->r.target];
+        const uint32_t k = n_fd++;
+        FiltDesc& d = H->fd[k];
+        memcpy(H->cols[kMaxReq - 1 - k], R->cols.data(), R->cols.size() * sizeof(prism_column));
+        d.P = fp.prog.as<FilterProg>();
+        d.cols = &reinterpret_cast<Control*>(sl.control.ptr)->cols[kMaxReq - 1 - k][0];
+        d.n = R->r.n_rows;
+        d.mask = R->r.out_mask;
+        d.count = R->d_out;
+        InitDesc& a = H->init[H->n_init++];
+        a.dst = R->d_out;
+        a.n_words = 2;
+        const uint64_t tr = std::max<uint64_t>(64 * 64, cand_of(S, K_FILT).tile_rows / 64 * 64);
+        for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
+          vt[fp.variant].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
+        vrows[fp.variant] += R->r.n_rows;
+      }
+    }
+  }
+  uint32_t off = 0;
+  int grid[kVariants] = {};
+  for (int v = 0; v < kVariants; ++v) {
+    H->q[v].n_tiles = static_cast<uint32_t>(vt[v].size());
+    H->q[v].next = 0;
+    H->q[v].offset = off;
+    if (!vt[v].empty()) {
+      if (off + vt[v].size() > static_cast<size_t>(kMaxTiles))
+        fail(PRISM_ERR_INTERNAL, "Internal error: batch tile overflow");
+      memcpy(&H->tiles[off], vt[v].data(), vt[v].size() * sizeof(Tile));
+      off += static_cast<uint32_t>(vt[v].size());
+      const Cand cd = cand_of(S, kind_of(v));
+      uint32_t bpc = cd.bpc;
+      if (lds[v]) bpc = std::min<uint32_t>(bpc, std::max<uint32_t>(1, static_cast<uint32_t>((160u << 10) / lds[v])));
+      grid[v] = static_cast<int>(std::min<uint64_t>(vt[v].size(), static_cast<uint64_t>(n_cu) * bpc));
+    }
+  }
+  const size_t ctl_bytes = offsetof(Control, tiles) + off * sizeof(Tile);
+  Control* dC = sl.control.as<Control>();
+  hipStream_t st = sl.st;
+  PRISM_HIP(hipMemcpyAsync(dC, H, ctl_bytes, hipMemcpyHostToDevice, st));
+  const int init_grid = static_cast<int>(std::max<uint32_t>(1, std::min<uint32_t>(H->n_init, n_cu * 4)));
+  const int fin_grid = static_cast<int>(std::max<uint32_t>(1, std::min<uint32_t>(H->n_gb, n_cu * 2)));
+  const bool timed = tuning(S);
+  auto issue = [&](bool with_events) {
+    hipLaunchKernelGGL(k_svc_init, dim3(init_grid), dim3(kSB), 0, st, dC);
+    for (int v = 0; v < kVariants; ++v) {
+      if (!grid[v]) continue;
+      hipEvent_t a = nullptr, b = nullptr;
+      if (with_events) {
+        PRISM_HIP(hipEventCreate(&a));
+        PRISM_HIP(hipEventCreate(&b));
+        PRISM_HIP(hipEventRecord(a, st));
+      }
+      launch_variant(v, dim3(grid[v]), lds[v], st, dC);
+      if (with_events) {
+        PRISM_HIP(hipEventRecord(b, st));
+        sl.timed.push_back(Slot::Timed{v, a, b, vrows[v], cand_index(S, kind_of(v))});
+      }
+    }
+    if (H->n_gb) hipLaunchKernelGGL(k_svc_gb_finish, dim3(fin_grid), dim3(kSB), 0, st, dC);
+  };
+  if (S->cfg.use_graphs && !timed) {
+    const std::string key = shape_key(grid, lds, init_grid, fin_grid);
+    auto it = sl.graphs.find(key);
+    if (it == sl.graphs.end()) {
+      hipGraph_t g;
+      PRISM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      issue(false);
+      PRISM_HIP(hipStreamEndCapture(st, &g));
+      hipGraphExec_t ex;
+      PRISM_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      PRISM_HIP(hipGraphDestroy(g));
+      it = sl.graphs.emplace(key, ex).first;
+    }
+    PRISM_HIP(hipGraphLaunch(it->second, st));
+    ++S->graph_launches;
+  } else {
+    issue(timed);
+  }
+  PRISM_CHECK_LAUNCH();
+  if (res_used) PRISM_HIP(hipMemcpyAsync(sl.h_results, res_base, res_used * 8, hipMemcpyDeviceToHost, st));
+  sl.results_words = res_used;
+  sl.table_words = tab_used;
+  PRISM_HIP(hipEventRecord(sl.ev, st));
+  if (S->cfg.autotune)
+    for (int k = 0; k < 3; ++k)
+      if (S->tuner[k].best < 0) ++S->tuner[k].next;
+}
+
+// Finish a completed batch (called with the service lock held).
+static void finalize(prism_service* S, Slot& sl) {
+  const auto now = clk::now();
+  for (Req* R : sl.reqs) {
+    const unsigned long long* o = sl.h_results + R->res_off;
+    prism_request_result& res = R->res;
+    res.status = PRISM_OK;
+    res.kind = R->r.kind;
+    res.n_rows = R->r.n_rows;
+    res.batch_id = sl.batch_id;
+    switch (R->r.kind) {
+      case PRISM_REQ_GROUPBY: {
+        const uint64_t n = o[0];
+        const uint32_t flags = static_cast<uint32_t>(o[1]);
+        res.count = n;
+        if (flags & gb::kFlagOverflow) res.status = PRISM_ERR_EXECUTION;
+        else if (flags & (gb::kFlagDropped | gb::kFlagGrow)) res.status = PRISM_ERR_RESOURCE;
+        const size_t stride = static_cast<size_t>(R->gbp->g->plan.stride);
+        R->rows.assign(o + 2, o + 2 + n * stride);
+        break;
+      }
+      case PRISM_REQ_PROBE:
+        res.count = o[0];
+        res.probe_sum_lo = o[1];
+        res.probe_sum_hi = static_cast<int64_t>(o[2]);
+        res.build_sum_lo = o[3];
+        res.build_sum_hi = static_cast<int64_t>(o[4]);
+        if (R->table) --R->table->pins;
+        break;
+      default:
+        res.count = o[0];
+        if (o[1] & gb::kFlagOverflow) res.status = PRISM_ERR_EXECUTION;
+    }
+    res.latency_ms = ms_between(R->t_submit, now);
+    res.queue_ms = ms_between(R->t_submit, R->t_launch);
+    S->latencies.push_back(res.latency_ms);
+    R->done = true;
+    ++S->completed;
+  }
+  for (auto& tm : sl.timed) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, tm.a, tm.b) == hipSuccess && ms > 0) {
+      Tuner& T = S->tuner[kind_of(tm.variant)];
+      if (T.best < 0 && tm.cand >= 0 && tm.cand < static_cast<int>(T.cands.size())) {
+        T.rows_per_us[tm.cand] += tm.rows / (ms * 1e3);
+        ++T.samples[tm.cand];
+        bool all = true;
+        for (int s : T.samples) all = all && s >= 2;
+        if (all) {
+          int best = 0;
+          for (size_t i = 1; i < T.cands.size(); ++i)
+            if (T.rows_per_us[i] / T.samples[i] > T.rows_per_us[best] / T.samples[best]) best = static_cast<int>(i);
+          T.best = best;
+        }
+      }
+    }
+    (void)hipEventDestroy(tm.a);
+    (void)hipEventDestroy(tm.b);
+  }
+  sl.timed.clear();
+  sl.reqs.clear();
+  sl.busy = false;
+}
+
+static bool admit_into(prism_service* S, Slot& sl) {
+  size_t res_used = 0, tab_used = 0;
+  uint64_t rows = 0;
+  uint32_t tiles = 0;
+  const size_t res_cap = kResultsBytes / 8, tab_cap = (S->cfg.arena_bytes - kResultsBytes) / 8;
+  while (!S->pending.empty() && sl.reqs.size() < S->cfg.max_batch_requests) {
+    Req* R = S->pending.front();
+    size_t rw, tw;
+    footprint(R, rw, tw);
+    if (rw > res_cap || tw > tab_cap) {  // can never fit: fail it
+      S->pending.pop_front();
+      R->res.status = PRISM_ERR_RESOURCE;
+      R->res.kind = R->r.kind;
+      R->done = true;
+      ++S->completed;
+      continue;
+    }
+    const uint64_t est_tiles = R->r.n_rows / 65536 + 1;
+    if (!sl.reqs.empty() && (res_used + rw > res_cap || tab_used + tw > tab_cap ||
+                             rows + R->r.n_rows > S->cfg.max_batch_rows || tiles + est_tiles > kMaxTiles / 4))
+      break;
+    if (R->r.kind == PRISM_REQ_PROBE) {
+      Table& T = S->tables[R->r.target];
+      bool ok = false;
+      try {
+        ok = make_resident(S, T);
+      } catch (const Error& e) {
+        S->pending.pop_front();
+        R->res.status = e.status;
+        R->res.kind = R->r.kind;
+        R->done = true;
+        ++S->completed;
+        continue;
+      }
+      if (!ok) {
+        bool busy = false;
+        for (const Slot& o : S->slots) busy = busy || o.busy;
+        if (busy || !sl.reqs.empty()) break;  // memory held by batches in flight: retry when they finish
+        S->pending.pop_front();                // nothing to wait for: the table can never fit
+        R->res.status = PRISM_ERR_RESOURCE;
+        R->res.kind = R->r.kind;
+        R->done = true;
+        ++S->completed;
+        continue;
+      }
+      ++T.pins;
+      R->table = &T;
+    }
+    S->pending.pop_front();
+    res_used += rw;
+    tab_used += tw;
+    rows += R->r.n_rows;
+    tiles += static_cast<uint32_t>(est_tiles);
+    R->t_launch = clk::now();
+    sl.reqs.push_back(R);
+  }
+  return !sl.reqs.empty();
+}
+
+static void dispatcher(prism_service* S) {
+  (void)hipSetDevice(S->ctx->device);
+  std::unique_lock<std::mutex> lock(S->mu);
+  while (true) {
+    bool progress = false;
+    for (Slot& sl : S->slots)
+      if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
+        finalize(S, sl);
+        progress = true;
+      }
+    if (progress) S->cv_done.notify_all();
+    for (Slot& sl : S->slots) {
+      if (sl.busy || S->pending.empty()) continue;
+      try {
+        if (admit_into(S, sl)) {
+          sl.batch_id = ++S->batches;
+          S->batch_reqs += sl.reqs.size();
+          launch_batch(S, sl);
+          sl.busy = true;
+          progress = true;
+        }
+      } catch (const std::exception& e) {
+        for (Req* R : sl.reqs) {
+          R->res.status = PRISM_ERR_EXECUTION;
+          R->done = true;
+          if (R->table) --R->table->pins;
+          ++S->completed;
+        }
+        sl.reqs.clear();
+        S->fatal = e.what();
+        S->cv_done.notify_all();
+      }
+    }
+    bool any_busy = false;
+    for (Slot& sl : S->slots) any_busy = any_busy || sl.busy;
+    if (S->stop && S->pending.empty() && !any_busy) break;
+    if (!progress) {
+      if (any_busy) S->cv_work.wait_for(lock, std::chrono::microseconds(20));
+      else S->cv_work.wait_for(lock, std::chrono::milliseconds(5));
+    }
+  }
+}
+
+}  // namespace svc
+}  // namespace prism
+
+using namespace prism;
+using namespace prism::svc;
+
+extern "C" {
+
+prism_status prism_service_create(prism_ctx* ctx, const prism_service_config* cfg, prism_service** out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or output");
+    *out = nullptr;
+    auto S = std::make_unique<prism_service>();
+    prism_service_config c{};
+    if (cfg) c = *cfg;
+    if (!c.hbm_budget) c.hbm_budget = 64ULL << 30;
+    if (!c.max_inflight) c.max_inflight = 3;
+    if (c.max_inflight > 8) c.max_inflight = 8;
+    if (!c.max_batch_requests || c.max_batch_requests > static_cast<uint32_t>(kMaxReq)) c.max_batch_requests = kMaxReq;
+    if (!c.max_batch_rows) c.max_batch_rows = 256ULL << 20;
+    if (!c.arena_bytes) c.arena_bytes = 256ULL << 20;
+    if (c.arena_bytes < kResultsBytes + (16u << 20)) c.arena_bytes = kResultsBytes + (16u << 20);
+    S->cfg = c;
+    prism_ctx* own = nullptr;
+    prism_status st = prism_ctx_create(ctx->device, &own);
+    if (st != PRISM_OK) fail(st, prism_last_error());
+    S->ctx = as_ctx(own);
+    use_device(own);
+    S->slots.resize(c.max_inflight);
+    for (Slot& sl : S->slots) {
+      PRISM_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+      PRISM_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+      sl.arena.alloc(S->ctx, c.arena_bytes);
+      sl.control.alloc(S->ctx, sizeof(Control));
+      PRISM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_control), sizeof(Control), hipHostMallocDefault));
+      PRISM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_results), kResultsBytes, hipHostMallocDefault));
+    }
+    S->tuner[K_GB].cands = {{131072, 2}, {524288, 2}, {2097152, 4}};
+    S->tuner[K_PROBE].cands = {{262144, 4}, {1048576, 8}, {4194304, 8}};
+    S->tuner[K_FILT].cands = {{131072, 4}, {524288, 8}, {2097152, 8}};
+    for (auto& t : S->tuner) {
+      t.rows_per_us.assign(t.cands.size(), 0.0);
+      t.samples.assign(t.cands.size(), 0);
+    }
+    prism_service* raw = S.get();
+    S->worker = std::thread([raw] { dispatcher(raw); });
+    *out = S.release();
+  });
+}
+
+void prism_service_destroy(prism_service* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->ctx->device);
+  for (Slot& sl : s->slots) {
+    (void)hipStreamSynchronize(sl.st);
+    for (auto& kv : sl.graphs) (void)hipGraphExecDestroy(kv.second);
+    if (sl.h_control) (void)hipHostFree(sl.h_control);
+    if (sl.h_results) (void)hipHostFree(sl.h_results);
+    sl.arena.release();
+    sl.control.release();
+    (void)hipEventDestroy(sl.ev);
+    (void)hipStreamDestroy(sl.st);
+  }
+  for (auto& kv : s->tables)
+    if (kv.second.t) prism_join_table_free(kv.second.t);
+  for (auto& kv : s->gb_plans) prism_groupby_destroy(kv.second.g);
+  s->filt_plans.clear();
+  for (auto& kv : s->reqs) delete kv.second;
+  prism_ctx* own = s->ctx;
+  delete s;
+  prism_ctx_destroy(own);
+}
+
+prism_status prism_service_register_table(prism_service* s, uint32_t table_id, const int64_t* h_keys,
+                                          const int32_t* h_payload, uint64_t n) {
+  return guarded([&] {
+    if (!s || (n && !h_keys)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service or keys");
+    if (n >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: build side exceeds u32 row indices");
+    std::lock_guard<std::mutex> lock(s->mu);
+    Table& T = s->tables[table_id];
+    if (T.t && T.pins) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: table in use by requests in flight");
+    if (T.t) {
+      s->resident_bytes -= T.bytes;
+      prism_join_table_free(T.t);
+      T.t = nullptr;
+    }
+    T.id = table_id;
+    T.keys.assign(h_keys, h_keys + n);
+    T.payload.clear();
+    if (h_payload) T.payload.assign(h_payload, h_payload + n);
+    use_device(s->ctx);
+    make_resident(s, T);
+  });
+}
+
+prism_status prism_service_register_groupby(prism_service* s, uint32_t plan_id, const prism_groupby_plan* plan,
+                                            uint64_t max_groups) {
+  return guarded([&] {
+    if (!s || !plan) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service or plan");
+    if (max_groups == 0 || max_groups > 1024)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: service group-by plans hold 1..1024 groups");
+    prism_groupby* g = nullptr;
+    prism_status st = prism_groupby_create(s->ctx, plan, max_groups, &g);
+    if (st != PRISM_OK) fail(st, prism_last_error());
+    GbPlan P;
+    P.g = g;
+    P.cap = std::max<uint64_t>(64, next_pow2(2 * max_groups));
+    const uint64_t pw = static_cast<uint64_t>(g->plan.stride - gb::kW_COUNT);
+    const uint64_t ps = next_pow2(max_groups > 2 ? max_groups : 2);
+    const size_t priv_bytes = (2 * ps + ps * pw * kSB) * 8;
+    if (ps <= 16 && priv_bytes <= gb::kPrivBudget) {
+      P.priv = true;
+      P.lds_slots = static_cast<int>(ps);
+      P.lds_shift = 64 - log2_pow2(ps);
+      P.lds_bytes = priv_bytes;
+    } else {
+      const size_t sb = static_cast<size_t>(g->plan.stride) * 8;
+      uint64_t sl = next_pow2(2 * max_groups);
+      while (sl * sb > gb::kLdsBudget && sl > 16) sl >>= 1;
+      P.priv = false;
+      P.lds_slots = static_cast<int>(sl);
+      P.lds_shift = 64 - log2_pow2(sl);
+      P.lds_bytes = sl * sb;
+    }
+    P.variant = gb_variant(g, P.priv);
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->gb_plans.find(plan_id);
+    if (it != s->gb_plans.end()) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: plan id already registered");
+    s->gb_plans[plan_id] = P;
+  });
+}
+
+prism_status prism_service_register_filter(prism_service* s, uint32_t plan_id, const prism_instr* code, uint32_t n_code,
+                                           const prism_string* strings, uint32_t n_strings, const int32_t* column_types,
+                                           uint32_t n_columns) {
+  return guarded([&] {
+    if (!s || !code || (n_columns && !column_types)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL argument");
+    if (n_columns > static_cast<uint32_t>(kMaxCols)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 columns");
+    int depth = 1;
+    const ValType vt = check_program(code, n_code, column_types, n_columns, n_strings, &depth);
+    if (vt != VT_B && vt != VT_NULL) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: filter program is not boolean");
+    auto P = std::make_unique<FilterProg>();
+    memset(P.get(), 0, sizeof(FilterProg));
+    P->len = static_cast<int32_t>(n_code);
+    memcpy(P->code, code, n_code * sizeof(prism_instr));
+    build_str_table(strings, n_strings, P->strs);
+    use_device(s->ctx);
+    FiltPlan F;
+    F.prog.alloc(s->ctx, sizeof(FilterProg));
+    PRISM_HIP(hipMemcpyAsync(F.prog.ptr, P.get(), sizeof(FilterProg), hipMemcpyHostToDevice, s->ctx->stream));
+    PRISM_HIP(hipStreamSynchronize(s->ctx->stream));
+    F.variant = depth <= 4 ? V_FILT_4 : V_FILT_2;
+    F.col_types.assign(column_types, column_types + n_columns);
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->filt_plans.count(plan_id)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: plan id already registered");
+    s->filt_plans[plan_id] = std::move(F);
+  });
+}
+
+prism_status prism_service_submit(prism_service* s, const prism_request* req, uint64_t* ticket) {
+  return guarded([&] {
+    if (!s || !req || !ticket) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service/request/ticket");
+    auto R = std::make_unique<Req>();
+    R->r = *req;
+    R->t_submit = clk::now();
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (!s->fatal.empty()) fail(PRISM_ERR_EXECUTION, "Execution error: service stopped: " + s->fatal);
+    switch (req->kind) {
+      case PRISM_REQ_GROUPBY: {
+        auto it = s->gb_plans.find(req->target);
+        if (it == s->gb_plans.end()) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown group-by plan");
+        const size_t nc = it->second.g->col_types.size();
+        if (nc && !req->columns) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL columns");
+        for (size_t i = 0; i < nc; ++i)
+          if (req->columns[i].type != it->second.g->col_types[i])
+            fail(PRISM_ERR_SCHEMA_MISMATCH, "Schema mismatch: column " + std::to_string(i) + " type differs from the plan");
+        R->cols.assign(req->columns, req->columns + nc);
+        R->gbp = &it->second;
+        break;
+      }
+      case PRISM_REQ_PROBE:
+        if (!s->tables.count(req->target)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown table");
+        if (req->n_rows && !req->probe_keys) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL probe keys");
+        if (req->n_rows >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: probe rows exceed u32 row indices");
+        break;
+      case PRISM_REQ_FILTER: {
+        auto it = s->filt_plans.find(req->target);
+        if (it == s->filt_plans.end()) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown filter plan");
+        const size_t nc = it->second.col_types.size();
+        if (nc && !req->columns) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL columns");
+        for (size_t i = 0; i < nc; ++i)
+          if (req->columns[i].type != it->second.col_types[i])
+            fail(PRISM_ERR_SCHEMA_MISMATCH, "Schema mismatch: column " + std::to_string(i) + " type differs from the plan");
+        R->cols.assign(req->columns, req->columns + nc);
+        break;
+      }
+      default:
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown request kind");
+    }
+    R->ticket = s->next_ticket++;
+    *ticket = R->ticket;
+    Req* raw = R.release();
+    s->reqs[raw->ticket] = raw;
+    s->pending.push_back(raw);
+    ++s->submitted;
+    s->cv_work.notify_one();
+  });
+}
+
+static Req* find_req(prism_service* s, uint64_t ticket) {
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown ticket");
+  return it->second;
+}
+
+prism_status prism_service_poll(prism_service* s, uint64_t ticket, int32_t* done) {
+  return guarded([&] {
+    if (!s || !done) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service/output");
+    std::lock_guard<std::mutex> lock(s->mu);
+    *done = find_req(s, ticket)->done ? 1 : 0;
+  });
+}
+
+prism_status prism_service_wait(prism_service* s, uint64_t ticket, prism_request_result* result) {
+  return guarded([&] {
+    if (!s || !result) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service/result");
+    std::unique_lock<std::mutex> lock(s->mu);
+    Req* R = find_req(s, ticket);
+    s->cv_done.wait(lock, [&] { return R->done; });
+    *result = R->res;
+  });
+}
+
+prism_status prism_service_read_groups(prism_service* s, uint64_t ticket, const prism_groupby_output* out) {
+  return guarded([&] {
+    if (!s || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service/output");
+    std::lock_guard<std::mutex> lock(s->mu);
+    Req* R = find_req(s, ticket);
+    if (!R->done || R->r.kind != PRISM_REQ_GROUPBY)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: not a finished group-by request");
+    gb::convert_rows(R->gbp->g, R->rows.data(), R->res.count, out);
+  });
+}
+
+prism_status prism_service_release(prism_service* s, uint64_t ticket) {
+  return guarded([&] {
+    if (!s) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service");
+    std::lock_guard<std::mutex> lock(s->mu);
+    Req* R = find_req(s, ticket);
+    if (!R->done) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: request still running");
+    s->reqs.erase(ticket);
+    delete R;
+  });
+}
+
+prism_status prism_service_drain(prism_service* s) {
+  return guarded([&] {
+    if (!s) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service");
+    std::unique_lock<std::mutex> lock(s->mu);
+    s->cv_done.wait(lock, [&] { return s->completed >= s->submitted; });
+  });
+}
+
+prism_status prism_service_stats_get(prism_service* s, prism_service_stats* out) {
+  return guarded([&] {
+    if (!s || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service/output");
+    std::lock_guard<std::mutex> lock(s->mu);
+    memset(out, 0, sizeof(*out));
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->batches = s->batches;
+    out->tables_built = s->built;
+    out->tables_evicted = s->evicted;
+    out->resident_table_bytes = s->resident_bytes;
+    out->graph_launches = s->graph_launches;
+    out->mean_batch_requests = s->batches ? static_cast<double>(s->batch_reqs) / s->batches : 0.0;
+    std::vector<double> l = s->latencies;
+    if (!l.empty()) {
+      auto pct = [&](double p) {
+        size_t k = static_cast<size_t>(p * (l.size() - 1) + 0.5);
+        std::nth_element(l.begin(), l.begin() + k, l.end());
+        return l[k];
+      };
+      out->p50_ms = pct(0.50);
+      out->p99_ms = pct(0.99);
+    }
+    for (int k = 0; k < 3; ++k) {
+      const Tuner& T = s->tuner[k];
+      const int i = T.best >= 0 ? T.best : 0;
+      out->tile_rows[k] = T.cands[i].tile_rows;
+      out->blocks_per_cu[k] = T.cands[i].bpc;
+      out->tuned[k] = T.best >= 0 ? 1 : 0;
+    }
+  });
+}
+
+prism_status prism_service_reset_stats(prism_service* s) {
+  return guarded([&] {
+    if (!s) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service");
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->latencies.clear();
+  });
+}
+
+}  // extern "C"

@@ -147,6 +147,39 @@ class ExprNode(Structure):
                 ("n_clauses", c_uint32), ("else_expr", c_int32)]
 
 
+class ServiceConfig(Structure):
+    _fields_ = [("hbm_budget", c_uint64), ("max_inflight", c_uint32), ("max_batch_requests", c_uint32),
+                ("max_batch_rows", c_uint64), ("arena_bytes", c_uint64), ("use_graphs", c_int32),
+                ("autotune", c_int32)]
+
+
+class Request(Structure):
+    _fields_ = [("kind", c_int32), ("target", c_uint32), ("columns", POINTER(Column)), ("probe_keys", c_void_p),
+                ("probe_payload", c_void_p), ("n_rows", c_uint64), ("out_mask", c_void_p)]
+
+
+class RequestResult(Structure):
+    _fields_ = [("status", c_int32), ("kind", c_int32), ("n_rows", c_uint64), ("count", c_uint64),
+                ("probe_sum_lo", c_uint64), ("probe_sum_hi", c_int64), ("build_sum_lo", c_uint64),
+                ("build_sum_hi", c_int64), ("latency_ms", c_double), ("queue_ms", c_double), ("batch_id", c_uint64)]
+
+    @property
+    def probe_sum(self) -> int:
+        return (self.probe_sum_hi << 64) + self.probe_sum_lo
+
+    @property
+    def build_sum(self) -> int:
+        return (self.build_sum_hi << 64) + self.build_sum_lo
+
+
+class ServiceStats(Structure):
+    _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("batches", c_uint64), ("tables_built", c_uint64),
+                ("tables_evicted", c_uint64), ("resident_table_bytes", c_uint64), ("graph_launches", c_uint64),
+                ("p50_ms", c_double), ("p99_ms", c_double), ("mean_batch_requests", c_double),
+                ("tile_rows", c_uint32 * 3), ("blocks_per_cu", c_uint32 * 3), ("tuned", c_int32 * 3)]
+
+
+REQ = dict(GROUPBY=1, PROBE=2, FILTER=3)
 SCALAR = dict(INT32=0, INT64=1, FLOAT64=2, UTF8=3, BOOLEAN=4, DATE32=5)
 PRED = dict(LITERAL=0, EQ=1, NE=2, LT=3, LE=4, GT=5, GE=6, IS_NULL=7, IS_NOT_NULL=8, AND=9, OR=10, NOT=11,
             LIKE=12, ILIKE=13)
@@ -210,6 +243,20 @@ _SIGNATURES = [
     ("prism_join_agg_read", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("prism_join_agg_reset", c_int32, [c_void_p]),
     ("prism_join_agg_destroy", None, [c_void_p]),
+    ("prism_service_create", c_int32, [c_void_p, POINTER(ServiceConfig), POINTER(c_void_p)]),
+    ("prism_service_destroy", None, [c_void_p]),
+    ("prism_service_register_table", c_int32, [c_void_p, c_uint32, c_void_p, c_void_p, c_uint64]),
+    ("prism_service_register_groupby", c_int32, [c_void_p, c_uint32, POINTER(GroupbyPlan), c_uint64]),
+    ("prism_service_register_filter", c_int32, [c_void_p, c_uint32, POINTER(Instr), c_uint32, POINTER(PString),
+                                                c_uint32, POINTER(c_int32), c_uint32]),
+    ("prism_service_submit", c_int32, [c_void_p, POINTER(Request), POINTER(c_uint64)]),
+    ("prism_service_poll", c_int32, [c_void_p, c_uint64, POINTER(c_int32)]),
+    ("prism_service_wait", c_int32, [c_void_p, c_uint64, POINTER(RequestResult)]),
+    ("prism_service_read_groups", c_int32, [c_void_p, c_uint64, POINTER(GroupbyOutput)]),
+    ("prism_service_release", c_int32, [c_void_p, c_uint64]),
+    ("prism_service_drain", c_int32, [c_void_p]),
+    ("prism_service_stats_get", c_int32, [c_void_p, POINTER(ServiceStats)]),
+    ("prism_service_reset_stats", c_int32, [c_void_p]),
     ("prism_evaluate_predicate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(PredNode), c_uint32, c_void_p,
                                            c_void_p]),
     ("prism_filter_batch", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -222,6 +269,8 @@ _SIGNATURES = [
     ("prism_project_batch_with_exprs", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_uint32), c_uint32,
                                                  POINTER(ExprNode), c_uint32, POINTER(c_int32), c_uint32,
                                                  POINTER(PredNode), c_uint32, c_void_p, c_void_p]),
+    ("prism_filter_mask", c_int32, [c_void_p, POINTER(Instr), c_uint32, POINTER(PString), c_uint32, POINTER(c_int32),
+                                    POINTER(Column), c_uint32, c_uint64, c_void_p, POINTER(c_uint64)]),
     ("prism_string_like", c_int32, [c_void_p, c_void_p, c_void_p, c_char_p, c_uint32, c_int32, c_void_p,
                                     c_void_p]),
     ("prism_datagen_uniform_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_int64,
@@ -232,6 +281,8 @@ _SIGNATURES = [
     ("prism_datagen_permutation_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32]),
     ("prism_datagen_zipf_i64", c_int32, [c_void_p, c_void_p, c_uint64, c_uint64, c_uint32, c_void_p,
                                          c_uint64, c_uint32]),
+    ("prism_datagen_lower_utf8", c_int32, [c_void_p, c_void_p, c_uint64, c_void_p, c_uint64, c_uint64, c_uint32,
+                                           c_uint32, c_uint32, POINTER(c_uint64)]),
     ("prism_datagen_flag_utf8", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_uint32,
                                           c_char_p, c_uint32]),
 ]

@@ -187,3 +187,14 @@ def skewed_join(n_build: int, n_probe: int, n_values: int, n_attr: int, seed: in
         "probe_payload": uniform_i64(count, seed, 34, 0, 1_000_000, start=start),
         "cdf": cdf,
     }
+
+
+def lower_utf8(n: int, seed: int, stream: int, min_len: int, max_len: int):
+    """(bytes uint8, offsets int32[n+1]) of random lowercase strings; bit-identical
+    to prism_datagen_lower_utf8."""
+    lens = (min_len + uniform_from(rnd(seed, stream, 0, n), max_len - min_len + 1)).astype(np.int64)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    total = int(off[-1])
+    b = (ord("a") + uniform_from(rnd(seed, stream + 1, 0, total), 26)).astype(np.uint8)
+    return b, off.astype(np.int32)

@@ -295,6 +295,55 @@ def instrs(prog: Sequence[tuple]) -> list:
     return out
 
 
+def build_plan(column_types: Sequence[int], keys: Sequence[int], values: Sequence[Sequence[tuple]],
+               aggs: Sequence[tuple], filter_prog=None, strings: Sequence[bytes] = ()):
+    """prism_groupby_plan + the ctypes arrays it points to (keep them alive)."""
+    ct = (ctypes.c_int32 * max(len(column_types), 1))(*column_types)
+    kc = (ctypes.c_uint32 * max(len(keys), 1))(*keys)
+    flat, lens = [], []
+    for v in values:
+        ins = instrs(v)
+        flat += ins
+        lens.append(len(ins))
+    vals = (L.Instr * max(len(flat), 1))(*flat)
+    vl = (ctypes.c_uint32 * max(len(lens), 1))(*lens)
+    ag = (L.GroupbyAgg * max(len(aggs), 1))(*[L.GroupbyAgg(L.AGG[f] if isinstance(f, str) else f, v)
+                                              for f, v in aggs])
+    fl = instrs(filter_prog) if filter_prog else []
+    fa = (L.Instr * max(len(fl), 1))(*fl)
+    strs = [bytes(x) for x in strings]
+    st = (L.PString * max(len(strs), 1))(*[L.PString(x, len(x)) for x in strs])
+    plan = L.GroupbyPlan(len(column_types), ct, kc, len(keys), fa if fl else None, len(fl), vals, vl,
+                         len(lens), ag, len(aggs), st, len(strs))
+    return plan, (ct, kc, vals, vl, ag, fa, st, strs)
+
+
+def read_groups(read_fn, n_groups: int, n_aggs: int) -> dict:
+    """Call read_fn(prism_groupby_output*) and return the groups as numpy arrays
+    (int128 sums as Python ints)."""
+    n = max(n_groups, 1)
+    first = np.zeros(n, np.uint64)
+    khash = np.zeros(n, np.uint64)
+    rcount = np.zeros(n, np.uint64)
+    f64 = [np.zeros(n, np.float64) for _ in range(n_aggs)]
+    i64 = [np.zeros(n, np.int64) for _ in range(n_aggs)]
+    s128 = [np.zeros(2 * n, np.uint64) for _ in range(n_aggs)]
+    P = ctypes.POINTER
+    fp = (P(ctypes.c_double) * max(n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_double)) for a in f64])
+    ip = (P(ctypes.c_int64) * max(n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_int64)) for a in i64])
+    sp = (P(ctypes.c_uint64) * max(n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_uint64)) for a in s128])
+    out = L.GroupbyOutput(first.ctypes.data_as(P(c_uint64)), khash.ctypes.data_as(P(c_uint64)),
+                          rcount.ctypes.data_as(P(c_uint64)), fp, ip, sp)
+    read_fn(byref(out))
+    sums = []
+    for a in s128:
+        lo = a[0::2][:n_groups].astype(object)
+        hi = a[1::2][:n_groups].view(np.int64).astype(object)
+        sums.append([int(h) * (1 << 64) + int(l) for l, h in zip(lo, hi)])
+    return {"first_row": first[:n_groups], "key_hash": khash[:n_groups], "row_count": rcount[:n_groups],
+            "f64": [a[:n_groups] for a in f64], "i64": [a[:n_groups] for a in i64], "sum128": sums}
+
+
 class GroupBy:
     """prism_groupby: resident group state fed batch by batch."""
 
@@ -304,24 +353,7 @@ class GroupBy:
         self.ctx = ctx
         self.lib = ctx.lib
         self.n_aggs = len(aggs)
-        ct = (ctypes.c_int32 * len(column_types))(*column_types)
-        kc = (ctypes.c_uint32 * max(len(keys), 1))(*keys)
-        flat, lens = [], []
-        for v in values:
-            ins = instrs(v)
-            flat += ins
-            lens.append(len(ins))
-        vals = (L.Instr * max(len(flat), 1))(*flat)
-        vl = (ctypes.c_uint32 * max(len(lens), 1))(*lens)
-        ag = (L.GroupbyAgg * max(len(aggs), 1))(*[L.GroupbyAgg(L.AGG[f] if isinstance(f, str) else f, v)
-                                                  for f, v in aggs])
-        fl = instrs(filter_prog) if filter_prog else []
-        fa = (L.Instr * max(len(fl), 1))(*fl)
-        self._strings = [bytes(s) for s in strings]
-        st = (L.PString * max(len(strings), 1))(*[L.PString(s, len(s)) for s in self._strings])
-        plan = L.GroupbyPlan(len(column_types), ct, kc, len(keys), fa if fl else None, len(fl), vals, vl,
-                             len(lens), ag, len(aggs), st, len(strings))
-        self._keep = (ct, kc, vals, vl, ag, fa, st)
+        plan, self._keep = build_plan(column_types, keys, values, aggs, filter_prog, strings)
         self.handle = c_void_p()
         L.check(self.lib.prism_groupby_create(ctx.handle, byref(plan), expected_groups, byref(self.handle)))
 
@@ -338,27 +370,7 @@ class GroupBy:
         return int(n.value)
 
     def read(self, n_groups: int) -> dict:
-        n = max(n_groups, 1)
-        first = np.zeros(n, np.uint64)
-        khash = np.zeros(n, np.uint64)
-        rcount = np.zeros(n, np.uint64)
-        f64 = [np.zeros(n, np.float64) for _ in range(self.n_aggs)]
-        i64 = [np.zeros(n, np.int64) for _ in range(self.n_aggs)]
-        s128 = [np.zeros(2 * n, np.uint64) for _ in range(self.n_aggs)]
-        P = ctypes.POINTER
-        fp = (P(ctypes.c_double) * max(self.n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_double)) for a in f64])
-        ip = (P(ctypes.c_int64) * max(self.n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_int64)) for a in i64])
-        sp = (P(ctypes.c_uint64) * max(self.n_aggs, 1))(*[a.ctypes.data_as(P(ctypes.c_uint64)) for a in s128])
-        out = L.GroupbyOutput(first.ctypes.data_as(P(c_uint64)), khash.ctypes.data_as(P(c_uint64)),
-                              rcount.ctypes.data_as(P(c_uint64)), fp, ip, sp)
-        L.check(self.lib.prism_groupby_read(self.handle, byref(out)))
-        sums = []
-        for a in s128:
-            lo = a[0::2][:n_groups].astype(object)
-            hi = a[1::2][:n_groups].view(np.int64).astype(object)
-            sums.append([int(h) * (1 << 64) + int(l) for l, h in zip(lo, hi)])
-        return {"first_row": first[:n_groups], "key_hash": khash[:n_groups], "row_count": rcount[:n_groups],
-                "f64": [a[:n_groups] for a in f64], "i64": [a[:n_groups] for a in i64], "sum128": sums}
+        return read_groups(lambda out: L.check(self.lib.prism_groupby_read(self.handle, out)), n_groups, self.n_aggs)
 
     def close(self) -> None:
         if self.handle:
