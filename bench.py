@@ -17,6 +17,7 @@ The CPU baseline times the oracle's restatement of the reference join on a
 bounded sample (1M build rows, 8M probe rows) on one host core.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -219,6 +220,176 @@ def bench_join_agg(ctx, n_build: int, n_probe: int, n_values: int, n_attr: int, 
     return res
 
 
+def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
+    """Served request stream: Poisson arrivals, rows log-uniform in [4096, 16M];
+    50% group-by (config 0 and config 2 shapes), 40% probes against 8 resident
+    build tables of 1M..32M unique keys, 10% LIKE '%abc%' filters over a Utf8
+    column of 8..64 random lowercase bytes.  Inputs are device-resident (slices
+    of generated pools).  Each mode runs the same schedule: "served" submits to
+    the operator service (continuous batching, persistent tile-queue kernels,
+    graph replay), "plain" runs each request with the one-request-per-launch
+    operators in arrival order.  Latency = result on the host - arrival."""
+    import threading
+
+    import numpy as np
+
+    from prism_executor import _lib as L
+    from prism_executor import datagen as dg
+    from prism_executor import device as D
+    from prism_executor.service import Service, filter_mask
+
+    lib = ctx.lib
+    S0 = dg.SEED
+    NP = 16_000_000 + 4096
+    I64, I32, UTF8 = L.TYPE_INT64, L.TYPE_INT32, L.TYPE_UTF8
+    k0, v0 = D.empty(ctx, NP, np.int64), D.empty(ctx, NP, np.int64)
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, k0.ptr, NP, S0, 1, 0, 1000))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, v0.ptr, NP, S0, 2, 0, 1_000_000))
+    date, price, disc = D.empty(ctx, NP, np.int32), D.empty(ctx, NP, np.int64), D.empty(ctx, NP, np.int64)
+    fa, fao, fb, fbo = (D.empty(ctx, NP, np.uint8), D.empty(ctx, NP + 1, np.int32), D.empty(ctx, NP, np.uint8),
+                        D.empty(ctx, NP + 1, np.int32))
+    L.check(lib.prism_datagen_uniform_i32(ctx.handle, date.ptr, NP, S0, 20, 0, 2557))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, price.ptr, NP, S0, 21, 90_000, 10_410_000))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, disc.ptr, NP, S0, 22, 0, 11))
+    L.check(lib.prism_datagen_flag_utf8(ctx.handle, fa.ptr, fao.ptr, NP, S0, 25, dg.FLAG_A, len(dg.FLAG_A)))
+    L.check(lib.prism_datagen_flag_utf8(ctx.handle, fb.ptr, fbo.ptr, NP, S0, 26, dg.FLAG_B, len(dg.FLAG_B)))
+    pk, pp = D.empty(ctx, NP, np.int64), D.empty(ctx, NP, np.int64)
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.ptr, NP, S0, 40, 0, 64_000_000))
+    L.check(lib.prism_datagen_bits_i64(ctx.handle, pp.ptr, NP, S0, 41))
+    sb, so = D.empty(ctx, NP * 64, np.uint8), D.empty(ctx, NP + 1, np.int32)
+    tb = ctypes.c_uint64()
+    L.check(lib.prism_datagen_lower_utf8(ctx.handle, sb.ptr, NP * 64, so.ptr, NP, S0, 42, 8, 64, ctypes.byref(tb)))
+    sizes = [1, 2, 3, 5, 8, 12, 20, 32]
+    tables = []
+    for i, m in enumerate(sizes):
+        n = m * 1_000_000
+        kk, pl = D.empty(ctx, n, np.int64), D.empty(ctx, n, np.int32)
+        L.check(lib.prism_datagen_permutation_i64(ctx.handle, kk.ptr, n, S0, 70 + i))
+        L.check(lib.prism_datagen_uniform_i32(ctx.handle, pl.ptr, n, S0, 80 + i, -(1 << 31), 1 << 32))
+        tables.append((kk, pl))
+    ctx.sync()
+
+    GB0 = dict(column_types=[I64, I64], keys=[0], values=[[("LOAD_I", 1)]],
+               aggs=[("SUM", 0), ("COUNT", 0), ("MIN", 0), ("MAX", 0)])
+    GB2 = dict(column_types=[I32, I64, I64, UTF8, UTF8], keys=[3, 4], values=[REVENUE_EXACT],
+               aggs=[("SUM", 0), ("COUNT", 0), ("AVG", 0)], filter_prog=FILTER_LE_2470)
+    LIKE = [("LIKE", 0, 0)]
+
+    def gb0_cols(o):
+        return [D.column(I64, k0, offset=o), D.column(I64, v0, offset=o)]
+
+    def gb2_cols(o):
+        return [D.column(I32, date, offset=o), D.column(I64, price, offset=o), D.column(I64, disc, offset=o),
+                D.column(UTF8, fa, fao, offset=o), D.column(UTF8, fb, fbo, offset=o)]
+
+    def str_cols(o):
+        return [D.column(UTF8, sb, so, offset=o)]
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_requests):
+        n = int(np.exp(rng.uniform(np.log(4096), np.log(16_000_000))))
+        off = int(rng.integers(0, NP - n))
+        u = rng.random()
+        kind = "gb0" if u < 0.25 else "gb2" if u < 0.5 else "probe" if u < 0.9 else "filter"
+        reqs.append((kind, n, off, int(rng.integers(0, len(sizes)))))
+    total_rows = sum(r[1] for r in reqs)
+    gap = rng.exponential(1.0, n_requests)
+
+    # plain: device tables built once (resident, like the service's)
+    plain_tabs = [D.JoinTable(ctx, kk, payload=pl) for kk, pl in tables]
+
+    def run_plain(arrivals):
+        lat = []
+        t0 = time.perf_counter()
+        for (kind, n, off, t), a in zip(reqs, arrivals):
+            now = time.perf_counter() - t0
+            if now < a:
+                time.sleep(a - now)
+            if kind in ("gb0", "gb2"):
+                g = D.GroupBy(ctx, expected_groups=1000 if kind == "gb0" else 8, **(GB0 if kind == "gb0" else GB2))
+                g.update(gb0_cols(off) if kind == "gb0" else gb2_cols(off), n, 0)
+                k = g.finish()
+                g.read(k)
+                g.close()
+            elif kind == "probe":
+                plain_tabs[t].probe(pk[off:off + n], pp[off:off + n], None, want_pairs=False)
+            else:
+                filter_mask(ctx, LIKE, [UTF8], str_cols(off), n, None, strings=[b"%abc%"])
+            lat.append((time.perf_counter() - t0 - a) * 1e3)
+        return lat, time.perf_counter() - t0
+
+    S = Service(ctx, hbm_budget=96 << 30)
+    for i, (kk, pl) in enumerate(tables):
+        S.register_table(i, kk.numpy(), pl.numpy())
+    S.register_groupby(0, max_groups=1000, **GB0)
+    S.register_groupby(2, max_groups=8, **GB2)
+    S.register_filter(4, LIKE, [UTF8], strings=[b"%abc%"])
+
+    def run_served(arrivals):
+        tickets = [0] * len(reqs)
+        delay = [0.0] * len(reqs)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, ((kind, n, off, t), a) in enumerate(zip(reqs, arrivals)):
+                now = time.perf_counter() - t0
+                if now < a:
+                    time.sleep(a - now)
+                delay[i] = (time.perf_counter() - t0 - a) * 1e3
+                if kind == "gb0":
+                    tickets[i] = S.submit_groupby(0, gb0_cols(off), n)
+                elif kind == "gb2":
+                    tickets[i] = S.submit_groupby(2, gb2_cols(off), n)
+                elif kind == "probe":
+                    tickets[i] = S.submit_probe(t, pk[off:off + n], pp[off:off + n])
+                else:
+                    tickets[i] = S.submit_filter(4, str_cols(off), n)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        S.drain()
+        wall = time.perf_counter() - t0
+        lat = []
+        for i, tk in enumerate(tickets):
+            r = S.wait(tk)
+            if r.status != 0:
+                raise RuntimeError(f"served request {i} failed with status {r.status}")
+            lat.append(delay[i] + r.latency_ms)
+            S.release(tk)
+        return lat, wall
+
+    def summary(lat, wall):
+        a = np.asarray(lat)
+        return {"rows_per_s": total_rows / wall, "requests_per_s": len(reqs) / wall, "p50_ms": float(np.percentile(a, 50)),
+                "p99_ms": float(np.percentile(a, 99)), "wall_s": wall}
+
+    run_served(np.zeros(len(reqs)))  # warm-up: builds tables, captures graphs, tunes tiles
+    out = {"workload": f"{n_requests} requests, rows log-uniform [4096, 16M] (total {total_rows}), 50% group-by "
+                       "(config 0 / config 2 shapes), 40% probes of 8 resident tables (1M..32M unique keys), 10% "
+                       "LIKE '%abc%' over 8..64-byte lowercase Utf8; inputs resident in HBM",
+           "rates": {}}
+    for rate in rates:
+        arr = np.cumsum(gap) / rate
+        arr -= arr[0]
+        S.reset_stats()
+        served = summary(*run_served(arr))
+        served["service"] = {k: v for k, v in S.stats().items() if k.startswith("mean_") or k.startswith("max_")}
+        out["rates"][str(rate)] = {"served": served, "plain": summary(*run_plain(arr))}
+    z = np.zeros(len(reqs))
+    sat = {"served": summary(*run_served(z)), "plain": summary(*run_plain(z))}
+    sat["throughput_ratio"] = sat["served"]["rows_per_s"] / sat["plain"]["rows_per_s"]
+    out["saturation"] = sat
+    out["service_stats"] = S.stats()
+    S.close()
+    for t in plain_tabs:
+        t.close()
+    for a in [k0, v0, date, price, disc, fa, fao, fb, fbo, pk, pp, sb, so] + [x for tp in tables for x in tp]:
+        a.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +403,9 @@ def main() -> int:
     ap.add_argument("--no-groupby", action="store_true")
     ap.add_argument("--no-join-agg", action="store_true")
     ap.add_argument("--join-agg-probe-rows", type=int, default=256_000_000)
+    ap.add_argument("--no-service", action="store_true")
+    ap.add_argument("--svc-requests", type=int, default=1000)
+    ap.add_argument("--svc-rates", type=str, default="200,500,1000")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="also run D2D copies of this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -357,6 +531,8 @@ def main() -> int:
         line["join_aggregate"] = bench_join_agg(ctx, 16_000_000, args.join_agg_probe_rows, 8_000_000, 1_000_000,
                                                 max(2, min(args.steps, 5)),
                                                 0 if args.no_cpu_baseline else 8_000_000)
+    if not args.no_service:
+        line["served_stream"] = bench_service(ctx, args.svc_requests, [int(x) for x in args.svc_rates.split(",")])
     if args.calibrate_copy:
         src = D.empty(ctx, args.calibrate_copy, np.uint8)
         dst = D.empty(ctx, args.calibrate_copy, np.uint8)

@@ -481,6 +481,11 @@ typedef struct {
 typedef struct {
   uint64_t submitted, completed, batches, tables_built, tables_evicted, resident_table_bytes, graph_launches;
   double p50_ms, p99_ms, mean_batch_requests;
+  double mean_batch_gpu_ms, max_batch_gpu_ms;  /* stream time from batch start to its results on the host */
+  double mean_batch_launch_ms;                 /* host time to admit and launch a batch */
+  double mean_batch_detect_ms;                 /* batch end on the device -> seen by the dispatcher */
+  double mean_batch_rows;
+  double mean_batch_kernel_ms;                 /* batch start -> its kernels done (before the results copy) */
   uint32_t tile_rows[3];      /* per kind: group-by, probe, filter */
   uint32_t blocks_per_cu[3];
   int32_t tuned[3];

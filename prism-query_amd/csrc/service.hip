@@ -185,13 +185,25 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
     const ProbeDesc& d = C->pr[tl.req];
     unsigned long long m = 0;
     __int128 ps = 0, bs = 0;
-    for (uint64_t r = tl.start + threadIdx.x; r < tl.end; r += kSB) {
-      uint32_t val, cnt;
-      int32_t pay = 0;
-      if (join::lookup<KIND, true>(d.T, d.keys[r], val, cnt, pay)) {
-        ++m;
-        if (d.pay) ps += d.pay[r];
-        if (d.T.has_payload) bs += pay;
+    constexpr int U = 4;  // independent lookups in flight per thread
+    for (uint64_t r0 = tl.start + threadIdx.x; r0 < tl.end; r0 += static_cast<uint64_t>(kSB) * U) {
+      int64_t key[U];
+      bool in[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t r = r0 + static_cast<uint64_t>(u) * kSB;
+        in[u] = r < tl.end;
+        key[u] = in[u] ? d.keys[r] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        uint32_t val, cnt;
+        int32_t pay = 0;
+        if (in[u] && join::lookup<KIND, true>(d.T, key[u], val, cnt, pay)) {
+          ++m;
+          if (d.pay) ps += d.pay[r0 + static_cast<uint64_t>(u) * kSB];
+          if (d.T.has_payload) bs += pay;
+        }
       }
     }
     for (int k = 32; k > 0; k >>= 1) m += __shfl_xor(m, k, 64);
@@ -385,6 +397,9 @@ struct Tuner {
 struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
+  hipEvent_t ev_start = nullptr;
+  clk::time_point t_admit, t_launched;
+  uint64_t batch_rows = 0;
   DeviceBuffer arena;      // results area, then group tables
   DeviceBuffer control;    // Control
   Control* h_control = nullptr;
@@ -402,12 +417,18 @@ struct Slot {
     int cand;
   };
   std::vector<Timed> timed;
+  hipEvent_t ev_kern = nullptr;       // after the kernels, before the results copy
+  std::atomic<int> signalled{0};      // set by the stream's host callback
+  prism_service* owner = nullptr;
+  Slot() = default;
+  Slot(Slot&&) noexcept {}
 };
 
 }  // namespace svc
 }  // namespace prism
 
 struct prism_service {
+  void wake();
   prism::Ctx* ctx = nullptr;           // owned context: table builds, allocations
   prism_service_config cfg{};
   std::vector<prism::svc::Slot> slots;
@@ -423,13 +444,29 @@ struct prism_service {
   uint64_t completed = 0, submitted = 0, batch_reqs = 0;
   uint64_t resident_bytes = 0;
   std::vector<double> latencies;
+  double sum_gpu_ms = 0, max_gpu_ms = 0, sum_launch_ms = 0, sum_detect_ms = 0, sum_batch_rows = 0, sum_kern_ms = 0;
   bool stop = false;
+  std::mutex wake_mu;
+  bool woke = false;
   std::thread worker;
   std::string fatal;
 };
 
+void prism_service::wake() {
+  { std::lock_guard<std::mutex> l(wake_mu); woke = true; }
+  cv_work.notify_one();
+}
+
 namespace prism {
 namespace svc {
+
+// Runs on the runtime's callback thread when a batch's results are on the
+// host: flag the slot and wake the dispatcher (no HIP calls here).
+void on_batch_done(void* p) {
+  Slot* sl = static_cast<Slot*>(p);
+  sl->signalled.store(1);
+  sl->owner->wake();
+}
 
 static double ms_between(clk::time_point a, clk::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -546,6 +583,8 @@ static void launch_variant(int v, dim3 grid, size_t lds, hipStream_t st, Control
   }
 }
 
+void on_batch_done(void* p);
+
 // Build the batch's control block, launch it on the slot's stream.
 static void launch_batch(prism_service* S, Slot& sl) {
   Control* H = sl.h_control;
@@ -553,6 +592,18 @@ static void launch_batch(prism_service* S, Slot& sl) {
   const int n_cu = S->ctx->num_cus;
   std::vector<Tile> vt[kVariants];
   uint64_t vrows[kVariants] = {};
+  // Tile rows per kind: the tuned size, shrunk for small batches so that every
+  // workgroup slot of the persistent grid gets several tiles.
+  uint64_t kind_rows[3] = {};
+  for (Req* R : sl.reqs)
+    kind_rows[R->r.kind == PRISM_REQ_GROUPBY ? K_GB : R->r.kind == PRISM_REQ_PROBE ? K_PROBE : K_FILT] += R->r.n_rows;
+  uint64_t tile_of[3];
+  const uint64_t min_tile[3] = {65536, 16384, 16384};
+  for (int k = 0; k < 3; ++k) {
+    const Cand cd = cand_of(S, static_cast<Kind>(k));
+    const uint64_t want = kind_rows[k] / (static_cast<uint64_t>(n_cu) * cd.bpc * 4);
+    tile_of[k] = std::min<uint64_t>(cd.tile_rows, std::max<uint64_t>(min_tile[k], want));
+  }
   size_t lds[kVariants] = {};
   unsigned long long* res_base = sl.arena.as<unsigned long long>();
   unsigned long long* tab_base = res_base + kResultsBytes / 8;
@@ -596,9 +647,8 @@ static void launch_batch(prism_service* S, Slot& sl) {
         InitDesc& b = H->init[H->n_init++];
         b.dst = table + (gp.cap + 1) * stride;
         b.n_words = 2;
-        const Cand cd = cand_of(S, K_GB);
         const uint64_t step = static_cast<uint64_t>(kSB) * (gp.variant >= V_GB_288P ? 2 : 4);
-        const uint64_t tr = std::max<uint64_t>(step, cd.tile_rows / step * step);
+        const uint64_t tr = std::max<uint64_t>(step, tile_of[K_GB] / step * step);
         for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
           vt[gp.variant].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
         vrows[gp.variant] += R->r.n_rows;
@@ -617,7 +667,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
         a.dst = R->d_out;
         a.n_words = 5;
         const int v = R->table->t->info.kind == PRISM_TABLE_DENSE ? V_PROBE_DENSE : V_PROBE_HASHED;
-        const uint64_t tr = std::max<uint64_t>(kSB, cand_of(S, K_PROBE).tile_rows);
+        const uint64_t tr = std::max<uint64_t>(kSB, tile_of[K_PROBE]);
         for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
           vt[v].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
         vrows[v] += R->r.n_rows;
@@ -636,7 +686,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
         InitDesc& a = H->init[H->n_init++];
         a.dst = R->d_out;
         a.n_words = 2;
-        const uint64_t tr = std::max<uint64_t>(64 * 64, cand_of(S, K_FILT).tile_rows / 64 * 64);
+        const uint64_t tr = std::max<uint64_t>(64 * 64, tile_of[K_FILT] / 64 * 64);
         for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
           vt[fp.variant].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
         vrows[fp.variant] += R->r.n_rows;
@@ -663,6 +713,10 @@ static void launch_batch(prism_service* S, Slot& sl) {
   const size_t ctl_bytes = offsetof(Control, tiles) + off * sizeof(Tile);
   Control* dC = sl.control.as<Control>();
   hipStream_t st = sl.st;
+  uint64_t brows = 0;
+  for (int v = 0; v < kVariants; ++v) brows += vrows[v];
+  sl.batch_rows = brows;
+  PRISM_HIP(hipEventRecord(sl.ev_start, st));
   PRISM_HIP(hipMemcpyAsync(dC, H, ctl_bytes, hipMemcpyHostToDevice, st));
   const int init_grid = static_cast<int>(std::max<uint32_t>(1, std::min<uint32_t>(H->n_init, n_cu * 4)));
   const int fin_grid = static_cast<int>(std::max<uint32_t>(1, std::min<uint32_t>(H->n_gb, n_cu * 2)));
@@ -704,10 +758,14 @@ static void launch_batch(prism_service* S, Slot& sl) {
     issue(timed);
   }
   PRISM_CHECK_LAUNCH();
+  PRISM_HIP(hipEventRecord(sl.ev_kern, st));
   if (res_used) PRISM_HIP(hipMemcpyAsync(sl.h_results, res_base, res_used * 8, hipMemcpyDeviceToHost, st));
   sl.results_words = res_used;
   sl.table_words = tab_used;
   PRISM_HIP(hipEventRecord(sl.ev, st));
+  sl.signalled.store(0);
+  PRISM_HIP(hipLaunchHostFunc(st, on_batch_done, &sl));
+  sl.t_launched = clk::now();
   if (S->cfg.autotune)
     for (int k = 0; k < 3; ++k)
       if (S->tuner[k].best < 0) ++S->tuner[k].next;
@@ -716,6 +774,16 @@ static void launch_batch(prism_service* S, Slot& sl) {
 // Finish a completed batch (called with the service lock held).
 static void finalize(prism_service* S, Slot& sl) {
   const auto now = clk::now();
+  float gms = 0;
+  if (hipEventElapsedTime(&gms, sl.ev_start, sl.ev) == hipSuccess) {
+    S->sum_gpu_ms += gms;
+    S->max_gpu_ms = std::max<double>(S->max_gpu_ms, gms);
+  }
+  S->sum_launch_ms += ms_between(sl.t_admit, sl.t_launched);
+  S->sum_detect_ms += ms_between(sl.t_launched, now) - gms;
+  float kms = 0;
+  if (hipEventElapsedTime(&kms, sl.ev_start, sl.ev_kern) == hipSuccess) S->sum_kern_ms += kms;
+  S->sum_batch_rows += static_cast<double>(sl.batch_rows);
   for (Req* R : sl.reqs) {
     const unsigned long long* o = sl.h_results + R->res_off;
     prism_request_result& res = R->res;
@@ -842,7 +910,8 @@ static void dispatcher(prism_service* S) {
   while (true) {
     bool progress = false;
     for (Slot& sl : S->slots)
-      if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
+      if (sl.busy && (sl.signalled.load() || hipEventQuery(sl.ev) == hipSuccess)) {
+        PRISM_HIP(hipEventSynchronize(sl.ev));
         finalize(S, sl);
         progress = true;
       }
@@ -850,6 +919,7 @@ static void dispatcher(prism_service* S) {
     for (Slot& sl : S->slots) {
       if (sl.busy || S->pending.empty()) continue;
       try {
+        sl.t_admit = clk::now();
         if (admit_into(S, sl)) {
           sl.batch_id = ++S->batches;
           S->batch_reqs += sl.reqs.size();
@@ -873,8 +943,15 @@ static void dispatcher(prism_service* S) {
     for (Slot& sl : S->slots) any_busy = any_busy || sl.busy;
     if (S->stop && S->pending.empty() && !any_busy) break;
     if (!progress) {
-      if (any_busy) S->cv_work.wait_for(lock, std::chrono::microseconds(20));
-      else S->cv_work.wait_for(lock, std::chrono::milliseconds(5));
+      // woken by submit() or by a batch's completion callback
+      lock.unlock();
+      {
+        std::unique_lock<std::mutex> wl(S->wake_mu);
+        S->cv_work.wait_for(wl, any_busy ? std::chrono::microseconds(200) : std::chrono::milliseconds(5),
+                            [&] { return S->woke; });
+        S->woke = false;
+      }
+      lock.lock();
     }
   }
 }
@@ -910,7 +987,10 @@ prism_status prism_service_create(prism_ctx* ctx, const prism_service_config* cf
     S->slots.resize(c.max_inflight);
     for (Slot& sl : S->slots) {
       PRISM_HIP(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
-      PRISM_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+      PRISM_HIP(hipEventCreate(&sl.ev));
+      PRISM_HIP(hipEventCreate(&sl.ev_start));
+      PRISM_HIP(hipEventCreate(&sl.ev_kern));
+      sl.owner = S.get();
       sl.arena.alloc(S->ctx, c.arena_bytes);
       sl.control.alloc(S->ctx, sizeof(Control));
       PRISM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_control), sizeof(Control), hipHostMallocDefault));
@@ -935,7 +1015,7 @@ void prism_service_destroy(prism_service* s) {
     std::lock_guard<std::mutex> lock(s->mu);
     s->stop = true;
   }
-  s->cv_work.notify_all();
+  s->wake();
   if (s->worker.joinable()) s->worker.join();
   (void)hipSetDevice(s->ctx->device);
   for (Slot& sl : s->slots) {
@@ -946,6 +1026,8 @@ void prism_service_destroy(prism_service* s) {
     sl.arena.release();
     sl.control.release();
     (void)hipEventDestroy(sl.ev);
+    (void)hipEventDestroy(sl.ev_start);
+    (void)hipEventDestroy(sl.ev_kern);
     (void)hipStreamDestroy(sl.st);
   }
   for (auto& kv : s->tables)
@@ -1089,8 +1171,8 @@ prism_status prism_service_submit(prism_service* s, const prism_request* req, ui
     Req* raw = R.release();
     s->reqs[raw->ticket] = raw;
     s->pending.push_back(raw);
+    s->wake();
     ++s->submitted;
-    s->cv_work.notify_one();
   });
 }
 
@@ -1161,6 +1243,13 @@ prism_status prism_service_stats_get(prism_service* s, prism_service_stats* out)
     out->resident_table_bytes = s->resident_bytes;
     out->graph_launches = s->graph_launches;
     out->mean_batch_requests = s->batches ? static_cast<double>(s->batch_reqs) / s->batches : 0.0;
+    const double nb = s->batches ? static_cast<double>(s->batches) : 1.0;
+    out->mean_batch_gpu_ms = s->sum_gpu_ms / nb;
+    out->max_batch_gpu_ms = s->max_gpu_ms;
+    out->mean_batch_launch_ms = s->sum_launch_ms / nb;
+    out->mean_batch_detect_ms = s->sum_detect_ms / nb;
+    out->mean_batch_rows = s->sum_batch_rows / nb;
+    out->mean_batch_kernel_ms = s->sum_kern_ms / nb;
     std::vector<double> l = s->latencies;
     if (!l.empty()) {
       auto pct = [&](double p) {
@@ -1186,6 +1275,9 @@ prism_status prism_service_reset_stats(prism_service* s) {
     if (!s) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service");
     std::lock_guard<std::mutex> lock(s->mu);
     s->latencies.clear();
+    s->batches = 0;
+    s->batch_reqs = 0;
+    s->sum_gpu_ms = s->max_gpu_ms = s->sum_launch_ms = s->sum_detect_ms = s->sum_batch_rows = s->sum_kern_ms = 0;
   });
 }
 

@@ -176,6 +176,8 @@ class ServiceStats(Structure):
     _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("batches", c_uint64), ("tables_built", c_uint64),
                 ("tables_evicted", c_uint64), ("resident_table_bytes", c_uint64), ("graph_launches", c_uint64),
                 ("p50_ms", c_double), ("p99_ms", c_double), ("mean_batch_requests", c_double),
+                ("mean_batch_gpu_ms", c_double), ("max_batch_gpu_ms", c_double), ("mean_batch_launch_ms", c_double),
+                ("mean_batch_detect_ms", c_double), ("mean_batch_rows", c_double), ("mean_batch_kernel_ms", c_double),
                 ("tile_rows", c_uint32 * 3), ("blocks_per_cu", c_uint32 * 3), ("tuned", c_int32 * 3)]
 
 

@@ -94,6 +94,9 @@ class Service:
                 "tables_built": s.tables_built, "tables_evicted": s.tables_evicted,
                 "resident_table_bytes": s.resident_table_bytes, "graph_launches": s.graph_launches,
                 "p50_ms": s.p50_ms, "p99_ms": s.p99_ms, "mean_batch_requests": s.mean_batch_requests,
+                "mean_batch_gpu_ms": s.mean_batch_gpu_ms, "max_batch_gpu_ms": s.max_batch_gpu_ms,
+                "mean_batch_launch_ms": s.mean_batch_launch_ms, "mean_batch_detect_ms": s.mean_batch_detect_ms,
+                "mean_batch_rows": s.mean_batch_rows, "mean_batch_kernel_ms": s.mean_batch_kernel_ms,
                 "tile_rows": list(s.tile_rows), "blocks_per_cu": list(s.blocks_per_cu), "tuned": list(s.tuned)}
 
     def reset_stats(self) -> None:
