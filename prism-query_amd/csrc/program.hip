@@ -168,6 +168,18 @@ ValType check_program(const prism_instr* code, uint32_t len, const int32_t* col_
   return st[0];
 }
 
+// Shape of a LIKE pattern: '%' only first and/or last, no '_' and no '\'.
+static uint8_t like_kind(const uint8_t* p, int32_t n) {
+  for (int32_t i = 0; i < n; ++i)
+    if (p[i] == '_' || p[i] == '\\' || (p[i] == '%' && i != 0 && i != n - 1)) return LK_GENERAL;
+  const bool lead = n >= 1 && p[0] == '%', trail = n >= 2 && p[n - 1] == '%';
+  if (n == 1 && lead) return LK_CONTAINS;  // "%": any string
+  if (lead && trail) return LK_CONTAINS;
+  if (trail) return LK_PREFIX;
+  if (lead) return LK_SUFFIX;
+  return LK_EXACT;
+}
+
 void build_str_table(const prism_string* s, uint32_t n, StrTable& out) {
   if (n > static_cast<uint32_t>(kMaxStrings)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: too many string literals");
   memset(&out, 0, sizeof(out));
@@ -179,6 +191,7 @@ void build_str_table(const prism_string* s, uint32_t n, StrTable& out) {
     if (pos + static_cast<int64_t>(s[i].len) > kMaxStrBytes)
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: string literals exceed 1 KiB");
     memcpy(out.bytes + pos, s[i].data, s[i].len);
+    out.like_kind[i] = like_kind(reinterpret_cast<const uint8_t*>(s[i].data), static_cast<int32_t>(s[i].len));
     pos += static_cast<int32_t>(s[i].len);
   }
   out.off[n] = pos;

@@ -28,9 +28,14 @@ struct ColsArg {
   prism_column c[kMaxCols];
 };
 
+// LIKE pattern shapes with a direct scan (no wildcard but leading / trailing
+// '%', no '_', no escape); anything else runs the general matcher.
+enum LikeKind : uint8_t { LK_GENERAL = 0, LK_CONTAINS = 1, LK_PREFIX = 2, LK_SUFFIX = 3, LK_EXACT = 4 };
+
 struct StrTable {
   int32_t n;
   int32_t off[kMaxStrings + 1];
+  uint8_t like_kind[kMaxStrings];
   uint8_t bytes[kMaxStrBytes];
 };
 
@@ -130,6 +135,56 @@ __device__ __forceinline__ int utf8_len(uint8_t c) {
 // SQL LIKE: '%' any sequence, '_' one character, '\' escapes the next pattern
 // character.  Greedy match with backtracking to the last '%' (linear for the
 // common '%lit%' shapes).
+// Bytes equal (ASCII letters folded when ci) between s and lit.
+__device__ __forceinline__ bool bytes_eq(const uint8_t* s, const uint8_t* lit, int32_t m, bool ci) {
+  for (int32_t j = 0; j < m; ++j) {
+    const uint8_t a = s[j], b = lit[j];
+    if (ci ? fold_ascii(a) != fold_ascii(b) : a != b) return false;
+  }
+  return true;
+}
+
+// Does s[0, n) contain lit[0, m)?  Four bytes per load: a SWAR test marks the
+// bytes equal to lit[0] (folded for ILIKE), and only those starts are
+// verified.  The aligned words may reach up to 3 bytes outside the string,
+// never outside the 4-byte word holding its last byte.
+__device__ __forceinline__ bool contains_lit(const uint8_t* s, int32_t n, const uint8_t* lit, int32_t m, bool ci) {
+  if (m == 0) return true;
+  if (n < m) return false;
+  const uint8_t f0 = lit[0];
+  const bool letter = ((f0 | 0x20) >= 'a' && (f0 | 0x20) <= 'z');
+  const bool fold = ci && letter;
+  const uint32_t first = (fold ? (f0 | 0x20u) : f0) * 0x01010101u;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(s);
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(a & ~static_cast<uintptr_t>(3));
+  const int32_t lead = static_cast<int32_t>(a & 3);
+  const int32_t last_start = n - m;  // last candidate position
+  for (int32_t k = 0;; ++k) {
+    const int32_t p0 = 4 * k - lead;  // string position of byte 0 of this word
+    if (p0 > last_start) return false;
+    uint32_t x = w[k];
+    if (fold) x |= 0x20202020u;
+    const uint32_t y = x ^ first;
+    uint32_t hit = (y - 0x01010101u) & ~y & 0x80808080u;  // bytes of x equal to first (may over-report above a hit)
+    while (hit) {
+      const int32_t b = __builtin_ctz(hit) >> 3;
+      hit &= hit - 1;
+      const int32_t p = p0 + b;
+      if (p < 0 || p > last_start) continue;
+      if (bytes_eq(s + p + 1, lit + 1, m - 1, ci) && (ci ? fold_ascii(s[p]) == fold_ascii(f0) : s[p] == f0)) return true;
+    }
+  }
+}
+
+__device__ __forceinline__ bool like_fast(int kind, const uint8_t* s, int32_t n, const uint8_t* p, int32_t np, bool ci) {
+  switch (kind) {
+    case LK_CONTAINS: return np <= 2 || contains_lit(s, n, p + 1, np - 2, ci);  // "%" and "%%" match all
+    case LK_PREFIX: return n >= np - 1 && bytes_eq(s, p, np - 1, ci);
+    case LK_SUFFIX: return n >= np - 1 && bytes_eq(s + n - (np - 1), p + 1, np - 1, ci);
+    default: return n == np && bytes_eq(s, p, np, ci);  // LK_EXACT
+  }
+}
+
 __device__ inline bool like_match(const uint8_t* s, int32_t ns, const uint8_t* p, int32_t np, bool ci) {
   int32_t si = 0, pi = 0, star_p = -1, star_s = 0;
   while (si < ns) {
@@ -364,7 +419,9 @@ __device__ __forceinline__ void run_program(const prism_instr* __restrict__ code
               int k = str_cmp(s, e - b, lit, nlit);
               res = cmp_i(cop, k, 0);
             } else {
-              res = like_match(s, e - b, lit, nlit, op == PRISM_OP_ILIKE);
+              const int lk = strs->like_kind[sidx];
+              res = lk == LK_GENERAL ? like_match(s, e - b, lit, nlit, op == PRISM_OP_ILIKE)
+                                     : like_fast(lk, s, e - b, lit, nlit, op == PRISM_OP_ILIKE);
             }
             v |= col_valid(c, row[r]) ? (1u << r) : 0u;
           }

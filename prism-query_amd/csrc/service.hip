@@ -598,7 +598,9 @@ static void launch_batch(prism_service* S, Slot& sl) {
   for (Req* R : sl.reqs)
     kind_rows[R->r.kind == PRISM_REQ_GROUPBY ? K_GB : R->r.kind == PRISM_REQ_PROBE ? K_PROBE : K_FILT] += R->r.n_rows;
   uint64_t tile_of[3];
-  const uint64_t min_tile[3] = {65536, 16384, 16384};
+  bool gb_shared = false;  // the shared LDS form flushes every group per tile: keep its tiles larger
+  for (Req* R : sl.reqs) gb_shared = gb_shared || (R->r.kind == PRISM_REQ_GROUPBY && !R->gbp->priv);
+  const uint64_t min_tile[3] = {gb_shared ? 65536u : 16384u, 4096, 4096};
   for (int k = 0; k < 3; ++k) {
     const Cand cd = cand_of(S, static_cast<Kind>(k));
     const uint64_t want = kind_rows[k] / (static_cast<uint64_t>(n_cu) * cd.bpc * 4);

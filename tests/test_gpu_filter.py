@@ -88,6 +88,8 @@ PREDS = [
     fp.Like(3, "%abc%"), fp.Like(3, "_"), fp.Like(3, "h_llo"), fp.Like(3, "%"), fp.Like(3, "ab\\\\c"),
     fp.Like(3, "a\\%b"), fp.ILike(3, "HELLO"), fp.ILike(3, "%ABC%"), fp.Literal(False),
     fp.Not(fp.And(fp.Or(fp.Gt(0, SV.Int32(-20)), fp.Lt(1, SV.Int64(5))), fp.Not(fp.IsNull(2)))),
+    fp.Like(3, "he%"), fp.Like(3, "%bc"), fp.Like(3, "zz"), fp.Like(3, "%%"), fp.Like(3, ""), fp.ILike(3, "%é%"),
+    fp.ILike(3, "HE%"), fp.ILike(3, "%LO"),
 ]
 
 
@@ -204,3 +206,17 @@ def test_empty_batch(ctx):
     assert fp.filter_and_project(b, fp.Gt(0, SV.Int32(0)), [0, 3]).num_rows == 0
     assert len(fp.evaluate_scalar_expr(b, EXPRS[0])) == 0
     assert len(fp.evaluate_scalar_expr(b, EXPRS[6])) == 0
+
+
+@pytest.mark.parametrize("pattern,ci", [("%abc%", False), ("%ab%", False), ("abc%", False), ("%abc", False),
+                                        ("%a%", False), ("%xyzw%", False), ("%q%z%", False), ("%ABC%", True),
+                                        ("aB%", True)])
+def test_like_shapes_on_long_strings(ctx, pattern, ci):
+    say(f"like shapes {pattern} ci={ci}")
+    from prism_executor import datagen as dg
+    b, o = dg.lower_utf8(60_001, 11, 3, 1, 64)
+    strs = pa.StringArray.from_buffers(60_001, pa.py_buffer(o.tobytes()), pa.py_buffer(b.tobytes()))
+    for arr in (strs, strs.slice(3, 40_000), strs.slice(1, 17)):
+        v, m = fo.eval_pred(pa.record_batch([arr], names=["s"]), (fp.ILike if ci else fp.Like)(0, pattern))
+        got = (so.string_ilike if ci else so.string_like)(arr, pattern)
+        assert got.to_pylist() == fo.mask_array(v, m).to_pylist()
