@@ -121,6 +121,21 @@ prism_status prism_ctx_kernel_timing(prism_ctx* ctx, int32_t enable);
 prism_status prism_ctx_kernel_timing_read(prism_ctx* ctx, const char* kernel, double* total_ms,
                                           uint64_t* launches);
 
+/* Kernels generated at run time (no counterpart in the reference, whose
+ * operators are compiled Rust): group-by update kernels are specialised to
+ * their plan and compiled with hipRTC on first use; PRISM_JIT=0 in the
+ * environment runs the prebuilt interpreting kernels instead.  Counters of
+ * this process; a failed compile falls back to the prebuilt kernel and keeps
+ * the compiler log for prism_jit_last_error. */
+typedef struct {
+  uint64_t kernels;  /* distinct kernels compiled */
+  uint64_t failures; /* failed compiles / loads */
+  uint64_t hits;     /* requests served by a kernel compiled earlier */
+  double compile_ms; /* total compile + load time */
+} prism_jit_stats;
+prism_status prism_jit_get_stats(prism_jit_stats* out);
+const char* prism_jit_last_error(void);
+
 /* ------------------------------------------------------------------------- */
 /* Column descriptors (device layer: every pointer is a device pointer).      */
 /* ------------------------------------------------------------------------- */
@@ -312,6 +327,16 @@ prism_status prism_groupby_read(prism_groupby* g, const prism_groupby_output* ou
 /* Clear the state for reuse without freeing device memory. */
 prism_status prism_groupby_reset(prism_groupby* g);
 void prism_groupby_destroy(prism_groupby* g);
+
+/* Source of the generated group-by update kernel of `plan` for one launch
+ * shape (form 0 = wave pre-aggregation, 1 = per-row slots; bit c of
+ * `nullable` = column c carries a validity bitmap), copied NUL-terminated into
+ * `buf` (truncated to `cap` bytes); *len = full length.  With `compile` set
+ * it is also compiled for gfx950 (no device needed) and a compile error
+ * returns PRISM_ERR_EXECUTION with the compiler log as the last error.  A
+ * diagnostic with no counterpart in the reference. */
+prism_status prism_groupby_jit_source(const prism_groupby_plan* plan, int32_t form, uint32_t nullable,
+                                      int32_t compile, char* buf, uint64_t cap, uint64_t* len);
 
 
 /* ========================================================================= */

@@ -3,7 +3,9 @@
 #include <memory>
 #include <vector>
 
+#include "gb_jit.hpp"
 #include "groupby.hpp"
+#include "jit.hpp"
 
 namespace prism {
 namespace gb {
@@ -137,10 +139,67 @@ static uint64_t group_estimate(const prism_groupby* g) {
   return seen ? seen : g->expected;
 }
 
-// Rows [first, n_rows) of the update.
-static void launch(prism_groupby* g, const prism_column* cols, uint64_t first, uint64_t n_rows, uint64_t row_base) {
-  Ctx* c = g->ctx;
+// LDS table of the shared form for `known` groups (0 = unknown): slots, hash
+// shift and bytes; no LDS table when the groups cannot fit.
+static size_t shared_slots(const prism_groupby* g, uint64_t known, int& lds_slots, int& lds_shift) {
   const size_t sb = g->slot_bytes();
+  const uint64_t want = known ? known : 512;
+  uint64_t s = next_pow2(want * 2 < 64 ? 64 : want * 2);
+  uint64_t fit = 1;
+  while ((fit * 2) * sb <= kLdsBudget) fit *= 2;
+  if (s > fit) s = fit;
+  lds_slots = 0;
+  lds_shift = 0;
+  if (want <= s && s >= 16) {
+    lds_slots = static_cast<int>(s);
+    lds_shift = 64 - log2_pow2(s);
+  }
+  return static_cast<size_t>(lds_slots) * sb;
+}
+
+// Generated update kernel of this launch shape, compiled on first use; nullptr
+// when generated kernels are off or the compile failed.  A kernel whose
+// registers sit just above the budget of one more wave per SIMD is compiled
+// again with that occupancy asked for, and kept if it spills nothing.
+static hipFunction_t jit_kernel(prism_groupby* g, int form, int rows, uint32_t nullable) {
+  if (!jit::enabled()) return nullptr;
+  const uint64_t key = static_cast<uint64_t>(form) | (static_cast<uint64_t>(rows) << 4) |
+                       (static_cast<uint64_t>(nullable) << 16);
+  auto it = g->jit_fns.find(key);
+  if (it != g->jit_fns.end()) return it->second;
+  const int dev = g->ctx->device;
+  const std::string src = jit_groupby_source(g, form, rows, nullable);
+  hipFunction_t fn = jit::get(dev, src, kJitGroupbyKernel);
+  int regs = 0;
+  if (fn && hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn) == hipSuccess && regs > 0) {
+    constexpr int kVgprs = 512, kGranule = 8;
+    const int used = (regs + kGranule - 1) / kGranule * kGranule;
+    const int waves = kVgprs / used;
+    if (waves < 8) {
+      const int budget = kVgprs / (waves + 1) / kGranule * kGranule;
+      if ((used - budget) * 8 <= used) {
+        std::string hinted = src;
+        const size_t at = hinted.find(kJitWavesHint);
+        if (at != std::string::npos) {
+          hinted.insert(at + strlen(kJitWavesHint),
+                        " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves + 1) + ")))");
+          hipFunction_t f2 = jit::get(dev, hinted, kJitGroupbyKernel);
+          int local = 1;
+          if (f2 && hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f2) == hipSuccess && local == 0)
+            fn = f2;
+        }
+      }
+    }
+  }
+  g->jit_fns.emplace(key, fn);
+  return fn;
+}
+
+// Rows [first, n_rows) of the update; bit c of `nullable` = column c has a
+// validity bitmap in this update.
+static void launch(prism_groupby* g, const prism_column* cols, uint64_t first, uint64_t n_rows, uint64_t row_base,
+                   uint32_t nullable) {
+  Ctx* c = g->ctx;
   const uint64_t known = group_estimate(g);
   // Private form when the group count is known to be small.
   const uint64_t pw = static_cast<uint64_t>(g->plan.stride - kW_COUNT);
@@ -154,16 +213,33 @@ static void launch(prism_groupby* g, const prism_column* cols, uint64_t first, u
     lds_shift = 64 - log2_pow2(ps);
     lds_bytes = priv_bytes;
   } else {
-    uint64_t want = known ? known : 512;
-    uint64_t s = next_pow2(want * 2 < 64 ? 64 : want * 2);
-    uint64_t fit = 1;
-    while ((fit * 2) * sb <= kLdsBudget) fit *= 2;
-    if (s > fit) s = fit;
-    if (want <= s && s >= 16) {
-      lds_slots = static_cast<int>(s);
-      lds_shift = 64 - log2_pow2(s);
+    lds_bytes = shared_slots(g, known, lds_slots, lds_shift);
+  }
+  // Generated kernel: the shared-form LDS table; wave pre-aggregation when
+  // every row falls in one group.
+  {
+    const int form = (g->plan.n_keys == 0 || known == 1) ? kJitWave : kJitShared;
+    const int rows = g->plan.n_pref > 6 ? 2 : 4;
+    hipFunction_t fn = jit_kernel(g, form, rows, nullable);
+    if (fn) {
+      int slots = 0, shift = 0;
+      const size_t bytes = shared_slots(g, known, slots, shift);
+      int per_cu = 0;
+      PRISM_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, bytes));
+      if (per_cu < 1) per_cu = 1;
+      uint64_t blocks = ceil_div(n_rows - first, static_cast<uint64_t>(kBlock) * rows);
+      const uint64_t max_blocks = static_cast<uint64_t>(c->num_cus) * per_cu;
+      if (blocks > max_blocks) blocks = max_blocks;
+      if (blocks == 0) blocks = 1;
+      const DevPlan* P = g->d_plan.as<DevPlan>();
+      GTable G = g->view();
+      uint32_t* flags = g->flags_ptr();
+      void* args[] = {&P, &cols, &first, &n_rows, &row_base, &G, &slots, &shift, &flags};
+      KernelTimer kt(c, "groupby_update");
+      PRISM_HIP(hipModuleLaunchKernel(fn, static_cast<unsigned>(blocks), 1, 1, kBlock, 1, 1,
+                                      static_cast<unsigned>(bytes), c->stream, args, nullptr));
+      return;
     }
-    lds_bytes = lds_slots * sb;
   }
   const bool shallow = g->max_depth <= 4;
   const uint64_t rows_per_lane = shallow ? 4 : 2;
@@ -188,6 +264,147 @@ static void launch(prism_groupby* g, const prism_column* cols, uint64_t first, u
 }  // namespace gb
 }  // namespace prism
 
+namespace prism {
+namespace gb {
+
+// Checks `plan` and fills the host side of `g` (device plan image, column
+// types, aggregate lanes); touches no device.
+static void build_plan(prism_groupby* g, const prism_groupby_plan* plan) {
+  if (plan->n_columns > static_cast<uint32_t>(kMaxCols))
+    fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 input columns");
+  if (plan->n_keys > static_cast<uint32_t>(kMaxKeys))
+    fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 8 group-by columns");
+  if (plan->n_values > static_cast<uint32_t>(kMaxValues))
+    fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 value programs");
+  if (plan->n_columns && !plan->column_types) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL column types");
+  g->col_types.assign(plan->column_types, plan->column_types + plan->n_columns);
+  DevPlan& D = g->plan;
+  memset(&D, 0, sizeof(D));
+  build_str_table(plan->strings, plan->n_strings, D.strs);
+
+  D.n_keys = static_cast<int32_t>(plan->n_keys);
+  for (uint32_t k = 0; k < plan->n_keys; ++k) {
+    uint32_t col = plan->key_columns[k];
+    if (col >= plan->n_columns) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: group-by column out of range");
+    D.key_cols[k] = static_cast<int32_t>(col);
+  }
+  int32_t code_len = 0;
+  auto append = [&](const prism_instr* code, uint32_t len) {
+    if (code_len + static_cast<int64_t>(len) > kMaxCode)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: row programs exceed 160 instructions");
+    memcpy(&D.code[code_len], code, len * sizeof(prism_instr));
+    int32_t start = code_len;
+    code_len += static_cast<int32_t>(len);
+    return start;
+  };
+  if (plan->filter && plan->n_filter) {
+    int depth = 1;
+    ValType t = check_program(plan->filter, plan->n_filter, plan->column_types, plan->n_columns,
+                              plan->n_strings, &depth);
+    g->max_depth = depth > g->max_depth ? depth : g->max_depth;
+    if (t != VT_B && t != VT_NULL) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: filter is not boolean");
+    D.filter_start = append(plan->filter, plan->n_filter);
+    D.filter_len = static_cast<int32_t>(plan->n_filter);
+  }
+  D.n_values = static_cast<int32_t>(plan->n_values);
+  uint32_t off = 0;
+  for (uint32_t v = 0; v < plan->n_values; ++v) {
+    uint32_t len = plan->value_lengths[v];
+    int depth = 1;
+    ValType t = check_program(plan->values + off, len, plan->column_types, plan->n_columns, plan->n_strings,
+                              &depth);
+    g->max_depth = depth > g->max_depth ? depth : g->max_depth;
+    g->value_types.push_back(t);
+    D.value_start[v] = append(plan->values + off, len);
+    D.value_len[v] = static_cast<int32_t>(len);
+    off += len;
+  }
+  // Prefetch slots: fixed-width columns read by the programs or hashed as
+  // keys; their loads become kOpLoadPre of the slot.
+  auto fixed = [&](uint32_t col) { return plan->column_types[col] != PRISM_TYPE_UTF8; };
+  auto pref_slot = [&](uint32_t col) -> int32_t {
+    for (int32_t j = 0; j < D.n_pref; ++j)
+      if (D.pref_col[j] == static_cast<int32_t>(col)) return j;
+    if (D.n_pref >= kMaxPref) return -1;
+    D.pref_col[D.n_pref] = static_cast<int32_t>(col);
+    return D.n_pref++;
+  };
+  for (int32_t pc = 0; pc < code_len; ++pc) {
+    prism_instr& ins = D.code[pc];
+    if ((ins.op == PRISM_OP_LOAD_I || ins.op == PRISM_OP_LOAD_F) && fixed(static_cast<uint32_t>(ins.arg))) {
+      const int32_t j = pref_slot(static_cast<uint32_t>(ins.arg));
+      if (j >= 0) {
+        ins.op = kOpLoadPre;
+        ins.arg = j;
+      }
+    }
+  }
+  for (int32_t k = 0; k < D.n_keys; ++k)
+    D.key_pref[k] = fixed(static_cast<uint32_t>(D.key_cols[k])) ? pref_slot(static_cast<uint32_t>(D.key_cols[k])) : -1;
+  const bool global = plan->n_keys == 0;
+  D.skip_nulls = global ? 1 : 0;
+  int32_t word = kW_LANES;
+  D.tmpl[kW_KEY] = kEmptyKey;
+  D.tmpl[kW_FIRST] = kNoRow;
+  D.tmpl[kW_COUNT] = 0;
+  auto add_lane = [&](int32_t op, int32_t value, int32_t is_int) {
+    for (int32_t i = 0; i < D.n_lanes; ++i)  // SUM and AVG of one value share a lane
+      if (D.lanes[i].op == op && D.lanes[i].value == value) return i;
+    if (D.n_lanes >= kMaxLanes) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: too many aggregates");
+    Lane& L = D.lanes[D.n_lanes];
+    L.op = op;
+    L.value = value;
+    L.word = word;
+    L.is_int = is_int;
+    int words = op == L_SUM_I ? 2 : op == L_SUM_F ? 3 : 1;
+    for (int i = 0; i < words; ++i) D.tmpl[word + i] = 0;
+    if (op == L_MIN_F) D.tmpl[word] = f64_order(INFINITY);
+    if (op == L_MAX_F) D.tmpl[word] = f64_order(-INFINITY);
+    word += words;
+    return D.n_lanes++;
+  };
+  for (uint32_t a = 0; a < plan->n_aggs; ++a) {
+    const prism_groupby_agg& A = plan->aggs[a];
+    g->aggs.push_back(A);
+    int32_t ls = -1, lm = -1, lc = -1;
+    if (A.func == PRISM_AGG_COUNT_DISTINCT)
+      fail(PRISM_ERR_UNSUPPORTED_AGGREGATION,
+           "Aggregation error: unsupported function COUNT DISTINCT in the device aggregation");
+    if (A.func < PRISM_AGG_SUM || A.func > PRISM_AGG_MAX)
+      fail(PRISM_ERR_UNSUPPORTED_AGGREGATION, "Aggregation error: unsupported function " + std::to_string(A.func));
+    if ((A.value < 0 || static_cast<uint32_t>(A.value) >= plan->n_values) && A.func != PRISM_AGG_COUNT)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: aggregate value index out of range");
+    if (A.value >= 0 && static_cast<uint32_t>(A.value) >= plan->n_values)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: aggregate value index out of range");
+    int32_t vt = (A.value >= 0 && static_cast<uint32_t>(A.value) < plan->n_values) ? g->value_types[A.value] : VT_I;
+    if (vt == VT_NULL) vt = VT_I;
+    bool is_int = vt != VT_F;
+    switch (A.func) {
+      case PRISM_AGG_SUM:
+      case PRISM_AGG_AVG:
+        ls = add_lane(is_int ? L_SUM_I : L_SUM_F, A.value, is_int);
+        break;
+      case PRISM_AGG_MIN:
+        lm = add_lane(L_MIN_F, A.value, is_int);
+        break;
+      case PRISM_AGG_MAX:
+        lm = add_lane(L_MAX_F, A.value, is_int);
+        break;
+      default:
+        break;
+    }
+    if (global && A.func != PRISM_AGG_SUM && A.func != PRISM_AGG_MIN && A.func != PRISM_AGG_MAX && A.value >= 0)
+      lc = add_lane(L_CNT_VALID, A.value, is_int);
+    g->agg_lane_sum.push_back(ls);
+    g->agg_lane_mm.push_back(lm);
+    g->agg_lane_cnt.push_back(lc);
+  }
+  D.stride = word;
+}
+
+}  // namespace gb
+}  // namespace prism
+
 extern "C" {
 
 prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan, uint64_t expected_groups,
@@ -195,140 +412,12 @@ prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan
   return guarded([&] {
     if (!ctx || !plan || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/plan/output");
     *out = nullptr;
-    if (plan->n_columns > static_cast<uint32_t>(kMaxCols))
-      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 input columns");
-    if (plan->n_keys > static_cast<uint32_t>(kMaxKeys))
-      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 8 group-by columns");
-    if (plan->n_values > static_cast<uint32_t>(kMaxValues))
-      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: at most 16 value programs");
-    if (plan->n_columns && !plan->column_types) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL column types");
-    use_device(ctx);
     auto g = std::make_unique<prism_groupby>();
+    build_plan(g.get(), plan);
+    use_device(ctx);
     g->ctx = as_ctx(ctx);
     g->expected = expected_groups;
-    g->col_types.assign(plan->column_types, plan->column_types + plan->n_columns);
-    DevPlan& D = g->plan;
-    memset(&D, 0, sizeof(D));
-    build_str_table(plan->strings, plan->n_strings, D.strs);
-
-    D.n_keys = static_cast<int32_t>(plan->n_keys);
-    for (uint32_t k = 0; k < plan->n_keys; ++k) {
-      uint32_t col = plan->key_columns[k];
-      if (col >= plan->n_columns) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: group-by column out of range");
-      D.key_cols[k] = static_cast<int32_t>(col);
-    }
-    int32_t code_len = 0;
-    auto append = [&](const prism_instr* code, uint32_t len) {
-      if (code_len + static_cast<int64_t>(len) > kMaxCode)
-        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: row programs exceed 160 instructions");
-      memcpy(&D.code[code_len], code, len * sizeof(prism_instr));
-      int32_t start = code_len;
-      code_len += static_cast<int32_t>(len);
-      return start;
-    };
-    if (plan->filter && plan->n_filter) {
-      int depth = 1;
-      ValType t = check_program(plan->filter, plan->n_filter, plan->column_types, plan->n_columns,
-                                plan->n_strings, &depth);
-      g->max_depth = depth > g->max_depth ? depth : g->max_depth;
-      if (t != VT_B && t != VT_NULL) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: filter is not boolean");
-      D.filter_start = append(plan->filter, plan->n_filter);
-      D.filter_len = static_cast<int32_t>(plan->n_filter);
-    }
-    D.n_values = static_cast<int32_t>(plan->n_values);
-    uint32_t off = 0;
-    for (uint32_t v = 0; v < plan->n_values; ++v) {
-      uint32_t len = plan->value_lengths[v];
-      int depth = 1;
-      ValType t = check_program(plan->values + off, len, plan->column_types, plan->n_columns, plan->n_strings,
-                                &depth);
-      g->max_depth = depth > g->max_depth ? depth : g->max_depth;
-      g->value_types.push_back(t);
-      D.value_start[v] = append(plan->values + off, len);
-      D.value_len[v] = static_cast<int32_t>(len);
-      off += len;
-    }
-    // Prefetch slots: fixed-width columns read by the programs or hashed as
-    // keys; their loads become kOpLoadPre of the slot.
-    auto fixed = [&](uint32_t col) { return plan->column_types[col] != PRISM_TYPE_UTF8; };
-    auto pref_slot = [&](uint32_t col) -> int32_t {
-      for (int32_t j = 0; j < D.n_pref; ++j)
-        if (D.pref_col[j] == static_cast<int32_t>(col)) return j;
-      if (D.n_pref >= kMaxPref) return -1;
-      D.pref_col[D.n_pref] = static_cast<int32_t>(col);
-      return D.n_pref++;
-    };
-    for (int32_t pc = 0; pc < code_len; ++pc) {
-      prism_instr& ins = D.code[pc];
-      if ((ins.op == PRISM_OP_LOAD_I || ins.op == PRISM_OP_LOAD_F) && fixed(static_cast<uint32_t>(ins.arg))) {
-        const int32_t j = pref_slot(static_cast<uint32_t>(ins.arg));
-        if (j >= 0) {
-          ins.op = kOpLoadPre;
-          ins.arg = j;
-        }
-      }
-    }
-    for (int32_t k = 0; k < D.n_keys; ++k)
-      D.key_pref[k] = fixed(static_cast<uint32_t>(D.key_cols[k])) ? pref_slot(static_cast<uint32_t>(D.key_cols[k])) : -1;
-    const bool global = plan->n_keys == 0;
-    D.skip_nulls = global ? 1 : 0;
-    int32_t word = kW_LANES;
-    D.tmpl[kW_KEY] = kEmptyKey;
-    D.tmpl[kW_FIRST] = kNoRow;
-    D.tmpl[kW_COUNT] = 0;
-    auto add_lane = [&](int32_t op, int32_t value, int32_t is_int) {
-      for (int32_t i = 0; i < D.n_lanes; ++i)  // SUM and AVG of one value share a lane
-        if (D.lanes[i].op == op && D.lanes[i].value == value) return i;
-      if (D.n_lanes >= kMaxLanes) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: too many aggregates");
-      Lane& L = D.lanes[D.n_lanes];
-      L.op = op;
-      L.value = value;
-      L.word = word;
-      L.is_int = is_int;
-      int words = op == L_SUM_I ? 2 : op == L_SUM_F ? 3 : 1;
-      for (int i = 0; i < words; ++i) D.tmpl[word + i] = 0;
-      if (op == L_MIN_F) D.tmpl[word] = f64_order(INFINITY);
-      if (op == L_MAX_F) D.tmpl[word] = f64_order(-INFINITY);
-      word += words;
-      return D.n_lanes++;
-    };
-    for (uint32_t a = 0; a < plan->n_aggs; ++a) {
-      const prism_groupby_agg& A = plan->aggs[a];
-      g->aggs.push_back(A);
-      int32_t ls = -1, lm = -1, lc = -1;
-      if (A.func == PRISM_AGG_COUNT_DISTINCT)
-        fail(PRISM_ERR_UNSUPPORTED_AGGREGATION,
-             "Aggregation error: unsupported function COUNT DISTINCT in the device aggregation");
-      if (A.func < PRISM_AGG_SUM || A.func > PRISM_AGG_MAX)
-        fail(PRISM_ERR_UNSUPPORTED_AGGREGATION, "Aggregation error: unsupported function " + std::to_string(A.func));
-      if ((A.value < 0 || static_cast<uint32_t>(A.value) >= plan->n_values) && A.func != PRISM_AGG_COUNT)
-        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: aggregate value index out of range");
-      if (A.value >= 0 && static_cast<uint32_t>(A.value) >= plan->n_values)
-        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: aggregate value index out of range");
-      int32_t vt = (A.value >= 0 && static_cast<uint32_t>(A.value) < plan->n_values) ? g->value_types[A.value] : VT_I;
-      if (vt == VT_NULL) vt = VT_I;
-      bool is_int = vt != VT_F;
-      switch (A.func) {
-        case PRISM_AGG_SUM:
-        case PRISM_AGG_AVG:
-          ls = add_lane(is_int ? L_SUM_I : L_SUM_F, A.value, is_int);
-          break;
-        case PRISM_AGG_MIN:
-          lm = add_lane(L_MIN_F, A.value, is_int);
-          break;
-        case PRISM_AGG_MAX:
-          lm = add_lane(L_MAX_F, A.value, is_int);
-          break;
-        default:
-          break;
-      }
-      if (global && A.func != PRISM_AGG_SUM && A.func != PRISM_AGG_MIN && A.func != PRISM_AGG_MAX && A.value >= 0)
-        lc = add_lane(L_CNT_VALID, A.value, is_int);
-      g->agg_lane_sum.push_back(ls);
-      g->agg_lane_mm.push_back(lm);
-      g->agg_lane_cnt.push_back(lc);
-    }
-    D.stride = word;
+    const DevPlan& D = g->plan;
     // Lanes are visited per value program; keep the order stable.
     g->d_plan.alloc(g->ctx, sizeof(DevPlan));
     g->counters.alloc(g->ctx, 256);
@@ -343,6 +432,27 @@ prism_status prism_groupby_create(prism_ctx* ctx, const prism_groupby_plan* plan
   });
 }
 
+prism_status prism_groupby_jit_source(const prism_groupby_plan* plan, int32_t form, uint32_t nullable,
+                                      int32_t compile, char* buf, uint64_t cap, uint64_t* len) {
+  return guarded([&] {
+    if (!plan || !len) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL plan/output");
+    if (form != kJitWave && form != kJitShared) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown form");
+    prism_groupby g;
+    build_plan(&g, plan);
+    const std::string src = jit_groupby_source(&g, form, g.plan.n_pref > 6 ? 2 : 4, nullable);
+    *len = src.size();
+    if (buf && cap) {
+      const size_t n = src.size() < cap - 1 ? src.size() : cap - 1;
+      memcpy(buf, src.data(), n);
+      buf[n] = '\0';
+    }
+    if (compile) {
+      std::string code, err;
+      if (!jit::compile_code(src, "gfx950", code, err)) fail(PRISM_ERR_EXECUTION, "Execution error: " + err);
+    }
+  });
+}
+
 prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns, uint64_t n_rows, uint64_t row_base) {
   return guarded([&] {
     if (!g) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL group-by state");
@@ -353,6 +463,7 @@ prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns,
     g->finished = false;
     ColsArg cols;
     memset(&cols, 0, sizeof(cols));
+    uint32_t nullable = 0;
     for (size_t i = 0; i < g->col_types.size(); ++i) {
       if (columns[i].type != g->col_types[i])
         fail(PRISM_ERR_SCHEMA_MISMATCH, "Schema mismatch: column " + std::to_string(i) + " type differs from the plan");
@@ -360,6 +471,7 @@ prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns,
       if (columns[i].type == PRISM_TYPE_UTF8 && !columns[i].offsets)
         fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: Utf8 column without offsets");
       cols.c[i] = columns[i];
+      if (columns[i].validity) nullable |= 1u << i;
     }
     hipStream_t st = g->ctx->stream;
     g->d_cols.reserve(g->ctx, sizeof(ColsArg));
@@ -376,7 +488,7 @@ prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns,
         g->backup.reserve(g->ctx, g->table.bytes);
         PRISM_HIP(hipMemcpyAsync(g->backup.ptr, g->table.ptr, g->table.bytes, hipMemcpyDeviceToDevice, st));
       }
-      launch(g, dcols, done, end, row_base);
+      launch(g, dcols, done, end, row_base, nullable);
       uint64_t groups;
       uint32_t flags;
       read_counters(g, groups, flags);

@@ -147,6 +147,10 @@ class ExprNode(Structure):
                 ("n_clauses", c_uint32), ("else_expr", c_int32)]
 
 
+class JitStats(Structure):
+    _fields_ = [("kernels", c_uint64), ("failures", c_uint64), ("hits", c_uint64), ("compile_ms", c_double)]
+
+
 class ServiceConfig(Structure):
     _fields_ = [("hbm_budget", c_uint64), ("max_inflight", c_uint32), ("max_batch_requests", c_uint32),
                 ("max_batch_rows", c_uint64), ("arena_bytes", c_uint64), ("use_graphs", c_int32),
@@ -223,6 +227,10 @@ _SIGNATURES = [
     ("prism_groupby_read", c_int32, [c_void_p, POINTER(GroupbyOutput)]),
     ("prism_groupby_reset", c_int32, [c_void_p]),
     ("prism_groupby_destroy", None, [c_void_p]),
+    ("prism_groupby_jit_source", c_int32, [POINTER(GroupbyPlan), c_int32, c_uint32, c_int32, c_char_p, c_uint64,
+                                           POINTER(c_uint64)]),
+    ("prism_jit_get_stats", c_int32, [POINTER(JitStats)]),
+    ("prism_jit_last_error", c_char_p, []),
     ("prism_hash_table_build", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(c_uint32), c_uint32,
                                          POINTER(c_void_p)]),
     ("prism_hash_table_probe", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_uint32), c_uint32,

@@ -318,6 +318,33 @@ def build_plan(column_types: Sequence[int], keys: Sequence[int], values: Sequenc
     return plan, (ct, kc, vals, vl, ag, fa, st, strs)
 
 
+def jit_source(column_types: Sequence[int], keys: Sequence[int], values: Sequence[Sequence[tuple]],
+               aggs: Sequence[tuple], filter_prog=None, strings: Sequence[bytes] = (), form: int = 0,
+               nullable: int = 0, compile: bool = False) -> str:
+    """Source of the generated group-by update kernel of this plan (form 0 =
+    wave pre-aggregation, 1 = per-row slots; bit c of `nullable` = column c has
+    a validity bitmap).  compile=True also compiles it for gfx950 (no device
+    needed) and raises PrismError with the compiler log when that fails."""
+    lib = L.load()
+    plan, keep = build_plan(column_types, keys, values, aggs, filter_prog, strings)
+    n = c_uint64(0)
+    L.check(lib.prism_groupby_jit_source(byref(plan), form, nullable, 0, None, 0, byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    L.check(lib.prism_groupby_jit_source(byref(plan), form, nullable, 1 if compile else 0, buf, n.value + 1,
+                                         byref(n)))
+    del keep
+    return buf.value.decode()
+
+
+def jit_stats() -> dict:
+    """Counters of the kernels generated at run time in this process."""
+    lib = L.load()
+    s = L.JitStats()
+    L.check(lib.prism_jit_get_stats(byref(s)))
+    return {"kernels": s.kernels, "failures": s.failures, "hits": s.hits, "compile_ms": s.compile_ms,
+            "last_error": (lib.prism_jit_last_error() or b"").decode(errors="replace")}
+
+
 def read_groups(read_fn, n_groups: int, n_aggs: int) -> dict:
     """Call read_fn(prism_groupby_output*) and return the groups as numpy arrays
     (int128 sums as Python ints)."""

@@ -1,5 +1,7 @@
 """Hash aggregation on the GPU against the oracle (hash_aggregate.rs:182-447):
-groups in first-appearance order, FxHash group ids, exact integer sums."""
+groups in first-appearance order, FxHash group ids, exact integer sums.  Every
+test runs twice: with the update kernels generated per plan (hipRTC) and with
+the prebuilt interpreting kernels (PRISM_JIT=0)."""
 import math
 
 import numpy as np
@@ -16,6 +18,13 @@ from prism_executor import device as D
 pytestmark = pytest.mark.gpu
 
 I64, I32, F64, UTF8 = L.TYPE_INT64, L.TYPE_INT32, L.TYPE_FLOAT64, L.TYPE_UTF8
+
+
+@pytest.fixture(autouse=True, params=["generated", "prebuilt"])
+def engine(request, monkeypatch):
+    monkeypatch.setenv("PRISM_JIT", "1" if request.param == "generated" else "0")
+    say("engine", request.param)
+    return request.param
 
 
 def gpu_groupby(ctx, types, cols, keys, values, aggs, n, filt=None, strings=(), batches=None, expected=0):
@@ -278,3 +287,10 @@ def test_private_form_global_aggregates(ctx):
     assert out["sum128"][1][0] == exact_sum(vals[valid])
     for a in (3, 4):
         assert out["f64"][a][0] == ref["f64"][a][0]
+
+
+def test_generated_kernels_ran_without_compile_failures(ctx, engine):
+    st = D.jit_stats()
+    say("jit stats", st)
+    assert st["failures"] == 0, st["last_error"]
+    assert st["kernels"] > 0
