@@ -32,6 +32,7 @@ struct Stats {
   unsigned long long n_distinct;
   unsigned long long max_dup;
   unsigned long long sentinel;  // HASHED: build rows holding INT64_MIN
+  unsigned long long dups;      // DENSE speculative build: second claims of an entry
 };
 
 
@@ -102,6 +103,7 @@ __global__ void k_init_stats(Stats* st) {
   st->n_distinct = 0;
   st->max_dup = 0;
   st->sentinel = 0;
+  st->dups = 0;
 }
 
 __global__ void k_copy_sentinel(const uint32_t* cnt_of_sentinel, Stats* st) { st->sentinel = *cnt_of_sentinel; }
@@ -200,6 +202,30 @@ __global__ void k_hashed_unique_rows(Slot* slots, const uint32_t* __restrict__ s
 __global__ void k_dense_unique_rows(uint32_t* dense, const uint32_t* __restrict__ slot_of_row, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     dense[slot_of_row[i]] = static_cast<uint32_t>(i);
+}
+
+// DENSE build speculating unique keys: entries start absent ({kNone, 0} in the
+// payload form), every row claims its key's entry with a CAS, and a second
+// claim of an entry counts a duplicate (the table is then built the general
+// way).  With unique keys every entry has one writer, so the result does not
+// depend on the order of the claims.
+__global__ void k_dense_fill2(uint32_t* dense, uint2* dense_up, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    dense[i] = kNone;
+    if (dense_up) dense_up[i] = make_uint2(kNone, 0u);
+  }
+}
+__global__ void k_dense_claim(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key, uint32_t* dense,
+                              uint2* dense_up, const int32_t* __restrict__ pay, unsigned long long* dups) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
+    const uint32_t old = atomicCAS(&dense[d], kNone, static_cast<uint32_t>(i));
+    if (old != kNone) {
+      atomicAdd(dups, 1ULL);
+      continue;
+    }
+    if (dense_up) dense_up[d] = make_uint2(static_cast<uint32_t>(i), static_cast<uint32_t>(pay[i]));
+  }
 }
 
 __global__ void k_dense_pack(uint2* dense, const uint32_t* __restrict__ start, const uint32_t* __restrict__ cnt,
@@ -1218,7 +1244,46 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
     T->info.max_key = hs.max_key;
 
     DeviceBuffer slot_of_row(c, (n_rows ? n_rows : 1) * 4);
-    if (dense) {
+    bool fast = false;  // unique DENSE table built by the speculative claim (payload included)
+    if (dense && !keys_only && n_rows) {
+      T->dense.alloc(c, range * 4);
+      uint32_t* d = T->dense.as<uint32_t>();
+      uint2* up = nullptr;
+      if (d_payload) {
+        T->payload.alloc(c, range * 8);
+        up = T->payload.as<uint2>();
+      }
+      hipLaunchKernelGGL(k_dense_fill2, grid_for(ctx, range), dim3(kBuildBlock), 0, st, d, up, range);
+      hipLaunchKernelGGL(k_dense_claim, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, hs.min_key, d,
+                         up, d_payload, &dst->dups);
+      PRISM_CHECK_LAUNCH();
+      Stats h2;
+      PRISM_HIP(hipMemcpyAsync(&h2, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
+      PRISM_HIP(hipStreamSynchronize(st));
+      if (h2.dups == 0) {
+        fast = true;
+        T->info.kind = PRISM_TABLE_DENSE;
+        T->info.slots = range;
+        T->info.n_distinct = n_rows;
+        T->info.max_dup = 1;
+        T->info.unique = true;
+        T->view.dense_u = d;
+        T->view.min_key = hs.min_key;
+        T->view.range = range;
+        if (d_payload) {  // the packed {row, payload} entries replace the row array
+          T->view.has_payload = 1;
+          T->view.dense_up = up;
+          T->dense.release();
+          T->view.dense_u = nullptr;
+        }
+      } else {
+        T->dense.release();
+        T->payload.release();
+      }
+    }
+    if (fast) {
+      // built above
+    } else if (dense) {
       T->info.kind = PRISM_TABLE_DENSE;
       T->info.slots = range;
       DeviceBuffer cnt(c, range * 4);
@@ -1302,7 +1367,7 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       T->view.shift = shift;
       T->view.sentinel_present = hs.sentinel != 0;
     }
-    if (d_payload && n_rows) {  // carry the build payload in the table entries
+    if (d_payload && n_rows && !fast) {  // carry the build payload in the table entries
       T->view.has_payload = 1;
       if (!T->info.unique) {
         T->payload.alloc(c, n_rows * 4);
