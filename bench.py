@@ -366,6 +366,13 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
                 "p99_ms": float(np.percentile(a, 99)), "wall_s": wall}
 
     run_served(np.zeros(len(reqs)))  # warm-up: builds tables, captures graphs, tunes tiles
+    # warm-up of the plain operators: the first request of each plan compiles
+    # its generated kernels
+    for kind, n in (("gb0", 4096), ("gb2", 4096)):
+        g = D.GroupBy(ctx, expected_groups=1000 if kind == "gb0" else 8, **(GB0 if kind == "gb0" else GB2))
+        g.update(gb0_cols(0) if kind == "gb0" else gb2_cols(0), n, 0)
+        g.read(g.finish())
+        g.close()
     out = {"workload": f"{n_requests} requests, rows log-uniform [4096, 16M] (total {total_rows}), 50% group-by "
                        "(config 0 / config 2 shapes), 40% probes of 8 resident tables (1M..32M unique keys), 10% "
                        "LIKE '%abc%' over 8..64-byte lowercase Utf8; inputs resident in HBM",

@@ -508,7 +508,8 @@ typedef struct {
   double p50_ms, p99_ms, mean_batch_requests;
   double mean_batch_gpu_ms, max_batch_gpu_ms;  /* stream time from batch start to its results on the host */
   double mean_batch_launch_ms;                 /* host time to admit and launch a batch */
-  double mean_batch_detect_ms;                 /* batch end on the device -> seen by the dispatcher */
+  double mean_batch_detect_ms;                 /* host wall time after the launch returned beyond the
+                                                  batch's device time (completion detection), >= 0 */
   double mean_batch_rows;
   double mean_batch_kernel_ms;                 /* batch start -> its kernels done (before the results copy) */
   uint32_t tile_rows[3];      /* per kind: group-by, probe, filter */

@@ -433,5 +433,44 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
   return umax64(umax64(readlane_u64(x, 15), readlane_u64(x, 31)), umax64(readlane_u64(x, 47), readlane_u64(x, 63)));
 }
 
+
+// ---------------------------------------------------------------------------
+// Persistent tile-queue form of the update (the operator service): a
+// workgroup takes (request, row range) tiles from a queue until it is empty.
+// ---------------------------------------------------------------------------
+struct TileQueue {
+  uint32_t n_tiles;
+  uint32_t next;
+  uint32_t offset;  // first tile of this queue in the tile array
+  uint32_t pad;
+};
+
+struct RowTile {
+  uint32_t req;
+  uint32_t pad;
+  uint64_t start, end;
+};
+
+// One group-by request of a batch.
+struct TileReq {
+  const DevPlan* P;
+  const prism_column* cols;
+  uint64_t n;
+  GTable G;
+  int32_t lds_slots, lds_shift;
+  unsigned long long* out;  // n groups, flags, then rows (stride words each)
+  uint32_t* flags;
+  uint32_t pad;
+};
+
+// Next tile index of the workgroup (one atomic per tile).
+__device__ __forceinline__ uint32_t next_tile(TileQueue* q, uint32_t* s_t) {
+  if (threadIdx.x == 0) *s_t = atomicAdd(&q->next, 1u);
+  __syncthreads();
+  const uint32_t t = *s_t;
+  __syncthreads();
+  return t;
+}
+
 }  // namespace gb
 }  // namespace prism

@@ -16,8 +16,10 @@
 //            group (global aggregates), where all lanes would hit one LDS word.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "gb_jit.hpp"
@@ -63,17 +65,15 @@ class Gen {
           "using namespace prism;\nusing namespace prism::gb;\n"
        << "#define R_ " << R_ << "\n#define STRIDE " << D_.stride << "\n"
        << kJitWavesHint << "\n"
-       << "extern \"C\" __global__ __launch_bounds__(256) PRISM_WAVES_HINT void " << kJitGroupbyKernel
-       << "(const DevPlan* __restrict__ P, const prism_column* __restrict__ cols, uint64_t first, uint64_t n,\n"
-          "    uint64_t row_base, GTable G, int lds_slots, int lds_shift, uint32_t* flags) {\n"
-          "  extern __shared__ unsigned long long lds[];\n"
+       // rows it0, it0 + step, ... below n, each batch 256 * R_ rows of this workgroup
+       << "__device__ __forceinline__ void gb_body(const DevPlan* __restrict__ P, const prism_column* __restrict__ cols,\n"
+          "    uint64_t it0, uint64_t step, uint64_t n, uint64_t row_base, GTable G, unsigned long long* lds,\n"
+          "    int lds_slots, int lds_shift, uint32_t* flags) {\n"
           "  const int tid = threadIdx.x;\n"
           "  const int lane = tid & 63;\n"
           "  (void)lane;\n";
     declare_columns();
-    o_ << "  gb_init<false>(P, lds, lds_slots);\n"
-          "  const uint64_t step = static_cast<uint64_t>(gridDim.x) * 256 * R_;\n"
-          "  for (uint64_t it = first + static_cast<uint64_t>(blockIdx.x) * 256 * R_; it < n; it += step) {\n"
+    o_ << "  for (uint64_t it = it0; it < n; it += step) {\n"
           "    uint64_t row[R_];\n    bool in[R_];\n"
           "#pragma unroll\n"
           "    for (int r = 0; r < R_; ++r) { row[r] = it + r * 256 + tid; in[r] = row[r] < n; }\n"
@@ -87,7 +87,33 @@ class Gen {
     values();
     if (form_ == kJitWave) wave_accumulate();
     else row_accumulate("in[r]");
-    o_ << "  }\n  gb_flush<false>(P, G, lds, lds_slots, flags);\n}\n";
+    o_ << "  }\n}\n";
+    // one launch over rows [first, n): grid-stride batches
+    o_ << "extern \"C\" __global__ __launch_bounds__(256) PRISM_WAVES_HINT void " << kJitGroupbyKernel
+       << "(const DevPlan* __restrict__ P, const prism_column* __restrict__ cols, uint64_t first, uint64_t n,\n"
+          "    uint64_t row_base, GTable G, int lds_slots, int lds_shift, uint32_t* flags) {\n"
+          "  extern __shared__ unsigned long long lds[];\n"
+          "  gb_init<false>(P, lds, lds_slots);\n"
+          "  gb_body(P, cols, first + static_cast<uint64_t>(blockIdx.x) * 256 * R_, static_cast<uint64_t>(gridDim.x) * 256 * R_,\n"
+          "          n, row_base, G, lds, lds_slots, lds_shift, flags);\n"
+          "  gb_flush<false>(P, G, lds, lds_slots, flags);\n}\n";
+    // persistent tile-queue form: each workgroup takes (request, row range)
+    // tiles of requests of this plan until the queue is empty
+    o_ << "extern \"C\" __global__ __launch_bounds__(256) PRISM_WAVES_HINT void " << kJitGroupbyTiles
+       << "(TileQueue* q, const RowTile* __restrict__ tiles, const TileReq* __restrict__ reqs) {\n"
+          "  extern __shared__ unsigned long long lds[];\n"
+          "  __shared__ uint32_t s_t;\n"
+          "  const uint32_t n_tiles = q->n_tiles;\n"
+          "  while (true) {\n"
+          "    const uint32_t t = next_tile(q, &s_t);\n"
+          "    if (t >= n_tiles) break;\n"
+          "    const RowTile tl = tiles[q->offset + t];\n"
+          "    const TileReq& d = reqs[tl.req];\n"
+          "    gb_init<false>(d.P, lds, d.lds_slots);\n"
+          "    gb_body(d.P, d.cols, tl.start, 256 * R_, tl.end, 0, d.G, lds, d.lds_slots, d.lds_shift, d.flags);\n"
+          "    gb_flush<false>(d.P, d.G, lds, d.lds_slots, d.flags);\n"
+          "    __syncthreads();\n"
+          "  }\n}\n";
     return o_.str();
   }
 
@@ -610,6 +636,57 @@ class Gen {
 
 std::string jit_groupby_source(const prism_groupby* g, int form, int rows, uint32_t nullable) {
   return Gen(g, form, rows, nullable).run();
+}
+
+namespace {
+std::mutex g_kmu;
+std::unordered_map<std::string, JitKernels> g_kernels;
+}  // namespace
+
+JitKernels jit_kernels(const prism_groupby* g, int form, int rows, uint32_t nullable) {
+  if (!jit::enabled()) return {};
+  const int dev = g->ctx->device;
+  // the plan image is zero-filled before it is built, so its bytes name it
+  std::string key(reinterpret_cast<const char*>(&g->plan), sizeof(DevPlan));
+  key.append(reinterpret_cast<const char*>(g->col_types.data()), g->col_types.size() * sizeof(int32_t));
+  const int32_t shape[4] = {dev, form, rows, static_cast<int32_t>(nullable)};
+  key.append(reinterpret_cast<const char*>(shape), sizeof(shape));
+  std::lock_guard<std::mutex> lock(g_kmu);
+  auto it = g_kernels.find(key);
+  if (it != g_kernels.end()) return it->second;
+  const std::string src = jit_groupby_source(g, form, rows, nullable);
+  std::string used_src = src;
+  hipFunction_t fn = jit::get(dev, src, kJitGroupbyKernel);
+  int regs = 0;
+  // Registers just above the budget of one more wave per SIMD: compile again
+  // asking for that occupancy, keep it if nothing spills.
+  if (fn && hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn) == hipSuccess && regs > 0) {
+    constexpr int kVgprs = 512, kGranule = 8;
+    const int used = (regs + kGranule - 1) / kGranule * kGranule;
+    const int waves = kVgprs / used;
+    if (waves < 8) {
+      const int budget = kVgprs / (waves + 1) / kGranule * kGranule;
+      const size_t at = src.find(kJitWavesHint);
+      if ((used - budget) * 8 <= used && at != std::string::npos) {
+        std::string hinted = src;
+        hinted.insert(at + strlen(kJitWavesHint),
+                      " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves + 1) + ")))");
+        hipFunction_t f2 = jit::get(dev, hinted, kJitGroupbyKernel);
+        int local = 1;
+        if (f2 && hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f2) == hipSuccess && local == 0) {
+          fn = f2;
+          used_src = hinted;
+        }
+      }
+    }
+  }
+  JitKernels k;
+  if (fn) {
+    k.update = fn;
+    k.tiles = jit::get(dev, used_src, kJitGroupbyTiles);
+  }
+  g_kernels.emplace(std::move(key), k);
+  return k;
 }
 
 }  // namespace gb

@@ -141,7 +141,7 @@ static uint64_t group_estimate(const prism_groupby* g) {
 
 // LDS table of the shared form for `known` groups (0 = unknown): slots, hash
 // shift and bytes; no LDS table when the groups cannot fit.
-static size_t shared_slots(const prism_groupby* g, uint64_t known, int& lds_slots, int& lds_shift) {
+size_t shared_slots(const prism_groupby* g, uint64_t known, int& lds_slots, int& lds_shift) {
   const size_t sb = g->slot_bytes();
   const uint64_t want = known ? known : 512;
   uint64_t s = next_pow2(want * 2 < 64 ? 64 : want * 2);
@@ -155,44 +155,6 @@ static size_t shared_slots(const prism_groupby* g, uint64_t known, int& lds_slot
     lds_shift = 64 - log2_pow2(s);
   }
   return static_cast<size_t>(lds_slots) * sb;
-}
-
-// Generated update kernel of this launch shape, compiled on first use; nullptr
-// when generated kernels are off or the compile failed.  A kernel whose
-// registers sit just above the budget of one more wave per SIMD is compiled
-// again with that occupancy asked for, and kept if it spills nothing.
-static hipFunction_t jit_kernel(prism_groupby* g, int form, int rows, uint32_t nullable) {
-  if (!jit::enabled()) return nullptr;
-  const uint64_t key = static_cast<uint64_t>(form) | (static_cast<uint64_t>(rows) << 4) |
-                       (static_cast<uint64_t>(nullable) << 16);
-  auto it = g->jit_fns.find(key);
-  if (it != g->jit_fns.end()) return it->second;
-  const int dev = g->ctx->device;
-  const std::string src = jit_groupby_source(g, form, rows, nullable);
-  hipFunction_t fn = jit::get(dev, src, kJitGroupbyKernel);
-  int regs = 0;
-  if (fn && hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn) == hipSuccess && regs > 0) {
-    constexpr int kVgprs = 512, kGranule = 8;
-    const int used = (regs + kGranule - 1) / kGranule * kGranule;
-    const int waves = kVgprs / used;
-    if (waves < 8) {
-      const int budget = kVgprs / (waves + 1) / kGranule * kGranule;
-      if ((used - budget) * 8 <= used) {
-        std::string hinted = src;
-        const size_t at = hinted.find(kJitWavesHint);
-        if (at != std::string::npos) {
-          hinted.insert(at + strlen(kJitWavesHint),
-                        " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves + 1) + ")))");
-          hipFunction_t f2 = jit::get(dev, hinted, kJitGroupbyKernel);
-          int local = 1;
-          if (f2 && hipFuncGetAttribute(&local, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f2) == hipSuccess && local == 0)
-            fn = f2;
-        }
-      }
-    }
-  }
-  g->jit_fns.emplace(key, fn);
-  return fn;
 }
 
 // Rows [first, n_rows) of the update; bit c of `nullable` = column c has a
@@ -220,7 +182,7 @@ static void launch(prism_groupby* g, const prism_column* cols, uint64_t first, u
   {
     const int form = (g->plan.n_keys == 0 || known == 1) ? kJitWave : kJitShared;
     const int rows = g->plan.n_pref > 6 ? 2 : 4;
-    hipFunction_t fn = jit_kernel(g, form, rows, nullable);
+    hipFunction_t fn = jit_kernels(g, form, rows, nullable).update;
     if (fn) {
       int slots = 0, shift = 0;
       const size_t bytes = shared_slots(g, known, slots, shift);

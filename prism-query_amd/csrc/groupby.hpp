@@ -202,9 +202,6 @@ struct prism_groupby {
   uint64_t expected = 0;
   int max_depth = 1;
   bool finished = false;
-  // Generated update kernels by launch shape (form, rows per lane, nullable
-  // columns); nullptr = compile failed, the prebuilt kernel runs.
-  std::map<uint64_t, hipFunction_t> jit_fns;
 
   unsigned long long* n_groups_ptr() { return counters.as<unsigned long long>(); }
   uint32_t* flags_ptr() { return reinterpret_cast<uint32_t*>(counters.as<char>() + 8); }
@@ -226,5 +223,8 @@ namespace gb {
 // Convert finished group rows (stride words each, first-appearance order, on
 // the host) into the caller's output arrays (prism_groupby_read).
 void convert_rows(const prism_groupby* g, const unsigned long long* rows, uint64_t n, const prism_groupby_output* out);
+// LDS table of the shared form for `known` groups (0 = unknown): slots, hash
+// shift and bytes; no LDS table (0 slots) when the groups cannot fit.
+size_t shared_slots(const prism_groupby* g, uint64_t known, int& lds_slots, int& lds_shift);
 }  // namespace gb
 }  // namespace prism

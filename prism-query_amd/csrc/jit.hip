@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cstdlib>
+#include <map>
 #include <unordered_map>
 
 namespace prism {
@@ -24,8 +25,8 @@ const Header kHeaders[] = {
 constexpr int kNumHeaders = sizeof(kHeaders) / sizeof(kHeaders[0]);
 
 struct Entry {
-  hipModule_t module = nullptr;
-  hipFunction_t fn = nullptr;
+  hipModule_t module = nullptr;  // nullptr: the compile or load failed
+  std::map<std::string, hipFunction_t> fns;
 };
 
 std::mutex g_mu;
@@ -79,18 +80,13 @@ bool compile_code(const std::string& src, const std::string& arch, std::string& 
 
 namespace {
 
-bool compile(int device, const std::string& src, const char* name, Entry& out, std::string& err) {
+bool load(int device, const std::string& src, Entry& out, std::string& err) {
   std::string code;
   if (!compile_code(src, arch_of(device), code, err)) return false;
-  hipError_t e = hipModuleLoadData(&out.module, code.data());
+  const hipError_t e = hipModuleLoadData(&out.module, code.data());
   if (e != hipSuccess) {
     err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
-    return false;
-  }
-  e = hipModuleGetFunction(&out.fn, out.module, name);
-  if (e != hipSuccess) {
-    err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e);
-    (void)hipModuleUnload(out.module);
+    out.module = nullptr;
     return false;
   }
   return true;
@@ -104,25 +100,36 @@ bool enabled() {
 }
 
 hipFunction_t get(int device, const std::string& src, const char* name) {
-  std::string key = std::to_string(device) + '\n' + name + '\n' + src;
+  std::string key = std::to_string(device) + '\n' + src;
   std::lock_guard<std::mutex> lock(g_mu);
   auto it = g_cache.find(key);
   if (it != g_cache.end()) {
     ++g_stats.hits;
-    return it->second.fn;
+  } else {
+    const auto t0 = std::chrono::steady_clock::now();
+    Entry e;
+    std::string err;
+    if (!load(device, src, e, err)) {
+      ++g_stats.failures;
+      g_last_error = err;
+    } else {
+      g_stats.compile_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      ++g_stats.kernels;
+    }
+    it = g_cache.emplace(std::move(key), std::move(e)).first;
   }
-  const auto t0 = std::chrono::steady_clock::now();
-  Entry e;
-  std::string err;
-  if (!compile(device, src, name, e, err)) {
+  Entry& e = it->second;
+  if (!e.module) return nullptr;
+  auto f = e.fns.find(name);
+  if (f != e.fns.end()) return f->second;
+  hipFunction_t fn = nullptr;
+  if (hipModuleGetFunction(&fn, e.module, name) != hipSuccess) {
     ++g_stats.failures;
-    g_last_error = err;
-    return nullptr;
+    g_last_error = std::string("hipModuleGetFunction: no kernel ") + name;
+    fn = nullptr;
   }
-  g_stats.compile_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  ++g_stats.kernels;
-  g_cache.emplace(std::move(key), e);
-  return e.fn;
+  e.fns.emplace(name, fn);
+  return fn;
 }
 
 Stats stats() {

@@ -15,9 +15,9 @@ namespace jit {
 // interpreting kernels run instead).  Read at every call.
 bool enabled();
 
-// Compile (once per device and source) and load `name` from `src`.  On a
-// compile or load failure returns nullptr and keeps the compiler log as the
-// last JIT error.
+// Kernel `name` of `src`, compiled and loaded once per device and source (a
+// failed compile is remembered too).  On a compile or load failure returns
+// nullptr and keeps the compiler log as the last JIT error.
 hipFunction_t get(int device, const std::string& src, const char* name);
 
 // Compile `src` for `arch` (e.g. "gfx950") into a code object; no device is
@@ -25,7 +25,7 @@ hipFunction_t get(int device, const std::string& src, const char* name);
 bool compile_code(const std::string& src, const std::string& arch, std::string& code, std::string& err);
 
 struct Stats {
-  uint64_t kernels = 0;   // distinct kernels compiled
+  uint64_t kernels = 0;   // distinct sources compiled
   uint64_t failures = 0;  // failed compiles / loads
   uint64_t hits = 0;      // lookups served from the cache
   double compile_ms = 0;  // total compile + load time

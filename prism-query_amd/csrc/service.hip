@@ -34,7 +34,9 @@
 #include <unordered_map>
 #include <vector>
 
+#include "gb_jit.hpp"
 #include "groupby.hpp"
+#include "jit.hpp"
 #include "join.hpp"
 #include "program.hpp"
 
@@ -49,6 +51,7 @@ constexpr int kMaxReq = 256;           // requests per batch
 constexpr int kMaxTiles = 32768;       // tiles per batch
 constexpr int kFinishCap = 4096;       // group-table slots a finish workgroup orders in LDS
 constexpr int kVariants = 10;          // persistent-kernel variants (see Variant)
+constexpr int kJitQueues = 16;         // queues of generated group-by tile kernels per batch
 constexpr size_t kResultsBytes = 48u << 20;
 
 enum Variant : int {
@@ -65,16 +68,7 @@ struct FilterProg {
   StrTable strs;
 };
 
-struct GbDesc {
-  const gb::DevPlan* P;
-  const prism_column* cols;
-  uint64_t n;
-  gb::GTable G;
-  int32_t lds_slots, lds_shift;
-  unsigned long long* out;  // n groups, flags, then rows (stride words each)
-  uint32_t* flags;
-  uint32_t pad;
-};
+using GbDesc = gb::TileReq;
 
 struct ProbeDesc {
   join::TableView T;
@@ -92,18 +86,8 @@ struct FiltDesc {
   unsigned long long* count;  // selected rows, then error flags
 };
 
-struct Tile {
-  uint32_t req;
-  uint32_t pad;
-  uint64_t start, end;
-};
-
-struct Queue {
-  uint32_t n_tiles;
-  uint32_t next;
-  uint32_t offset;  // first tile of this variant in the tile array
-  uint32_t pad;
-};
+using Tile = gb::RowTile;
+using Queue = gb::TileQueue;
 
 struct InitDesc {
   unsigned long long* dst;
@@ -115,7 +99,7 @@ struct InitDesc {
 
 // Control block (device mirror of a pinned host staging block).
 struct Control {
-  Queue q[kVariants];
+  Queue q[kVariants + kJitQueues];  // prebuilt variants, then generated group-by kernels
   uint32_t n_init, n_gb, pad0, pad1;
   InitDesc init[2 * kMaxReq];
   GbDesc gb[kMaxReq];
@@ -127,13 +111,7 @@ struct Control {
 };
 
 // ------------------------------------------------------------------ kernels
-__device__ __forceinline__ uint32_t next_tile(Queue* q, uint32_t* s_t) {
-  if (threadIdx.x == 0) *s_t = atomicAdd(&q->next, 1u);
-  __syncthreads();
-  const uint32_t t = *s_t;
-  __syncthreads();
-  return t;
-}
+using gb::next_tile;
 
 __global__ __launch_bounds__(kSB) void k_svc_init(Control* C) {
   const uint32_t n = C->n_init;
@@ -358,6 +336,11 @@ struct GbPlan {
   int lds_slots = 0, lds_shift = 0;
   size_t lds_bytes = 0;
   uint64_t cap = 0;
+  // generated tile kernels (one per set of nullable columns): their form,
+  // rows per lane and shared-form LDS table
+  int jit_form = 0, jit_rows = 4;
+  int jit_slots = 0, jit_shift = 0;
+  size_t jit_lds = 0;
 };
 
 struct FiltPlan {
@@ -562,9 +545,13 @@ static void footprint(const Req* R, size_t& res_words, size_t& table_words) {
 }
 
 static std::string shape_key(const int (&grid)[kVariants], const size_t (&lds)[kVariants], int n_init_grid,
-                             int n_fin_grid) {
+                             int n_fin_grid, const std::vector<hipFunction_t>& jf, const std::vector<int>& jgrid,
+                             const std::vector<size_t>& jlds) {
   std::string k;
   for (int v = 0; v < kVariants; ++v) k += std::to_string(grid[v]) + ":" + std::to_string(lds[v]) + ",";
+  for (size_t i = 0; i < jf.size(); ++i)
+    k += std::to_string(reinterpret_cast<uintptr_t>(jf[i])) + "/" + std::to_string(jgrid[i]) + ":" +
+         std::to_string(jlds[i]) + ",";
   return k + std::to_string(n_init_grid) + "," + std::to_string(n_fin_grid);
 }
 
@@ -599,7 +586,8 @@ static void launch_batch(prism_service* S, Slot& sl) {
     kind_rows[R->r.kind == PRISM_REQ_GROUPBY ? K_GB : R->r.kind == PRISM_REQ_PROBE ? K_PROBE : K_FILT] += R->r.n_rows;
   uint64_t tile_of[3];
   bool gb_shared = false;  // the shared LDS form flushes every group per tile: keep its tiles larger
-  for (Req* R : sl.reqs) gb_shared = gb_shared || (R->r.kind == PRISM_REQ_GROUPBY && !R->gbp->priv);
+  for (Req* R : sl.reqs)
+    gb_shared = gb_shared || (R->r.kind == PRISM_REQ_GROUPBY && (!R->gbp->priv || jit::enabled()));
   const uint64_t min_tile[3] = {gb_shared ? 65536u : 16384u, 4096, 4096};
   for (int k = 0; k < 3; ++k) {
     const Cand cd = cand_of(S, static_cast<Kind>(k));
@@ -607,6 +595,11 @@ static void launch_batch(prism_service* S, Slot& sl) {
     tile_of[k] = std::min<uint64_t>(cd.tile_rows, std::max<uint64_t>(min_tile[k], want));
   }
   size_t lds[kVariants] = {};
+  // generated group-by tile kernels of this batch: one queue per kernel
+  std::vector<hipFunction_t> jf;
+  std::vector<std::vector<Tile>> jt;
+  std::vector<uint64_t> jrows;
+  std::vector<size_t> jlds;
   unsigned long long* res_base = sl.arena.as<unsigned long long>();
   unsigned long long* tab_base = res_base + kResultsBytes / 8;
   size_t res_used = 0, tab_used = 0;
@@ -637,9 +630,25 @@ static void launch_batch(prism_service* S, Slot& sl) {
         d.G.stride = stride;
         d.G.n_groups = table + (gp.cap + 1) * stride;
         d.flags = reinterpret_cast<uint32_t*>(table + (gp.cap + 1) * stride + 1);
-        d.lds_slots = gp.lds_slots;
-        d.lds_shift = gp.lds_shift;
         d.out = R->d_out;
+        uint32_t nullable = 0;
+        for (size_t c = 0; c < R->cols.size(); ++c)
+          if (R->cols[c].validity) nullable |= 1u << c;
+        hipFunction_t tf = gb::jit_kernels(gp.g, gp.jit_form, gp.jit_rows, nullable).tiles;
+        int qi = -1;
+        if (tf) {
+          for (size_t i = 0; i < jf.size(); ++i)
+            if (jf[i] == tf) qi = static_cast<int>(i);
+          if (qi < 0 && jf.size() < static_cast<size_t>(kJitQueues)) {
+            qi = static_cast<int>(jf.size());
+            jf.push_back(tf);
+            jt.emplace_back();
+            jrows.push_back(0);
+            jlds.push_back(0);
+          }
+        }
+        d.lds_slots = qi >= 0 ? gp.jit_slots : gp.lds_slots;
+        d.lds_shift = qi >= 0 ? gp.jit_shift : gp.lds_shift;
         H->gb_idx[H->n_gb++] = k;
         InitDesc& a = H->init[H->n_init++];
         a.dst = table;
@@ -649,12 +658,19 @@ static void launch_batch(prism_service* S, Slot& sl) {
         InitDesc& b = H->init[H->n_init++];
         b.dst = table + (gp.cap + 1) * stride;
         b.n_words = 2;
-        const uint64_t step = static_cast<uint64_t>(kSB) * (gp.variant >= V_GB_288P ? 2 : 4);
+        const uint64_t step =
+            static_cast<uint64_t>(kSB) * (qi >= 0 ? gp.jit_rows : (gp.variant >= V_GB_288P ? 2 : 4));
         const uint64_t tr = std::max<uint64_t>(step, tile_of[K_GB] / step * step);
+        std::vector<Tile>& dst = qi >= 0 ? jt[qi] : vt[gp.variant];
         for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
-          vt[gp.variant].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
-        vrows[gp.variant] += R->r.n_rows;
-        lds[gp.variant] = std::max(lds[gp.variant], gp.lds_bytes);
+          dst.push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
+        if (qi >= 0) {
+          jrows[qi] += R->r.n_rows;
+          jlds[qi] = std::max(jlds[qi], gp.jit_lds);
+        } else {
+          vrows[gp.variant] += R->r.n_rows;
+          lds[gp.variant] = std::max(lds[gp.variant], gp.lds_bytes);
+        }
         break;
       }
       case PRISM_REQ_PROBE: {
@@ -712,11 +728,26 @@ static void launch_batch(prism_service* S, Slot& sl) {
       grid[v] = static_cast<int>(std::min<uint64_t>(vt[v].size(), static_cast<uint64_t>(n_cu) * bpc));
     }
   }
+  std::vector<int> jgrid(jf.size(), 0);
+  for (size_t i = 0; i < jf.size(); ++i) {
+    Queue& q = H->q[kVariants + i];
+    q.n_tiles = static_cast<uint32_t>(jt[i].size());
+    q.next = 0;
+    q.offset = off;
+    if (off + jt[i].size() > static_cast<size_t>(kMaxTiles)) fail(PRISM_ERR_INTERNAL, "Internal error: batch tile overflow");
+    memcpy(&H->tiles[off], jt[i].data(), jt[i].size() * sizeof(Tile));
+    off += static_cast<uint32_t>(jt[i].size());
+    const Cand cd = cand_of(S, K_GB);
+    uint32_t bpc = cd.bpc;
+    if (jlds[i]) bpc = std::min<uint32_t>(bpc, std::max<uint32_t>(1, static_cast<uint32_t>((160u << 10) / jlds[i])));
+    jgrid[i] = static_cast<int>(std::min<uint64_t>(jt[i].size(), static_cast<uint64_t>(n_cu) * bpc));
+  }
   const size_t ctl_bytes = offsetof(Control, tiles) + off * sizeof(Tile);
   Control* dC = sl.control.as<Control>();
   hipStream_t st = sl.st;
   uint64_t brows = 0;
   for (int v = 0; v < kVariants; ++v) brows += vrows[v];
+  for (uint64_t r : jrows) brows += r;
   sl.batch_rows = brows;
   PRISM_HIP(hipEventRecord(sl.ev_start, st));
   PRISM_HIP(hipMemcpyAsync(dC, H, ctl_bytes, hipMemcpyHostToDevice, st));
@@ -739,10 +770,28 @@ static void launch_batch(prism_service* S, Slot& sl) {
         sl.timed.push_back(Slot::Timed{v, a, b, vrows[v], cand_index(S, kind_of(v))});
       }
     }
+    Tile* d_tiles = &dC->tiles[0];
+    GbDesc* d_gb = &dC->gb[0];
+    for (size_t i = 0; i < jf.size(); ++i) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (with_events) {
+        PRISM_HIP(hipEventCreate(&a));
+        PRISM_HIP(hipEventCreate(&b));
+        PRISM_HIP(hipEventRecord(a, st));
+      }
+      Queue* d_q = &dC->q[kVariants + i];
+      void* args[] = {&d_q, &d_tiles, &d_gb};
+      PRISM_HIP(hipModuleLaunchKernel(jf[i], static_cast<unsigned>(jgrid[i]), 1, 1, kSB, 1, 1,
+                                      static_cast<unsigned>(jlds[i]), st, args, nullptr));
+      if (with_events) {
+        PRISM_HIP(hipEventRecord(b, st));
+        sl.timed.push_back(Slot::Timed{V_GB_444S, a, b, jrows[i], cand_index(S, K_GB)});
+      }
+    }
     if (H->n_gb) hipLaunchKernelGGL(k_svc_gb_finish, dim3(fin_grid), dim3(kSB), 0, st, dC);
   };
   if (S->cfg.use_graphs && !timed) {
-    const std::string key = shape_key(grid, lds, init_grid, fin_grid);
+    const std::string key = shape_key(grid, lds, init_grid, fin_grid, jf, jgrid, jlds);
     auto it = sl.graphs.find(key);
     if (it == sl.graphs.end()) {
       hipGraph_t g;
@@ -782,7 +831,9 @@ static void finalize(prism_service* S, Slot& sl) {
     S->max_gpu_ms = std::max<double>(S->max_gpu_ms, gms);
   }
   S->sum_launch_ms += ms_between(sl.t_admit, sl.t_launched);
-  S->sum_detect_ms += ms_between(sl.t_launched, now) - gms;
+  // host wall time from the launch call's return to this point beyond the
+  // batch's device time (the device may start before the launch call returns)
+  S->sum_detect_ms += std::max(0.0, ms_between(sl.t_launched, now) - gms);
   float kms = 0;
   if (hipEventElapsedTime(&kms, sl.ev_start, sl.ev_kern) == hipSuccess) S->sum_kern_ms += kms;
   S->sum_batch_rows += static_cast<double>(sl.batch_rows);
@@ -1094,6 +1145,10 @@ prism_status prism_service_register_groupby(prism_service* s, uint32_t plan_id, 
       P.lds_bytes = sl * sb;
     }
     P.variant = gb_variant(g, P.priv);
+    P.jit_form = g->plan.n_keys == 0 ? gb::kJitWave : gb::kJitShared;
+    P.jit_rows = g->plan.n_pref > 6 ? 2 : 4;
+    P.jit_lds = gb::shared_slots(g, max_groups, P.jit_slots, P.jit_shift);
+    (void)gb::jit_kernels(g, P.jit_form, P.jit_rows, 0);  // null-free columns: compiled now
     std::lock_guard<std::mutex> lock(s->mu);
     auto it = s->gb_plans.find(plan_id);
     if (it != s->gb_plans.end()) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: plan id already registered");

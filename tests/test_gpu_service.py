@@ -2,7 +2,8 @@
 mix equals the one-request-per-launch operator's result on the same input
 (group-by exact words, probe matches and int128 sums, filter counts and
 bitmaps), with and without graph replay, under table eviction and from many
-submitting threads."""
+submitting threads.  Every test runs with the group-by tile kernels generated
+per plan and with the prebuilt interpreting ones (PRISM_JIT=0)."""
 import threading
 
 import numpy as np
@@ -25,6 +26,13 @@ GB2 = dict(column_types=[I32, I64, I64, UTF8, UTF8], keys=[3, 4],
            aggs=[("SUM", 0), ("COUNT", 0), ("AVG", 0)],
            filter_prog=[("LOAD_I", 0), ("CONST_I", 0, 2470), ("CMP_I", L.CMP["LE"])])
 LIKE_ABC = [("LIKE", 0, 0)]
+
+
+@pytest.fixture(autouse=True, params=["generated", "prebuilt"])
+def engine(request, monkeypatch):
+    monkeypatch.setenv("PRISM_JIT", "1" if request.param == "generated" else "0")
+    say("engine", request.param)
+    return request.param
 
 
 class Pools:
