@@ -120,14 +120,44 @@ __global__ void k_init_slots(Slot* s, uint64_t n) {
   }
 }
 
-// DENSE: count rows per key offset; remember each row's offset.
-__global__ void k_dense_count(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key,
-                              uint32_t* cnt, uint32_t* slot_of_row) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
-    slot_of_row[i] = d;
-    atomicAdd(&cnt[d], 1u);
+// DENSE: count rows per key offset; remember each row's offset.  Counts
+// gather in a workgroup LDS table first (skewed keys would otherwise
+// serialise on one counter); offsets it cannot place are counted in HBM.
+constexpr int kCountSlots = 4096, kCountProbes = 4;
+__global__ __launch_bounds__(256) void k_dense_count(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key,
+                                                     uint32_t* cnt, uint32_t* slot_of_row) {
+  __shared__ uint32_t sk[kCountSlots], sc[kCountSlots];
+  for (int i = threadIdx.x; i < kCountSlots; i += blockDim.x) {
+    sk[i] = kNone;
+    sc[i] = 0;
   }
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
+    if (slot_of_row) slot_of_row[i] = d;
+    uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(d) * 0x9E3779B97F4A7C15ULL) >> 52);  // 12 bits
+    bool placed = false;
+    for (int p = 0; p < kCountProbes && d != kNone; ++p) {
+      const uint32_t cur = __hip_atomic_load(&sk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (cur == kNone) {
+        const uint32_t old = atomicCAS(&sk[h], kNone, d);
+        if (old == kNone || old == d) {
+          atomicAdd(&sc[h], 1u);
+          placed = true;
+          break;
+        }
+      } else if (cur == d) {
+        atomicAdd(&sc[h], 1u);
+        placed = true;
+        break;
+      }
+      h = (h + 1) & (kCountSlots - 1);
+    }
+    if (!placed) atomicAdd(&cnt[d], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kCountSlots; i += blockDim.x)
+    if (sk[i] != kNone) atomicAdd(&cnt[sk[i]], sc[i]);
 }
 
 // HASHED: claim a slot per distinct key (CAS on the key word), count rows.
@@ -970,16 +1000,45 @@ __device__ __forceinline__ int64_t slot_of(const TableView& T, int64_t k, bool u
   }
 }
 
+// Probe-side aggregates per key slot, one array per word (the atomics of a
+// row spread over memory channels): row count, int128 payload sum, first row.
 struct JaAcc {
   unsigned long long* cnt;    // per key slot
   unsigned long long* sum;    // per key slot: lo, hi words
   unsigned long long* first;  // per key slot: first probe row
 };
 
+// Probe key -> key slot.  A DENSE table tests presence in a bitmap of the
+// key range (1 bit per key: small enough to stay in L2) when one is given.
+template <int KIND>
+__device__ __forceinline__ int64_t probe_slot(const TableView& T, int64_t k, bool unique, const uint32_t* present) {
+  if constexpr (KIND == PRISM_TABLE_DENSE) {
+    if (!present) return slot_of<KIND>(T, k, unique);
+    const uint64_t d = static_cast<uint64_t>(k) - static_cast<uint64_t>(T.min_key);
+    if (d >= T.range) return -1;
+    return ((present[d >> 5] >> (d & 31)) & 1u) ? static_cast<int64_t>(d) : -1;
+  } else {
+    return slot_of<KIND>(T, k, unique);
+  }
+}
+
+// Presence bitmap of a DENSE table: bit d = some build row has key min + d.
+__global__ void k_presence_bits(TableView T, uint32_t* bits, uint64_t n_words) {
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < n_words; w += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    for (int b = 0; b < 32; ++b) {
+      const uint64_t d = w * 32 + b;
+      if (d < T.range && slot_of<PRISM_TABLE_DENSE>(T, static_cast<int64_t>(d + static_cast<uint64_t>(T.min_key)), true) >= 0)
+        v |= 1u << b;
+    }
+    bits[w] = v;
+  }
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kJaBlock) void k_ja_probe(TableView T, bool unique, const int64_t* __restrict__ keys,
                                                        const int64_t* __restrict__ pay, uint64_t n, uint64_t row_base,
-                                                       JaAcc A) {
+                                                       JaAcc A, const uint32_t* __restrict__ present) {
   __shared__ uint32_t ck[kJaCache];
   __shared__ unsigned long long cc[kJaCache], clo[kJaCache], cf[kJaCache];
   __shared__ long long chi[kJaCache];
@@ -993,7 +1052,7 @@ __global__ __launch_bounds__(kJaBlock) void k_ja_probe(TableView T, bool unique,
   __syncthreads();
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kJaBlock;
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kJaBlock) + threadIdx.x; r < n; r += stride) {
-    const int64_t s = slot_of<KIND>(T, keys[r], unique);
+    const int64_t s = probe_slot<KIND>(T, keys[r], unique, present);
     if (s < 0) continue;
     const int64_t x = pay ? pay[r] : 0;
     const unsigned long long row = row_base + r;
@@ -1120,6 +1179,7 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     p[i] = v;
 }
 
+
 // min / max of an int32 column (two stages, no contended atomics).
 __global__ void k_minmax_i32_part(const int32_t* __restrict__ v, uint64_t n, int64_t* __restrict__ part) {
   __shared__ int64_t lds[16];
@@ -1151,6 +1211,7 @@ struct prism_join_agg {
   prism::DeviceBuffer acc;     // cnt | sum (2 words) | first, per key slot
   prism::DeviceBuffer groups;  // cnt | sum | first | key, per group
   prism::DeviceBuffer counters;
+  prism::DeviceBuffer present;  // DENSE: presence bitmap of the key range
   int64_t attr_min = 0;
   uint64_t n_groups_cap = 0;
   bool dense_groups = true;
@@ -1538,6 +1599,13 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
     J->n_slots = t->info.kind == PRISM_TABLE_DENSE ? (t->info.slots ? t->info.slots : 1) : t->info.slots + 2;
     if (J->n_slots >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: build side too large for the join aggregate");
     J->acc.alloc(c, J->n_slots * 4 * 8);
+    if (t->info.kind == PRISM_TABLE_DENSE && n_build) {
+      const uint64_t words = ceil_div(t->info.slots, 32);
+      J->present.alloc(c, words * 4);
+      hipLaunchKernelGGL(k_presence_bits, grid_for(c, words), dim3(kBuildBlock), 0, ctx->stream, t->view,
+                         J->present.as<uint32_t>(), words);
+      PRISM_CHECK_LAUNCH();
+    }
     J->counters.alloc(c, 64);
     // attribute range: dense groups when it is small, else a hashed group table
     hipStream_t st = ctx->stream;
@@ -1593,10 +1661,12 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     KernelTimer kt(J->ctx, "join_agg_probe");
     if (J->table->info.kind == PRISM_TABLE_DENSE)
       hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_DENSE>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0,
-                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs());
+                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(),
+                         J->present.as<uint32_t>());
     else
       hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0,
-                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs());
+                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(),
+                         static_cast<const uint32_t*>(nullptr));
     PRISM_CHECK_LAUNCH();
   });
 }
