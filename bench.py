@@ -28,9 +28,12 @@ sys.path.insert(0, os.path.join(ROOT, "prism-query_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each dominant kernel on the default workloads, from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script, each scaled by
-# the counter's reading for a plain device copy of known size (profiles/).
-TRAFFIC = {}
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script (each counter
+# scaled by its reading for a plain 1 GiB device copy: FETCH 2.0, WRITE 1.0),
+# recorded in profiles/bench_default_hbm_traffic.md.  Measured offline, not by
+# this run: they hold for the default sizes only.
+TRAFFIC = {"join_probe": 3_494_900_000, "groupby_update": 6_373_400_000}
+DEFAULT_SIZES = (1_000_000, 64_000_000, 200_000_000)
 METRIC = "rows/s & %HBM peak (join probe, group-by); p99 latency batched vs plain"
 
 
@@ -109,7 +112,7 @@ def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
            "rows_per_s": n / wall, "update_ms": upd_ms,
            "roofline": {"bound": "hbm", "kernel": "groupby_update", "achieved": n * 30 / (upd_ms * 1e-3) / 1e9,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": n * 30 / (upd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                        "traffic": TRAFFIC.get("groupby_update")}}
+                        "traffic": TRAFFIC.get("groupby_update") if n == DEFAULT_SIZES[2] else None}}
     if cpu_sample:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyarrow as pa
@@ -519,7 +522,8 @@ def main() -> int:
                                f"rows (int64 payload) -> ordered row-index pairs + int128 payload sums",
                    "build_rows": nb, "probe_rows": npr, "table": "dense" if info.kind == L.TABLE_DENSE else "hashed"},
         "roofline": {"bound": "hbm", "kernel": "join_probe", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC.get("join_probe")},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC.get("join_probe") if (nb, npr) == DEFAULT_SIZES[:2] else None},
         "kernels": kernels,
         "probe_ms": probe_ms,
         "probe_GBps": alg_bytes / (probe_ms * 1e-3) / 1e9,
