@@ -252,3 +252,25 @@ def test_errors(ctx, world):
     with pytest.raises(L.PrismError):
         S.register_groupby(0, max_groups=5000, **GB0)
     S.close()
+
+
+def test_nullable_columns_match_plain(ctx, world):
+    say("service nullable columns")
+    P, _, _ = world
+    rng = np.random.default_rng(5)
+    valid = to_dev(np.packbits(rng.random(P.n) > 0.1, bitorder="little"))
+    S = Service(ctx)
+    S.register_groupby(0, max_groups=1000, **GB0)
+    for n, off in [(50_000, 0), (300_000, 123_457), (1_000_000, 64)]:
+        cols = [D.column(I64, P.k0, validity=valid, offset=off), D.column(I64, P.v0, validity=valid, offset=off)]
+        tk = S.submit_groupby(0, cols, n)
+        r = S.wait(tk)
+        assert r.status == 0
+        got = S.read_groups(tk, 0, r.count)
+        S.release(tk)
+        g = D.GroupBy(ctx, expected_groups=8, **GB0)
+        g.update(cols, n, 0)
+        ref = g.read(g.finish())
+        g.close()
+        assert same_groups(got, ref), (n, off)
+    S.close()
