@@ -109,6 +109,7 @@ struct prism_ctx {
   prism::DeviceBuffer scratch;
   void* pinned = nullptr;   // small pinned host staging for results
   size_t pinned_bytes = 0;
+  void* readback = nullptr;  // 4 KiB pinned block for small device -> host reads under `mu`
   hipEvent_t t0 = nullptr, t1 = nullptr;  // timer markers
   // Stream-ordered cache of device blocks released by library-owned buffers:
   // a block is reused only by work queued later on the context's stream, so
@@ -131,6 +132,10 @@ void use_device(prism_ctx* ctx);
 
 // Pinned host staging of at least n bytes owned by the context.
 void* pinned_staging(prism_ctx* ctx, size_t n);
+
+// Pinned 4 KiB block for small synchronous device -> host reads (statistics,
+// counters); callers hold ctx->mu.
+void* readback_block(prism_ctx* ctx);
 
 // Return every cached block to the device (hipFree).
 void pool_trim(prism_ctx* ctx);

@@ -127,6 +127,11 @@ void* pinned_staging(prism_ctx* ctx, size_t n) {
   return ctx->pinned;
 }
 
+void* readback_block(prism_ctx* ctx) {
+  if (!ctx->readback) PRISM_HIP(hipHostMalloc(&ctx->readback, 4096, hipHostMallocDefault));
+  return ctx->readback;
+}
+
 }  // namespace prism
 
 using namespace prism;
@@ -175,6 +180,7 @@ void prism_ctx_destroy(prism_ctx* ctx) {
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->readback) (void)hipHostFree(c->readback);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }

@@ -235,27 +235,68 @@ __global__ void k_dense_unique_rows(uint32_t* dense, const uint32_t* __restrict_
 }
 
 // DENSE build speculating unique keys: entries start absent ({kNone, 0} in the
-// payload form), every row claims its key's entry with a CAS, and a second
-// claim of an entry counts a duplicate (the table is then built the general
-// way).  With unique keys every entry has one writer, so the result does not
-// depend on the order of the claims.
+// payload form), every row stores its index (and payload) into its key's
+// entry, then a second pass checks that each row's entry holds that row.  With
+// unique keys every entry has one writer; a duplicate key leaves some row
+// whose entry holds another row, which counts a duplicate (the table is then
+// built the general way).  Plain stores: no atomics on the table.
+// Key range from the device statistics; 0 when it exceeds `cap` entries.
+__device__ __forceinline__ uint64_t dense_range(const Stats* st, uint64_t cap) {
+  const uint64_t r = static_cast<uint64_t>(st->max_key) - static_cast<uint64_t>(st->min_key) + 1;  // 0: 2^64 wrapped
+  return (st->max_key >= st->min_key && r != 0 && r <= cap) ? r : 0;
+}
+__global__ void k_dense_fill2_dev(uint32_t* dense, uint2* dense_up, uint64_t cap, const Stats* st) {
+  const uint64_t n = dense_range(st, cap);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    dense[i] = kNone;
+    if (dense_up) dense_up[i] = make_uint2(kNone, 0u);
+  }
+}
+__global__ void k_dense_scatter_dev(const int64_t* __restrict__ keys, uint64_t n, uint64_t cap, const Stats* st,
+                                    uint32_t* dense, uint2* dense_up, const int32_t* __restrict__ pay) {
+  if (dense_range(st, cap) == 0) return;
+  const int64_t min_key = st->min_key;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
+    dense[d] = static_cast<uint32_t>(i);
+    if (dense_up) dense_up[d] = make_uint2(static_cast<uint32_t>(i), static_cast<uint32_t>(pay[i]));
+  }
+}
+__global__ void k_dense_verify_dev(const int64_t* __restrict__ keys, uint64_t n, uint64_t cap, Stats* st,
+                                   const uint32_t* __restrict__ dense, const uint2* __restrict__ dense_up) {
+  if (dense_range(st, cap) == 0) return;
+  const int64_t min_key = st->min_key;
+  uint32_t bad = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
+    bad += (dense[d] != static_cast<uint32_t>(i) || (dense_up && dense_up[d].x != static_cast<uint32_t>(i))) ? 1u : 0u;
+  }
+  if (bad) atomicAdd(&st->dups, static_cast<unsigned long long>(bad));
+}
+
 __global__ void k_dense_fill2(uint32_t* dense, uint2* dense_up, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     dense[i] = kNone;
     if (dense_up) dense_up[i] = make_uint2(kNone, 0u);
   }
 }
-__global__ void k_dense_claim(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key, uint32_t* dense,
-                              uint2* dense_up, const int32_t* __restrict__ pay, unsigned long long* dups) {
+__global__ void k_dense_scatter(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key, uint32_t* dense,
+                                uint2* dense_up, const int32_t* __restrict__ pay) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
-    const uint32_t old = atomicCAS(&dense[d], kNone, static_cast<uint32_t>(i));
-    if (old != kNone) {
-      atomicAdd(dups, 1ULL);
-      continue;
-    }
+    dense[d] = static_cast<uint32_t>(i);
     if (dense_up) dense_up[d] = make_uint2(static_cast<uint32_t>(i), static_cast<uint32_t>(pay[i]));
   }
+}
+__global__ void k_dense_verify(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key,
+                               const uint32_t* __restrict__ dense, const uint2* __restrict__ dense_up,
+                               unsigned long long* dups) {
+  uint32_t bad = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
+    bad += (dense[d] != static_cast<uint32_t>(i) || (dense_up && dense_up[d].x != static_cast<uint32_t>(i))) ? 1u : 0u;
+  }
+  if (bad) atomicAdd(dups, static_cast<unsigned long long>(bad));
 }
 
 __global__ void k_dense_pack(uint2* dense, const uint32_t* __restrict__ start, const uint32_t* __restrict__ cnt,
@@ -1283,12 +1324,60 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
     DeviceBuffer sbuf(c, sizeof(Stats) + 16 * kRedBlocks);
     Stats* dst = sbuf.as<Stats>();
     void* part = sbuf.as<char>() + sizeof(Stats);
+    Stats* pinned = static_cast<Stats*>(readback_block(ctx));
     hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(1), 0, st, dst);
     if (n_rows) launch_minmax(ctx, d_keys, n_rows, static_cast<int64_t*>(part), dst);
     PRISM_CHECK_LAUNCH();
-    Stats hs;
-    PRISM_HIP(hipMemcpyAsync(&hs, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
+    // Unique keys whose range fits n + 1024 entries (a permutation, dense ids):
+    // the dense table is built from the device-side range without waiting for
+    // it, and one read at the end confirms range and uniqueness.
+    if (kind != PRISM_TABLE_HASHED && !keys_only && n_rows) {
+      const uint64_t cap = n_rows + 1024;
+      T->dense.alloc(c, cap * 4);
+      uint32_t* d = T->dense.as<uint32_t>();
+      uint2* up = nullptr;
+      if (d_payload) {
+        T->payload.alloc(c, cap * 8);
+        up = T->payload.as<uint2>();
+      }
+      hipLaunchKernelGGL(k_dense_fill2_dev, grid_for(ctx, cap), dim3(kBuildBlock), 0, st, d, up, cap,
+                         static_cast<const Stats*>(dst));
+      hipLaunchKernelGGL(k_dense_scatter_dev, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, cap,
+                         static_cast<const Stats*>(dst), d, up, d_payload);
+      hipLaunchKernelGGL(k_dense_verify_dev, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, cap, dst,
+                         static_cast<const uint32_t*>(d), static_cast<const uint2*>(up));
+      PRISM_CHECK_LAUNCH();
+      PRISM_HIP(hipMemcpyAsync(pinned, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
+      PRISM_HIP(hipStreamSynchronize(st));
+      const Stats h1 = *pinned;
+      const uint64_t range = static_cast<uint64_t>(h1.max_key) - static_cast<uint64_t>(h1.min_key) + 1;
+      if (h1.max_key >= h1.min_key && range != 0 && range <= cap && h1.dups == 0) {  // range 0: 2^64 wrapped
+        T->info.kind = PRISM_TABLE_DENSE;
+        T->info.slots = range;
+        T->info.min_key = h1.min_key;
+        T->info.max_key = h1.max_key;
+        T->info.n_distinct = n_rows;
+        T->info.max_dup = 1;
+        T->info.unique = true;
+        T->view.dense_u = d;
+        T->view.min_key = h1.min_key;
+        T->view.range = range;
+        if (d_payload) {  // the packed {row, payload} entries replace the row array
+          T->view.has_payload = 1;
+          T->view.dense_up = up;
+          T->dense.release();
+          T->view.dense_u = nullptr;
+        }
+        T->info.device_bytes = T->slots.bytes + T->dense.bytes + T->rows.bytes + T->payload.bytes;
+        *out = guard.release();
+        return;
+      }
+      T->dense.release();
+      T->payload.release();
+    }
+    PRISM_HIP(hipMemcpyAsync(pinned, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
     PRISM_HIP(hipStreamSynchronize(st));
+    Stats hs = *pinned;
     if (n_rows == 0) hs.min_key = hs.max_key = 0;
     const uint64_t range = static_cast<uint64_t>(hs.max_key) - static_cast<uint64_t>(hs.min_key) + 1;
     bool dense;
@@ -1315,8 +1404,10 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
         up = T->payload.as<uint2>();
       }
       hipLaunchKernelGGL(k_dense_fill2, grid_for(ctx, range), dim3(kBuildBlock), 0, st, d, up, range);
-      hipLaunchKernelGGL(k_dense_claim, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, hs.min_key, d,
-                         up, d_payload, &dst->dups);
+      hipLaunchKernelGGL(k_dense_scatter, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, hs.min_key,
+                         d, up, d_payload);
+      hipLaunchKernelGGL(k_dense_verify, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, hs.min_key,
+                         static_cast<const uint32_t*>(d), static_cast<const uint2*>(up), &dst->dups);
       PRISM_CHECK_LAUNCH();
       Stats h2;
       PRISM_HIP(hipMemcpyAsync(&h2, dst, sizeof(Stats), hipMemcpyDeviceToHost, st));
