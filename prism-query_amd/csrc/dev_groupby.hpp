@@ -472,5 +472,63 @@ __device__ __forceinline__ uint32_t next_tile(TileQueue* q, uint32_t* s_t) {
   return t;
 }
 
+
+// ---------------------------------------------------------------------------
+// Small finish (one workgroup of kFinishThreads): the occupied slots of a
+// group table with at most kFinishCap slots, ordered by first row (bitonic
+// sort in LDS) and written stride words each to `rows`; returns the group
+// count.  s_first / s_slot are LDS arrays of kFinishCap entries.
+// ---------------------------------------------------------------------------
+constexpr int kFinishCap = 4096;
+constexpr int kFinishThreads = 256;
+
+__device__ __forceinline__ uint32_t order_groups(const GTable& G, unsigned long long* rows, unsigned long long* s_first,
+                                                 uint32_t* s_slot, uint32_t* s_n) {
+  const int stride = G.stride;
+  const uint64_t slots = G.cap + 1;
+  if (threadIdx.x == 0) *s_n = 0;
+  __syncthreads();
+  for (uint64_t s = threadIdx.x; s < slots; s += kFinishThreads) {
+    const unsigned long long f = G.w[s * stride + kW_FIRST];
+    if (f != kNoRow) {
+      const uint32_t i = atomicAdd(s_n, 1u);
+      s_first[i] = f;
+      s_slot[i] = static_cast<uint32_t>(s);
+    }
+  }
+  __syncthreads();
+  const uint32_t n = *s_n;
+  uint32_t p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  for (uint32_t i = n + threadIdx.x; i < p2; i += kFinishThreads) {
+    s_first[i] = ~0ULL;
+    s_slot[i] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= p2; size <<= 1)
+    for (uint32_t half = size >> 1; half > 0; half >>= 1) {
+      for (uint32_t i = threadIdx.x; i < p2; i += kFinishThreads) {
+        const uint32_t j = i ^ half;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const unsigned long long a = s_first[i], b = s_first[j];
+          if ((a > b) == up) {
+            s_first[i] = b;
+            s_first[j] = a;
+            const uint32_t t = s_slot[i];
+            s_slot[i] = s_slot[j];
+            s_slot[j] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (uint64_t e = threadIdx.x; e < static_cast<uint64_t>(n) * stride; e += kFinishThreads) {
+    const uint64_t i = e / stride, j = e % stride;
+    rows[e] = G.w[static_cast<uint64_t>(s_slot[i]) * stride + j];
+  }
+  return n;
+}
+
 }  // namespace gb
 }  // namespace prism

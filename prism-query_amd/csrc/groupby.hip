@@ -22,6 +22,17 @@ __global__ __launch_bounds__(kBlock) void k_groupby(const DevPlan* __restrict__ 
   gb_flush<kPriv>(P, G, lds, lds_slots, flags);
 }
 
+// Finish of a table of at most kFinishCap slots in one workgroup: groups in
+// first-row order into `rows`, their count into *n_found.
+__global__ __launch_bounds__(kFinishThreads) void k_finish_small(GTable G, unsigned long long* rows,
+                                                                 unsigned long long* n_found) {
+  __shared__ unsigned long long s_first[kFinishCap];
+  __shared__ uint32_t s_slot[kFinishCap];
+  __shared__ uint32_t s_n;
+  const uint32_t n = order_groups(G, rows, s_first, s_slot, &s_n);
+  if (threadIdx.x == 0) *n_found = n;
+}
+
 __global__ void k_fill_tmpl(unsigned long long* w, uint64_t n_words, const DevPlan* __restrict__ P) {
   const int stride = P->stride;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += (uint64_t)gridDim.x * blockDim.x)
@@ -94,16 +105,19 @@ static void alloc_table(prism_groupby* g, uint64_t cap) {
   PRISM_CHECK_LAUNCH();
 }
 
+// Group count and flags (pinned readback: the context lock is held and the
+// stream drained before the block is reused).
 static void read_counters(prism_groupby* g, uint64_t& groups, uint32_t& flags) {
-  struct {
+  struct Counters {
     unsigned long long n;
     uint32_t f;
     uint32_t pad;
-  } h;
-  PRISM_HIP(hipMemcpyAsync(&h, g->counters.ptr, 16, hipMemcpyDeviceToHost, g->ctx->stream));
+  };
+  Counters* h = static_cast<Counters*>(readback_block(g->ctx));
+  PRISM_HIP(hipMemcpyAsync(h, g->counters.ptr, 16, hipMemcpyDeviceToHost, g->ctx->stream));
   PRISM_HIP(hipStreamSynchronize(g->ctx->stream));
-  groups = h.n;
-  flags = h.f;
+  groups = h->n;
+  flags = h->f;
 }
 
 // Move the table into one of `new_cap` slots.
@@ -437,7 +451,12 @@ prism_status prism_groupby_update(prism_groupby* g, const prism_column* columns,
     }
     hipStream_t st = g->ctx->stream;
     g->d_cols.reserve(g->ctx, sizeof(ColsArg));
-    PRISM_HIP(hipMemcpyAsync(g->d_cols.ptr, &cols, sizeof(ColsArg), hipMemcpyHostToDevice, st));
+    // column descriptors go up from the pinned readback block (second KiB);
+    // every update drains the stream before it returns
+    static_assert(sizeof(ColsArg) <= 3072, "column descriptors fit the readback block");
+    char* pin = static_cast<char*>(readback_block(g->ctx)) + 1024;
+    memcpy(pin, &cols, sizeof(ColsArg));
+    PRISM_HIP(hipMemcpyAsync(g->d_cols.ptr, pin, sizeof(ColsArg), hipMemcpyHostToDevice, st));
     const prism_column* dcols = g->d_cols.as<prism_column>();
     uint64_t done = 0;
     while (done < n_rows) {
@@ -487,6 +506,19 @@ prism_status prism_groupby_finish(prism_groupby* g, uint64_t* n_groups) {
     hipStream_t st = g->ctx->stream;
     Ctx* c = g->ctx;
     const uint64_t slots = g->cap + 1;
+    if (slots <= static_cast<uint64_t>(kFinishCap)) {  // one workgroup sorts in LDS; one read of the count
+      g->result.reserve(c, slots * g->slot_bytes());
+      hipLaunchKernelGGL(k_finish_small, dim3(1), dim3(kFinishThreads), 0, st, g->view(),
+                         g->result.as<unsigned long long>(), g->n_found_ptr());
+      PRISM_CHECK_LAUNCH();
+      unsigned long long* h = static_cast<unsigned long long*>(readback_block(c));
+      PRISM_HIP(hipMemcpyAsync(h, g->n_found_ptr(), 8, hipMemcpyDeviceToHost, st));
+      PRISM_HIP(hipStreamSynchronize(st));
+      g->n_result = *h;
+      g->finished = true;
+      *n_groups = *h;
+      return;
+    }
     DeviceBuffer first(c, slots * 8), first_sorted(c, slots * 8), idx(c, slots * 4), idx_sorted(c, slots * 4);
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, g->n_found_ptr(), 0ULL);
     hipLaunchKernelGGL(k_collect, fill_grid(g->ctx, slots), dim3(256), 0, st, g->table.as<unsigned long long>(),

@@ -49,7 +49,7 @@ constexpr int kSB = 256;               // threads per workgroup (group-by kernel
 static_assert(kSB == gb::kBlock, "service group-by tiles use the group-by workgroup size");
 constexpr int kMaxReq = 256;           // requests per batch
 constexpr int kMaxTiles = 32768;       // tiles per batch
-constexpr int kFinishCap = 4096;       // group-table slots a finish workgroup orders in LDS
+using gb::kFinishCap;                  // group-table slots a finish workgroup orders in LDS
 constexpr int kVariants = 10;          // persistent-kernel variants (see Variant)
 constexpr int kJitQueues = 16;         // queues of generated group-by tile kernels per batch
 constexpr size_t kResultsBytes = 48u << 20;
@@ -259,57 +259,14 @@ __global__ __launch_bounds__(kSB) void k_svc_filter(Control* C, int v) {
 
 // Group-by finish: one workgroup per request collects the occupied slots,
 // orders them by first row (bitonic sort in LDS) and writes the rows.
-__global__ __launch_bounds__(kSB) void k_svc_gb_finish(Control* C) {
+__global__ __launch_bounds__(gb::kFinishThreads) void k_svc_gb_finish(Control* C) {
   __shared__ unsigned long long s_first[kFinishCap];
   __shared__ uint32_t s_slot[kFinishCap];
   __shared__ uint32_t s_n;
   const uint32_t n_gb = C->n_gb;
   for (uint32_t k = blockIdx.x; k < n_gb; k += gridDim.x) {
     const GbDesc& d = C->gb[C->gb_idx[k]];
-    const int stride = d.G.stride;
-    const uint64_t slots = d.G.cap + 1;
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
-    for (uint64_t s = threadIdx.x; s < slots; s += kSB) {
-      const unsigned long long f = d.G.w[s * stride + gb::kW_FIRST];
-      if (f != gb::kNoRow) {
-        const uint32_t i = atomicAdd(&s_n, 1u);
-        s_first[i] = f;
-        s_slot[i] = static_cast<uint32_t>(s);
-      }
-    }
-    __syncthreads();
-    const uint32_t n = s_n;
-    uint32_t p2 = 1;
-    while (p2 < n) p2 <<= 1;
-    for (uint32_t i = n + threadIdx.x; i < p2; i += kSB) {
-      s_first[i] = ~0ULL;
-      s_slot[i] = 0xFFFFFFFFu;
-    }
-    __syncthreads();
-    for (uint32_t size = 2; size <= p2; size <<= 1)
-      for (uint32_t stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
-        for (uint32_t i = threadIdx.x; i < p2; i += kSB) {
-          const uint32_t j = i ^ stride2;
-          if (j > i) {
-            const bool up = (i & size) == 0;
-            const unsigned long long a = s_first[i], b = s_first[j];
-            if ((a > b) == up) {
-              s_first[i] = b;
-              s_first[j] = a;
-              const uint32_t t = s_slot[i];
-              s_slot[i] = s_slot[j];
-              s_slot[j] = t;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    unsigned long long* rows = d.out + 2;
-    for (uint64_t e = threadIdx.x; e < static_cast<uint64_t>(n) * stride; e += kSB) {
-      const uint64_t i = e / stride, j = e % stride;
-      rows[e] = d.G.w[static_cast<uint64_t>(s_slot[i]) * stride + j];
-    }
+    const uint32_t n = gb::order_groups(d.G, d.out + 2, s_first, s_slot, &s_n);
     if (threadIdx.x == 0) {
       d.out[0] = n;
       d.out[1] = *d.flags;
@@ -788,7 +745,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
         sl.timed.push_back(Slot::Timed{V_GB_444S, a, b, jrows[i], cand_index(S, K_GB)});
       }
     }
-    if (H->n_gb) hipLaunchKernelGGL(k_svc_gb_finish, dim3(fin_grid), dim3(kSB), 0, st, dC);
+    if (H->n_gb) hipLaunchKernelGGL(k_svc_gb_finish, dim3(fin_grid), dim3(gb::kFinishThreads), 0, st, dC);
   };
   if (S->cfg.use_graphs && !timed) {
     const std::string key = shape_key(grid, lds, init_grid, fin_grid, jf, jgrid, jlds);
