@@ -19,6 +19,8 @@
 // (probe row ascending, then build row ascending).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "join.hpp"
@@ -636,6 +638,8 @@ __global__ __launch_bounds__(kPB) void k_probe_unique(TableView T, const int64_t
   }
   __syncthreads();
   uint2* dst = staged + base;
+  // Plain stores: the move pass reads the staged pairs back from the
+  // Infinity Cache (non-temporal stores made it slower).
   for (uint32_t i = t; i < tile_total; i += kPB) dst[i] = s_pair[i];
 }
 
@@ -1041,12 +1045,10 @@ __device__ __forceinline__ int64_t slot_of(const TableView& T, int64_t k, bool u
   }
 }
 
-// Probe-side aggregates per key slot, one array per word (the atomics of a
-// row spread over memory channels): row count, int128 payload sum, first row.
+// Probe-side aggregates per key slot in one 32-byte record (a slot's reads
+// and writes touch one cache line): row count, int128 payload sum, first row.
 struct JaAcc {
-  unsigned long long* cnt;    // per key slot
-  unsigned long long* sum;    // per key slot: lo, hi words
-  unsigned long long* first;  // per key slot: first probe row
+  unsigned long long* w;  // per key slot, 32 B: count, sum lo, sum hi, first probe row
 };
 
 // Probe key -> key slot.  A DENSE table tests presence in a bitmap of the
@@ -1076,20 +1078,99 @@ __global__ void k_presence_bits(TableView T, uint32_t* bits, uint64_t n_words) {
   }
 }
 
+// LDS cache of hot key slots: per entry the slot, row count, split int128
+// payload sum (low 32-bit halves unsigned, high halves signed) and first row.
+// The cached slots are fixed at create time (the slots with the most build
+// rows, the probe's likely Zipf head), laid out in buckets of four read with
+// one 16-byte LDS load; a slot may sit in either of two buckets, so a lookup
+// costs at most two loads and no entry is ever claimed at run time.
+constexpr int kJaWays = 4;
+constexpr int kJaBuckets = kJaCache / kJaWays;
+static_assert(kJaBuckets == 256, "two 8-bit bucket indices");
+
+__host__ __device__ __forceinline__ uint32_t ja_bucket(uint32_t s, int t) {
+  const uint64_t h = static_cast<uint64_t>(s) * 0x9E3779B97F4A7C15ULL;
+  return static_cast<uint32_t>(h >> (t ? 48 : 56)) & (kJaBuckets - 1);
+}
+
+constexpr int kJaRegHot = 4;                    // slots summed in registers (the hottest)
+constexpr int kJaCacheAll = kJaCache + kJaRegHot;  // accumulators: cache entries, then register slots
+
+struct JaCache {
+  uint32_t key[kJaCache];
+  unsigned long long cnt[kJaCacheAll], lo[kJaCacheAll], first[kJaCacheAll];
+  long long hi[kJaCacheAll];
+};
+
+__device__ __forceinline__ void ja_cache_init(JaCache& C, const uint32_t* __restrict__ keys) {
+  for (int i = threadIdx.x; i < kJaCacheAll; i += blockDim.x) {
+    if (i < kJaCache) C.key[i] = keys[i];
+    C.cnt[i] = 0;
+    C.lo[i] = 0;
+    C.hi[i] = 0;
+    C.first[i] = ~0ULL;
+  }
+}
+
+__device__ __forceinline__ int ja_cache_find(const JaCache& C, uint32_t s) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const uint32_t b = ja_bucket(s, t);
+    const v4u32 e = *reinterpret_cast<const v4u32*>(&C.key[kJaWays * b]);
+    if (e.x == s) return kJaWays * b;
+    if (e.y == s) return kJaWays * b + 1;
+    if (e.z == s) return kJaWays * b + 2;
+    if (e.w == s) return kJaWays * b + 3;
+  }
+  return -1;
+}
+
+// Adds one row to slot s's cache entry; false when the slot is not cached.
+__device__ __forceinline__ bool ja_cache_add(JaCache& C, uint32_t s, int64_t x, unsigned long long row) {
+  const int c = ja_cache_find(C, s);
+  if (c < 0) return false;
+  atomicAdd(&C.cnt[c], 1ULL);
+  const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(x));
+  const long long hi = x >> 32;
+  if (lo) atomicAdd(&C.lo[c], static_cast<unsigned long long>(lo));
+  if (hi) atomicAdd(reinterpret_cast<unsigned long long*>(&C.hi[c]), static_cast<unsigned long long>(hi));
+  if (row < __hip_atomic_load(&C.first[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(&C.first[c], row);
+  return true;
+}
+
+// One row added to its slot with device atomics.
+__device__ __forceinline__ void ja_add_direct(const JaAcc& A, uint64_t s, int64_t x, unsigned long long row) {
+  atomicAdd(&A.w[4 * s], 1ULL);
+  if (x) atomic_add_i128(&A.w[4 * s + 1], &A.w[4 * s + 2], static_cast<uint64_t>(x), x >> 63);
+  if (row < __hip_atomic_load(&A.w[4 * s + 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&A.w[4 * s + 3], row);
+}
+
+// Cache entries merged into the slot arrays (after a barrier).
+__device__ __forceinline__ void ja_cache_flush(JaCache& C, const JaAcc& A) {
+  for (int c = threadIdx.x; c < kJaCache; c += blockDim.x) {
+    if (C.key[c] == kJaEmpty || C.cnt[c] == 0) continue;
+    const uint64_t s = C.key[c];
+    atomicAdd(&A.w[4 * s], C.cnt[c]);
+    Split128 sp;
+    sp.lo32 = C.lo[c];
+    sp.hi32 = C.hi[c];
+    uint64_t lo;
+    int64_t hi;
+    sp.to_i128(lo, hi);
+    if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&A.w[4 * s + 1], &A.w[4 * s + 2], lo, hi);
+    atomicMin(&A.w[4 * s + 3], C.first[c]);
+  }
+}
+
+// Probe pass with every miss of the LDS cache added by device atomics (used
+// when the slot space is too large for the partitioned tail below).
 template <int KIND>
 __global__ __launch_bounds__(kJaBlock) void k_ja_probe(TableView T, bool unique, const int64_t* __restrict__ keys,
                                                        const int64_t* __restrict__ pay, uint64_t n, uint64_t row_base,
-                                                       JaAcc A, const uint32_t* __restrict__ present) {
-  __shared__ uint32_t ck[kJaCache];
-  __shared__ unsigned long long cc[kJaCache], clo[kJaCache], cf[kJaCache];
-  __shared__ long long chi[kJaCache];
-  for (int i = threadIdx.x; i < kJaCache; i += kJaBlock) {
-    ck[i] = kJaEmpty;
-    cc[i] = 0;
-    clo[i] = 0;
-    chi[i] = 0;
-    cf[i] = ~0ULL;
-  }
+                                                       JaAcc A, const uint32_t* __restrict__ present,
+                                                       const uint32_t* __restrict__ cache_keys) {
+  __shared__ JaCache C;
+  ja_cache_init(C, cache_keys);
   __syncthreads();
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kJaBlock;
   for (uint64_t r = blockIdx.x * static_cast<uint64_t>(kJaBlock) + threadIdx.x; r < n; r += stride) {
@@ -1097,49 +1178,561 @@ __global__ __launch_bounds__(kJaBlock) void k_ja_probe(TableView T, bool unique,
     if (s < 0) continue;
     const int64_t x = pay ? pay[r] : 0;
     const unsigned long long row = row_base + r;
-    uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(s) * 0x9E3779B97F4A7C15ULL) >> 54);  // 10 bits
-    int c = -1;
-    for (int p = 0; p < kJaProbes; ++p) {
-      const uint32_t cur = __hip_atomic_load(&ck[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (cur == static_cast<uint32_t>(s)) {
-        c = static_cast<int>(h);
-        break;
-      }
-      if (cur == kJaEmpty) {
-        const uint32_t old = atomicCAS(&ck[h], kJaEmpty, static_cast<uint32_t>(s));
-        if (old == kJaEmpty || old == static_cast<uint32_t>(s)) {
-          c = static_cast<int>(h);
-          break;
-        }
-      }
-      h = (h + 1) & (kJaCache - 1);
+    if (!ja_cache_add(C, static_cast<uint32_t>(s), x, row)) ja_add_direct(A, static_cast<uint64_t>(s), x, row);
+  }
+  __syncthreads();
+  ja_cache_flush(C, A);
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned tail.  Device atomics on random slots run at about 24 G/s on
+// MI355X whatever their scope or footprint (measured), so rows whose slot
+// misses the LDS cache are not added to their slot directly.  They are
+// radix-partitioned by slot in two passes - level 1 by the slot's high bits
+// into at most 64 partitions, then into ranges of kJaSub slots - and each
+// range is summed by one workgroup with LDS atomics and written back with
+// plain read-modify-writes (no other workgroup touches that range).  Level-1
+// regions have a fixed capacity; rows beyond it (a partition far above its
+// share) fall back to device atomics, so any input stays exact.
+// ---------------------------------------------------------------------------
+constexpr int kJaSubShift = 12;
+constexpr int kJaSub = 1 << kJaSubShift;  // slots per aggregation range (LDS: 24 B per slot)
+constexpr int kJaP1Max = 64;              // level-1 partitions
+constexpr int kJaFanMax = 64;             // aggregation ranges per level-1 partition
+constexpr int kJaStage = 32;              // K1 LDS staging records per level-1 partition
+constexpr int kJaFlushAt = 16;            // K1 flushes a staging buffer at this fill
+constexpr int kJaBlock1 = 512;            // K1 threads (two workgroups per CU)
+constexpr int kJaRows1 = 4;               // rows per thread per K1 round
+constexpr int kJaWaves1 = kJaBlock1 / 64;
+constexpr uint32_t kJaBlk = 256;          // level-1 records a K1 workgroup reserves at a time
+constexpr uint32_t kJaHole = 0xFFFFFFFFu; // slot of an unused reserved record
+constexpr int kJaPartBlock = 1024;
+constexpr int kJaWavesPB = kJaPartBlock / 64;
+constexpr uint32_t kJaPartRecs = 16384;   // level-1 records per workgroup in the repartition passes
+constexpr int kJaStage2 = 4096;           // repartition LDS staging records (all ranges together)
+
+struct JaPart {
+  unsigned long long* l1_sr;  // level-1 regions, partition p at p * cap1: slot | row << 32 (holes: slot kJaHole)
+  long long* l1_pay;          //   and the row's payload
+  unsigned long long* fill1;  // per level-1 partition: records reserved (may pass cap1)
+  uint64_t cap1;
+  int shift1;                 // slot >> shift1 = level-1 partition
+  int fan;                    // aggregation ranges per level-1 partition (power of two)
+  int n_p1;                   // level-1 partitions
+  uint32_t nb;                // repartition workgroups per level-1 partition
+  uint32_t* cnt2;             // per range: records
+  unsigned long long* off2;   // per range: first record in l2
+  unsigned long long* blk2;   // per (partition, block, range): records
+  uint4* l2;                  // {slot & (kJaSub - 1), row, payload lo, payload hi}, grouped by range
+};
+
+// Barrier for LDS traffic only: global stores and atomics in flight are not
+// waited for (the compiler still waits for loaded values before their use).
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+struct JaStage1 {
+  unsigned long long sr[kJaP1Max][kJaStage];  // slot | row << 32
+  long long pay[kJaP1Max][kJaStage];
+  uint32_t n[kJaP1Max];
+  unsigned long long cur[kJaP1Max];  // next free record of the reserved block
+  unsigned long long end[kJaP1Max];  // end of the reserved block
+  unsigned long long nxt[kJaP1Max];  // block reserved ahead (~0 = none)
+};
+
+// Staging buffers holding at least `at_least` records go to their level-1
+// region, half a wave per buffer.  Space comes in blocks of kJaBlk records,
+// reserved a round ahead (the reservation's latency hides behind that
+// round's loads); records past the region's capacity are added to their
+// slots with device atomics.
+__device__ __forceinline__ void ja_stage1_flush(JaStage1& S, const JaPart& P, const JaAcc& A, uint32_t at_least) {
+  static_assert(kJaStage == 32, "half a wave per staging buffer");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t i = static_cast<uint32_t>(lane & 31);
+  for (int p = 2 * wave + (lane >> 5); p < P.n_p1; p += 2 * kJaWaves1) {
+    const uint32_t n0 = S.n[p];
+    const uint32_t m = n0 < static_cast<uint32_t>(kJaStage) ? n0 : static_cast<uint32_t>(kJaStage);
+    if (m == 0 || m < at_least) continue;
+    const unsigned long long cur = S.cur[p], end = S.end[p];
+    const uint32_t k = end - cur < m ? static_cast<uint32_t>(end - cur) : m;  // records into the current block
+    unsigned long long nb = S.nxt[p];
+    if (k < m && nb == ~0ULL) {  // no block reserved ahead (the last flush): reserve now
+      if (i == 0) nb = atomicAdd(&P.fill1[p], static_cast<unsigned long long>(kJaBlk));
+      nb = __shfl(nb, lane & 32, 64);
     }
-    if (c >= 0) {
-      atomicAdd(&cc[c], 1ULL);
-      const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(x));
-      const long long hi = x >> 32;
-      if (lo) atomicAdd(&clo[c], static_cast<unsigned long long>(lo));
-      if (hi) atomicAdd(reinterpret_cast<unsigned long long*>(&chi[c]), static_cast<unsigned long long>(hi));
-      if (row < __hip_atomic_load(&cf[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(&cf[c], row);
-    } else {
-      atomicAdd(&A.cnt[s], 1ULL);
-      if (x) atomic_add_i128(&A.sum[2 * s], &A.sum[2 * s + 1], static_cast<uint64_t>(x), x >> 63);
-      if (row < __hip_atomic_load(&A.first[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&A.first[s], row);
+    if (i < m) {
+      const unsigned long long sr = S.sr[p][i];
+      const long long x = S.pay[p][i];
+      const uint64_t at = i < k ? cur + i : nb + (i - k);
+      if (at < P.cap1) {
+        const uint64_t d = static_cast<uint64_t>(p) * P.cap1 + at;
+        P.l1_sr[d] = sr;
+        P.l1_pay[d] = x;
+      } else {
+        ja_add_direct(A, static_cast<uint32_t>(sr), x, sr >> 32);
+      }
+    }
+    if (i == 0) {
+      S.n[p] = 0;
+      if (k < m) {
+        S.cur[p] = nb + (m - k);
+        S.end[p] = nb + kJaBlk;
+        S.nxt[p] = ~0ULL;
+      } else {
+        S.cur[p] = cur + m;
+      }
+    }
+  }
+}
+
+// K1: probe rows -> LDS cache, misses staged by level-1 partition.
+// The hottest slots (by build-side row count) are summed in registers: a
+// Zipf head would otherwise serialise on one LDS address.
+
+struct JaReg {
+  uint32_t cnt, first;
+  uint64_t lo;
+  int64_t hi;
+  __device__ __forceinline__ void clear() {
+    cnt = 0;
+    first = 0xFFFFFFFFu;
+    lo = 0;
+    hi = 0;
+  }
+  // branch-free: every row updates all register slots under its match flag
+  __device__ __forceinline__ void add(bool m, int64_t x, uint32_t row) {
+    cnt += m ? 1u : 0u;
+    lo += m ? static_cast<uint32_t>(static_cast<uint64_t>(x)) : 0u;
+    hi += m ? (x >> 32) : 0;
+    first = (m && row < first) ? row : first;
+  }
+  // wave totals added to the workgroup's accumulators of register entry j
+  __device__ __forceinline__ void to_lds(JaCache& C, int j) {
+    unsigned long long c = cnt, l = lo;
+    long long h = hi;
+    uint32_t f = first;
+    for (int d = 32; d > 0; d >>= 1) {
+      c += __shfl_xor(c, d, 64);
+      l += __shfl_xor(l, d, 64);
+      h += __shfl_xor(h, d, 64);
+      const uint32_t o = __shfl_xor(f, d, 64);
+      f = o < f ? o : f;
+    }
+    if (lane_id() == 0 && c) {
+      const int e = kJaCache + j;
+      atomicAdd(&C.cnt[e], c);
+      atomicAdd(&C.lo[e], l);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&C.hi[e]), static_cast<unsigned long long>(h));
+      atomicMin(&C.first[e], static_cast<unsigned long long>(f));
+    }
+  }
+  // wave totals, added to slot s's record by lane 0
+  __device__ __forceinline__ void flush(uint32_t s, const JaAcc& A) {
+    unsigned long long c = cnt, l = lo;
+    long long h = hi;
+    uint32_t f = first;
+    for (int d = 32; d > 0; d >>= 1) {
+      c += __shfl_xor(c, d, 64);
+      l += __shfl_xor(l, d, 64);
+      h += __shfl_xor(h, d, 64);
+      const uint32_t o = __shfl_xor(f, d, 64);
+      f = o < f ? o : f;
+    }
+    if (lane_id() == 0 && c) {
+      atomicAdd(&A.w[4 * static_cast<uint64_t>(s)], c);
+      Split128 sp;
+      sp.lo32 = l;
+      sp.hi32 = h;
+      uint64_t wl;
+      int64_t wh;
+      sp.to_i128(wl, wh);
+      if (wl | static_cast<uint64_t>(wh))
+        atomic_add_i128(&A.w[4 * static_cast<uint64_t>(s) + 1], &A.w[4 * static_cast<uint64_t>(s) + 2], wl, wh);
+      atomicMin(&A.w[4 * static_cast<uint64_t>(s) + 3], static_cast<unsigned long long>(f));
+    }
+  }
+};
+
+template <int KIND>
+__global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ja_probe_part(TableView T, bool unique, const int64_t* __restrict__ keys,
+                                                             const int64_t* __restrict__ pay, uint64_t n,
+                                                             uint64_t row_base, JaAcc A,
+                                                             const uint32_t* __restrict__ present, JaPart P,
+                                                             const uint32_t* __restrict__ hot_slots,
+                                                             const uint32_t* __restrict__ cache_keys,
+                                                             unsigned long long* __restrict__ cache_part) {
+  __shared__ JaCache C;
+  __shared__ JaStage1 S;
+  static_assert(kJaRegHot == 4, "four register slots");
+  const uint32_t h0 = hot_slots[0], h1 = hot_slots[1], h2 = hot_slots[2], h3 = hot_slots[3];
+  JaReg r0, r1, r2, r3;
+  r0.clear();
+  r1.clear();
+  r2.clear();
+  r3.clear();
+  ja_cache_init(C, cache_keys);
+  for (int p = threadIdx.x; p < kJaP1Max; p += blockDim.x) {
+    S.n[p] = 0;
+    S.cur[p] = 0;
+    S.end[p] = 0;
+    S.nxt[p] = ~0ULL;
+  }
+  __syncthreads();
+  constexpr uint64_t kRound = static_cast<uint64_t>(kJaBlock1) * kJaRows1;
+  for (uint64_t base = blockIdx.x * kRound; base < n; base += static_cast<uint64_t>(gridDim.x) * kRound) {
+    // a partition whose block may run out in this round's flush gets its
+    // next block now; the reply is read after the round's loads are issued
+    const bool want = static_cast<int>(threadIdx.x) < P.n_p1 && S.nxt[threadIdx.x] == ~0ULL &&
+                      S.end[threadIdx.x] - S.cur[threadIdx.x] < static_cast<unsigned long long>(2 * kJaStage);
+    unsigned long long ahead = 0;
+    if (want) ahead = atomicAdd(&P.fill1[threadIdx.x], static_cast<unsigned long long>(kJaBlk));
+    int64_t k[kJaRows1], x[kJaRows1];
+#pragma unroll
+    for (int j = 0; j < kJaRows1; ++j) {
+      const uint64_t r = base + static_cast<uint64_t>(j) * kJaBlock1 + threadIdx.x;
+      k[j] = r < n ? keys[r] : 0;
+      x[j] = (pay && r < n) ? pay[r] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kJaRows1; ++j) {
+      const uint64_t r = base + static_cast<uint64_t>(j) * kJaBlock1 + threadIdx.x;
+      if (r >= n) continue;
+      const int64_t s = probe_slot<KIND>(T, k[j], unique, present);
+      if (s < 0) continue;
+      const unsigned long long row = row_base + r;
+      const uint32_t su = static_cast<uint32_t>(s);
+      const bool m0 = su == h0, m1 = su == h1, m2 = su == h2, m3 = su == h3;
+      r0.add(m0, x[j], static_cast<uint32_t>(row));
+      r1.add(m1, x[j], static_cast<uint32_t>(row));
+      r2.add(m2, x[j], static_cast<uint32_t>(row));
+      r3.add(m3, x[j], static_cast<uint32_t>(row));
+      if (m0 | m1 | m2 | m3) continue;
+      if (ja_cache_add(C, static_cast<uint32_t>(s), x[j], row)) continue;
+      const uint32_t p = static_cast<uint32_t>(static_cast<uint64_t>(s) >> P.shift1);
+      const uint32_t i = atomicAdd(&S.n[p], 1u);
+      if (i < static_cast<uint32_t>(kJaStage)) {
+        S.sr[p][i] = static_cast<unsigned long long>(su) | (row << 32);
+        S.pay[p][i] = x[j];
+      } else {
+        ja_add_direct(A, static_cast<uint64_t>(s), x[j], row);
+      }
+    }
+    if (want) S.nxt[threadIdx.x] = ahead;
+    lds_barrier();
+    ja_stage1_flush(S, P, A, kJaFlushAt);
+    lds_barrier();
+  }
+  ja_stage1_flush(S, P, A, 1);
+  lds_barrier();
+  // unused reserved records (the current block's rest, the block reserved
+  // ahead) are marked as holes
+  for (int p = threadIdx.x >> 6; p < P.n_p1; p += kJaWaves1) {
+    const unsigned long long cur = S.cur[p], end = S.end[p] < P.cap1 ? S.end[p] : P.cap1;
+    for (unsigned long long at = cur + (threadIdx.x & 63); at < end; at += 64)
+      P.l1_sr[static_cast<uint64_t>(p) * P.cap1 + at] = kJaHole;
+    const unsigned long long nx = S.nxt[p];
+    if (nx != ~0ULL) {
+      const unsigned long long ne = nx + kJaBlk < P.cap1 ? nx + kJaBlk : P.cap1;
+      for (unsigned long long at = nx + (threadIdx.x & 63); at < ne; at += 64)
+        P.l1_sr[static_cast<uint64_t>(p) * P.cap1 + at] = kJaHole;
+    }
+  }
+  // this workgroup's cache and register totals are added to its own row of
+  // per-workgroup partials (the same row in every pass of the probe call);
+  // one reduction after the last pass adds them to the slot records
+  __syncthreads();
+  r0.to_lds(C, 0);
+  r1.to_lds(C, 1);
+  r2.to_lds(C, 2);
+  r3.to_lds(C, 3);
+  __syncthreads();
+  unsigned long long* row = cache_part + static_cast<uint64_t>(blockIdx.x) * kJaCacheAll * 4;
+  for (int c = threadIdx.x; c < kJaCacheAll; c += blockDim.x) {
+    const unsigned long long cnt = C.cnt[c];
+    if (!cnt) continue;
+    unsigned long long* e = row + 4 * c;
+    e[0] += cnt;
+    e[1] += C.lo[c];
+    e[2] += static_cast<unsigned long long>(C.hi[c]);
+    if (C.first[c] < e[3]) e[3] = C.first[c];
+  }
+}
+
+// Per-workgroup cache partials summed per entry and added to the slot records:
+// a block takes 32 entries, its 8 thread groups split the workgroup rows.
+constexpr int kJaRedEntries = 32;
+__global__ __launch_bounds__(256) void k_ja_cache_reduce(const unsigned long long* __restrict__ cache_part,
+                                                         uint32_t n_wg, const uint32_t* __restrict__ hot, JaAcc A) {
+  __shared__ unsigned long long r_cnt[8][kJaRedEntries], r_lo[8][kJaRedEntries], r_f[8][kJaRedEntries];
+  __shared__ long long r_hi[8][kJaRedEntries];
+  const uint32_t ce = threadIdx.x % kJaRedEntries, g = threadIdx.x / kJaRedEntries;
+  const uint32_t c = blockIdx.x * kJaRedEntries + ce;
+  unsigned long long cnt = 0, lo = 0, f = ~0ULL;
+  long long hi = 0;
+  if (c < static_cast<uint32_t>(kJaCacheAll))
+    for (uint32_t w = g; w < n_wg; w += 8) {
+      const unsigned long long* e = cache_part + (static_cast<uint64_t>(w) * kJaCacheAll + c) * 4;
+      cnt += e[0];
+      lo += e[1];
+      hi += static_cast<long long>(e[2]);
+      f = e[3] < f ? e[3] : f;
+    }
+  r_cnt[g][ce] = cnt;
+  r_lo[g][ce] = lo;
+  r_hi[g][ce] = hi;
+  r_f[g][ce] = f;
+  __syncthreads();
+  if (g != 0 || c >= static_cast<uint32_t>(kJaCacheAll)) return;
+  for (int k = 1; k < 8; ++k) {
+    cnt += r_cnt[k][ce];
+    lo += r_lo[k][ce];
+    hi += r_hi[k][ce];
+    f = r_f[k][ce] < f ? r_f[k][ce] : f;
+  }
+  const uint32_t s = c < static_cast<uint32_t>(kJaCache) ? hot[16 + c] : hot[c - kJaCache];
+  if (s == kJaEmpty || !cnt) return;
+  atomicAdd(&A.w[4 * static_cast<uint64_t>(s)], cnt);
+  Split128 sp;
+  sp.lo32 = lo;
+  sp.hi32 = hi;
+  uint64_t l;
+  int64_t h;
+  sp.to_i128(l, h);
+  if (l | static_cast<uint64_t>(h))
+    atomic_add_i128(&A.w[4 * static_cast<uint64_t>(s) + 1], &A.w[4 * static_cast<uint64_t>(s) + 2], l, h);
+  atomicMin(&A.w[4 * static_cast<uint64_t>(s) + 3], f);
+}
+
+// Slot's build-side row count (the table keeps key multiplicities).
+template <int KIND>
+__device__ __forceinline__ uint32_t ja_slot_count(const TableView& T, uint64_t s, bool unique) {
+  if constexpr (KIND == PRISM_TABLE_DENSE) {
+    if (T.presence_only) return T.dense_u[s];
+    if (unique) return T.has_payload ? (T.dense_up[s].x != kNone ? 1u : 0u) : (T.dense_u[s] != kNone ? 1u : 0u);
+    return T.dense_nu[s].y;
+  } else {
+    const Slot e = T.slots[s];
+    return e.key == kEmpty ? 0u : (unique ? 1u : e.cnt);
+  }
+}
+
+// Hot-slot selection: a histogram of build-side row counts on a log scale
+// (8 bins per doubling) gives the host a count threshold; the slots at or
+// above it are listed, and the host ranks them.
+constexpr int kJaHotBins = 256;
+constexpr uint32_t kJaHotListMax = 4096;
+
+__device__ __forceinline__ int ja_count_bin(uint32_t c) {
+  if (c == 0) return -1;
+  const int b = static_cast<int>(8.0f * __log2f(static_cast<float>(c)));
+  return b < kJaHotBins - 1 ? b : kJaHotBins - 1;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_ja_count_hist(TableView T, bool unique, uint64_t n_slots,
+                                                       unsigned int* hist) {
+  __shared__ unsigned int h[kJaHotBins];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * 256ULL + threadIdx.x; i < n_slots; i += static_cast<uint64_t>(gridDim.x) * 256) {
+    const int b = ja_count_bin(ja_slot_count<KIND>(T, i, unique));
+    if (b >= 0) atomicAdd(&h[b], 1u);
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_ja_hot_list(TableView T, bool unique, uint64_t n_slots, int min_bin,
+                                                     unsigned long long* list, unsigned int* n_list) {
+  __shared__ unsigned long long l[256];
+  __shared__ unsigned int ln, lbase;
+  for (uint64_t i0 = blockIdx.x * 256ULL; i0 < n_slots; i0 += static_cast<uint64_t>(gridDim.x) * 256) {
+    if (threadIdx.x == 0) ln = 0;
+    __syncthreads();
+    const uint64_t i = i0 + threadIdx.x;
+    if (i < n_slots) {
+      const uint32_t c = ja_slot_count<KIND>(T, i, unique);
+      if (ja_count_bin(c) >= min_bin) l[atomicAdd(&ln, 1u)] = (static_cast<unsigned long long>(c) << 32) | i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && ln) lbase = atomicAdd(n_list, ln);
+    __syncthreads();
+    if (threadIdx.x < ln && lbase + threadIdx.x < kJaHotListMax) list[lbase + threadIdx.x] = l[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// Level-1 records of repartition workgroup b: partition b / nb, records [lo, hi).
+__device__ __forceinline__ bool ja_part_block(const JaPart& P, uint32_t& p, uint32_t& blk, uint64_t& lo, uint64_t& hi) {
+  p = blockIdx.x / P.nb;
+  blk = blockIdx.x % P.nb;
+  const uint64_t f = P.fill1[p];
+  const uint64_t fill = f < P.cap1 ? f : P.cap1;
+  lo = static_cast<uint64_t>(blk) * kJaPartRecs;
+  if (lo >= fill) return false;
+  hi = lo + kJaPartRecs < fill ? lo + kJaPartRecs : fill;
+  return true;
+}
+
+// K1b (count): records per (partition, block, range); holes are skipped.
+__global__ __launch_bounds__(kJaPartBlock) void k_ja_count2(JaPart P) {
+  __shared__ uint32_t hist[kJaFanMax];
+  uint32_t p, blk;
+  uint64_t lo, hi;
+  const bool any = ja_part_block(P, p, blk, lo, hi);
+  for (int i = threadIdx.x; i < P.fan; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  if (any) {
+    const unsigned long long* sr = P.l1_sr + static_cast<uint64_t>(p) * P.cap1;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      const uint32_t s = static_cast<uint32_t>(sr[i]);
+      if (s != kJaHole) atomicAdd(&hist[(s >> kJaSubShift) & static_cast<uint32_t>(P.fan - 1)], 1u);
     }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < kJaCache; c += kJaBlock) {
-    if (ck[c] == kJaEmpty) continue;
-    const uint64_t s = ck[c];
-    atomicAdd(&A.cnt[s], cc[c]);
+  for (int i = threadIdx.x; i < P.fan; i += blockDim.x)
+    P.blk2[static_cast<uint64_t>(blockIdx.x) * P.fan + i] = hist[i];
+}
+
+// Range totals over the repartition blocks (one thread per range).
+__global__ void k_ja_tot2(JaPart P, uint32_t n_ranges) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_ranges) return;
+  const uint32_t fan = static_cast<uint32_t>(P.fan);
+  const uint32_t p = r / fan, g = r % fan;
+  uint32_t t = 0;
+  for (uint32_t k = 0; k < P.nb; ++k) t += static_cast<uint32_t>(P.blk2[(static_cast<uint64_t>(p) * P.nb + k) * fan + g]);
+  P.cnt2[r] = t;
+}
+
+// Exclusive scan of the range totals: each range's first place in l2 (one
+// workgroup; a few ranges per thread).
+__global__ __launch_bounds__(kJaPartBlock) void k_ja_scan2(JaPart P, uint32_t n_ranges) {
+  __shared__ unsigned long long part[kJaPartBlock];
+  const uint32_t per = (n_ranges + kJaPartBlock - 1) / kJaPartBlock;
+  const uint32_t b = threadIdx.x * per;
+  unsigned long long s = 0;
+  for (uint32_t i = 0; i < per; ++i)
+    if (b + i < n_ranges) s += P.cnt2[b + i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < kJaPartBlock; d <<= 1) {
+    const unsigned long long v = threadIdx.x >= static_cast<uint32_t>(d) ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned long long run = part[threadIdx.x] - s;
+  for (uint32_t i = 0; i < per; ++i)
+    if (b + i < n_ranges) {
+      P.off2[b + i] = run;
+      run += P.cnt2[b + i];
+    }
+}
+
+// K1b (scatter): level-1 records to their range's place in l2, staged in LDS;
+// each block writes after the records of the blocks before it (no device
+// atomics).
+__global__ __launch_bounds__(kJaPartBlock) void k_ja_scatter2(JaPart P) {
+  __shared__ uint4 st[kJaStage2];
+  __shared__ uint32_t st_n[kJaFanMax];
+  __shared__ unsigned long long cur[kJaFanMax];
+  uint32_t p, blk;
+  uint64_t lo, hi;
+  if (!ja_part_block(P, p, blk, lo, hi)) return;
+  const uint32_t fan = static_cast<uint32_t>(P.fan);
+  const uint32_t cap = kJaStage2 / fan;  // staging records per range
+  const uint32_t at = cap / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < fan; i += blockDim.x) {
+    // this block's first place in range i: after the earlier blocks' records
+    unsigned long long at = P.off2[static_cast<uint64_t>(p) * fan + i];
+    for (uint32_t k = 0; k < blk; ++k) at += P.blk2[(static_cast<uint64_t>(p) * P.nb + k) * fan + i];
+    st_n[i] = 0;
+    cur[i] = at;
+  }
+  __syncthreads();
+  const uint64_t r0 = static_cast<uint64_t>(p) * P.cap1;
+  for (uint64_t base = lo;; base += kJaPartBlock) {
+    const uint64_t i = base + threadIdx.x;
+    if (i < hi) {
+      const unsigned long long sr = P.l1_sr[r0 + i];
+      const uint32_t s = static_cast<uint32_t>(sr);
+      if (s != kJaHole) {
+        const uint64_t x = static_cast<uint64_t>(P.l1_pay[r0 + i]);
+        const uint4 rec = make_uint4(s & (kJaSub - 1), static_cast<uint32_t>(sr >> 32), static_cast<uint32_t>(x),
+                                     static_cast<uint32_t>(x >> 32));
+        const uint32_t g = (s >> kJaSubShift) & (fan - 1);
+        const uint32_t k = atomicAdd(&st_n[g], 1u);
+        if (k < cap)
+          st[g * cap + k] = rec;
+        else
+          P.l2[atomicAdd(&cur[g], 1ULL)] = rec;
+      }
+    }
+    const bool last = base + kJaPartBlock >= hi;
+    lds_barrier();
+    for (uint32_t g = wave; g < fan; g += kJaWavesPB) {
+      const uint32_t n0 = st_n[g];
+      const uint32_t m = n0 < cap ? n0 : cap;
+      if (m == 0 || (!last && m < at)) continue;
+      const unsigned long long pos = cur[g];
+      for (uint32_t j = lane; j < m; j += 64) P.l2[pos + j] = st[g * cap + j];
+      if (lane == 0) {
+        st_n[g] = 0;
+        cur[g] = pos + m;
+      }
+    }
+    lds_barrier();
+    if (last) break;
+  }
+}
+
+// K2: one workgroup per range sums its records in LDS and adds the totals to
+// the slot arrays.
+__global__ __launch_bounds__(kJaPartBlock) void k_ja_agg2(JaPart P, JaAcc A) {
+  __shared__ uint32_t a_cnt[kJaSub], a_first[kJaSub];
+  __shared__ unsigned long long a_lo[kJaSub];
+  __shared__ long long a_hi[kJaSub];
+  const uint32_t g = blockIdx.x;
+  const uint32_t m = P.cnt2[g];
+  if (m == 0) return;
+  for (int i = threadIdx.x; i < kJaSub; i += blockDim.x) {
+    a_cnt[i] = 0;
+    a_first[i] = 0xFFFFFFFFu;
+    a_lo[i] = 0;
+    a_hi[i] = 0;
+  }
+  __syncthreads();
+  const uint4* rec = P.l2 + P.off2[g];
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+    const uint4 r = rec[i];
+    atomicAdd(&a_cnt[r.x], 1u);
+    if (r.z) atomicAdd(&a_lo[r.x], static_cast<unsigned long long>(r.z));
+    if (r.w) atomicAdd(reinterpret_cast<unsigned long long*>(&a_hi[r.x]),
+                       static_cast<unsigned long long>(static_cast<long long>(static_cast<int32_t>(r.w))));
+    if (r.y < a_first[r.x]) atomicMin(&a_first[r.x], r.y);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kJaSub; i += blockDim.x) {
+    const uint32_t c = a_cnt[i];
+    if (!c) continue;
+    const uint64_t s = static_cast<uint64_t>(g) * kJaSub + i;
+    A.w[4 * s] += c;
     Split128 sp;
-    sp.lo32 = clo[c];
-    sp.hi32 = chi[c];
+    sp.lo32 = a_lo[i];
+    sp.hi32 = a_hi[i];
     uint64_t lo;
     int64_t hi;
     sp.to_i128(lo, hi);
-    if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&A.sum[2 * s], &A.sum[2 * s + 1], lo, hi);
-    atomicMin(&A.first[s], cf[c]);
+    const uint64_t olo = A.w[4 * s + 1];
+    const uint64_t nlo = olo + lo;
+    A.w[4 * s + 1] = nlo;
+    A.w[4 * s + 2] += static_cast<uint64_t>(hi) + (nlo < olo ? 1u : 0u);
+    const unsigned long long f = a_first[i];
+    if (f < A.w[4 * s + 3]) A.w[4 * s + 3] = f;
   }
 }
 
@@ -1177,26 +1770,168 @@ __global__ void k_ja_build(TableView T, bool unique, const int64_t* __restrict__
        b += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const int64_t s = slot_of<KIND>(T, bkeys[b], unique);
     if (s < 0) continue;
-    const unsigned long long c = A.cnt[s];
+    const unsigned long long c = A.w[4 * s];
     if (c == 0) continue;
     const int64_t g = ja_group(G, battr[b]);
     if (g < 0) continue;
     atomicAdd(&G.cnt[g], c);
-    const uint64_t lo = A.sum[2 * s];
-    const int64_t hi = static_cast<int64_t>(A.sum[2 * s + 1]);
+    const uint64_t lo = A.w[4 * s + 1];
+    const int64_t hi = static_cast<int64_t>(A.w[4 * s + 2]);
     if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&G.sum[2 * g], &G.sum[2 * g + 1], lo, hi);
-    atomicMin(&G.first[g], (A.first[s] << 32) | b);
+    atomicMin(&G.first[g], (A.w[4 * s + 3] << 32) | b);
   }
 }
 
-__global__ void k_ja_collect(JaGroups G, unsigned long long* n_found, uint64_t* first_out, uint32_t* idx_out) {
-  for (uint64_t g = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; g < G.n;
-       g += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    if (G.cnt[g] == 0) continue;
-    const unsigned long long i = atomicAdd(n_found, 1ULL);
-    first_out[i] = G.first[g];
-    idx_out[i] = static_cast<uint32_t>(g);
+// Dense groups: build rows sorted by attribute at create time (stable, so
+// rows ascend within an attribute).  Sort key = attribute offset, value =
+// (slot << 32) | build row.
+template <int KIND>
+__global__ void k_ja_attr_key(TableView T, bool unique, const int64_t* __restrict__ bkeys,
+                              const int32_t* __restrict__ attr, uint64_t n, int64_t attr_min, uint32_t* __restrict__ key,
+                              unsigned long long* __restrict__ val) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    key[i] = static_cast<uint32_t>(static_cast<int64_t>(attr[i]) - attr_min);
+    const int64_t s = slot_of<KIND>(T, bkeys[i], unique);
+    const unsigned long long sl = s < 0 ? 0xFFFFFFFFULL : static_cast<unsigned long long>(s);
+    val[i] = (sl << 32) | i;
   }
+}
+
+// Fold of the per-slot probe totals into the attribute groups over the
+// attribute-sorted build rows: each thread sums the runs of equal attribute
+// in its kFoldRows rows; a run wholly inside the thread's rows is the whole
+// group and is stored, a run cut by a chunk edge is added with atomics.
+constexpr int kFoldRows = 32;
+
+__device__ __forceinline__ void ja_group_put(const JaGroups& G, uint32_t g, bool partial, unsigned long long c,
+                                             uint64_t lo, int64_t hi, unsigned long long f) {
+  if (partial) {
+    atomicAdd(&G.cnt[g], c);
+    if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&G.sum[2 * g], &G.sum[2 * g + 1], lo, hi);
+    atomicMin(&G.first[g], f);
+  } else {
+    G.cnt[g] = c;
+    G.sum[2 * g] = lo;
+    G.sum[2 * g + 1] = static_cast<unsigned long long>(hi);
+    G.first[g] = f;
+  }
+}
+
+__global__ void k_ja_fold(const uint32_t* __restrict__ sattr, const unsigned long long* __restrict__ sval, uint64_t n,
+                          JaAcc A, JaGroups G) {
+  constexpr int kAhead = 8;  // slot records in flight per thread
+  const uint64_t lo = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) * kFoldRows;
+  if (lo >= n) return;
+  const uint64_t hi = lo + kFoldRows < n ? lo + kFoldRows : n;
+  uint32_t a = sattr[lo];
+  bool partial = lo > 0 && sattr[lo - 1] == a;
+  unsigned long long c = 0, f = ~0ULL;
+  uint64_t slo = 0;
+  int64_t shi = 0;
+  const uint4* rec = reinterpret_cast<const uint4*>(A.w);
+  for (uint64_t i0 = lo; i0 < hi; i0 += kAhead) {
+    uint32_t at[kAhead];
+    unsigned long long v[kAhead];
+    uint4 r0[kAhead], r1[kAhead];
+#pragma unroll
+    for (int j = 0; j < kAhead; ++j) {
+      const uint64_t i = i0 + j < hi ? i0 + j : hi - 1;
+      at[j] = sattr[i];
+      v[j] = sval[i];
+    }
+#pragma unroll
+    for (int j = 0; j < kAhead; ++j) {
+      const uint32_t s = static_cast<uint32_t>(v[j] >> 32);
+      const uint64_t q = s == 0xFFFFFFFFu ? 0 : 2 * static_cast<uint64_t>(s);
+      r0[j] = rec[q];
+      r1[j] = rec[q + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < kAhead; ++j) {
+      if (i0 + j >= hi) break;
+      if (at[j] != a) {
+        if (c) ja_group_put(G, a, partial, c, slo, shi, f);
+        a = at[j];
+        partial = false;
+        c = 0;
+        f = ~0ULL;
+        slo = 0;
+        shi = 0;
+      }
+      if (static_cast<uint32_t>(v[j] >> 32) == 0xFFFFFFFFu) continue;
+      const unsigned long long cc = (static_cast<unsigned long long>(r0[j].y) << 32) | r0[j].x;
+      if (!cc) continue;
+      c += cc;
+      const uint64_t l = (static_cast<uint64_t>(r0[j].w) << 32) | r0[j].z;
+      const uint64_t nl = slo + l;
+      shi += static_cast<int64_t>((static_cast<uint64_t>(r1[j].y) << 32) | r1[j].x) + (nl < slo ? 1 : 0);
+      slo = nl;
+      const unsigned long long fi = (static_cast<unsigned long long>(r1[j].z) << 32) | (v[j] & 0xFFFFFFFFULL);
+      f = fi < f ? fi : f;
+    }
+  }
+  // the last run goes on in the next thread's rows when the next attribute is the same
+  if (hi < n && sattr[hi] == a) partial = true;
+  if (c) ja_group_put(G, a, partial, c, slo, shi, f);
+}
+
+// Groups holding rows, compacted without atomics: each block counts the
+// groups of its span, then writes them after the earlier blocks' groups.
+constexpr int kCollectBlocks = 1024;
+
+__device__ __forceinline__ void ja_span(const JaGroups& G, uint64_t& lo, uint64_t& hi) {
+  const uint64_t span = ceil_div(G.n, static_cast<uint64_t>(gridDim.x));
+  lo = blockIdx.x * span;
+  hi = lo + span < G.n ? lo + span : G.n;
+}
+
+__global__ __launch_bounds__(256) void k_ja_collect_count(JaGroups G, unsigned int* block_count) {
+  __shared__ unsigned int tot;
+  if (threadIdx.x == 0) tot = 0;
+  __syncthreads();
+  uint64_t lo, hi;
+  ja_span(G, lo, hi);
+  unsigned int c = 0;
+  for (uint64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) c += G.cnt[g] != 0;
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+  if (lane_id() == 0 && c) atomicAdd(&tot, c);
+  __syncthreads();
+  if (threadIdx.x == 0) block_count[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_ja_collect(JaGroups G, const unsigned int* __restrict__ block_count,
+                                                    unsigned long long* n_found, uint64_t* first_out, uint32_t* idx_out) {
+  __shared__ unsigned long long base;
+  __shared__ unsigned int wave_n[4];
+  if (threadIdx.x < 64) {  // this block's offset: the groups of the blocks before it
+    unsigned long long s = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 64) s += block_count[b];
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (threadIdx.x == 0) base = s;
+  }
+  __syncthreads();
+  uint64_t lo, hi;
+  ja_span(G, lo, hi);
+  unsigned long long at = base;
+  const int wave = threadIdx.x >> 6;
+  for (uint64_t g0 = lo; g0 < hi; g0 += blockDim.x) {
+    const uint64_t g = g0 + threadIdx.x;
+    const bool found = g < hi && G.cnt[g] != 0;
+    const uint64_t m = ballot64(found);
+    if (lane_id() == 0) wave_n[wave] = __popcll(m);
+    __syncthreads();
+    unsigned long long before = at;
+    for (int w = 0; w < wave; ++w) before += wave_n[w];
+    if (found) {
+      const uint64_t i = before + lanes_below(m);
+      first_out[i] = G.first[g];
+      idx_out[i] = static_cast<uint32_t>(g);
+    }
+    for (int w = 0; w < 4; ++w) at += wave_n[w];
+    __syncthreads();
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *n_found = at;
 }
 
 // Output row i: attribute, count, sum lo, sum hi, first pair.
@@ -1211,6 +1946,16 @@ __global__ void k_ja_gather(JaGroups G, const uint32_t* __restrict__ order, uint
     sum_lo[i] = G.sum[2 * g];
     sum_hi[i] = static_cast<long long>(G.sum[2 * g + 1]);
     first[i] = G.first[g];
+  }
+}
+
+// Slot records cleared: count 0, sum 0, first probe row "none".
+__global__ void k_fill_acc(unsigned long long* p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+  {
+    reinterpret_cast<uint4*>(p)[2 * i] = make_uint4(0, 0, 0, 0);
+    reinterpret_cast<uint4*>(p)[2 * i + 1] = make_uint4(0, 0, ~0u, ~0u);
   }
 }
 
@@ -1259,12 +2004,22 @@ struct prism_join_agg {
   prism::DeviceBuffer result;  // finished rows: attr(i32) cnt sum_lo sum_hi first
   uint64_t n_result = 0;
   bool finished = false;
+  // partitioned tail: level-1 regions (slot, row, payload arrays), level-2
+  // records, per-partition and per-range counters
+  prism::DeviceBuffer part_l1, part_l2, part_ctl;
+  prism::DeviceBuffer hot;  // the slots with the most build rows: register slots, then LDS cache keys (u32)
+  prism::DeviceBuffer cache_part;  // per K1 workgroup: cache and register totals (4 words per entry)
+  // dense groups: build rows sorted by attribute: attribute offsets (u32), then
+  // (slot << 32 | row) values (u64, 16-byte aligned)
+  prism::DeviceBuffer fold;
+  unsigned long long* fold_vals() const {
+    return reinterpret_cast<unsigned long long*>(fold.as<char>() + ((n_build * 4 + 15) & ~15ULL));
+  }
+  uint64_t part_chunk = 0;  // probe rows per partitioned pass the scratch is sized for
 
   prism::join::JaAcc accs() {
     prism::join::JaAcc A;
-    A.cnt = acc.as<unsigned long long>();
-    A.sum = A.cnt + n_slots;
-    A.first = A.sum + 2 * n_slots;
+    A.w = acc.as<unsigned long long>();
     return A;
   }
   prism::join::JaGroups groups_view() {
@@ -1287,8 +2042,7 @@ static void ja_clear(prism_join_agg* J) {
   hipStream_t st = J->ctx->stream;
   const uint64_t S = J->n_slots;
   unsigned long long* a = J->acc.as<unsigned long long>();
-  hipLaunchKernelGGL(k_fill_u64, grid_for(J->ctx, 3 * S), dim3(kBuildBlock), 0, st, a, 3 * S, 0ULL);
-  hipLaunchKernelGGL(k_fill_u64, grid_for(J->ctx, S), dim3(kBuildBlock), 0, st, a + 3 * S, S, ~0ULL);
+  hipLaunchKernelGGL(k_fill_acc, grid_for(J->ctx, S), dim3(kBuildBlock), 0, st, a, S);
   PRISM_CHECK_LAUNCH();
   J->finished = false;
   J->n_result = 0;
@@ -1668,6 +2422,73 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
   });
 }
 
+static bool t_kind_dense(const prism_join_table* t) { return t->info.kind == PRISM_TABLE_DENSE; }
+
+// The slots with the most build rows: the first kJaRegHot go to registers,
+// the next ones into the LDS cache's buckets (first free way of either
+// bucket, in rank order).  hot buffer: u32 [0, 16) register slots, then the
+// kJaCache cache keys.
+static void ja_pick_hot(prism_join_agg* J, hipStream_t st) {
+  Ctx* c = J->ctx;
+  prism_join_table* t = J->table;
+  const bool dense = t->info.kind == PRISM_TABLE_DENSE;
+  const bool unique = t->info.unique;
+  const uint64_t n_slots = t->info.slots;  // HASHED: the INT64_MIN sentinel slot is never hot
+  DeviceBuffer scratch(c, kJaHotBins * 4 + 16 + kJaHotListMax * 8);
+  unsigned int* hist = scratch.as<unsigned int>();
+  unsigned int* n_list = hist + kJaHotBins;
+  unsigned long long* list = reinterpret_cast<unsigned long long*>(scratch.as<char>() + kJaHotBins * 4 + 16);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, hist, static_cast<uint64_t>(kJaHotBins + 4), 0u);
+  const dim3 g(n_slots ? static_cast<unsigned>(std::min<uint64_t>(ceil_div(n_slots, 256), 1024)) : 1);
+  if (dense)
+    hipLaunchKernelGGL(k_ja_count_hist<PRISM_TABLE_DENSE>, g, dim3(256), 0, st, t->view, unique, n_slots, hist);
+  else
+    hipLaunchKernelGGL(k_ja_count_hist<PRISM_TABLE_HASHED>, g, dim3(256), 0, st, t->view, unique, n_slots, hist);
+  PRISM_CHECK_LAUNCH();
+  std::vector<unsigned int> h(kJaHotBins);
+  PRISM_HIP(hipMemcpyAsync(h.data(), hist, kJaHotBins * 4, hipMemcpyDeviceToHost, st));
+  PRISM_HIP(hipStreamSynchronize(st));
+  int min_bin = kJaHotBins;
+  uint64_t taken = 0;
+  while (min_bin > 0 && taken + h[min_bin - 1] <= kJaHotListMax) taken += h[--min_bin];
+  std::vector<uint32_t> hot(16 + kJaCache, kJaEmpty);
+  if (taken) {
+    if (dense)
+      hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_DENSE>, g, dim3(256), 0, st, t->view, unique, n_slots, min_bin, list,
+                         n_list);
+    else
+      hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_HASHED>, g, dim3(256), 0, st, t->view, unique, n_slots, min_bin,
+                         list, n_list);
+    PRISM_CHECK_LAUNCH();
+    std::vector<unsigned long long> l(taken);
+    PRISM_HIP(hipMemcpyAsync(l.data(), list, taken * 8, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+    // most rows first, then the lower slot
+    std::sort(l.begin(), l.end(), [](unsigned long long a, unsigned long long b) {
+      const uint32_t ca = static_cast<uint32_t>(a >> 32), cb = static_cast<uint32_t>(b >> 32);
+      return ca != cb ? ca > cb : static_cast<uint32_t>(a) < static_cast<uint32_t>(b);
+    });
+    size_t i = 0;
+    for (int j = 0; j < kJaRegHot && i < l.size(); ++j, ++i) hot[j] = static_cast<uint32_t>(l[i]);
+    uint32_t* keys = hot.data() + 16;
+    for (; i < l.size(); ++i) {
+      const uint32_t sl = static_cast<uint32_t>(l[i]);
+      bool placed = false;
+      for (int tb = 0; tb < 2 && !placed; ++tb) {
+        const uint32_t b = ja_bucket(sl, tb);
+        for (int w = 0; w < kJaWays && !placed; ++w)
+          if (keys[kJaWays * b + w] == kJaEmpty) {
+            keys[kJaWays * b + w] = sl;
+            placed = true;
+          }
+      }
+    }
+  }
+  J->hot.alloc(c, hot.size() * 4);
+  PRISM_HIP(hipMemcpyAsync(J->hot.ptr, hot.data(), hot.size() * 4, hipMemcpyHostToDevice, st));
+  PRISM_HIP(hipStreamSynchronize(st));
+}
+
 prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, const int32_t* d_build_attr,
                                    uint64_t n_build, prism_join_agg** out) {
   return guarded([&] {
@@ -1698,6 +2519,7 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
       PRISM_CHECK_LAUNCH();
     }
     J->counters.alloc(c, 64);
+    ja_pick_hot(J.get(), ctx->stream);
     // attribute range: dense groups when it is small, else a hashed group table
     hipStream_t st = ctx->stream;
     int64_t lo = 0, hi = -1;
@@ -1722,6 +2544,31 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
       J->dense_groups = true;
       J->n_groups_cap = range;
       J->groups.alloc(c, range * 4 * 8);
+      if (n_build) {
+        // build rows sorted by attribute once, for the atomic-free fold
+        J->fold.alloc(c, n_build * 12 + 16);
+        uint32_t* sattr = J->fold.as<uint32_t>();
+        unsigned long long* sval = J->fold_vals();
+        DeviceBuffer tmp_kv(c, n_build * 12 + 16);
+        uint32_t* k_in = tmp_kv.as<uint32_t>();
+        unsigned long long* v_in = reinterpret_cast<unsigned long long*>(tmp_kv.as<char>() + ((n_build * 4 + 15) & ~15ULL));
+        if (t_kind_dense(J->table))
+          hipLaunchKernelGGL(k_ja_attr_key<PRISM_TABLE_DENSE>, grid_for(c, n_build), dim3(kBuildBlock), 0, st,
+                             J->table->view, J->table->info.unique, d_build_keys, d_build_attr, n_build, lo, k_in, v_in);
+        else
+          hipLaunchKernelGGL(k_ja_attr_key<PRISM_TABLE_HASHED>, grid_for(c, n_build), dim3(kBuildBlock), 0, st,
+                             J->table->view, J->table->info.unique, d_build_keys, d_build_attr, n_build, lo, k_in, v_in);
+        PRISM_CHECK_LAUNCH();
+        int bits = 1;
+        while (bits < 32 && (1ULL << bits) < range) ++bits;
+        size_t tmp = 0;
+        PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k_in, sattr, v_in, sval, static_cast<int>(n_build), 0,
+                                                     bits, st));
+        DeviceBuffer t(c, tmp);
+        PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(t.ptr, tmp, k_in, sattr, v_in, sval, static_cast<int>(n_build), 0,
+                                                     bits, st));
+        PRISM_HIP(hipStreamSynchronize(st));
+      }
     } else {
       J->dense_groups = false;
       J->n_groups_cap = next_pow2(2 * n_build > 1024 ? 2 * n_build : 1024);
@@ -1732,6 +2579,36 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
     *out = J.release();
   });
 }
+
+namespace prism {
+namespace join {
+
+// Shape of the partitioned tail for a slot space: ranges of kJaSub slots, at
+// most kJaP1Max level-1 partitions of `fan` ranges each.  False when the slot
+// space needs more ranges than two levels of staging cover.
+struct JaShape {
+  int n_p1 = 0, fan = 1, shift1 = kJaSubShift;
+};
+static bool ja_shape(uint64_t n_slots, JaShape& sh) {
+  const uint64_t ranges = ceil_div(n_slots, kJaSub);
+  uint64_t fan = next_pow2(ceil_div(ranges, kJaP1Max));
+  if (fan < 1) fan = 1;
+  if (fan > static_cast<uint64_t>(kJaFanMax)) return false;
+  sh.fan = static_cast<int>(fan);
+  sh.shift1 = kJaSubShift + log2_pow2(fan);
+  sh.n_p1 = static_cast<int>(ceil_div(ranges, fan));
+  return true;
+}
+
+static uint64_t env_u64(const char* name, uint64_t dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  const unsigned long long x = std::strtoull(v, nullptr, 10);
+  return x ? x : dflt;
+}
+
+}  // namespace join
+}  // namespace prism
 
 prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys, const int64_t* d_probe_payload,
                                   uint64_t n_probe, uint64_t probe_row_base) {
@@ -1744,20 +2621,93 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     std::lock_guard<std::mutex> lock(J->ctx->mu);
     J->finished = false;
     if (n_probe == 0 || J->n_build == 0) return;
+    Ctx* c = J->ctx;
+    hipStream_t st = c->stream;
     const TableView& T = J->table->view;
     const bool unique = J->table->info.unique;
-    uint64_t blocks = ceil_div(n_probe, kJaBlock * 8);
-    const uint64_t cap = static_cast<uint64_t>(J->ctx->num_cus) * 4;
-    if (blocks > cap) blocks = cap;
-    KernelTimer kt(J->ctx, "join_agg_probe");
-    if (J->table->info.kind == PRISM_TABLE_DENSE)
-      hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_DENSE>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0,
-                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(),
-                         J->present.as<uint32_t>());
-    else
-      hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0,
-                         J->ctx->stream, T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(),
-                         static_cast<const uint32_t*>(nullptr));
+    const bool dense = J->table->info.kind == PRISM_TABLE_DENSE;
+    const uint32_t* present = dense ? J->present.as<uint32_t>() : nullptr;
+    JaShape sh;
+    // PRISM_JA_PART=0 keeps every cache miss on device atomics (the
+    // unpartitioned pass); PRISM_JA_CHUNK sets the probe rows per partitioned
+    // pass (scratch: 48 B per row of a pass).
+    const bool part = env_u64("PRISM_JA_PART", 1) != 0 && ja_shape(J->n_slots, sh);
+    KernelTimer kt(c, "join_agg_probe");
+    if (!part) {
+      uint64_t blocks = ceil_div(n_probe, kJaBlock * 8);
+      const uint64_t cap = static_cast<uint64_t>(c->num_cus) * 4;
+      if (blocks > cap) blocks = cap;
+      if (dense)
+        hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_DENSE>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0, st,
+                           T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(), present,
+                           J->hot.as<uint32_t>() + 16);
+      else
+        hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0, st,
+                           T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(), present,
+                           J->hot.as<uint32_t>() + 16);
+      PRISM_CHECK_LAUNCH();
+      return;
+    }
+    uint64_t chunk = env_u64("PRISM_JA_CHUNK", 32ULL << 20);
+    if (chunk > n_probe) chunk = n_probe;
+    const uint64_t n_ranges = static_cast<uint64_t>(sh.n_p1) * sh.fan;
+    // twice a partition's share of the pass, plus the unused tails of the
+    // blocks K1 workgroups reserve
+    uint64_t g1max = ceil_div(chunk, static_cast<uint64_t>(kJaBlock1) * kJaRows1);
+    if (g1max > 2 * static_cast<uint64_t>(c->num_cus)) g1max = 2 * static_cast<uint64_t>(c->num_cus);
+    uint64_t cap1 = 2 * chunk / sh.n_p1 + 64;
+    if (cap1 > chunk) cap1 = chunk;
+    cap1 += g1max * kJaBlk;
+    if (chunk > J->part_chunk || J->part_l1.bytes < sh.n_p1 * cap1 * 16) {
+      J->part_l1.reserve(c, sh.n_p1 * cap1 * 16);
+      J->part_l2.reserve(c, chunk * 16);
+      J->part_chunk = chunk;
+    }
+    const uint32_t nb = static_cast<uint32_t>(ceil_div(cap1, kJaPartRecs));
+    J->part_ctl.reserve(c, sh.n_p1 * 8 + n_ranges * (4 + 8) + static_cast<uint64_t>(sh.n_p1) * nb * sh.fan * 8 + 64);
+    JaPart P;
+    char* l1 = J->part_l1.as<char>();
+    P.l1_sr = reinterpret_cast<unsigned long long*>(l1);
+    P.l1_pay = reinterpret_cast<long long*>(P.l1_sr + sh.n_p1 * cap1);
+    P.cap1 = cap1;
+    P.shift1 = sh.shift1;
+    P.fan = sh.fan;
+    P.n_p1 = sh.n_p1;
+    char* ctl = J->part_ctl.as<char>();
+    P.fill1 = reinterpret_cast<unsigned long long*>(ctl);
+    P.off2 = P.fill1 + sh.n_p1;
+    P.blk2 = P.off2 + n_ranges;
+    P.cnt2 = reinterpret_cast<uint32_t*>(P.blk2 + static_cast<uint64_t>(sh.n_p1) * nb * sh.fan);
+    P.nb = nb;
+    P.l2 = J->part_l2.as<uint4>();
+    const JaAcc A = J->accs();
+    J->cache_part.reserve(c, g1max * kJaCacheAll * 32);
+    unsigned long long* cpart = J->cache_part.as<unsigned long long>();
+    hipLaunchKernelGGL(k_fill_acc, grid_for(c, g1max * kJaCacheAll), dim3(kBuildBlock), 0, st, cpart,
+                       g1max * kJaCacheAll);
+    for (uint64_t at = 0; at < n_probe; at += chunk) {
+      const uint64_t m = n_probe - at < chunk ? n_probe - at : chunk;
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, P.fill1, static_cast<uint64_t>(sh.n_p1), 0ULL);
+      uint64_t g1 = ceil_div(m, static_cast<uint64_t>(kJaBlock1) * kJaRows1);
+      if (g1 > 2 * static_cast<uint64_t>(c->num_cus)) g1 = 2 * static_cast<uint64_t>(c->num_cus);
+      const int64_t* kp = d_probe_keys + at;
+      const int64_t* pp = d_probe_payload ? d_probe_payload + at : nullptr;
+      if (dense)
+        hipLaunchKernelGGL(k_ja_probe_part<PRISM_TABLE_DENSE>, dim3(static_cast<unsigned>(g1)), dim3(kJaBlock1), 0, st,
+                           T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), J->hot.as<uint32_t>() + 16, cpart);
+      else
+        hipLaunchKernelGGL(k_ja_probe_part<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(g1)), dim3(kJaBlock1), 0,
+                           st, T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), J->hot.as<uint32_t>() + 16, cpart);
+      const dim3 gp(static_cast<unsigned>(sh.n_p1) * nb);
+      hipLaunchKernelGGL(k_ja_count2, gp, dim3(kJaPartBlock), 0, st, P);
+      hipLaunchKernelGGL(k_ja_tot2, grid_for(c, n_ranges), dim3(kBuildBlock), 0, st, P, static_cast<uint32_t>(n_ranges));
+      hipLaunchKernelGGL(k_ja_scan2, dim3(1), dim3(kJaPartBlock), 0, st, P, static_cast<uint32_t>(n_ranges));
+      hipLaunchKernelGGL(k_ja_scatter2, gp, dim3(kJaPartBlock), 0, st, P);
+      hipLaunchKernelGGL(k_ja_agg2, dim3(static_cast<unsigned>(n_ranges)), dim3(kJaPartBlock), 0, st, P, A);
+      PRISM_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_ja_cache_reduce, dim3(static_cast<unsigned>(ceil_div(kJaCacheAll, kJaRedEntries))), dim3(256), 0, st,
+                       cpart, static_cast<uint32_t>(g1max), J->hot.as<uint32_t>(), A);
     PRISM_CHECK_LAUNCH();
   });
 }
@@ -1779,7 +2729,13 @@ prism_status prism_join_agg_finish(prism_join_agg* J, uint64_t* n_groups) {
     const TableView& T = J->table->view;
     const bool unique = J->table->info.unique;
     JaGroups GV = J->groups_view();
-    if (J->n_build) {
+    if (J->n_build && J->dense_groups) {
+      KernelTimer kt(c, "join_agg_build");
+      const uint64_t threads = ceil_div(J->n_build, kFoldRows);
+      hipLaunchKernelGGL(k_ja_fold, dim3(static_cast<unsigned>(ceil_div(threads, kBuildBlock))), dim3(kBuildBlock), 0,
+                         st, J->fold.as<uint32_t>(), J->fold_vals(), J->n_build, J->accs(), GV);
+      PRISM_CHECK_LAUNCH();
+    } else if (J->n_build) {
       KernelTimer kt(c, "join_agg_build");
       if (J->table->info.kind == PRISM_TABLE_DENSE)
         hipLaunchKernelGGL(k_ja_build<PRISM_TABLE_DENSE>, grid_for(c, J->n_build), dim3(kBuildBlock), 0, st, T, unique,
@@ -1791,8 +2747,10 @@ prism_status prism_join_agg_finish(prism_join_agg* J, uint64_t* n_groups) {
     }
     DeviceBuffer first(c, G * 8), first_sorted(c, G * 8), idx(c, G * 4), idx_sorted(c, G * 4);
     unsigned long long* found_p = J->counters.as<unsigned long long>();
-    hipLaunchKernelGGL(k_ja_collect, grid_for(c, G), dim3(kBuildBlock), 0, st, GV, found_p, first.as<uint64_t>(),
-                       idx.as<uint32_t>());
+    DeviceBuffer bcount(c, kCollectBlocks * 4);
+    hipLaunchKernelGGL(k_ja_collect_count, dim3(kCollectBlocks), dim3(256), 0, st, GV, bcount.as<unsigned int>());
+    hipLaunchKernelGGL(k_ja_collect, dim3(kCollectBlocks), dim3(256), 0, st, GV, bcount.as<unsigned int>(), found_p,
+                       first.as<uint64_t>(), idx.as<uint32_t>());
     PRISM_CHECK_LAUNCH();
     unsigned long long h[2];
     PRISM_HIP(hipMemcpyAsync(h, J->counters.ptr, 16, hipMemcpyDeviceToHost, st));

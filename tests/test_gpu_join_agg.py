@@ -108,3 +108,71 @@ def test_medium_against_eager_oracle(ctx):
         assert c == cnt[a] and s == sums[a]
     fp = out["first_probe_row"]
     assert np.all(np.diff(fp) >= 0)
+
+
+def eager_check(out, bk, attr, pk, pay, n_attr):
+    cnt, sums = orc.join_agg_eager(bk, attr, pk, pay, n_attr)
+    nz = [a for a in range(n_attr) if cnt[a] > 0]
+    assert sorted(out["attr"].tolist()) == nz
+    for a, c, s in zip(out["attr"].tolist(), out["count"].tolist(), out["sum"]):
+        assert c == cnt[a] and s == sums[a]
+    assert np.all(np.diff(out["first_probe_row"]) >= 0)
+
+
+@pytest.mark.parametrize("chunk", ["0", "5000", "70001"])
+def test_partitioned_tail_chunks_and_overflow(ctx, monkeypatch, chunk):
+    """Misses of the LDS cache go through the two-level slot partition: small
+    passes (several per probe call) leave level-1 regions only a few hundred
+    records deep, so hot partitions overflow into device atomics; the result
+    must equal the unpartitioned pass and the eager restatement."""
+    say(f"join-agg partitioned tail chunk={chunk}")
+    if chunk != "0":
+        monkeypatch.setenv("PRISM_JA_CHUNK", chunk)
+    d = dg.skewed_join(3_000, 200_000, 60_000, 500, seed=21)
+    pay = d["probe_payload"] - 400_000
+    pay[::5] = pay[::5] << 30  # some negative, some wide payloads
+    batches = [(d["probe_key"][:123_457], pay[:123_457], 0), (d["probe_key"][123_457:], pay[123_457:], 123_457)]
+    out = run(ctx, d["build_key"], d["build_attr"], batches)
+    monkeypatch.setenv("PRISM_JA_PART", "0")
+    ref = run(ctx, d["build_key"], d["build_attr"], batches)
+    for k in ("attr", "count", "first_probe_row", "first_build_row"):
+        assert np.array_equal(out[k], ref[k]), k
+    assert out["sum"] == ref["sum"]
+    eager_check(out, d["build_key"], d["build_attr"], d["probe_key"], pay, 500)
+
+
+@pytest.mark.parametrize("chunk", ["0", "300000"])
+def test_partitioned_tail_wide_key_range(ctx, monkeypatch, chunk):
+    """A 1M-key range needs several aggregation ranges per level-1 partition
+    (fan > 1); uniform probe keys put almost every row in the tail."""
+    say(f"join-agg wide key range chunk={chunk}")
+    if chunk != "0":
+        monkeypatch.setenv("PRISM_JA_CHUNK", chunk)
+    rng = np.random.default_rng(22)
+    n_attr = 5_000
+    bk = rng.integers(0, 1_000_000, 300_000)
+    bk[0], bk[1] = 0, 999_999
+    attr = rng.integers(0, n_attr, 300_000).astype(np.int32)
+    pk = rng.integers(-1_000, 1_001_000, 2_000_000)
+    pay = rng.integers(-(1 << 62), 1 << 62, 2_000_000)
+    out = run(ctx, bk, attr, [(pk, pay, 0)])
+    eager_check(out, bk, attr, pk, pay, n_attr)
+
+
+@pytest.mark.parametrize("chunk", ["0", "5000"])
+def test_partitioned_tail_one_hot_partition(ctx, monkeypatch, chunk):
+    """Every probe key falls in the first 4,096 slots of a 60K-slot range while
+    the rows hold more distinct keys than the LDS cache: the whole tail lands
+    in one level-1 partition, overflowing both the LDS staging buffer of a
+    round and (with 5,000-row passes) the partition's level-1 region."""
+    say(f"join-agg one hot partition chunk={chunk}")
+    if chunk != "0":
+        monkeypatch.setenv("PRISM_JA_CHUNK", chunk)
+    rng = np.random.default_rng(23)
+    bk = np.arange(60_000, dtype=np.int64)[rng.permutation(60_000)]
+    bk = np.concatenate([bk, bk[:7_000]])  # duplicate keys: several build rows per key
+    attr = rng.integers(0, 300, len(bk)).astype(np.int32)
+    pk = rng.integers(0, 4_000, 150_000)
+    pay = rng.integers(-10**15, 10**15, 150_000)
+    out = run(ctx, bk, attr, [(pk, pay, 0)])
+    check(out, *materialized(bk, attr, pk, pay))
