@@ -49,7 +49,7 @@ constexpr int kBuildBlock = 256;
 
 // Reductions run in two stages (per-block partials, then one block) so that
 // no global address takes more than one atomic per launch.
-constexpr int kRedBlocks = 256;
+constexpr int kRedBlocks = 1024;
 
 template <class T, class Op>
 __device__ __forceinline__ T block_reduce(T v, Op op, T* lds) {

@@ -46,6 +46,12 @@ namespace svc {
 using clk = std::chrono::steady_clock;
 
 constexpr int kSB = 256;               // threads per workgroup (group-by kernels assume kBlock)
+// Probe and filter tiles add their totals to one of kAccCopies copies of the
+// request's accumulator (tile index modulo kAccCopies): one address per
+// request would serialise the atomics of thousands of tiles.
+constexpr int kAccCopies = 16;
+// Group-by plans with at most this many slots flush a tile's groups in no time.
+constexpr uint64_t kSmallGroupCap = 64;
 static_assert(kSB == gb::kBlock, "service group-by tiles use the group-by workgroup size");
 constexpr int kMaxReq = 256;           // requests per batch
 constexpr int kMaxTiles = 32768;       // tiles per batch
@@ -75,7 +81,7 @@ struct ProbeDesc {
   const int64_t* keys;
   const int64_t* pay;
   uint64_t n;
-  unsigned long long* acc;  // matches, probe sum lo/hi, build sum lo/hi
+  unsigned long long* acc;  // kAccCopies x {matches, probe sum lo/hi, build sum lo/hi}
 };
 
 struct FiltDesc {
@@ -83,7 +89,7 @@ struct FiltDesc {
   const prism_column* cols;
   uint64_t n;
   unsigned long long* mask;   // optional caller bitmap
-  unsigned long long* count;  // selected rows, then error flags
+  unsigned long long* count;  // kAccCopies selected-row counts, then error flags
 };
 
 using Tile = gb::RowTile;
@@ -163,15 +169,16 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
     const ProbeDesc& d = C->pr[tl.req];
     unsigned long long m = 0;
     __int128 ps = 0, bs = 0;
-    constexpr int U = 4;  // independent lookups in flight per thread
+    constexpr int U = 8;  // independent lookups in flight per thread
     for (uint64_t r0 = tl.start + threadIdx.x; r0 < tl.end; r0 += static_cast<uint64_t>(kSB) * U) {
-      int64_t key[U];
+      int64_t key[U], x[U];
       bool in[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const uint64_t r = r0 + static_cast<uint64_t>(u) * kSB;
         in[u] = r < tl.end;
         key[u] = in[u] ? d.keys[r] : 0;
+        x[u] = (in[u] && d.pay) ? d.pay[r] : 0;  // loaded with the key: no dependent load after the lookup
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
         int32_t pay = 0;
         if (in[u] && join::lookup<KIND, true>(d.T, key[u], val, cnt, pay)) {
           ++m;
-          if (d.pay) ps += d.pay[r0 + static_cast<uint64_t>(u) * kSB];
+          ps += x[u];
           if (d.T.has_payload) bs += pay;
         }
       }
@@ -202,9 +209,10 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
         ps += s_p[w];
         bs += s_b[w];
       }
-      if (m) atomicAdd(&d.acc[0], m);
-      if (ps) atomic_add_i128(&d.acc[1], &d.acc[2], static_cast<uint64_t>(ps), static_cast<int64_t>(ps >> 64));
-      if (bs) atomic_add_i128(&d.acc[3], &d.acc[4], static_cast<uint64_t>(bs), static_cast<int64_t>(bs >> 64));
+      unsigned long long* acc = d.acc + 5 * (t % kAccCopies);
+      if (m) atomicAdd(&acc[0], m);
+      if (ps) atomic_add_i128(&acc[1], &acc[2], static_cast<uint64_t>(ps), static_cast<int64_t>(ps >> 64));
+      if (bs) atomic_add_i128(&acc[3], &acc[4], static_cast<uint64_t>(bs), static_cast<int64_t>(bs >> 64));
     }
     __syncthreads();
   }
@@ -237,7 +245,7 @@ __global__ __launch_bounds__(kSB) void k_svc_filter(Control* C, int v) {
       uint64_t val[R];
       uint32_t valid;
       run_program<R, DEPTH, 1>(d.P->code, d.P->len, d.cols, &d.P->strs, row, in, pre, 0u, val, valid,
-                               reinterpret_cast<uint32_t*>(d.count + 1));
+                               reinterpret_cast<uint32_t*>(d.count + kAccCopies));
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (w0 + r >= w_end) break;
@@ -251,7 +259,7 @@ __global__ __launch_bounds__(kSB) void k_svc_filter(Control* C, int v) {
     if (threadIdx.x == 0) {
       unsigned long long s = 0;
       for (int w = 0; w < kSB / 64; ++w) s += s_c[w];
-      if (s) atomicAdd(d.count, s);
+      if (s) atomicAdd(d.count + t % kAccCopies, s);
     }
     __syncthreads();
   }
@@ -494,10 +502,10 @@ static void footprint(const Req* R, size_t& res_words, size_t& table_words) {
       break;
     }
     case PRISM_REQ_PROBE:
-      res_words = 5;
+      res_words = 5 * kAccCopies;
       break;
     default:
-      res_words = 2;
+      res_words = kAccCopies + 1;
   }
 }
 
@@ -542,10 +550,13 @@ static void launch_batch(prism_service* S, Slot& sl) {
   for (Req* R : sl.reqs)
     kind_rows[R->r.kind == PRISM_REQ_GROUPBY ? K_GB : R->r.kind == PRISM_REQ_PROBE ? K_PROBE : K_FILT] += R->r.n_rows;
   uint64_t tile_of[3];
-  bool gb_shared = false;  // the shared LDS form flushes every group per tile: keep its tiles larger
+  // the shared LDS form flushes every group slot per tile: keep its tiles
+  // larger when a plan has many slots (a few slots flush in no time)
+  bool gb_big_flush = false;
   for (Req* R : sl.reqs)
-    gb_shared = gb_shared || (R->r.kind == PRISM_REQ_GROUPBY && (!R->gbp->priv || jit::enabled()));
-  const uint64_t min_tile[3] = {gb_shared ? 65536u : 16384u, 4096, 4096};
+    gb_big_flush = gb_big_flush || (R->r.kind == PRISM_REQ_GROUPBY && (!R->gbp->priv || jit::enabled()) &&
+                                    R->gbp->cap > kSmallGroupCap);
+  const uint64_t min_tile[3] = {gb_big_flush ? 65536u : 16384u, 4096, 4096};
   for (int k = 0; k < 3; ++k) {
     const Cand cd = cand_of(S, static_cast<Kind>(k));
     const uint64_t want = kind_rows[k] / (static_cast<uint64_t>(n_cu) * cd.bpc * 4);
@@ -557,6 +568,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
   std::vector<std::vector<Tile>> jt;
   std::vector<uint64_t> jrows;
   std::vector<size_t> jlds;
+  std::vector<uint64_t> jcap;  // largest group capacity per generated kernel
   unsigned long long* res_base = sl.arena.as<unsigned long long>();
   unsigned long long* tab_base = res_base + kResultsBytes / 8;
   size_t res_used = 0, tab_used = 0;
@@ -602,7 +614,9 @@ static void launch_batch(prism_service* S, Slot& sl) {
             jt.emplace_back();
             jrows.push_back(0);
             jlds.push_back(0);
+            jcap.push_back(0);
           }
+          if (qi >= 0) jcap[qi] = std::max<uint64_t>(jcap[qi], gp.cap);
         }
         d.lds_slots = qi >= 0 ? gp.jit_slots : gp.lds_slots;
         d.lds_shift = qi >= 0 ? gp.jit_shift : gp.lds_shift;
@@ -640,7 +654,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
         d.acc = R->d_out;
         InitDesc& a = H->init[H->n_init++];
         a.dst = R->d_out;
-        a.n_words = 5;
+        a.n_words = 5 * kAccCopies;
         const int v = R->table->t->info.kind == PRISM_TABLE_DENSE ? V_PROBE_DENSE : V_PROBE_HASHED;
         const uint64_t tr = std::max<uint64_t>(kSB, tile_of[K_PROBE]);
         for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
@@ -660,7 +674,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
         d.count = R->d_out;
         InitDesc& a = H->init[H->n_init++];
         a.dst = R->d_out;
-        a.n_words = 2;
+        a.n_words = kAccCopies + 1;
         const uint64_t tr = std::max<uint64_t>(64 * 64, tile_of[K_FILT] / 64 * 64);
         for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
           vt[fp.variant].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
@@ -696,6 +710,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
     off += static_cast<uint32_t>(jt[i].size());
     const Cand cd = cand_of(S, K_GB);
     uint32_t bpc = cd.bpc;
+    if (jcap[i] <= kSmallGroupCap) bpc = std::max<uint32_t>(bpc, 4);  // cheap flushes: fill the CUs
     if (jlds[i]) bpc = std::min<uint32_t>(bpc, std::max<uint32_t>(1, static_cast<uint32_t>((160u << 10) / jlds[i])));
     jgrid[i] = static_cast<int>(std::min<uint64_t>(jt[i].size(), static_cast<uint64_t>(n_cu) * bpc));
   }
@@ -812,17 +827,29 @@ static void finalize(prism_service* S, Slot& sl) {
         R->rows.assign(o + 2, o + 2 + n * stride);
         break;
       }
-      case PRISM_REQ_PROBE:
-        res.count = o[0];
-        res.probe_sum_lo = o[1];
-        res.probe_sum_hi = static_cast<int64_t>(o[2]);
-        res.build_sum_lo = o[3];
-        res.build_sum_hi = static_cast<int64_t>(o[4]);
+      case PRISM_REQ_PROBE: {
+        unsigned long long cnt = 0;
+        unsigned __int128 ps = 0, bs = 0;
+        for (int c = 0; c < kAccCopies; ++c) {
+          const unsigned long long* a = o + 5 * c;
+          cnt += a[0];
+          ps += (static_cast<unsigned __int128>(a[2]) << 64) | a[1];
+          bs += (static_cast<unsigned __int128>(a[4]) << 64) | a[3];
+        }
+        res.count = cnt;
+        res.probe_sum_lo = static_cast<uint64_t>(ps);
+        res.probe_sum_hi = static_cast<int64_t>(static_cast<uint64_t>(ps >> 64));
+        res.build_sum_lo = static_cast<uint64_t>(bs);
+        res.build_sum_hi = static_cast<int64_t>(static_cast<uint64_t>(bs >> 64));
         if (R->table) --R->table->pins;
         break;
-      default:
-        res.count = o[0];
-        if (o[1] & gb::kFlagOverflow) res.status = PRISM_ERR_EXECUTION;
+      }
+      default: {
+        unsigned long long cnt = 0;
+        for (int c = 0; c < kAccCopies; ++c) cnt += o[c];
+        res.count = cnt;
+        if (o[kAccCopies] & gb::kFlagOverflow) res.status = PRISM_ERR_EXECUTION;
+      }
     }
     res.latency_ms = ms_between(R->t_submit, now);
     res.queue_ms = ms_between(R->t_submit, R->t_launch);
