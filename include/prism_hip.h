@@ -431,6 +431,31 @@ prism_status prism_hash_aggregate_batches(prism_ctx* ctx, const struct ArrowArra
                                           struct ArrowArray* out, struct ArrowSchema* out_schema);
 
 /* ------------------------------------------------------------------------- */
+/* Distributed aggregate (prism-bench handler.rs:270-344, 397-526): each     */
+/* producer runs the plan with the expanded aggregate list and ships its     */
+/* partial batch (Flight stays on the host); the reducer merges them.        */
+/* ------------------------------------------------------------------------- */
+/* expand_worker_aggregates (handler.rs:397-428): AVG becomes SUM + COUNT of
+ * the same column; worker aggregate i is named "agg_<i>".  out_worker has
+ * room for 2 * n entries; out_map[2i] (and out_map[2i+1] for an AVG, else
+ * UINT32_MAX) are the worker indices of original aggregate i.  The names
+ * stay valid until the next call on the same thread. */
+prism_status prism_expand_worker_aggregates(const prism_agg_expr* original, uint32_t n, prism_agg_expr* out_worker,
+                                            uint32_t* n_worker, uint32_t* out_map);
+/* Reducer merge of partial batches (group columns first, then the worker
+ * aggregates of `original` in expansion order): build_final_merge_config
+ * (handler.rs:430-457: SUM for SUM / COUNT / COUNT DISTINCT partials, MIN,
+ * MAX), hash_aggregate_batches on the device, then
+ * reconstruct_original_aggregate_batch (handler.rs:459-526): COUNT and COUNT
+ * DISTINCT as Int64 (`as i64` of the merged sum), AVG as Float64 sum / count
+ * (0.0 when the count is 0), SUM / MIN / MAX as Float64; nulls read as 0. */
+prism_status prism_merge_partial_aggregates(prism_ctx* ctx, const struct ArrowArray* const* partials,
+                                            uint32_t n_partials, const struct ArrowSchema* schema,
+                                            uint32_t group_by_count, const prism_agg_expr* original,
+                                            uint32_t n_original, struct ArrowArray* out,
+                                            struct ArrowSchema* out_schema);
+
+/* ------------------------------------------------------------------------- */
 /* Join -> aggregate by a build-side attribute without materialising pairs:  */
 /* the fused join-aggregate of the prism-substrait executor                  */
 /* (executor.rs:333-458) for                                                 */

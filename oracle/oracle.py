@@ -218,3 +218,42 @@ def join_agg_eager(build_key, build_attr, probe_key, probe_payload, n_groups):
                            cnt.ctypes.data, lo.ctypes.data, hi.ctypes.data)
     sums = [int(h) * (1 << 64) + int(l) for l, h in zip(lo.tolist(), hi.tolist())]
     return cnt, sums
+
+
+def expand_worker_aggregates(original):
+    """expand_worker_aggregates restated (prism-bench handler.rs:397-428).
+    original: [(column, func_name)].  Returns worker [(column, func_name)] and
+    per original aggregate its worker indices (AVG -> SUM, COUNT)."""
+    worker, mapping = [], []
+    for col, f in original:
+        if f == "AVG":
+            mapping.append([len(worker), len(worker) + 1])
+            worker += [(col, "SUM"), (col, "COUNT")]
+        else:
+            mapping.append([len(worker)])
+            worker.append((col, f))
+    return worker, mapping
+
+
+def reducer_merge(partials, group_by_count, original):
+    """The reducer of execute_distributed_aggregate restated (handler.rs:311-343):
+    build_final_merge_config (430-457) over the partial batches, the merge
+    through hash_aggregate_batches, then reconstruct_original_aggregate_batch
+    (459-526).  Returns (first (batch, row) of each group, per original
+    aggregate its values: int for COUNT / COUNT_DISTINCT, float otherwise)."""
+    worker, mapping = expand_worker_aggregates(original)
+    merge = [(group_by_count + i, f if f in ("MIN", "MAX") else "SUM") for i, (_, f) in enumerate(worker)]
+    r = hash_aggregate(partials, list(range(group_by_count)), merge)
+    g = r["n_groups"]
+    out = []
+    for (col, f), m in zip(original, mapping):
+        src = r["f64"][m[0]]
+        if f in ("COUNT", "COUNT_DISTINCT"):
+            # numeric_value_as_i64 of the merged Float64 sum: `as i64` (truncation)
+            out.append([int(v) for v in src.tolist()])
+        elif f == "AVG":
+            cnt = r["f64"][m[1]]
+            out.append([float(s) / float(c) if c > 0 else 0.0 for s, c in zip(src.tolist(), cnt.tolist())])
+        else:
+            out.append([float(v) for v in src.tolist()])
+    return list(zip(r["first_batch"][:g].tolist(), r["first_row"][:g].tolist())), out

@@ -5,6 +5,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <unordered_map>
 #include <vector>
@@ -393,6 +394,229 @@ prism_status prism_hash_aggregate(prism_ctx* ctx, const struct ArrowArray* batch
                                   struct ArrowSchema* out_schema) {
   const struct ArrowArray* one[1] = {batch};
   return prism_hash_aggregate_batches(ctx, one, 1, schema, cfg, out, out_schema);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Distributed aggregate, reducer side (prism-bench handler.rs:270-344,
+// 397-526): producers run the plan with AVG split into SUM + COUNT; the
+// reducer merges every producer's partial batch with one device aggregation
+// and rebuilds the original aggregates.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Expanded {
+  std::vector<prism_agg_expr> worker;
+  std::vector<std::string> names;  // storage of worker[i].output_name
+  std::vector<uint32_t> map;       // 2 per original aggregate; second UINT32_MAX unless AVG
+};
+
+// expand_worker_aggregates (handler.rs:397-428).
+void expand(const prism_agg_expr* original, uint32_t n, Expanded& e) {
+  e.worker.clear();
+  e.names.clear();
+  e.map.assign(2 * static_cast<size_t>(n), UINT32_MAX);
+  auto push = [&](uint32_t column, int32_t func) {
+    const uint32_t idx = static_cast<uint32_t>(e.worker.size());
+    prism_agg_expr w;
+    w.column = column;
+    w.func = func;
+    w.output_name = nullptr;
+    e.worker.push_back(w);
+    e.names.push_back("agg_" + std::to_string(idx));
+    return idx;
+  };
+  for (uint32_t i = 0; i < n; ++i) {
+    const prism_agg_expr& a = original[i];
+    if (a.func < PRISM_AGG_SUM || a.func > PRISM_AGG_COUNT_DISTINCT)
+      fail(PRISM_ERR_UNSUPPORTED_AGGREGATION, "Aggregation error: unsupported function " + std::to_string(a.func));
+    if (a.func == PRISM_AGG_AVG) {
+      e.map[2 * i] = push(a.column, PRISM_AGG_SUM);
+      e.map[2 * i + 1] = push(a.column, PRISM_AGG_COUNT);
+    } else {
+      e.map[2 * i] = push(a.column, a.func);
+    }
+  }
+  for (size_t i = 0; i < e.worker.size(); ++i) e.worker[i].output_name = e.names[i].c_str();
+}
+
+// numeric_value_as_f64 / numeric_value_as_i64 (handler.rs:528-566): nulls read as 0.
+double value_f64(const HostColumn& c, int64_t row) {
+  if (!host_valid(c, row)) return 0.0;
+  const int64_t i = c.offset + row;
+  switch (c.type) {
+    case DType::Float64: return static_cast<const double*>(c.data)[i];
+    case DType::Int64: return static_cast<double>(static_cast<const int64_t*>(c.data)[i]);
+    case DType::Int32: return static_cast<double>(static_cast<const int32_t*>(c.data)[i]);
+    default:
+      fail(PRISM_ERR_INVALID_ARGUMENT,
+           "Invalid argument: unsupported numeric type for distributed aggregate merge: " + dtype_name(c.type));
+  }
+}
+
+// Rust's `f64 as i64`: truncation toward zero, saturating, NaN -> 0.
+int64_t f64_as_i64(double v) {
+  if (std::isnan(v)) return 0;
+  if (v >= 9223372036854775807.0) return INT64_MAX;
+  if (v <= -9223372036854775808.0) return INT64_MIN;
+  return static_cast<int64_t>(v);
+}
+
+int64_t value_i64(const HostColumn& c, int64_t row) {
+  if (!host_valid(c, row)) return 0;
+  const int64_t i = c.offset + row;
+  switch (c.type) {
+    case DType::Int64: return static_cast<const int64_t*>(c.data)[i];
+    case DType::Int32: return static_cast<const int32_t*>(c.data)[i];
+    case DType::Float64: return f64_as_i64(static_cast<const double*>(c.data)[i]);
+    default:
+      fail(PRISM_ERR_INVALID_ARGUMENT,
+           "Invalid argument: unsupported integer type for distributed aggregate merge: " + dtype_name(c.type));
+  }
+}
+
+// A host column copied into an output column (group keys pass through).
+OutColumn copy_out(const HostColumn& c, int64_t n) {
+  OutColumn o;
+  o.type = c.type;
+  o.name = c.name;
+  o.nullable = c.nullable;
+  o.length = n;
+  if (c.validity && c.null_count != 0) {
+    o.validity.assign(static_cast<size_t>((n + 7) / 8), 0);
+    for (int64_t r = 0; r < n; ++r)
+      if (host_valid(c, r)) o.validity[r >> 3] |= static_cast<uint8_t>(1u << (r & 7));
+      else ++o.null_count;
+  }
+  if (c.type == DType::Utf8) {
+    o.offsets.resize(static_cast<size_t>(n) + 1);
+    const int32_t base = n ? c.offsets[c.offset] : 0;
+    for (int64_t r = 0; r <= n; ++r) o.offsets[r] = c.offsets[c.offset + r] - base;
+    const uint8_t* b = static_cast<const uint8_t*>(c.data);
+    o.data.assign(b + base, b + base + o.offsets[n]);
+  } else if (c.type == DType::Bool) {
+    o.data.assign(static_cast<size_t>((n + 7) / 8), 0);
+    const uint8_t* b = static_cast<const uint8_t*>(c.data);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t i = c.offset + r;
+      if ((b[i >> 3] >> (i & 7)) & 1) o.data[r >> 3] |= static_cast<uint8_t>(1u << (r & 7));
+    }
+  } else {
+    const int w = dtype_width(c.type);
+    const uint8_t* b = static_cast<const uint8_t*>(c.data) + c.offset * w;
+    o.data.assign(b, b + n * w);
+  }
+  return o;
+}
+
+OutColumn f64_out(const char* name, std::vector<double>&& v) {
+  OutColumn o;
+  o.type = DType::Float64;
+  o.name = name ? name : "";
+  o.length = static_cast<int64_t>(v.size());
+  o.data.resize(v.size() * 8);
+  memcpy(o.data.data(), v.data(), v.size() * 8);
+  return o;
+}
+
+OutColumn i64_out(const char* name, std::vector<int64_t>&& v) {
+  OutColumn o;
+  o.type = DType::Int64;
+  o.name = name ? name : "";
+  o.length = static_cast<int64_t>(v.size());
+  o.data.resize(v.size() * 8);
+  memcpy(o.data.data(), v.data(), v.size() * 8);
+  return o;
+}
+
+// Releases an exported Arrow array / schema pair when it goes out of scope.
+struct ArrowHold {
+  ArrowArray a{};
+  ArrowSchema s{};
+  ~ArrowHold() {
+    if (a.release) a.release(&a);
+    if (s.release) s.release(&s);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+prism_status prism_expand_worker_aggregates(const prism_agg_expr* original, uint32_t n, prism_agg_expr* out_worker,
+                                            uint32_t* n_worker, uint32_t* out_map) {
+  return guarded([&] {
+    if ((n && !original) || !out_worker || !n_worker || !out_map)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL aggregate list or output");
+    static thread_local Expanded e;  // names stay valid until the next call on this thread
+    expand(original, n, e);
+    for (size_t i = 0; i < e.worker.size(); ++i) out_worker[i] = e.worker[i];
+    *n_worker = static_cast<uint32_t>(e.worker.size());
+    for (size_t i = 0; i < e.map.size(); ++i) out_map[i] = e.map[i];
+  });
+}
+
+prism_status prism_merge_partial_aggregates(prism_ctx* ctx, const struct ArrowArray* const* partials,
+                                            uint32_t n_partials, const struct ArrowSchema* schema,
+                                            uint32_t group_by_count, const prism_agg_expr* original,
+                                            uint32_t n_original, struct ArrowArray* out,
+                                            struct ArrowSchema* out_schema) {
+  return guarded([&] {
+    if (!ctx || !schema || !out || !out_schema) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL argument");
+    if ((n_partials && !partials) || (n_original && !original))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL partial batches or aggregate list");
+    use_device(ctx);
+    Expanded e;
+    expand(original, n_original, e);
+    std::vector<HostColumn> sch = import_schema(schema);
+    if (sch.size() < group_by_count + e.worker.size())
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: partial batches hold " + std::to_string(sch.size()) +
+                                           " columns, expected the group columns and " +
+                                           std::to_string(e.worker.size()) + " worker aggregates");
+    // build_final_merge_config (handler.rs:430-457)
+    std::vector<uint32_t> gb(group_by_count);
+    for (uint32_t i = 0; i < group_by_count; ++i) gb[i] = i;
+    std::vector<prism_agg_expr> merge(e.worker.size());
+    for (size_t i = 0; i < e.worker.size(); ++i) {
+      const int32_t f = e.worker[i].func;
+      merge[i].column = group_by_count + static_cast<uint32_t>(i);
+      merge[i].func = (f == PRISM_AGG_MIN || f == PRISM_AGG_MAX) ? f : PRISM_AGG_SUM;
+      merge[i].output_name = e.worker[i].output_name;
+    }
+    prism_hash_agg_config cfg{gb.data(), group_by_count, merge.data(), static_cast<uint32_t>(merge.size()), 0};
+    std::vector<HostBatch> bs;
+    for (uint32_t i = 0; i < n_partials; ++i) bs.push_back(import_batch(partials[i], schema));
+    ArrowHold merged;
+    aggregate(ctx, bs, sch, &cfg, &merged.a, &merged.s);
+    // reconstruct_original_aggregate_batch (handler.rs:459-526)
+    const HostBatch m = import_batch(&merged.a, &merged.s);
+    const int64_t n = m.n_rows;
+    std::vector<OutColumn> cols;
+    for (uint32_t k = 0; k < group_by_count; ++k) cols.push_back(copy_out(m.cols[k], n));
+    for (uint32_t i = 0; i < n_original; ++i) {
+      const prism_agg_expr& a = original[i];
+      const HostColumn& src = m.cols[group_by_count + e.map[2 * i]];
+      if (a.func == PRISM_AGG_COUNT || a.func == PRISM_AGG_COUNT_DISTINCT) {
+        std::vector<int64_t> v(static_cast<size_t>(n));
+        for (int64_t r = 0; r < n; ++r) v[r] = value_i64(src, r);
+        cols.push_back(i64_out(a.output_name, std::move(v)));
+      } else if (a.func == PRISM_AGG_AVG) {
+        const HostColumn& cnt = m.cols[group_by_count + e.map[2 * i + 1]];
+        std::vector<double> v(static_cast<size_t>(n));
+        for (int64_t r = 0; r < n; ++r) {
+          const double s = value_f64(src, r), c = value_f64(cnt, r);
+          v[r] = c > 0.0 ? s / c : 0.0;
+        }
+        cols.push_back(f64_out(a.output_name, std::move(v)));
+      } else {
+        std::vector<double> v(static_cast<size_t>(n));
+        for (int64_t r = 0; r < n; ++r) v[r] = value_f64(src, r);
+        cols.push_back(f64_out(a.output_name, std::move(v)));
+      }
+    }
+    export_batch(std::move(cols), n, out, out_schema);
+  });
 }
 
 }  // extern "C"

@@ -247,6 +247,10 @@ _SIGNATURES = [
     ("prism_hash_aggregate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HashAggConfigC), c_void_p, c_void_p]),
     ("prism_hash_aggregate_batches", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p,
                                                POINTER(HashAggConfigC), c_void_p, c_void_p]),
+    ("prism_expand_worker_aggregates", c_int32, [POINTER(AggExprC), c_uint32, POINTER(AggExprC), POINTER(c_uint32),
+                                                 POINTER(c_uint32)]),
+    ("prism_merge_partial_aggregates", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p, c_uint32,
+                                                 POINTER(AggExprC), c_uint32, c_void_p, c_void_p]),
     ("prism_join_agg_create", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(c_void_p)]),
     ("prism_join_agg_probe", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64]),
     ("prism_join_agg_finish", c_int32, [c_void_p, POINTER(c_uint64)]),
