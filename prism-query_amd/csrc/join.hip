@@ -1080,10 +1080,11 @@ __global__ void k_presence_bits(TableView T, uint32_t* bits, uint64_t n_words) {
 
 // LDS cache of hot key slots: per entry the slot, row count, split int128
 // payload sum (low 32-bit halves unsigned, high halves signed) and first row.
-// The cached slots are fixed at create time (the slots with the most build
-// rows, the probe's likely Zipf head), laid out in buckets of four read with
-// one 16-byte LDS load; a slot may sit in either of two buckets, so a lookup
-// costs at most two loads and no entry is ever claimed at run time.
+// The cache starts with the slots placed at create time (the slots with the
+// most build rows, the probe's likely Zipf head) and fills its free ways
+// with the first other slots the workgroup meets.  Entries sit in buckets of
+// four read with one 16-byte LDS load; a slot may sit in either of two
+// buckets, so a lookup costs at most two loads.
 constexpr int kJaWays = 4;
 constexpr int kJaBuckets = kJaCache / kJaWays;
 static_assert(kJaBuckets == 256, "two 8-bit bucket indices");
@@ -1112,22 +1113,32 @@ __device__ __forceinline__ void ja_cache_init(JaCache& C, const uint32_t* __rest
   }
 }
 
-__device__ __forceinline__ int ja_cache_find(const JaCache& C, uint32_t s) {
+// Entry of slot s: a slot placed at create time, or a free way of either
+// bucket claimed now (the first slots a workgroup meets, which covers a probe
+// side skewed on keys the build side does not rank); -1 when both buckets
+// are full.  A slot held by two entries would only split its totals.
+__device__ __forceinline__ int ja_cache_entry(JaCache& C, uint32_t s) {
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const uint32_t b = ja_bucket(s, t);
-    const v4u32 e = *reinterpret_cast<const v4u32*>(&C.key[kJaWays * b]);
-    if (e.x == s) return kJaWays * b;
-    if (e.y == s) return kJaWays * b + 1;
-    if (e.z == s) return kJaWays * b + 2;
-    if (e.w == s) return kJaWays * b + 3;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      const v4u32 e = *reinterpret_cast<const volatile v4u32*>(&C.key[kJaWays * b]);
+      if (e.x == s) return kJaWays * b;
+      if (e.y == s) return kJaWays * b + 1;
+      if (e.z == s) return kJaWays * b + 2;
+      if (e.w == s) return kJaWays * b + 3;
+      const int f = e.x == kJaEmpty ? 0 : e.y == kJaEmpty ? 1 : e.z == kJaEmpty ? 2 : e.w == kJaEmpty ? 3 : -1;
+      if (f < 0) break;
+      const uint32_t old = atomicCAS(&C.key[kJaWays * b + f], kJaEmpty, s);
+      if (old == kJaEmpty || old == s) return kJaWays * b + f;
+    }
   }
   return -1;
 }
 
 // Adds one row to slot s's cache entry; false when the slot is not cached.
 __device__ __forceinline__ bool ja_cache_add(JaCache& C, uint32_t s, int64_t x, unsigned long long row) {
-  const int c = ja_cache_find(C, s);
+  const int c = ja_cache_entry(C, s);
   if (c < 0) return false;
   atomicAdd(&C.cnt[c], 1ULL);
   const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(x));
@@ -1438,9 +1449,11 @@ __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4)
         P.l1_sr[static_cast<uint64_t>(p) * P.cap1 + at] = kJaHole;
     }
   }
-  // this workgroup's cache and register totals are added to its own row of
-  // per-workgroup partials (the same row in every pass of the probe call);
-  // one reduction after the last pass adds them to the slot records
+  // this workgroup's totals of the slots placed at create time and of the
+  // register slots are added to its own row of per-workgroup partials (the
+  // same row in every pass of the probe call; one reduction after the last
+  // pass adds them to the slot records); entries claimed at run time hold
+  // slots of this workgroup only and go to the slot records directly
   __syncthreads();
   r0.to_lds(C, 0);
   r1.to_lds(C, 1);
@@ -1451,6 +1464,19 @@ __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4)
   for (int c = threadIdx.x; c < kJaCacheAll; c += blockDim.x) {
     const unsigned long long cnt = C.cnt[c];
     if (!cnt) continue;
+    if (c < kJaCache && C.key[c] != cache_keys[c]) {  // claimed by this workgroup: its own slot
+      const uint64_t s = C.key[c];
+      atomicAdd(&A.w[4 * s], cnt);
+      Split128 sp;
+      sp.lo32 = C.lo[c];
+      sp.hi32 = C.hi[c];
+      uint64_t lo;
+      int64_t hi;
+      sp.to_i128(lo, hi);
+      if (lo | static_cast<uint64_t>(hi)) atomic_add_i128(&A.w[4 * s + 1], &A.w[4 * s + 2], lo, hi);
+      atomicMin(&A.w[4 * s + 3], C.first[c]);
+      continue;
+    }
     unsigned long long* e = row + 4 * c;
     e[0] += cnt;
     e[1] += C.lo[c];

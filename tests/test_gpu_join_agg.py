@@ -176,3 +176,21 @@ def test_partitioned_tail_one_hot_partition(ctx, monkeypatch, chunk):
     pay = rng.integers(-10**15, 10**15, 150_000)
     out = run(ctx, bk, attr, [(pk, pay, 0)])
     check(out, *materialized(bk, attr, pk, pay))
+
+
+@pytest.mark.parametrize("chunk", ["0", "300000"])
+def test_unique_build_keys_zipf_probe(ctx, monkeypatch, chunk):
+    """Unique build keys give the create-time ranking nothing to go on; the
+    probe's Zipf head must then be caught by the cache ways claimed at run time
+    (and stay exact either way)."""
+    say(f"join-agg unique build, zipf probe chunk={chunk}")
+    if chunk != "0":
+        monkeypatch.setenv("PRISM_JA_CHUNK", chunk)
+    n_vals, n_attr = 50_000, 700
+    bk = dg.permutation_i64(n_vals, 31, 5)
+    attr = np.random.default_rng(31).integers(0, n_attr, n_vals).astype(np.int32)
+    cdf = dg.zipf_cdf(n_vals, 1.1)
+    pk = dg.zipf_i64(2_000_000, 32, 6, cdf, 7)
+    pay = np.random.default_rng(33).integers(-10**9, 10**9, 2_000_000)
+    out = run(ctx, bk, attr, [(pk, pay, 0)])
+    eager_check(out, bk, attr, pk, pay, n_attr)
