@@ -503,8 +503,11 @@ def main() -> int:
     kern_ms = kernels["join_probe"]["avg_us"] * 1e-3
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     kernels["join_probe"]["alg_bytes"] = alg_bytes
-    kernels["join_probe_move"]["alg_bytes"] = r.matches * 16
-    kernels["join_probe_move"]["GBps"] = r.matches * 16 / (kernels["join_probe_move"]["avg_us"] * 1e-6) / 1e9
+    if kernels["join_probe_move"]["launches"]:  # the staged probe form moves pairs in a second kernel
+        kernels["join_probe_move"]["alg_bytes"] = r.matches * 16
+        kernels["join_probe_move"]["GBps"] = r.matches * 16 / (kernels["join_probe_move"]["avg_us"] * 1e-6) / 1e9
+    else:
+        del kernels["join_probe_move"]
     line = {
         "metric": METRIC,
         "value": npr / (ms_per_step * 1e-3),
