@@ -357,6 +357,7 @@ struct Slot {
   bool busy = false;
   uint64_t batch_id = 0;
   std::map<std::string, hipGraphExec_t> graphs;
+  std::map<std::string, int> shapes_seen;  // launch shapes issued without a graph so far
   // tuning: (variant, events, rows, candidate)
   struct Timed {
     int variant;
@@ -508,6 +509,10 @@ static void footprint(const Req* R, size_t& res_words, size_t& table_words) {
       res_words = kAccCopies + 1;
   }
 }
+
+// Grid sizes rounded up to a power of two: batches of similar size share a
+// launch shape (and its graph); the extra workgroups find the queue empty.
+static uint64_t pow2_grid(uint64_t n) { return n <= 1 ? 1 : next_pow2(n); }
 
 static std::string shape_key(const int (&grid)[kVariants], const size_t (&lds)[kVariants], int n_init_grid,
                              int n_fin_grid, const std::vector<hipFunction_t>& jf, const std::vector<int>& jgrid,
@@ -696,7 +701,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
       const Cand cd = cand_of(S, kind_of(v));
       uint32_t bpc = cd.bpc;
       if (lds[v]) bpc = std::min<uint32_t>(bpc, std::max<uint32_t>(1, static_cast<uint32_t>((160u << 10) / lds[v])));
-      grid[v] = static_cast<int>(std::min<uint64_t>(vt[v].size(), static_cast<uint64_t>(n_cu) * bpc));
+      grid[v] = static_cast<int>(std::min<uint64_t>(pow2_grid(vt[v].size()), static_cast<uint64_t>(n_cu) * bpc));
     }
   }
   std::vector<int> jgrid(jf.size(), 0);
@@ -712,7 +717,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
     uint32_t bpc = cd.bpc;
     if (jcap[i] <= kSmallGroupCap) bpc = std::max<uint32_t>(bpc, 4);  // cheap flushes: fill the CUs
     if (jlds[i]) bpc = std::min<uint32_t>(bpc, std::max<uint32_t>(1, static_cast<uint32_t>((160u << 10) / jlds[i])));
-    jgrid[i] = static_cast<int>(std::min<uint64_t>(jt[i].size(), static_cast<uint64_t>(n_cu) * bpc));
+    jgrid[i] = static_cast<int>(std::min<uint64_t>(pow2_grid(jt[i].size()), static_cast<uint64_t>(n_cu) * bpc));
   }
   const size_t ctl_bytes = offsetof(Control, tiles) + off * sizeof(Tile);
   Control* dC = sl.control.as<Control>();
@@ -723,8 +728,8 @@ static void launch_batch(prism_service* S, Slot& sl) {
   sl.batch_rows = brows;
   PRISM_HIP(hipEventRecord(sl.ev_start, st));
   PRISM_HIP(hipMemcpyAsync(dC, H, ctl_bytes, hipMemcpyHostToDevice, st));
-  const int init_grid = static_cast<int>(std::max<uint32_t>(1, std::min<uint32_t>(H->n_init, n_cu * 4)));
-  const int fin_grid = static_cast<int>(std::max<uint32_t>(1, std::min<uint32_t>(H->n_gb, n_cu * 2)));
+  const int init_grid = static_cast<int>(std::min<uint64_t>(pow2_grid(H->n_init), static_cast<uint64_t>(n_cu) * 4));
+  const int fin_grid = static_cast<int>(std::min<uint64_t>(pow2_grid(H->n_gb), static_cast<uint64_t>(n_cu) * 2));
   const bool timed = tuning(S);
   auto issue = [&](bool with_events) {
     hipLaunchKernelGGL(k_svc_init, dim3(init_grid), dim3(kSB), 0, st, dC);
@@ -762,8 +767,12 @@ static void launch_batch(prism_service* S, Slot& sl) {
     }
     if (H->n_gb) hipLaunchKernelGGL(k_svc_gb_finish, dim3(fin_grid), dim3(gb::kFinishThreads), 0, st, dC);
   };
-  if (S->cfg.use_graphs && !timed) {
-    const std::string key = shape_key(grid, lds, init_grid, fin_grid, jf, jgrid, jlds);
+  // A launch sequence is captured into a graph the second time its shape
+  // comes (a shape met once would pay capture + instantiation for nothing).
+  const std::string key = S->cfg.use_graphs && !timed ? shape_key(grid, lds, init_grid, fin_grid, jf, jgrid, jlds)
+                                                     : std::string();
+  const bool recurring = !key.empty() && (sl.graphs.count(key) || ++sl.shapes_seen[key] >= 2);
+  if (recurring) {
     auto it = sl.graphs.find(key);
     if (it == sl.graphs.end()) {
       hipGraph_t g;
