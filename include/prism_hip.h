@@ -456,6 +456,33 @@ prism_status prism_merge_partial_aggregates(prism_ctx* ctx, const struct ArrowAr
                                             struct ArrowSchema* out_schema);
 
 /* ------------------------------------------------------------------------- */
+/* Arrow IPC stream ingest: prism-jni's executePlan receives its input as an  */
+/* Arrow IPC stream in a JNI byte array and decodes it with arrow's           */
+/* StreamReader (prism-jni/src/lib.rs:96-121).  Streams of Int32 / Int64 /    */
+/* Float64 / Utf8 / Bool / Date32 fields; dictionaries, body compression and  */
+/* nested types are rejected with PRISM_ERR_ARROW.                            */
+/* ------------------------------------------------------------------------- */
+typedef struct prism_ipc_stream prism_ipc_stream; /* a decoded stream resident in HBM */
+
+/* The stream's record batches as Arrow C Data struct arrays whose buffers
+ * point into `bytes` (no copy: `bytes` must outlive the arrays), with one
+ * schema; out_batches has room for `capacity` arrays.  No device needed. */
+prism_status prism_ipc_read_host(const uint8_t* bytes, uint64_t n_bytes, struct ArrowArray* out_batches,
+                                 uint32_t capacity, uint32_t* n_batches, struct ArrowSchema* out_schema);
+/* The stream decoded straight into HBM: every column's buffers copied once
+ * from `bytes` through pinned staging; the columns feed the device-level
+ * entries (group-by update, join build/probe, service requests). */
+prism_status prism_ipc_read_device(prism_ctx* ctx, const uint8_t* bytes, uint64_t n_bytes, prism_ipc_stream** out);
+prism_status prism_ipc_stream_info(const prism_ipc_stream* s, uint32_t* n_batches, uint32_t* n_columns);
+/* Field `column`: name (valid while the stream lives), PRISM_TYPE_*, nullability. */
+prism_status prism_ipc_field(const prism_ipc_stream* s, uint32_t column, const char** name, int32_t* type,
+                             int32_t* nullable);
+/* Rows of record batch `batch` and the device descriptor of its column `column`. */
+prism_status prism_ipc_column(const prism_ipc_stream* s, uint32_t batch, uint32_t column, uint64_t* n_rows,
+                              prism_column* out);
+void prism_ipc_stream_free(prism_ipc_stream* s);
+
+/* ------------------------------------------------------------------------- */
 /* Join -> aggregate by a build-side attribute without materialising pairs:  */
 /* the fused join-aggregate of the prism-substrait executor                  */
 /* (executor.rs:333-458) for                                                 */

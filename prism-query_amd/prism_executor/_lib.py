@@ -247,6 +247,12 @@ _SIGNATURES = [
     ("prism_hash_aggregate", c_int32, [c_void_p, c_void_p, c_void_p, POINTER(HashAggConfigC), c_void_p, c_void_p]),
     ("prism_hash_aggregate_batches", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p,
                                                POINTER(HashAggConfigC), c_void_p, c_void_p]),
+    ("prism_ipc_read_host", c_int32, [c_void_p, c_uint64, c_void_p, c_uint32, POINTER(c_uint32), c_void_p]),
+    ("prism_ipc_read_device", c_int32, [c_void_p, c_void_p, c_uint64, POINTER(c_void_p)]),
+    ("prism_ipc_stream_info", c_int32, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
+    ("prism_ipc_field", c_int32, [c_void_p, c_uint32, POINTER(c_char_p), POINTER(c_int32), POINTER(c_int32)]),
+    ("prism_ipc_column", c_int32, [c_void_p, c_uint32, c_uint32, POINTER(c_uint64), POINTER(Column)]),
+    ("prism_ipc_stream_free", None, [c_void_p]),
     ("prism_expand_worker_aggregates", c_int32, [POINTER(AggExprC), c_uint32, POINTER(AggExprC), POINTER(c_uint32),
                                                  POINTER(c_uint32)]),
     ("prism_merge_partial_aggregates", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p, c_uint32,

@@ -1,0 +1,86 @@
+"""Arrow IPC stream ingest (prism-jni lib.rs:96-121 decodes the JNI byte array
+with arrow's StreamReader): the library's decoder against pyarrow's writer, on
+the CPU (the host form needs no device)."""
+import ctypes
+import datetime
+
+import numpy as np
+import pyarrow as pa
+import pytest
+
+from prism_executor import _lib as L
+from prism_executor.ipc import read_ipc_host
+
+
+def stream(batches, schema=None, **opts):
+    sink = pa.BufferOutputStream()
+    with pa.ipc.new_stream(sink, schema or batches[0].schema, options=pa.ipc.IpcWriteOptions(**opts)) as w:
+        for b in batches:
+            w.write_batch(b)
+    return sink.getvalue().to_pybytes()
+
+
+def sample(n, seed):
+    rng = np.random.default_rng(seed)
+    nulls = rng.random(n) < 0.1
+    strs = ["".join(chr(97 + c) for c in rng.integers(0, 26, rng.integers(0, 12))) for _ in range(n)]
+    return pa.record_batch([
+        pa.array(rng.integers(-2**31, 2**31, n).astype(np.int32)),
+        pa.array(rng.integers(-2**62, 2**62, n), mask=nulls),
+        pa.array(rng.standard_normal(n)),
+        pa.array(strs, pa.utf8(), mask=rng.random(n) < 0.05),
+        pa.array(rng.random(n) < 0.5, pa.bool_()),
+        pa.array([datetime.date(1992, 1, 1) + datetime.timedelta(days=int(d)) for d in rng.integers(0, 2557, n)],
+                 pa.date32()),
+    ], names=["i32", "i64", "f64", "s", "b", "d"])
+
+
+def test_round_trip_several_batches():
+    bs = [sample(1000, 1), sample(0, 2), sample(4097, 3), sample(1000, 4).slice(13, 500)]
+    got, keep = read_ipc_host(stream(bs))
+    assert len(got) == len(bs)
+    for g, b in zip(got, bs):
+        assert g.schema.names == b.schema.names and g.schema.types == b.schema.types
+        assert g.equals(b)
+
+
+def test_nullability_and_schema_only():
+    sch = pa.schema([pa.field("k", pa.int64(), nullable=False), pa.field("v", pa.float64())])
+    got, keep = read_ipc_host(stream([], schema=sch))
+    assert got == []
+    b = pa.record_batch([pa.array([1, 2, 3]), pa.array([0.5, None, 2.0])], schema=sch)
+    got, keep = read_ipc_host(stream([b]))
+    assert got[0].schema.field("k").nullable is False and got[0].schema.field("v").nullable is True
+    assert got[0].equals(b)
+
+
+def status_of(data):
+    lib = L.load()
+    arr = (L.ArrowArray * 8)()
+    n = L.c_uint32()
+    sch = L.ArrowSchema()
+
+    st = lib.prism_ipc_read_host(ctypes.cast(ctypes.c_char_p(data), ctypes.c_void_p), len(data), arr, 8,
+                                 ctypes.byref(n), ctypes.addressof(sch))
+    return st, lib.prism_last_error().decode()
+
+
+@pytest.mark.parametrize("arr", [pa.array([1.5], pa.float32()), pa.array([1], pa.uint8()),
+                                 pa.array([[1, 2]], pa.list_(pa.int64())), pa.array(["x"], pa.large_utf8()),
+                                 pa.array(["a", "b", "a"]).dictionary_encode()])
+def test_unsupported_fields_are_arrow_errors(arr):
+    st, msg = status_of(stream([pa.record_batch([arr], names=["c"])]))
+    assert st == 1 and "IPC stream" in msg  # PRISM_ERR_ARROW
+
+
+def test_truncated_and_compressed_streams_fail():
+    data = stream([sample(100, 5)])
+    for cut in (3, 20, len(data) // 2, len(data) - 9):
+        st, msg = status_of(data[:cut])
+        assert st != 0, cut
+    try:
+        comp = stream([sample(100, 6)], compression="zstd")
+    except (pa.ArrowNotImplementedError, ValueError, TypeError):
+        pytest.skip("this pyarrow build writes no compressed IPC")
+    st, msg = status_of(comp)
+    assert st != 0 and "compression" in msg
