@@ -483,6 +483,57 @@ prism_status prism_ipc_column(const prism_ipc_stream* s, uint32_t batch, uint32_
 void prism_ipc_stream_free(prism_ipc_stream* s);
 
 /* ------------------------------------------------------------------------- */
+/* Parquet decode: prism-executor's parquet_scan reads each file's footer,    */
+/* skips row groups whose statistics rule the predicate out and decodes the  */
+/* projected column chunks with the `parquet` crate (parquet_scan.rs:55-258, */
+/* 167-258, 357-494).  Here the footer and statistics come from           */
+/* prism_parquet_open / prism_parquet_statistics (the caller keeps           */
+/* row_group_might_match), and the chosen column chunks are decoded into     */
+/* HBM.  Flat schemas of BOOLEAN, INT32 (int32, date), INT64, DOUBLE and      */
+/* BYTE_ARRAY (utf8) leaves; PLAIN, dictionary (PLAIN_DICTIONARY /           */
+/* RLE_DICTIONARY) and RLE boolean pages, v1 and v2; codecs UNCOMPRESSED,    */
+/* SNAPPY, GZIP.  Anything else is rejected with PRISM_ERR_ARROW.            */
+/* ------------------------------------------------------------------------- */
+typedef struct prism_parquet_file prism_parquet_file;   /* parsed footer over the caller's bytes */
+typedef struct prism_parquet_table prism_parquet_table; /* decoded column chunks resident in HBM */
+
+typedef struct {
+  int32_t present;       /* the column chunk carries Statistics */
+  int32_t physical_type; /* Parquet physical type: 0 BOOLEAN, 1 INT32, 2 INT64, 5 DOUBLE, 6 BYTE_ARRAY, ... */
+  int64_t null_count;    /* -1 when absent */
+  int32_t has_min, has_max;
+  const uint8_t* min;    /* PLAIN encoding of the value (little-endian numbers, raw bytes) */
+  uint32_t min_len;
+  const uint8_t* max;
+  uint32_t max_len;
+} prism_parquet_stats;
+
+/* Parse the footer of a whole Parquet file held in `bytes` (which must
+ * outlive the handle).  Host only. */
+prism_status prism_parquet_open(const uint8_t* bytes, uint64_t n_bytes, prism_parquet_file** out);
+prism_status prism_parquet_file_info(const prism_parquet_file* f, uint32_t* n_row_groups, uint32_t* n_columns,
+                                     int64_t* n_rows);
+/* Leaf `column`: name, PRISM_TYPE_* (0 when the leaf cannot be decoded), nullability, physical type. */
+prism_status prism_parquet_field(const prism_parquet_file* f, uint32_t column, const char** name, int32_t* type,
+                                 int32_t* nullable, int32_t* physical_type);
+prism_status prism_parquet_row_group(const prism_parquet_file* f, uint32_t row_group, int64_t* n_rows);
+/* Column chunk statistics (row-group skipping, parquet_scan.rs:357-494); pointers live with the handle. */
+prism_status prism_parquet_statistics(const prism_parquet_file* f, uint32_t row_group, uint32_t column,
+                                      prism_parquet_stats* out);
+void prism_parquet_close(prism_parquet_file* f);
+/* Decode row groups x columns into HBM (NULL lists mean all), one batch per row group. */
+prism_status prism_parquet_read_device(prism_ctx* ctx, const prism_parquet_file* f, const uint32_t* row_groups,
+                                       uint32_t n_row_groups, const uint32_t* columns, uint32_t n_columns,
+                                       prism_parquet_table** out);
+prism_status prism_parquet_table_info(const prism_parquet_table* t, uint32_t* n_batches, uint32_t* n_columns);
+prism_status prism_parquet_table_column(const prism_parquet_table* t, uint32_t batch, uint32_t column,
+                                        uint64_t* n_rows, prism_column* out);
+/* Batch `batch` copied to the host as an Arrow C Data struct array (the RecordBatch parquet_scan returns). */
+prism_status prism_parquet_table_export(const prism_parquet_table* t, uint32_t batch, struct ArrowArray* out,
+                                        struct ArrowSchema* out_schema);
+void prism_parquet_table_free(prism_parquet_table* t);
+
+/* ------------------------------------------------------------------------- */
 /* Join -> aggregate by a build-side attribute without materialising pairs:  */
 /* the fused join-aggregate of the prism-substrait executor                  */
 /* (executor.rs:333-458) for                                                 */

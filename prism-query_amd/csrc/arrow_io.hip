@@ -115,7 +115,7 @@ prism_column DevColumn::desc() const {
 
 // Host -> HBM through two pinned staging halves, so the memcpy into one half
 // overlaps the DMA out of the other.
-static void staged_h2d(prism_ctx* ctx, void* dst, const void* src, size_t bytes) {
+void staged_h2d(prism_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!bytes) return;
   constexpr size_t kHalf = 8u << 20;
   char* pin = static_cast<char*>(pinned_staging(ctx, 2 * kHalf));

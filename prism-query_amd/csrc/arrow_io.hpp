@@ -56,6 +56,10 @@ struct DevColumn {
   prism_column desc() const;
 };
 
+// Copy host bytes into HBM through the context's pinned staging (returns once
+// the copies are done, so `src` may be reused).
+void staged_h2d(prism_ctx* ctx, void* dst, const void* src, size_t bytes);
+
 // Copy a host column into HBM (pinned staging, asynchronous on ctx->stream).
 DevColumn upload(prism_ctx* ctx, const HostColumn& c);
 

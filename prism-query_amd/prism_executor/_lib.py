@@ -98,6 +98,13 @@ class GroupbyOutput(Structure):
                 ("agg_i64", POINTER(POINTER(c_int64))), ("agg_sum128", POINTER(POINTER(c_uint64)))]
 
 
+class ParquetStats(Structure):
+    """prism_parquet_stats."""
+    _fields_ = [("present", c_int32), ("physical_type", c_int32), ("null_count", c_int64), ("has_min", c_int32),
+                ("has_max", c_int32), ("min", c_void_p), ("min_len", c_uint32), ("max", c_void_p),
+                ("max_len", c_uint32)]
+
+
 class ArrowSchema(Structure):
     pass
 
@@ -253,6 +260,19 @@ _SIGNATURES = [
     ("prism_ipc_field", c_int32, [c_void_p, c_uint32, POINTER(c_char_p), POINTER(c_int32), POINTER(c_int32)]),
     ("prism_ipc_column", c_int32, [c_void_p, c_uint32, c_uint32, POINTER(c_uint64), POINTER(Column)]),
     ("prism_ipc_stream_free", None, [c_void_p]),
+    ("prism_parquet_open", c_int32, [c_void_p, c_uint64, POINTER(c_void_p)]),
+    ("prism_parquet_file_info", c_int32, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int64)]),
+    ("prism_parquet_field", c_int32, [c_void_p, c_uint32, POINTER(c_char_p), POINTER(c_int32), POINTER(c_int32),
+                                      POINTER(c_int32)]),
+    ("prism_parquet_row_group", c_int32, [c_void_p, c_uint32, POINTER(c_int64)]),
+    ("prism_parquet_statistics", c_int32, [c_void_p, c_uint32, c_uint32, POINTER(ParquetStats)]),
+    ("prism_parquet_close", None, [c_void_p]),
+    ("prism_parquet_read_device", c_int32, [c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_uint32,
+                                            POINTER(c_void_p)]),
+    ("prism_parquet_table_info", c_int32, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
+    ("prism_parquet_table_column", c_int32, [c_void_p, c_uint32, c_uint32, POINTER(c_uint64), POINTER(Column)]),
+    ("prism_parquet_table_export", c_int32, [c_void_p, c_uint32, c_void_p, c_void_p]),
+    ("prism_parquet_table_free", None, [c_void_p]),
     ("prism_expand_worker_aggregates", c_int32, [POINTER(AggExprC), c_uint32, POINTER(AggExprC), POINTER(c_uint32),
                                                  POINTER(c_uint32)]),
     ("prism_merge_partial_aggregates", c_int32, [c_void_p, POINTER(c_void_p), c_uint32, c_void_p, c_uint32,
