@@ -223,6 +223,72 @@ def bench_join_agg(ctx, n_build: int, n_probe: int, n_values: int, n_attr: int, 
     return res
 
 
+def bench_parquet(ctx, n: int, steps: int) -> dict:
+    """Parquet scan of a table shaped like the filter + group-by input (date i32,
+    price i64, discount i64, quantity i32, flag A / flag B utf8), written as the
+    reference's data generator writes its files (UNCOMPRESSED, chunk statistics,
+    dictionary encoding on, 1,048,576-row row groups).  Timed: footer parse and
+    decode of every column chunk from host bytes into HBM (host page walk, H2D,
+    device index resolution and gather).  pyarrow's multi-threaded reader of
+    the same bytes into host memory is reported beside it."""
+    import io
+
+    import numpy as np
+    import pyarrow as pa
+    import pyarrow.parquet as pq
+
+    from prism_executor.parquet_scan import ParquetFile, ParquetTable
+
+    rng = np.random.default_rng(11)
+    flag = lambda vals, k: pa.DictionaryArray.from_arrays(pa.array(rng.integers(0, k, n).astype(np.int32)),
+                                                          pa.array(vals)).cast(pa.utf8())
+    t = pa.table({"date": pa.array(rng.integers(0, 2557, n).astype(np.int32)).cast(pa.date32()),
+                  "price": pa.array(rng.integers(90_000, 10_500_000, n)),
+                  "discount": pa.array(rng.integers(0, 11, n)),
+                  "quantity": pa.array(rng.integers(1, 51, n).astype(np.int32)),
+                  "flag_a": flag(["A", "N", "R"], 3), "flag_b": flag(["F", "O"], 2)})
+    buf = io.BytesIO()
+    pq.write_table(t, buf, compression="none", row_group_size=1 << 20, write_statistics=True)
+    data = buf.getvalue()
+    out_bytes = n * (4 + 8 + 8 + 4 + 2 * (4 + 1))  # decoded columns: values + utf8 offsets and bytes
+
+    def step():
+        f = ParquetFile(data)
+        tab = ParquetTable(ctx, f, list(range(f.num_row_groups)), list(range(f.num_columns)))
+        return f, tab
+
+    f, tab = step()
+    exp = pq.ParquetFile(io.BytesIO(data)).read_row_group(3)
+    ok = tab.to_arrow(3).equals(exp.to_batches()[0]) if exp.num_rows else True
+    tab.close()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f, tab = step()
+        tab.close()
+    wall = (time.perf_counter() - t0) / steps
+    ctx.kernel_timing(True)
+    f, tab = step()
+    tab.close()
+    kern = {}
+    for name in ("parquet_index", "parquet_gather"):
+        tot, cnt = ctx.kernel_time(name)
+        kern[name] = {"total_ms": tot, "launches": cnt}
+    ctx.kernel_timing(False)
+    dev_ms = sum(k["total_ms"] for k in kern.values())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pq.read_table(io.BytesIO(data))
+    pa_wall = (time.perf_counter() - t0) / steps
+    return {"workload": f"Parquet scan into HBM: {n} rows x 6 columns (date, price, discount, quantity, flag A, "
+                        "flag B), UNCOMPRESSED, dictionary encoding, 1,048,576-row row groups, file bytes in host "
+                        "memory", "file_bytes": len(data), "row_groups": f.num_row_groups, "equal_to_pyarrow": bool(ok),
+            "ms_per_scan": wall * 1e3, "rows_per_s": n / wall, "file_GBps": len(data) / wall / 1e9,
+            "device_kernels": kern, "device_ms": dev_ms,
+            "device_output_GBps": out_bytes / (dev_ms * 1e-3) / 1e9 if dev_ms else None,
+            "pyarrow_read_table_ms": pa_wall * 1e3}
+
+
 def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
     """Served request stream: Poisson arrivals, rows log-uniform in [4096, 16M];
     50% group-by (config 0 and config 2 shapes), 40% probes against 8 resident
@@ -414,6 +480,8 @@ def main() -> int:
     ap.add_argument("--no-join-agg", action="store_true")
     ap.add_argument("--join-agg-probe-rows", type=int, default=256_000_000)
     ap.add_argument("--no-service", action="store_true")
+    ap.add_argument("--no-parquet", action="store_true")
+    ap.add_argument("--parquet-rows", type=int, default=20_000_000)
     ap.add_argument("--svc-requests", type=int, default=1000)
     ap.add_argument("--svc-rates", type=str, default="200,500,1000")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="also run D2D copies of this many bytes")
@@ -545,6 +613,8 @@ def main() -> int:
         line["join_aggregate"] = bench_join_agg(ctx, 16_000_000, args.join_agg_probe_rows, 8_000_000, 1_000_000,
                                                 max(2, min(args.steps, 5)),
                                                 0 if args.no_cpu_baseline else 8_000_000)
+    if not args.no_parquet:
+        line["parquet_scan"] = bench_parquet(ctx, args.parquet_rows, max(2, min(args.steps, 5)))
     if not args.no_service:
         line["served_stream"] = bench_service(ctx, args.svc_requests, [int(x) for x in args.svc_rates.split(",")])
     if args.calibrate_copy:

@@ -921,7 +921,9 @@ static dim3 pgrid(prism_ctx* ctx, uint64_t n) {
 }
 
 // Host part done: move the chunk to HBM and resolve it there.
-static DevColumn decode_device(prism_ctx* ctx, HostChunk& H) {
+// `err` (device) is set when a dictionary index exceeds the source; the caller
+// reads the flags of all chunks back once.
+static DevColumn decode_device(prism_ctx* ctx, HostChunk& H, uint32_t* err) {
   Ctx* c = as_ctx(ctx);
   hipStream_t st = ctx->stream;
   HostColumn src;
@@ -951,11 +953,8 @@ static DevColumn decode_device(prism_ctx* ctx, HostChunk& H) {
   DeviceBuffer words(c, std::max<size_t>(H.words.size(), 2) * 4);
   DeviceBuffer valid, counts, prefix, tmp;
   DeviceBuffer idx(c, std::max<uint64_t>(rows, 1) * 4);
-  DeviceBuffer errb(c, 4);
-  uint32_t* err = errb.as<uint32_t>();
   staged_h2d(ctx, runs.ptr, H.runs.data(), H.runs.size() * sizeof(Run));
   staged_h2d(ctx, words.ptr, H.words.data(), H.words.size() * 4);
-  PRISM_HIP(hipMemsetAsync(err, 0, 4, st));
   if (H.null_count) {
     valid.alloc(c, n_words * 8);
     counts.alloc(c, n_words * 4);
@@ -985,10 +984,6 @@ static DevColumn decode_device(prism_ctx* ctx, HostChunk& H) {
     KernelTimer kt(ctx, "parquet_gather");
     out = take(ctx, S, idx.as<uint32_t>(), H.null_count ? valid.as<uint8_t>() : nullptr, rows);
   }
-  uint32_t e = 0;
-  PRISM_HIP(hipMemcpyAsync(&e, err, 4, hipMemcpyDeviceToHost, st));
-  PRISM_HIP(hipStreamSynchronize(st));
-  if (e) bad("column '" + H.leaf.name + "': a dictionary index exceeds the dictionary");
   out.name = H.leaf.name;
   out.nullable = H.leaf.optional;
   out.null_count = H.null_count;
@@ -1099,44 +1094,72 @@ prism_status prism_parquet_read_device(prism_ctx* ctx, const prism_parquet_file*
     auto T = std::make_unique<prism_parquet_table>();
     T->ctx = as_ctx(ctx);
     T->cols = cols;
-    // host part of the chunks on a pool of threads, a wave at a time; device part in order
+    // host part of the chunks on a pool of threads, a wave at a time, the next
+    // wave's host part overlapping this wave's device part (chunk order kept)
     const size_t n_chunks = rgs.size() * cols.size();
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<pq::HostChunk> wave;
-    std::vector<std::string> errs;
     for (uint32_t g : rgs) {
       T->rows.push_back(F.rgs[g].num_rows);
       T->batches.emplace_back();
     }
-    for (size_t base = 0; base < n_chunks; base += hw) {
-      const size_t m = std::min<size_t>(hw, n_chunks - base);
-      wave.assign(m, pq::HostChunk());
-      errs.assign(m, std::string());
-      std::vector<prism_status> codes(m, PRISM_OK);
+    struct Wave {
+      size_t base = 0, m = 0;
+      std::vector<pq::HostChunk> chunks;
+      std::vector<prism_status> codes;
+      std::vector<std::string> errs;
       std::vector<std::thread> th;
-      for (size_t i = 0; i < m; ++i)
-        th.emplace_back([&, i] {
-          const size_t k = base + i;
+      void join() {
+        for (auto& t : th) t.join();
+        th.clear();
+      }
+      ~Wave() { join(); }
+    };
+    auto start = [&](Wave& w, size_t base) {
+      w.base = base;
+      w.m = std::min<size_t>(hw, n_chunks - base);
+      w.chunks.assign(w.m, pq::HostChunk());
+      w.codes.assign(w.m, PRISM_OK);
+      w.errs.assign(w.m, std::string());
+      for (size_t i = 0; i < w.m; ++i)
+        w.th.emplace_back([&w, &F, &rgs, &cols, f, i] {
+          const size_t k = w.base + i;
           try {
-            wave[i] = pq::decode_host(f->bytes, f->n, F, rgs[k / cols.size()], cols[k % cols.size()]);
+            w.chunks[i] = pq::decode_host(f->bytes, f->n, F, rgs[k / cols.size()], cols[k % cols.size()]);
           } catch (const Error& e) {
-            codes[i] = e.status;
-            errs[i] = e.what();
+            w.codes[i] = e.status;
+            w.errs[i] = e.what();
           } catch (const std::exception& e) {
-            codes[i] = PRISM_ERR_INTERNAL;
-            errs[i] = std::string("Internal error: ") + e.what();
+            w.codes[i] = PRISM_ERR_INTERNAL;
+            w.errs[i] = std::string("Internal error: ") + e.what();
           }
         });
-      for (auto& t : th) t.join();
-      for (size_t i = 0; i < m; ++i)
-        if (codes[i] != PRISM_OK) fail(codes[i], errs[i]);
-      for (size_t i = 0; i < m; ++i) {
+    };
+    DeviceBuffer flags(T->ctx, std::max<size_t>(n_chunks, 1) * 4);
+    PRISM_HIP(hipMemsetAsync(flags.ptr, 0, std::max<size_t>(n_chunks, 1) * 4, ctx->stream));
+    Wave a, b;
+    Wave* cur = &a;
+    Wave* nxt = &b;
+    if (n_chunks) start(*cur, 0);
+    for (size_t base = 0; base < n_chunks; base += hw) {
+      cur->join();
+      for (size_t i = 0; i < cur->m; ++i)
+        if (cur->codes[i] != PRISM_OK) fail(cur->codes[i], cur->errs[i]);
+      if (base + hw < n_chunks) start(*nxt, base + hw);
+      for (size_t i = 0; i < cur->m; ++i) {
         const size_t k = base + i;
-        T->batches[k / cols.size()].push_back(pq::decode_device(ctx, wave[i]));
-        wave[i] = pq::HostChunk();
+        T->batches[k / cols.size()].push_back(pq::decode_device(ctx, cur->chunks[i], flags.as<uint32_t>() + k));
+        cur->chunks[i] = pq::HostChunk();
       }
+      std::swap(cur, nxt);
     }
+    std::vector<uint32_t> hf(n_chunks);
+    if (n_chunks)
+      PRISM_HIP(hipMemcpyAsync(hf.data(), flags.ptr, n_chunks * 4, hipMemcpyDeviceToHost, ctx->stream));
     PRISM_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t k = 0; k < n_chunks; ++k)
+      if (hf[k])
+        fail(PRISM_ERR_ARROW, "Arrow error: Parquet column '" + F.leaves[cols[k % cols.size()]].name +
+                                  "': a dictionary index exceeds the dictionary");
     *out = T.release();
   });
 }
