@@ -247,11 +247,13 @@ __device__ __forceinline__ uint64_t dense_range(const Stats* st, uint64_t cap) {
   const uint64_t r = static_cast<uint64_t>(st->max_key) - static_cast<uint64_t>(st->min_key) + 1;  // 0: 2^64 wrapped
   return (st->max_key >= st->min_key && r != 0 && r <= cap) ? r : 0;
 }
+// The speculative build keeps one array: `dense` (row per key) without a
+// payload, `dense_up` ({row, payload} per key) with one (dense is then NULL).
 __global__ void k_dense_fill2_dev(uint32_t* dense, uint2* dense_up, uint64_t cap, const Stats* st) {
   const uint64_t n = dense_range(st, cap);
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    dense[i] = kNone;
     if (dense_up) dense_up[i] = make_uint2(kNone, 0u);
+    else dense[i] = kNone;
   }
 }
 __global__ void k_dense_scatter_dev(const int64_t* __restrict__ keys, uint64_t n, uint64_t cap, const Stats* st,
@@ -260,8 +262,8 @@ __global__ void k_dense_scatter_dev(const int64_t* __restrict__ keys, uint64_t n
   const int64_t min_key = st->min_key;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
-    dense[d] = static_cast<uint32_t>(i);
     if (dense_up) dense_up[d] = make_uint2(static_cast<uint32_t>(i), static_cast<uint32_t>(pay[i]));
+    else dense[d] = static_cast<uint32_t>(i);
   }
 }
 __global__ void k_dense_verify_dev(const int64_t* __restrict__ keys, uint64_t n, uint64_t cap, Stats* st,
@@ -271,7 +273,7 @@ __global__ void k_dense_verify_dev(const int64_t* __restrict__ keys, uint64_t n,
   uint32_t bad = 0;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
-    bad += (dense[d] != static_cast<uint32_t>(i) || (dense_up && dense_up[d].x != static_cast<uint32_t>(i))) ? 1u : 0u;
+    bad += ((dense_up ? dense_up[d].x : dense[d]) != static_cast<uint32_t>(i)) ? 1u : 0u;
   }
   if (bad) atomicAdd(&st->dups, static_cast<unsigned long long>(bad));
 }
@@ -2113,12 +2115,14 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
     // it, and one read at the end confirms range and uniqueness.
     if (kind != PRISM_TABLE_HASHED && !keys_only && n_rows) {
       const uint64_t cap = n_rows + 1024;
-      T->dense.alloc(c, cap * 4);
-      uint32_t* d = T->dense.as<uint32_t>();
+      uint32_t* d = nullptr;
       uint2* up = nullptr;
       if (d_payload) {
         T->payload.alloc(c, cap * 8);
         up = T->payload.as<uint2>();
+      } else {
+        T->dense.alloc(c, cap * 4);
+        d = T->dense.as<uint32_t>();
       }
       hipLaunchKernelGGL(k_dense_fill2_dev, grid_for(ctx, cap), dim3(kBuildBlock), 0, st, d, up, cap,
                          static_cast<const Stats*>(dst));
