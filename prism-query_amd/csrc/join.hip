@@ -68,8 +68,15 @@ struct MaxOp { __device__ int64_t operator()(int64_t a, int64_t b) const { retur
 struct AddOp { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
 struct UMaxOp { __device__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; } };
 
-__global__ void k_minmax_part(const int64_t* __restrict__ keys, uint64_t n, int64_t* __restrict__ part) {
+// Also clears the speculative dense table (fill_n entries of fill_u32, or of
+// fill_up when a payload rides in the entries) in the same pass.
+__global__ void k_minmax_part(const int64_t* __restrict__ keys, uint64_t n, int64_t* __restrict__ part,
+                              uint32_t* fill_u32, uint2* fill_up, uint64_t fill_n) {
   __shared__ int64_t lds[16];
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < fill_n; i += (uint64_t)gridDim.x * blockDim.x) {
+    if (fill_up) fill_up[i] = make_uint2(kNone, 0u);
+    else fill_u32[i] = kNone;
+  }
   int64_t mn = INT64_MAX, mx = INT64_MIN;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     int64_t k = keys[i];
@@ -93,9 +100,13 @@ __global__ void k_minmax_final(const int64_t* __restrict__ part, int n_part, Sta
   }
   mn = block_reduce(mn, MinOp(), lds);
   mx = block_reduce(mx, MaxOp(), lds);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {  // every field: no separate initialisation
     st->min_key = mn;
     st->max_key = mx;
+    st->n_distinct = 0;
+    st->max_dup = 0;
+    st->sentinel = 0;
+    st->dups = 0;
   }
 }
 
@@ -248,14 +259,8 @@ __device__ __forceinline__ uint64_t dense_range(const Stats* st, uint64_t cap) {
   return (st->max_key >= st->min_key && r != 0 && r <= cap) ? r : 0;
 }
 // The speculative build keeps one array: `dense` (row per key) without a
-// payload, `dense_up` ({row, payload} per key) with one (dense is then NULL).
-__global__ void k_dense_fill2_dev(uint32_t* dense, uint2* dense_up, uint64_t cap, const Stats* st) {
-  const uint64_t n = dense_range(st, cap);
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    if (dense_up) dense_up[i] = make_uint2(kNone, 0u);
-    else dense[i] = kNone;
-  }
-}
+// payload, `dense_up` ({row, payload} per key) with one (dense is then NULL);
+// k_minmax_part clears it.
 __global__ void k_dense_scatter_dev(const int64_t* __restrict__ keys, uint64_t n, uint64_t cap, const Stats* st,
                                     uint32_t* dense, uint2* dense_up, const int32_t* __restrict__ pay) {
   if (dense_range(st, cap) == 0) return;
@@ -344,10 +349,12 @@ static dim3 grid_for(prism_ctx* ctx, uint64_t n) {
   return dim3(static_cast<unsigned>(blocks < cap ? blocks : cap));
 }
 
-static void launch_minmax(prism_ctx* ctx, const int64_t* keys, uint64_t n, int64_t* part, Stats* st) {
+static void launch_minmax(prism_ctx* ctx, const int64_t* keys, uint64_t n, int64_t* part, Stats* st,
+                          uint32_t* fill_u32 = nullptr, uint2* fill_up = nullptr, uint64_t fill_n = 0) {
   uint64_t b = ceil_div(n ? n : 1, kBuildBlock);
   int nb = static_cast<int>(b < kRedBlocks ? b : kRedBlocks);
-  hipLaunchKernelGGL(k_minmax_part, dim3(nb), dim3(kBuildBlock), 0, ctx->stream, keys, n, part);
+  hipLaunchKernelGGL(k_minmax_part, dim3(nb), dim3(kBuildBlock), 0, ctx->stream, keys, n, part, fill_u32, fill_up,
+                     fill_n);
   hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBuildBlock), 0, ctx->stream, part, nb, st);
   PRISM_CHECK_LAUNCH();
 }
@@ -1022,7 +1029,6 @@ static void launch_probe_vec(prism_ctx* ctx, bool vec, dim3 g, const TableView& 
 // ---------------------------------------------------------------------------
 constexpr int kJaBlock = 256;
 constexpr int kJaCache = 1024;   // LDS cache entries per workgroup
-constexpr int kJaProbes = 8;
 constexpr uint32_t kJaEmpty = 0xFFFFFFFFu;
 
 // Slot of a key in the table (DENSE: key offset; HASHED: slot index, the
@@ -2107,16 +2113,14 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
     Stats* dst = sbuf.as<Stats>();
     void* part = sbuf.as<char>() + sizeof(Stats);
     Stats* pinned = static_cast<Stats*>(readback_block(ctx));
-    hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(1), 0, st, dst);
-    if (n_rows) launch_minmax(ctx, d_keys, n_rows, static_cast<int64_t*>(part), dst);
-    PRISM_CHECK_LAUNCH();
     // Unique keys whose range fits n + 1024 entries (a permutation, dense ids):
     // the dense table is built from the device-side range without waiting for
     // it, and one read at the end confirms range and uniqueness.
-    if (kind != PRISM_TABLE_HASHED && !keys_only && n_rows) {
-      const uint64_t cap = n_rows + 1024;
-      uint32_t* d = nullptr;
-      uint2* up = nullptr;
+    const bool speculate = kind != PRISM_TABLE_HASHED && !keys_only && n_rows;
+    const uint64_t cap = n_rows + 1024;
+    uint32_t* d = nullptr;
+    uint2* up = nullptr;
+    if (speculate) {
       if (d_payload) {
         T->payload.alloc(c, cap * 8);
         up = T->payload.as<uint2>();
@@ -2124,8 +2128,11 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
         T->dense.alloc(c, cap * 4);
         d = T->dense.as<uint32_t>();
       }
-      hipLaunchKernelGGL(k_dense_fill2_dev, grid_for(ctx, cap), dim3(kBuildBlock), 0, st, d, up, cap,
-                         static_cast<const Stats*>(dst));
+    }
+    if (n_rows) launch_minmax(ctx, d_keys, n_rows, static_cast<int64_t*>(part), dst, d, up, speculate ? cap : 0);
+    else hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(1), 0, st, dst);
+    PRISM_CHECK_LAUNCH();
+    if (speculate) {
       hipLaunchKernelGGL(k_dense_scatter_dev, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, cap,
                          static_cast<const Stats*>(dst), d, up, d_payload);
       hipLaunchKernelGGL(k_dense_verify_dev, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, cap, dst,
