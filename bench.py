@@ -37,6 +37,11 @@ DEFAULT_SIZES = (1_000_000, 64_000_000, 200_000_000)
 METRIC = "rows/s & %HBM peak (join probe, group-by); p99 latency batched vs plain"
 
 
+def progress(section: str, t_start: float) -> None:
+    """One stderr line per finished section (stdout carries only the JSON line)."""
+    print(f"[bench] {section} done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(nb: int, sample: int) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -453,9 +458,16 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
         served = summary(*run_served(arr))
         served["service"] = {k: v for k, v in S.stats().items() if k.startswith("mean_") or k.startswith("max_")}
         out["rates"][str(rate)] = {"served": served, "plain": summary(*run_plain(arr))}
+    # saturation: every request submitted at once; three runs of each form,
+    # the run with the median served / plain ratio reported
     z = np.zeros(len(reqs))
-    sat = {"served": summary(*run_served(z)), "plain": summary(*run_plain(z))}
-    sat["throughput_ratio"] = sat["served"]["rows_per_s"] / sat["plain"]["rows_per_s"]
+    runs = []
+    for _ in range(3):
+        sv, pl = summary(*run_served(z)), summary(*run_plain(z))
+        runs.append((sv["rows_per_s"] / pl["rows_per_s"], sv, pl))
+    runs.sort(key=lambda x: x[0])
+    sat = {"served": runs[1][1], "plain": runs[1][2], "throughput_ratio": runs[1][0],
+           "throughput_ratios_all_runs": [r[0] for r in runs]}
     out["saturation"] = sat
     out["service_stats"] = S.stats()
     S.close()
@@ -498,6 +510,7 @@ def main() -> int:
     from prism_executor import device as D
 
     nb, npr = args.build_rows, args.probe_rows
+    t_start = time.perf_counter()
     ctx = D.Context(0)
     lib = ctx.lib
     bk = D.empty(ctx, nb, np.int64)
@@ -602,21 +615,27 @@ def main() -> int:
         "build_ms": build_ms,
         "matches": int(r.matches),
     }
+    progress("join", t_start)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(nb, args.cpu_sample)
+        progress("join cpu baseline", t_start)
     for a in (op, ob, pk, pp):
         a.free()
     if not args.no_groupby:
         line["groupby_filter"] = bench_groupby(ctx, args.groupby_rows, args.steps,
                                                0 if args.no_cpu_baseline else 10_000_000)
+        progress("group-by", t_start)
     if not args.no_join_agg:
         line["join_aggregate"] = bench_join_agg(ctx, 16_000_000, args.join_agg_probe_rows, 8_000_000, 1_000_000,
                                                 max(2, min(args.steps, 5)),
                                                 0 if args.no_cpu_baseline else 8_000_000)
+        progress("join-aggregate", t_start)
     if not args.no_parquet:
         line["parquet_scan"] = bench_parquet(ctx, args.parquet_rows, max(2, min(args.steps, 5)))
+        progress("parquet scan", t_start)
     if not args.no_service:
         line["served_stream"] = bench_service(ctx, args.svc_requests, [int(x) for x in args.svc_rates.split(",")])
+        progress("served stream", t_start)
     if args.calibrate_copy:
         src = D.empty(ctx, args.calibrate_copy, np.uint8)
         dst = D.empty(ctx, args.calibrate_copy, np.uint8)
