@@ -134,7 +134,8 @@ __global__ __launch_bounds__(kSB) void k_svc_groupby(Control* C, int v) {
   Queue* q = &C->q[v];
   const uint32_t n_tiles = q->n_tiles;
   while (true) {
-    const uint32_t t = next_tile(q, &s_t);
+    // uniform per workgroup: scalar loads of the tile and its request descriptor
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t));
     if (t >= n_tiles) break;
     const Tile tl = C->tiles[q->offset + t];
     const GbDesc& d = C->gb[tl.req];
@@ -163,7 +164,8 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
   const uint32_t n_tiles = q->n_tiles;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   while (true) {
-    const uint32_t t = next_tile(q, &s_t);
+    // uniform per workgroup: scalar loads of the tile and its request descriptor
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t));
     if (t >= n_tiles) break;
     const Tile tl = C->tiles[q->offset + t];
     const ProbeDesc& d = C->pr[tl.req];
@@ -177,8 +179,9 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
       for (int u = 0; u < U; ++u) {
         const uint64_t r = r0 + static_cast<uint64_t>(u) * kSB;
         in[u] = r < tl.end;
-        key[u] = in[u] ? d.keys[r] : 0;
-        x[u] = (in[u] && d.pay) ? d.pay[r] : 0;  // loaded with the key: no dependent load after the lookup
+        // streamed once: non-temporal, so the table's lines keep their place in L2 / MALL
+        key[u] = in[u] ? __builtin_nontemporal_load(d.keys + r) : 0;
+        x[u] = (in[u] && d.pay) ? __builtin_nontemporal_load(d.pay + r) : 0;  // with the key: no dependent load
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
 
 // Filter tiles start on 64-row boundaries: every wave step covers whole words.
 template <int R, int DEPTH>
-__global__ __launch_bounds__(kSB) void k_svc_filter(Control* C, int v) {
+__global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(4))) void k_svc_filter(Control* C, int v) {
   __shared__ uint32_t s_t;
   __shared__ unsigned long long s_c[kSB / 64];
   Queue* q = &C->q[v];
@@ -228,7 +231,8 @@ __global__ __launch_bounds__(kSB) void k_svc_filter(Control* C, int v) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t pre[1][R] = {};
   while (true) {
-    const uint32_t t = next_tile(q, &s_t);
+    // uniform per workgroup: scalar loads of the tile and its request descriptor
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t));
     if (t >= n_tiles) break;
     const Tile tl = C->tiles[q->offset + t];
     const FiltDesc& d = C->fd[tl.req];
