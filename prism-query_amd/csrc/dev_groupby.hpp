@@ -438,11 +438,13 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
 // Persistent tile-queue form of the update (the operator service): a
 // workgroup takes (request, row range) tiles from a queue until it is empty.
 // ---------------------------------------------------------------------------
+constexpr int kQueueLanes = 8;  // claim counters per queue: one per XCD
+
 struct TileQueue {
   uint32_t n_tiles;
-  uint32_t next;
   uint32_t offset;  // first tile of this queue in the tile array
-  uint32_t pad;
+  uint32_t pad[14];
+  uint32_t next[kQueueLanes * 16];  // counter of XCD x at next[16 * x], 64 bytes apart
 };
 
 struct RowTile {
@@ -463,13 +465,25 @@ struct TileReq {
   uint32_t pad;
 };
 
-// Next tile index of the workgroup (one atomic per tile).
-__device__ __forceinline__ uint32_t next_tile(TileQueue* q, uint32_t* s_t) {
-  if (threadIdx.x == 0) *s_t = atomicAdd(&q->next, 1u);
+// Next tile index of the workgroup.  Claims on one device-wide counter
+// serialise (about 0.1 us each), so the first tile of a workgroup is its own
+// index, and later ones come from the counter of its XCD (workgroup index mod
+// kQueueLanes), which hands out that XCD's share of the remaining tiles:
+// t = grid + x + kQueueLanes * k.  Every tile >= grid is handed out once as
+// long as the grid holds every residue, i.e. grid >= kQueueLanes whenever
+// tiles remain past the first round (grids are then a multiple of 8).
+// `first` starts true; the result is uniform over the workgroup.
+__device__ __forceinline__ uint32_t next_tile(TileQueue* q, uint32_t* s_t, bool& first) {
+  if (first) {
+    first = false;
+    return blockIdx.x;
+  }
+  const uint32_t x = blockIdx.x % kQueueLanes;
+  if (threadIdx.x == 0) *s_t = atomicAdd(&q->next[16 * x], 1u);
   __syncthreads();
-  const uint32_t t = *s_t;
+  const uint32_t k = *s_t;
   __syncthreads();
-  return t;
+  return gridDim.x + x + kQueueLanes * k;
 }
 
 

@@ -104,8 +104,9 @@ class Gen {
           "  extern __shared__ unsigned long long lds[];\n"
           "  __shared__ uint32_t s_t;\n"
           "  const uint32_t n_tiles = q->n_tiles;\n"
+          "  bool first = true;\n"
           "  while (true) {\n"
-          "    const uint32_t t = next_tile(q, &s_t);\n"
+          "    const uint32_t t = next_tile(q, &s_t, first);\n"
           "    if (t >= n_tiles) break;\n"
           "    const RowTile tl = tiles[q->offset + t];\n"
           "    const TileReq& d = reqs[tl.req];\n"

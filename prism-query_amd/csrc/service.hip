@@ -46,9 +46,9 @@ namespace svc {
 using clk = std::chrono::steady_clock;
 
 constexpr int kSB = 256;               // threads per workgroup (group-by kernels assume kBlock)
-// Probe and filter tiles add their totals to one of kAccCopies copies of the
-// request's accumulator (tile index modulo kAccCopies): one address per
-// request would serialise the atomics of thousands of tiles.
+// Probe workgroups and filter waves add their totals to one of kAccCopies
+// copies of the request's accumulator (workgroup or wave index modulo
+// kAccCopies), once per request: one address would serialise the atomics.
 constexpr int kAccCopies = 16;
 // Group-by plans with at most this many slots flush a tile's groups in no time.
 constexpr uint64_t kSmallGroupCap = 64;
@@ -133,9 +133,10 @@ __global__ __launch_bounds__(kSB) void k_svc_groupby(Control* C, int v) {
   __shared__ uint32_t s_t;
   Queue* q = &C->q[v];
   const uint32_t n_tiles = q->n_tiles;
+  bool first = true;  // first tile: the workgroup's own index
   while (true) {
     // uniform per workgroup: scalar loads of the tile and its request descriptor
-    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t));
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t, first));
     if (t >= n_tiles) break;
     const Tile tl = C->tiles[q->offset + t];
     const GbDesc& d = C->gb[tl.req];
@@ -155,6 +156,42 @@ __device__ __forceinline__ void wave_i128(__int128& v) {
   }
 }
 
+// Sums of one request's tiles stay in registers while the workgroup's tiles
+// belong to that request; they are added to the request's accumulator copy
+// (chosen by workgroup) when the request changes and at the end, so that
+// contended carry-propagating atomics are paid once per workgroup and request.
+__device__ __forceinline__ void svc_probe_flush(unsigned long long* acc_base, unsigned long long& m, __int128& ps,
+                                                __int128& bs, unsigned long long* s_m, __int128* s_p, __int128* s_b) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = 32; k > 0; k >>= 1) m += __shfl_xor(m, k, 64);
+  wave_i128(ps);
+  wave_i128(bs);
+  if (lane == 0) {
+    s_m[wave] = m;
+    s_p[wave] = ps;
+    s_b[wave] = bs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = 0;
+    ps = 0;
+    bs = 0;
+    for (int w = 0; w < kSB / 64; ++w) {
+      m += s_m[w];
+      ps += s_p[w];
+      bs += s_b[w];
+    }
+    unsigned long long* acc = acc_base + 5 * (blockIdx.x % kAccCopies);
+    if (m) atomicAdd(&acc[0], m);
+    if (ps) atomic_add_i128(&acc[1], &acc[2], static_cast<uint64_t>(ps), static_cast<int64_t>(ps >> 64));
+    if (bs) atomic_add_i128(&acc[3], &acc[4], static_cast<uint64_t>(bs), static_cast<int64_t>(bs >> 64));
+  }
+  __syncthreads();
+  m = 0;
+  ps = 0;
+  bs = 0;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
   __shared__ uint32_t s_t;
@@ -162,15 +199,20 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
   __shared__ __int128 s_p[kSB / 64], s_b[kSB / 64];
   Queue* q = &C->q[v];
   const uint32_t n_tiles = q->n_tiles;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t cur = 0xFFFFFFFFu;  // request whose sums the registers hold (uniform per workgroup)
+  unsigned long long m = 0;
+  __int128 ps = 0, bs = 0;
+  bool first = true;  // first tile: the workgroup's own index
   while (true) {
     // uniform per workgroup: scalar loads of the tile and its request descriptor
-    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t));
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t, first));
     if (t >= n_tiles) break;
     const Tile tl = C->tiles[q->offset + t];
     const ProbeDesc& d = C->pr[tl.req];
-    unsigned long long m = 0;
-    __int128 ps = 0, bs = 0;
+    if (tl.req != cur) {
+      if (cur != 0xFFFFFFFFu) svc_probe_flush(C->pr[cur].acc, m, ps, bs, s_m, s_p, s_b);
+      cur = tl.req;
+    }
     constexpr int U = 8;  // independent lookups in flight per thread
     for (uint64_t r0 = tl.start + threadIdx.x; r0 < tl.end; r0 += static_cast<uint64_t>(kSB) * U) {
       int64_t key[U], x[U];
@@ -194,49 +236,35 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
         }
       }
     }
-    for (int k = 32; k > 0; k >>= 1) m += __shfl_xor(m, k, 64);
-    wave_i128(ps);
-    wave_i128(bs);
-    if (lane == 0) {
-      s_m[wave] = m;
-      s_p[wave] = ps;
-      s_b[wave] = bs;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      m = 0;
-      ps = 0;
-      bs = 0;
-      for (int w = 0; w < kSB / 64; ++w) {
-        m += s_m[w];
-        ps += s_p[w];
-        bs += s_b[w];
-      }
-      unsigned long long* acc = d.acc + 5 * (t % kAccCopies);
-      if (m) atomicAdd(&acc[0], m);
-      if (ps) atomic_add_i128(&acc[1], &acc[2], static_cast<uint64_t>(ps), static_cast<int64_t>(ps >> 64));
-      if (bs) atomic_add_i128(&acc[3], &acc[4], static_cast<uint64_t>(bs), static_cast<int64_t>(bs >> 64));
-    }
-    __syncthreads();
   }
+  if (cur != 0xFFFFFFFFu) svc_probe_flush(C->pr[cur].acc, m, ps, bs, s_m, s_p, s_b);
 }
 
 // Filter tiles start on 64-row boundaries: every wave step covers whole words.
 template <int R, int DEPTH>
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(4))) void k_svc_filter(Control* C, int v) {
   __shared__ uint32_t s_t;
-  __shared__ unsigned long long s_c[kSB / 64];
   Queue* q = &C->q[v];
   const uint32_t n_tiles = q->n_tiles;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t copy = (blockIdx.x * (kSB / 64) + wave) % kAccCopies;
   const uint64_t pre[1][R] = {};
+  bool first = true;  // first tile: the workgroup's own index
+  // selected rows of one request, counted per wave across that request's
+  // tiles and added to its count copy when the request changes and at the end
+  uint32_t cur = 0xFFFFFFFFu;
+  unsigned long long sel = 0;
   while (true) {
     // uniform per workgroup: scalar loads of the tile and its request descriptor
-    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t));
+    const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t, first));
     if (t >= n_tiles) break;
     const Tile tl = C->tiles[q->offset + t];
     const FiltDesc& d = C->fd[tl.req];
-    unsigned long long sel = 0;
+    if (tl.req != cur) {
+      if (cur != 0xFFFFFFFFu && lane == 0 && sel) atomicAdd(C->fd[cur].count + copy, sel);
+      sel = 0;
+      cur = tl.req;
+    }
     const uint64_t w_end = ceil_div(tl.end, 64);
     for (uint64_t w0 = tl.start / 64 + static_cast<uint64_t>(wave) * R; w0 < w_end; w0 += (kSB / 64) * R) {
       uint64_t row[R];
@@ -258,15 +286,8 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (lane == 0 && d.mask) d.mask[w0 + r] = bw;
       }
     }
-    if (lane == 0) s_c[wave] = sel;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      unsigned long long s = 0;
-      for (int w = 0; w < kSB / 64; ++w) s += s_c[w];
-      if (s) atomicAdd(d.count + t % kAccCopies, s);
-    }
-    __syncthreads();
   }
+  if (cur != 0xFFFFFFFFu && lane == 0 && sel) atomicAdd(C->fd[cur].count + copy, sel);
 }
 
 // Group-by finish: one workgroup per request collects the occupied slots,
@@ -565,10 +586,13 @@ static void launch_batch(prism_service* S, Slot& sl) {
   for (Req* R : sl.reqs)
     gb_big_flush = gb_big_flush || (R->r.kind == PRISM_REQ_GROUPBY && (!R->gbp->priv || jit::enabled()) &&
                                     R->gbp->cap > kSmallGroupCap);
-  const uint64_t min_tile[3] = {gb_big_flush ? 65536u : 16384u, 4096, 4096};
+  // probe tiles may be small (a workgroup flushes its sums once per request,
+  // not per tile): small requests still spread over every workgroup
+  const uint64_t min_tile[3] = {gb_big_flush ? 65536u : 16384u, 1024, 4096};
+  const uint64_t per_wg[3] = {4, 2, 4};
   for (int k = 0; k < 3; ++k) {
     const Cand cd = cand_of(S, static_cast<Kind>(k));
-    const uint64_t want = kind_rows[k] / (static_cast<uint64_t>(n_cu) * cd.bpc * 4);
+    const uint64_t want = kind_rows[k] / (static_cast<uint64_t>(n_cu) * cd.bpc * per_wg[k]);
     tile_of[k] = std::min<uint64_t>(cd.tile_rows, std::max<uint64_t>(min_tile[k], want));
   }
   size_t lds[kVariants] = {};
@@ -695,7 +719,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
   int grid[kVariants] = {};
   for (int v = 0; v < kVariants; ++v) {
     H->q[v].n_tiles = static_cast<uint32_t>(vt[v].size());
-    H->q[v].next = 0;
+    memset(H->q[v].next, 0, sizeof(H->q[v].next));
     H->q[v].offset = off;
     if (!vt[v].empty()) {
       if (off + vt[v].size() > static_cast<size_t>(kMaxTiles))
@@ -705,14 +729,18 @@ static void launch_batch(prism_service* S, Slot& sl) {
       const Cand cd = cand_of(S, kind_of(v));
       uint32_t bpc = cd.bpc;
       if (lds[v]) bpc = std::min<uint32_t>(bpc, std::max<uint32_t>(1, static_cast<uint32_t>((160u << 10) / lds[v])));
-      grid[v] = static_cast<int>(std::min<uint64_t>(pow2_grid(vt[v].size()), static_cast<uint64_t>(n_cu) * bpc));
+      // tiles past the first round are claimed per residue of the workgroup
+      // index mod kQueueLanes: such grids hold every residue
+      grid[v] = static_cast<int>(std::min<uint64_t>(pow2_grid(vt[v].size()),
+                                                    std::max<uint64_t>(static_cast<uint64_t>(n_cu) * bpc,
+                                                                       gb::kQueueLanes)));
     }
   }
   std::vector<int> jgrid(jf.size(), 0);
   for (size_t i = 0; i < jf.size(); ++i) {
     Queue& q = H->q[kVariants + i];
     q.n_tiles = static_cast<uint32_t>(jt[i].size());
-    q.next = 0;
+    memset(q.next, 0, sizeof(q.next));
     q.offset = off;
     if (off + jt[i].size() > static_cast<size_t>(kMaxTiles)) fail(PRISM_ERR_INTERNAL, "Internal error: batch tile overflow");
     memcpy(&H->tiles[off], jt[i].data(), jt[i].size() * sizeof(Tile));
@@ -721,7 +749,9 @@ static void launch_batch(prism_service* S, Slot& sl) {
     uint32_t bpc = cd.bpc;
     if (jcap[i] <= kSmallGroupCap) bpc = std::max<uint32_t>(bpc, 4);  // cheap flushes: fill the CUs
     if (jlds[i]) bpc = std::min<uint32_t>(bpc, std::max<uint32_t>(1, static_cast<uint32_t>((160u << 10) / jlds[i])));
-    jgrid[i] = static_cast<int>(std::min<uint64_t>(pow2_grid(jt[i].size()), static_cast<uint64_t>(n_cu) * bpc));
+    jgrid[i] = static_cast<int>(std::min<uint64_t>(pow2_grid(jt[i].size()),
+                                                   std::max<uint64_t>(static_cast<uint64_t>(n_cu) * bpc,
+                                                                      gb::kQueueLanes)));
   }
   const size_t ctl_bytes = offsetof(Control, tiles) + off * sizeof(Tile);
   Control* dC = sl.control.as<Control>();
