@@ -1410,8 +1410,9 @@ __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4)
 #pragma unroll
     for (int j = 0; j < kJaRows1; ++j) {
       const uint64_t r = base + static_cast<uint64_t>(j) * kJaBlock1 + threadIdx.x;
-      k[j] = r < n ? keys[r] : 0;
-      x[j] = (pay && r < n) ? pay[r] : 0;
+      // streamed once: non-temporal, so the presence bits and slot records stay in L2
+      k[j] = r < n ? __builtin_nontemporal_load(keys + r) : 0;
+      x[j] = (pay && r < n) ? __builtin_nontemporal_load(pay + r) : 0;
     }
 #pragma unroll
     for (int j = 0; j < kJaRows1; ++j) {
@@ -1692,10 +1693,10 @@ __global__ __launch_bounds__(kJaPartBlock) void k_ja_scatter2(JaPart P) {
   for (uint64_t base = lo;; base += kJaPartBlock) {
     const uint64_t i = base + threadIdx.x;
     if (i < hi) {
-      const unsigned long long sr = P.l1_sr[r0 + i];
+      const unsigned long long sr = __builtin_nontemporal_load(P.l1_sr + r0 + i);  // last read of the record
       const uint32_t s = static_cast<uint32_t>(sr);
       if (s != kJaHole) {
-        const uint64_t x = static_cast<uint64_t>(P.l1_pay[r0 + i]);
+        const uint64_t x = static_cast<uint64_t>(__builtin_nontemporal_load(P.l1_pay + r0 + i));
         const uint4 rec = make_uint4(s & (kJaSub - 1), static_cast<uint32_t>(sr >> 32), static_cast<uint32_t>(x),
                                      static_cast<uint32_t>(x >> 32));
         const uint32_t g = (s >> kJaSubShift) & (fan - 1);
@@ -1740,9 +1741,9 @@ __global__ __launch_bounds__(kJaPartBlock) void k_ja_agg2(JaPart P, JaAcc A) {
     a_hi[i] = 0;
   }
   __syncthreads();
-  const uint4* rec = P.l2 + P.off2[g];
+  const v4u32* rec = reinterpret_cast<const v4u32*>(P.l2 + P.off2[g]);
   for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-    const uint4 r = rec[i];
+    const v4u32 r = __builtin_nontemporal_load(rec + i);  // the record's only read
     atomicAdd(&a_cnt[r.x], 1u);
     if (r.z) atomicAdd(&a_lo[r.x], static_cast<unsigned long long>(r.z));
     if (r.w) atomicAdd(reinterpret_cast<unsigned long long*>(&a_hi[r.x]),
