@@ -208,15 +208,16 @@ class Gen {
       const bool f = type(c) == PRISM_TYPE_FLOAT64;
       std::string ld;
       switch (type(c)) {
+        // columns are streamed once: non-temporal loads leave L2 to the group table's lines
         case PRISM_TYPE_INT64:
         case PRISM_TYPE_FLOAT64:
-          ld = "d" + C + "[rr[r]]";
+          ld = "__builtin_nontemporal_load(d" + C + " + rr[r])";
           break;
         case PRISM_TYPE_BOOL:
           ld = "static_cast<int64_t>(bit_at(d" + C + ", o" + C + " + rr[r]))";
           break;
         default:
-          ld = "static_cast<int64_t>(d" + C + "[rr[r]])";
+          ld = "static_cast<int64_t>(__builtin_nontemporal_load(d" + C + " + rr[r]))";
       }
       o_ << "    " << (f ? "double" : "int64_t") << " x" << C << "[R_];\n    bool x" << C << "v[R_];\n"
          << "#pragma unroll\n    for (int r = 0; r < R_; ++r) { x" << C << "[r] = " << ld << "; x" << C
