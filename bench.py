@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # scaled by its reading for a plain 1 GiB device copy: FETCH 2.0, WRITE 1.0),
 # recorded in profiles/bench_default_hbm_traffic.md.  Measured offline, not by
 # this run: they hold for the default sizes only.
-TRAFFIC = {"join_probe": 3_494_900_000, "groupby_update": 6_373_400_000}
+TRAFFIC = {"join_probe": 3_494_300_000, "groupby_update": 6_118_900_000}
 DEFAULT_SIZES = (1_000_000, 64_000_000, 200_000_000)
 METRIC = "rows/s & %HBM peak (join probe, group-by); p99 latency batched vs plain"
 
