@@ -646,7 +646,8 @@ static uint64_t def_levels(const uint8_t* q, const uint8_t* e, uint64_t nv, std:
       const uint64_t groups = h >> 1;
       if (!groups) bad("bit-packed run of length 0");
       if (groups > static_cast<uint64_t>(e - q)) bad("bit-packed definition levels are truncated");
-      const uint64_t cnt = std::min<uint64_t>(groups * 8, nv - done);
+      const uint64_t left = nv - done;
+      const uint64_t cnt = groups >= (left + 7) / 8 ? left : groups * 8;  // no product overflow
       put_bits(valid, row0 + done, q, cnt);
       for (uint64_t k = 0; k * 8 < cnt; ++k) {
         uint8_t b = q[k];
@@ -681,9 +682,12 @@ static void index_runs(const uint8_t* q, const uint8_t* e, int bw, uint64_t coun
     } else {
       const uint64_t groups = h >> 1;
       if (!groups) bad("bit-packed run of length 0");
+      // bounded before the product: a forged header must not wrap it past the check
+      if (bw && groups > static_cast<uint64_t>(e - q) / static_cast<uint64_t>(bw))
+        bad("bit-packed dictionary indices are truncated");
       const uint64_t nbytes = groups * static_cast<uint64_t>(bw);
-      if (nbytes > static_cast<uint64_t>(e - q)) bad("bit-packed dictionary indices are truncated");
-      const uint64_t cnt = std::min<uint64_t>(groups * 8, count - done);
+      const uint64_t left = count - done;
+      const uint64_t cnt = groups >= (left + 7) / 8 ? left : groups * 8;  // no product overflow
       if (bw == 0) {  // every index is 0 and no bytes follow
         if (!n_dict) bad("dictionary index exceeds the empty dictionary");
         H.runs.push_back({v0 + done, RUN_RLE, 0u});
