@@ -829,8 +829,7 @@ static HostChunk decode_host(const uint8_t* bytes, uint64_t n, const FileMeta& F
         q += 4;
         if (len > static_cast<uint64_t>(qe - q)) bad("RLE booleans are truncated");
         std::vector<uint8_t> bits((nonnull + 7) / 8 + 8, 0);
-        const uint64_t ones = def_levels(q, q + len, nonnull, bits, 0);
-        (void)ones;
+        def_levels(q, q + len, nonnull, bits, 0);  // the same bit-width-1 hybrid stream as the levels
         H.runs.push_back({vals, RUN_PLAIN, static_cast<uint32_t>(H.src.n)});
         put_bits(H.src.bits, H.src.n, bits.data(), nonnull);
         H.src.n += nonnull;
