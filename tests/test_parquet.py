@@ -198,3 +198,4 @@ def test_unsupported_leaves_and_bad_files():
     with pytest.raises(L.PrismError):
         ParquetFile(good[:-40] + good[-8:])
     assert lib.prism_parquet_open(None, 0, None) != 0
+    ParquetFile(good).close()  # a successful call clears the thread's last error for later tests
