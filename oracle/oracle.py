@@ -257,3 +257,109 @@ def reducer_merge(partials, group_by_count, original):
         else:
             out.append([float(v) for v in src.tolist()])
     return list(zip(r["first_batch"][:g].tolist(), r["first_row"][:g].tolist())), out
+
+
+# ---------------------------------------------------------------------------
+# Join types and joined-batch assembly (pure Python over numpy / pyarrow;
+# small inputs only).
+# ---------------------------------------------------------------------------
+INNER, LEFT, RIGHT, FULL, LEFT_SEMI, LEFT_ANTI = 0, 1, 2, 3, 4, 5
+
+
+def left_join_indices(probe_rows, mp, mb):
+    """left_join_indices (hash_join.rs:455-484): matched pairs in order, then
+    every unmatched probe row paired with build row 0 (not null)."""
+    matched = np.zeros(probe_rows, bool)
+    matched[np.asarray(mp, np.int64)] = True
+    un = np.nonzero(~matched)[0]
+    p = list(np.asarray(mp).tolist()) + un.tolist()
+    b = list(np.asarray(mb).tolist()) + [0] * len(un)
+    return p, b
+
+
+def semi_join_mask(probe_rows, mp):
+    """semi_join_mask (hash_join.rs:486-492)."""
+    m = np.zeros(probe_rows, bool)
+    m[np.asarray(mp, np.int64)] = True
+    return m
+
+
+def right_join_indices(build_rows, mp, mb):
+    """right_join_indices (hash_join.rs:494-524): matched pairs, then every
+    unmatched build row with a null probe index."""
+    matched = np.zeros(build_rows, bool)
+    matched[np.asarray(mb, np.int64)] = True
+    un = np.nonzero(~matched)[0]
+    return list(np.asarray(mp).tolist()) + [None] * len(un), list(np.asarray(mb).tolist()) + un.tolist()
+
+
+def full_join_indices(probe_rows, build_rows, mp, mb):
+    """full_join_indices (hash_join.rs:527-566): matched pairs, unmatched
+    probe rows (null build), then unmatched build rows (null probe)."""
+    pm = np.zeros(probe_rows, bool)
+    bm = np.zeros(build_rows, bool)
+    pm[np.asarray(mp, np.int64)] = True
+    bm[np.asarray(mb, np.int64)] = True
+    up, ub = np.nonzero(~pm)[0].tolist(), np.nonzero(~bm)[0].tolist()
+    p = list(np.asarray(mp).tolist()) + up + [None] * len(ub)
+    b = list(np.asarray(mb).tolist()) + [None] * len(up) + ub
+    return p, b
+
+
+def assemble_joined_batch(probe, p_idx, build, b_idx, probe_nullable=False, build_nullable=False):
+    """assemble_joined_batch[_nullable_probe/_both_nullable] (hash_join.rs:424-453,
+    569-625): take of every probe column, then every build column, a build
+    name that clashes with a probe name gets "_right"; the nullable forms mark
+    the probe (and for FULL the build) fields nullable."""
+    import pyarrow as pa
+    pi = pa.array(p_idx, pa.uint32())
+    bi = pa.array(b_idx, pa.uint32())
+    cols, fields = [], []
+    for i, f in enumerate(probe.schema):
+        fields.append(pa.field(f.name, f.type, True if probe_nullable else f.nullable))
+        cols.append(probe.column(i).take(pi))
+    names = set(probe.schema.names)
+    for i, f in enumerate(build.schema):
+        name = f.name + "_right" if f.name in names else f.name
+        fields.append(pa.field(name, f.type, True if build_nullable else f.nullable))
+        cols.append(build.column(i).take(bi))
+    return pa.RecordBatch.from_arrays(cols, schema=pa.schema(fields))
+
+
+def _filter_probe(probe, mask):
+    import pyarrow as pa
+    return pa.RecordBatch.from_arrays([probe.column(i).filter(pa.array(mask)) for i in range(probe.num_columns)],
+                                      schema=probe.schema)
+
+
+def hash_join(probe, build, join_type, build_keys, probe_keys):
+    """hash_join (hash_join.rs:312-365) over pyarrow RecordBatches."""
+    mp, mb = join_probe(build, build_keys, probe, probe_keys)
+    npr, nb = probe.num_rows, build.num_rows
+    if join_type == INNER:
+        return assemble_joined_batch(probe, mp, build, mb)
+    if join_type == LEFT:
+        p, b = left_join_indices(npr, mp, mb)
+        return assemble_joined_batch(probe, p, build, b)
+    if join_type == LEFT_SEMI:
+        return _filter_probe(probe, semi_join_mask(npr, mp))
+    if join_type == LEFT_ANTI:
+        return _filter_probe(probe, ~semi_join_mask(npr, mp))
+    if join_type == RIGHT:
+        p, b = right_join_indices(nb, mp, mb)
+        return assemble_joined_batch(probe, p, build, b, probe_nullable=True)
+    p, b = full_join_indices(npr, nb, mp, mb)
+    return assemble_joined_batch(probe, p, build, b, probe_nullable=True, build_nullable=True)
+
+
+def hash_join_probe_chunked(probe_batches, build, join_type, build_keys, probe_keys):
+    """hash_join_probe_chunked (hash_join.rs:206-282): per probe batch as
+    hash_join does, except Right/Full, which fall back to the inner assembly
+    (261-270); empty outputs are dropped (274-280)."""
+    out = []
+    for p in probe_batches:
+        jt = INNER if join_type in (RIGHT, FULL) else join_type
+        r = hash_join(p, build, jt, build_keys, probe_keys)
+        if r.num_rows > 0:
+            out.append(r)
+    return out

@@ -118,6 +118,7 @@ prism_column DevColumn::desc() const {
 void staged_h2d(prism_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!bytes) return;
   constexpr size_t kHalf = 8u << 20;
+  std::lock_guard<std::mutex> lock(ctx->staging_mu);
   char* pin = static_cast<char*>(pinned_staging(ctx, 2 * kHalf));
   static thread_local hipEvent_t ev[2] = {nullptr, nullptr};
   for (auto& e : ev)
@@ -153,7 +154,12 @@ DevColumn upload(prism_ctx* ctx, const HostColumn& c) {
   if (c.type == DType::Utf8) {
     std::vector<int32_t> o(static_cast<size_t>(n + 1));
     const int32_t base = c.offsets ? c.offsets[start] : 0;
-    for (int64_t i = 0; i <= n; ++i) o[i] = c.offsets ? c.offsets[start + i] - base : 0;
+    for (int64_t i = 0; i <= n; ++i) {
+      const int64_t v = c.offsets ? static_cast<int64_t>(c.offsets[start + i]) - base : 0;
+      if (i && (v < o[i - 1] || v > INT32_MAX))
+        fail(PRISM_ERR_ARROW, "Arrow error: Utf8 offsets decrease at row " + std::to_string(start + i - 1));
+      o[i] = static_cast<int32_t>(v);
+    }
     d.offsets.alloc(cx, o.size() * 4);
     staged_h2d(ctx, d.offsets.ptr, o.data(), o.size() * 4);
     const size_t nb = static_cast<size_t>(o[n]);
@@ -236,7 +242,6 @@ struct SchemaHolder {
 };
 
 const uint8_t kEmpty[16] = {0};
-const int32_t kZeroOffset[1] = {0};
 
 void release_col(ArrowArray* a) {
   delete static_cast<ColHolder*>(a->private_data);

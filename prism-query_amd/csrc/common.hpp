@@ -121,6 +121,17 @@ struct prism_ctx {
   // Per-kernel timing (event pairs per kernel name) while enabled.
   bool ktime_on = false;
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> ktime;
+  // The pinned staging buffer is shared by every upload on this context.
+  std::mutex staging_mu;
+  // Lanes: child contexts (own stream, scratch, staging, block cache) that the
+  // Arrow-layer operator entries borrow for one call, so that calls from
+  // several threads on one context run side by side on separate streams.
+  prism_ctx* parent = nullptr;
+  std::mutex lane_mu;
+  std::vector<prism_ctx*> lanes, idle_lanes;
+  // Held by a call that runs on a caller-chosen stream (prism_ctx_set_stream):
+  // such calls run one at a time, in the caller's stream order.
+  std::mutex serial_mu;
 };
 
 namespace prism {
@@ -139,6 +150,24 @@ void* readback_block(prism_ctx* ctx);
 
 // Return every cached block to the device (hipFree).
 void pool_trim(prism_ctx* ctx);
+
+// Borrows a lane of `ctx` for the lifetime of the guard and points `ctx` at
+// it.  Work queued on the lane must be finished (or be ordered before any
+// later use of what it writes) when the guard ends: the Arrow-layer entries
+// synchronise before they return.  Already a lane: nothing to do.  A context
+// whose stream the caller chose keeps that stream and runs calls one at a time.
+class Lane {
+ public:
+  explicit Lane(prism_ctx*& ctx);
+  ~Lane();
+  Lane(const Lane&) = delete;
+  Lane& operator=(const Lane&) = delete;
+
+ private:
+  prism_ctx* parent_ = nullptr;
+  prism_ctx* lane_ = nullptr;
+  std::unique_lock<std::mutex> serial_;
+};
 
 // Brackets the launches made during its lifetime with events on the context
 // stream when per-kernel timing is enabled.

@@ -115,6 +115,63 @@ void pool_trim(prism_ctx* ctx) {
 
 void use_device(prism_ctx* ctx) { PRISM_HIP(hipSetDevice(ctx->device)); }
 
+static prism_ctx* new_lane(prism_ctx* parent) {
+  auto* c = new Ctx();
+  c->device = parent->device;
+  c->num_cus = parent->num_cus;
+  c->num_xcds = parent->num_xcds;
+  c->lds_per_block = parent->lds_per_block;
+  c->parent = parent;
+  hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    fail(PRISM_ERR_EXECUTION, std::string("Execution error: lane stream creation failed: ") + hipGetErrorString(e));
+  }
+  c->stream = c->own_stream;
+  return c;
+}
+
+// Lanes this thread holds: an operator entry called from inside another runs
+// on the outer call's lane (or under its serial lock).
+static thread_local int g_lane_depth = 0;
+
+Lane::Lane(prism_ctx*& ctx) {
+  use_device(ctx);
+  if (g_lane_depth++ > 0 || ctx->parent) return;
+  if (ctx->stream != ctx->own_stream) {
+    serial_ = std::unique_lock<std::mutex>(ctx->serial_mu);
+    return;
+  }
+  parent_ = ctx;
+  {
+    std::lock_guard<std::mutex> lock(ctx->lane_mu);
+    if (!ctx->idle_lanes.empty()) {
+      lane_ = ctx->idle_lanes.back();
+      ctx->idle_lanes.pop_back();
+    }
+  }
+  if (!lane_) {
+    try {
+      lane_ = new_lane(ctx);
+    } catch (...) {
+      --g_lane_depth;
+      throw;
+    }
+    std::lock_guard<std::mutex> lock(ctx->lane_mu);
+    ctx->lanes.push_back(lane_);
+  }
+  lane_->ktime_on = false;
+  ctx = lane_;
+}
+
+Lane::~Lane() {
+  if (g_lane_depth > 0) --g_lane_depth;
+  if (!lane_) return;
+  std::lock_guard<std::mutex> lock(parent_->lane_mu);
+  parent_->idle_lanes.push_back(lane_);
+}
+
+// Callers hold ctx->staging_mu while they use the buffer.
 void* pinned_staging(prism_ctx* ctx, size_t n) {
   if (n > ctx->pinned_bytes) {
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -172,6 +229,9 @@ prism_status prism_ctx_create(int device, prism_ctx** out) {
 void prism_ctx_destroy(prism_ctx* ctx) {
   if (!ctx) return;
   auto* c = as_ctx(ctx);
+  for (prism_ctx* l : c->lanes) prism_ctx_destroy(l);
+  c->lanes.clear();
+  c->idle_lanes.clear();
   (void)hipSetDevice(c->device);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   c->scratch.release();
@@ -205,7 +265,13 @@ prism_status prism_ctx_synchronize(prism_ctx* ctx) {
   });
 }
 
-uint64_t prism_ctx_device_bytes(prism_ctx* ctx) { return ctx ? ctx->device_bytes.load() : 0; }
+uint64_t prism_ctx_device_bytes(prism_ctx* ctx) {
+  if (!ctx) return 0;
+  uint64_t b = ctx->device_bytes.load();
+  std::lock_guard<std::mutex> lock(ctx->lane_mu);
+  for (prism_ctx* l : ctx->lanes) b += l->device_bytes.load();
+  return b;
+}
 
 prism_status prism_device_alloc(prism_ctx* ctx, uint64_t bytes, void** out) {
   return guarded([&] {

@@ -17,19 +17,28 @@ static dim3 ggrid(prism_ctx* ctx, uint64_t n) {
 
 __device__ __forceinline__ bool bitv(const uint8_t* bm, uint64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
 
+// Every take kernel checks its valid indices against the source length; an
+// index past it reads nothing, and sets *err (the host then fails the call).
 template <class T>
-__global__ void k_take_fixed(const T* __restrict__ src, int64_t src_off, const uint32_t* __restrict__ idx,
-                             const uint8_t* __restrict__ idx_valid, uint64_t n, T* __restrict__ dst) {
+__global__ void k_take_fixed(const T* __restrict__ src, int64_t src_off, uint64_t src_len,
+                             const uint32_t* __restrict__ idx, const uint8_t* __restrict__ idx_valid, uint64_t n,
+                             T* __restrict__ dst, uint32_t* __restrict__ err) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     bool ok = idx_valid == nullptr || bitv(idx_valid, i);
-    dst[i] = ok ? src[src_off + idx[i]] : T{};
+    const uint32_t r = ok ? idx[i] : 0;
+    if (ok && r >= src_len) {
+      *err = 1;
+      ok = false;
+    }
+    dst[i] = ok ? src[src_off + r] : T{};
   }
 }
 
 // Bit gather, one 64-row word per wave: out bit i = index valid && src bit.
 // src == nullptr means "every source row is set" (validity of a column without nulls).
-__global__ void k_take_bits(const uint8_t* __restrict__ src, int64_t src_off, const uint32_t* __restrict__ idx,
-                            const uint8_t* __restrict__ idx_valid, uint64_t n, unsigned long long* __restrict__ dst) {
+__global__ void k_take_bits(const uint8_t* __restrict__ src, int64_t src_off, uint64_t src_len,
+                            const uint32_t* __restrict__ idx, const uint8_t* __restrict__ idx_valid, uint64_t n,
+                            unsigned long long* __restrict__ dst, uint32_t* __restrict__ err) {
   const uint64_t n_words = ceil_div(n, 64);
   const uint64_t waves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) / 64;
   const uint64_t wave0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 64;
@@ -39,32 +48,44 @@ __global__ void k_take_bits(const uint8_t* __restrict__ src, int64_t src_off, co
     bool bit = false;
     if (i < n) {
       bool ok = idx_valid == nullptr || bitv(idx_valid, i);
-      bit = ok && (src == nullptr || bitv(src, src_off + idx[i]));
+      const uint32_t r = ok ? idx[i] : 0;
+      if (ok && r >= src_len) {
+        *err = 1;
+        ok = false;
+      }
+      bit = ok && (src == nullptr || bitv(src, src_off + r));
     }
     uint64_t word = __ballot(bit);
     if (lane == 0) dst[w] = word;
   }
 }
 
-__global__ void k_take_str_len(const int32_t* __restrict__ off, int64_t src_off, const uint32_t* __restrict__ idx,
-                               const uint8_t* __restrict__ idx_valid, uint64_t n, uint64_t* __restrict__ len) {
+// Offsets were checked non-decreasing when the column was uploaded, so each
+// length is >= 0 and the copy below stays inside the scanned extents.
+__global__ void k_take_str_len(const int32_t* __restrict__ off, int64_t src_off, uint64_t src_len,
+                               const uint32_t* __restrict__ idx, const uint8_t* __restrict__ idx_valid, uint64_t n,
+                               uint64_t* __restrict__ len, uint32_t* __restrict__ err) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     bool ok = idx_valid == nullptr || bitv(idx_valid, i);
-    uint64_t r = src_off + (ok ? idx[i] : 0);
+    const uint32_t ri = ok ? idx[i] : 0;
+    if (ok && ri >= src_len) {
+      *err = 1;
+      ok = false;
+    }
+    const uint64_t r = src_off + ri;
     len[i] = ok ? static_cast<uint64_t>(off[r + 1] - off[r]) : 0;
   }
 }
 
-__global__ void k_take_str_copy(const int32_t* __restrict__ off, int64_t src_off, const uint8_t* __restrict__ bytes,
-                                const uint32_t* __restrict__ idx, const uint8_t* __restrict__ idx_valid, uint64_t n,
-                                const uint64_t* __restrict__ start, int32_t* __restrict__ out_off,
-                                uint8_t* __restrict__ out_bytes) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
-    if (i == n) continue;
+__global__ void k_take_str_copy(const int32_t* __restrict__ off, int64_t src_off, uint64_t src_len,
+                                const uint8_t* __restrict__ bytes, const uint32_t* __restrict__ idx,
+                                const uint8_t* __restrict__ idx_valid, uint64_t n, const uint64_t* __restrict__ start,
+                                int32_t* __restrict__ out_off, uint8_t* __restrict__ out_bytes) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint64_t s = start[i];
     out_off[i] = static_cast<int32_t>(s);
     bool ok = idx_valid == nullptr || bitv(idx_valid, i);
-    if (!ok) continue;
+    if (!ok || idx[i] >= src_len) continue;
     uint64_t r = src_off + idx[i];
     const int32_t b = off[r], e = off[r + 1];
     for (int32_t k = b; k < e; ++k) out_bytes[s + (k - b)] = bytes[k];
@@ -141,33 +162,46 @@ DevColumn take(prism_ctx* ctx, const DevColumn& src, const uint32_t* idx, const 
   d.length = static_cast<int64_t>(n);
   d.dev_offset = 0;
   const dim3 g = ggrid(ctx, n), b(kGB);
+  const uint64_t sl = src.length > 0 ? static_cast<uint64_t>(src.length) : 0;
+  DeviceBuffer err(c, 4);
+  fill_bytes(ctx, err.as<uint8_t>(), 4, 0);
+  uint32_t* e = err.as<uint32_t>();
+  auto check_err = [&] {
+    uint32_t h = 0;
+    PRISM_HIP(hipMemcpyAsync(&h, e, 4, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+    if (h)
+      fail(PRISM_ERR_ARROW, "Arrow error: take index out of bounds (source length " + std::to_string(sl) + ")");
+  };
   switch (src.type) {
     case DType::Int64: case DType::Float64:
       d.data.alloc(c, n ? n * 8 : 8);
-      if (n) hipLaunchKernelGGL(k_take_fixed<uint64_t>, g, b, 0, st, src.data.as<uint64_t>(), src.dev_offset, idx,
-                                idx_valid, n, d.data.as<uint64_t>());
+      if (n) hipLaunchKernelGGL(k_take_fixed<uint64_t>, g, b, 0, st, src.data.as<uint64_t>(), src.dev_offset, sl, idx,
+                                idx_valid, n, d.data.as<uint64_t>(), e);
       break;
     case DType::Int32: case DType::Date32: case DType::UInt32:
       d.data.alloc(c, n ? n * 4 : 4);
-      if (n) hipLaunchKernelGGL(k_take_fixed<uint32_t>, g, b, 0, st, src.data.as<uint32_t>(), src.dev_offset, idx,
-                                idx_valid, n, d.data.as<uint32_t>());
+      if (n) hipLaunchKernelGGL(k_take_fixed<uint32_t>, g, b, 0, st, src.data.as<uint32_t>(), src.dev_offset, sl, idx,
+                                idx_valid, n, d.data.as<uint32_t>(), e);
       break;
     case DType::Decimal128:
       d.data.alloc(c, n ? n * 16 : 16);
-      if (n) hipLaunchKernelGGL(k_take_fixed<uint4>, g, b, 0, st, src.data.as<uint4>(), src.dev_offset, idx,
-                                idx_valid, n, d.data.as<uint4>());
+      if (n) hipLaunchKernelGGL(k_take_fixed<uint4>, g, b, 0, st, src.data.as<uint4>(), src.dev_offset, sl, idx,
+                                idx_valid, n, d.data.as<uint4>(), e);
       break;
     case DType::Bool:
       d.data.alloc(c, ceil_div(n ? n : 1, 64) * 8);
-      if (n) hipLaunchKernelGGL(k_take_bits, g, b, 0, st, src.data.as<uint8_t>(), src.dev_offset, idx, idx_valid, n,
-                                d.data.as<unsigned long long>());
+      if (n) hipLaunchKernelGGL(k_take_bits, g, b, 0, st, src.data.as<uint8_t>(), src.dev_offset, sl, idx, idx_valid,
+                                n, d.data.as<unsigned long long>(), e);
       break;
     case DType::Utf8: {
       DeviceBuffer len(c, (n ? n : 1) * 8), start(c, (n ? n : 1) * 8);
       d.offsets.alloc(c, (n + 1) * 4);
       if (n) {
-        hipLaunchKernelGGL(k_take_str_len, g, b, 0, st, src.offsets.as<int32_t>(), src.dev_offset, idx, idx_valid, n,
-                           len.as<uint64_t>());
+        hipLaunchKernelGGL(k_take_str_len, g, b, 0, st, src.offsets.as<int32_t>(), src.dev_offset, sl, idx, idx_valid,
+                           n, len.as<uint64_t>(), e);
+        PRISM_CHECK_LAUNCH();
+        check_err();
         size_t tmp = 0;
         PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, len.as<uint64_t>(), start.as<uint64_t>(),
                                                    static_cast<int>(n), st));
@@ -181,7 +215,7 @@ DevColumn take(prism_ctx* ctx, const DevColumn& src, const uint32_t* idx, const 
         const uint64_t total = last[0] + last[1];
         if (total > 0x7fffffffULL) fail(PRISM_ERR_ARROW, "Arrow error: Utf8 output exceeds i32 offsets");
         d.data.alloc(c, total ? total : 1);
-        hipLaunchKernelGGL(k_take_str_copy, g, b, 0, st, src.offsets.as<int32_t>(), src.dev_offset,
+        hipLaunchKernelGGL(k_take_str_copy, g, b, 0, st, src.offsets.as<int32_t>(), src.dev_offset, sl,
                            src.data.as<uint8_t>(), idx, idx_valid, n, start.as<uint64_t>(), d.offsets.as<int32_t>(),
                            d.data.as<uint8_t>());
         hipLaunchKernelGGL(k_set_last_offset, dim3(1), dim3(1), 0, st, d.offsets.as<int32_t>(), n,
@@ -198,10 +232,11 @@ DevColumn take(prism_ctx* ctx, const DevColumn& src, const uint32_t* idx, const 
   PRISM_CHECK_LAUNCH();
   if (src.validity.ptr || idx_valid) {
     d.validity.alloc(c, ceil_div(n ? n : 1, 64) * 8);
-    if (n) hipLaunchKernelGGL(k_take_bits, g, b, 0, st, src.validity.as<uint8_t>(), src.dev_offset, idx, idx_valid, n,
-                              d.validity.as<unsigned long long>());
+    if (n) hipLaunchKernelGGL(k_take_bits, g, b, 0, st, src.validity.as<uint8_t>(), src.dev_offset, sl, idx,
+                              idx_valid, n, d.validity.as<unsigned long long>(), e);
     PRISM_CHECK_LAUNCH();
   }
+  if (n) check_err();
   return d;
 }
 

@@ -6,7 +6,8 @@
 namespace prism {
 
 // out[i] = src[idx[i]] for i < n; a null index (bit clear in idx_valid) yields
-// a null output row.  The result has offset 0.
+// a null output row.  The result has offset 0.  A valid index >= src.length
+// fails with PRISM_ERR_ARROW (arrow's take refuses it too); synchronises.
 DevColumn take(prism_ctx* ctx, const DevColumn& src, const uint32_t* idx, const uint8_t* idx_valid, uint64_t n);
 
 // flags[r] = 1 for every r in idx[0..n) (flags has n_rows bytes, cleared first).

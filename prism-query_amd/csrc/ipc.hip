@@ -180,28 +180,39 @@ static void read_batch(const Table& rb, const uint8_t* body, uint64_t body_len, 
     const void* vp;
     int64_t vlen;
     buffer(&vp, &vlen);
+    // Lengths are untrusted: every size test divides the buffer length
+    // instead of multiplying the row count, so nothing can wrap.
+    auto bitmap_short = [len](int64_t blen) { return blen < len / 8 + (len % 8 != 0); };
     if (nulls > 0) {
-      if (vlen < (len + 7) / 8) bad("validity buffer is too short");
+      if (bitmap_short(vlen)) bad("validity buffer is too short");
       c.validity = static_cast<const uint8_t*>(vp);
     }
     const void* dp;
     int64_t dlen;
     if (c.type == DType::Utf8) {
       buffer(&dp, &dlen);
-      if (dlen < 4 * (len + 1)) bad("offsets buffer is too short");
+      if (dlen < 4 || dlen / 4 - 1 < len) bad("offsets buffer is too short");
       c.offsets = static_cast<const int32_t*>(dp);
-      int32_t first, last;
-      memcpy(&first, c.offsets, 4);
-      memcpy(&last, c.offsets + len, 4);
       const void* sp;
       int64_t slen;
       buffer(&sp, &slen);
-      if (first < 0 || last < first || last > slen) bad("string offsets lie outside the data buffer");
+      // every offset in [0, data length] and non-decreasing, so each string
+      // lies inside the data buffer
+      int32_t prev;
+      memcpy(&prev, c.offsets, 4);
+      if (prev < 0) bad("string offsets lie outside the data buffer");
+      for (int64_t i = 1; i <= len; ++i) {
+        int32_t cur;
+        memcpy(&cur, c.offsets + i, 4);
+        if (cur < prev) bad("string offsets decrease at row " + std::to_string(i - 1));
+        prev = cur;
+      }
+      if (prev > slen) bad("string offsets lie outside the data buffer");
       c.data = sp;
     } else {
       buffer(&dp, &dlen);
-      const int64_t need = c.type == DType::Bool ? (len + 7) / 8 : len * dtype_width(c.type);
-      if (dlen < need) bad("values buffer is too short");
+      const bool short_values = c.type == DType::Bool ? bitmap_short(dlen) : dlen / dtype_width(c.type) < len;
+      if (short_values) bad("values buffer is too short");
       c.data = dp;
     }
     b.cols.push_back(c);

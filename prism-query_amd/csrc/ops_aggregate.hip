@@ -8,6 +8,7 @@
 #include <cmath>
 #include <map>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "gather.hpp"
@@ -37,11 +38,14 @@ __device__ __forceinline__ uint64_t smix(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// (group id, DistinctValue word) for every non-null value (hash_aggregate.rs:457-481):
-// Float64 folds -0.0 into 0.0 and every NaN into one NaN; Utf8 values are
-// represented by a 64-bit fingerprint of their bytes.
-__global__ void k_distinct_pairs(KeyArg K, prism_column v, uint64_t n, unsigned long long* __restrict__ cursor,
-                                 uint64_t* __restrict__ g_out, uint64_t* __restrict__ v_out) {
+// (group id, DistinctValue word, source row) for every non-null value
+// (hash_aggregate.rs:457-481): Float64 folds -0.0 into 0.0 and every NaN into
+// one NaN; a Utf8 value is represented by a 64-bit fingerprint of its bytes
+// (masked by fp_mask), and runs of equal fingerprints are checked byte by
+// byte afterwards.  Source row = batch << 40 | row.
+__global__ void k_distinct_pairs(KeyArg K, prism_column v, uint64_t n, uint64_t batch, uint64_t fp_mask,
+                                 unsigned long long* __restrict__ cursor, uint64_t* __restrict__ v_out,
+                                 ulonglong2* __restrict__ gi_out) {
   for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
     if (!col_valid(v, r)) continue;
     uint64_t h = 0;
@@ -62,33 +66,87 @@ __global__ void k_distinct_pairs(KeyArg K, prism_column v, uint64_t n, unsigned 
         const uint8_t* s = static_cast<const uint8_t*>(v.data) + b;
         uint64_t x = smix(static_cast<uint64_t>(e - b) + 0x51ED27ULL);
         for (int32_t p = 0; p < e - b; ++p) x = smix(x ^ (static_cast<uint64_t>(s[p]) << ((p & 7) * 8)) ^ p);
-        w = x;
+        w = x & fp_mask;
         break;
       }
       default: w = static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(v.data)[i]));
     }
     unsigned long long p = atomicAdd(cursor, 1ULL);
-    g_out[p] = h;
     v_out[p] = w;
+    gi_out[p] = make_ulonglong2(h, (batch << 40) | r);
   }
 }
 
-__global__ void k_first_of_run(const uint64_t* __restrict__ g, const uint64_t* __restrict__ v, uint64_t n,
-                               uint32_t* __restrict__ flag) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    flag[i] = (i == 0 || g[i] != g[i - 1] || v[i] != v[i - 1]) ? 1u : 0u;
+// Between the two sort passes: group ids become the keys, (value, row) the payload.
+__global__ void k_distinct_regroup(const uint64_t* __restrict__ v, const ulonglong2* __restrict__ gi, uint64_t n,
+                                   uint64_t* __restrict__ g_out, ulonglong2* __restrict__ vi_out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const ulonglong2 x = gi[i];
+    g_out[i] = x.x;
+    vi_out[i] = make_ulonglong2(v[i], x.y);
+  }
 }
 
-// Distinct counts per group id for one aggregate column across all batches.
+// Byte equality of two Utf8 values named by source row (batch << 40 | row);
+// cols[b] = the device descriptor of batch b's value column.
+__device__ __forceinline__ bool str_eq(const prism_column* __restrict__ cols, uint64_t a, uint64_t b) {
+  const prism_column& A = cols[a >> 40];
+  const prism_column& B = cols[b >> 40];
+  const uint64_t ra = A.offset + (a & ((1ULL << 40) - 1)), rb = B.offset + (b & ((1ULL << 40) - 1));
+  const int32_t a0 = A.offsets[ra], a1 = A.offsets[ra + 1], b0 = B.offsets[rb], b1 = B.offsets[rb + 1];
+  if (a1 - a0 != b1 - b0) return false;
+  const uint8_t* x = static_cast<const uint8_t*>(A.data) + a0;
+  const uint8_t* y = static_cast<const uint8_t*>(B.data) + b0;
+  for (int32_t k = 0; k < a1 - a0; ++k)
+    if (x[k] != y[k]) return false;
+  return true;
+}
+
+// flag[i] = 1 at the first element of each run of equal (group, word).  For
+// Utf8 (check_bytes), neighbours inside a run whose bytes differ are a
+// fingerprint collision: *collisions counts them and the host recounts the
+// affected runs exactly.
+__global__ void k_first_of_run(const uint64_t* __restrict__ g, const ulonglong2* __restrict__ vi, uint64_t n,
+                               const prism_column* __restrict__ str_cols, uint32_t* __restrict__ flag,
+                               unsigned long long* __restrict__ collisions) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const bool first = i == 0 || g[i] != g[i - 1] || vi[i].x != vi[i - 1].x;
+    flag[i] = first ? 1u : 0u;
+    if (!first && str_cols && !str_eq(str_cols, vi[i - 1].y, vi[i].y)) atomicAdd(collisions, 1ULL);
+  }
+}
+
+static uint64_t distinct_fp_mask() {
+  // test hook, read at each call: PRISM_DISTINCT_FP_BITS=b keeps b bits of the
+  // Utf8 fingerprint, so that tests can force collisions through the exact recount
+  const char* e = getenv("PRISM_DISTINCT_FP_BITS");
+  const int b = e ? atoi(e) : 64;
+  return b <= 0 ? 0ULL : b >= 64 ? ~0ULL : (1ULL << b) - 1;
+}
+
+static std::string host_str(const HostColumn& c, int64_t row) {
+  const int64_t i = c.offset + row;
+  const int32_t b = c.offsets[i], e = c.offsets[i + 1];
+  return std::string(static_cast<const char*>(c.data) + b, static_cast<size_t>(e - b));
+}
+
+// Distinct counts per group id for one aggregate column across all batches:
+// sort (group, word) pairs, count runs per group.
 static std::unordered_map<uint64_t, int64_t> count_distinct(prism_ctx* ctx, const std::vector<HostBatch>& batches,
                                                             const std::vector<uint32_t>& group_by, uint32_t column) {
   Ctx* c = as_ctx(ctx);
   hipStream_t st = ctx->stream;
   uint64_t total = 0;
   for (const HostBatch& b : batches) total += static_cast<uint64_t>(b.n_rows);
-  DeviceBuffer g(c, (total ? total : 1) * 8), v(c, (total ? total : 1) * 8), cur(c, 8);
+  const bool utf8 = !batches.empty() && batches[0].cols[column].type == DType::Utf8;
+  DeviceBuffer v(c, (total ? total : 1) * 8), gi(c, (total ? total : 1) * 16), cur(c, 8);
   hipLaunchKernelGGL(k_fill_u64_zero, dim3(1), dim3(1), 0, st, cur.as<unsigned long long>());
-  for (const HostBatch& b : batches) {
+  std::vector<DevColumn> vcols(batches.size());  // Utf8: kept for the byte check
+  std::vector<prism_column> str_desc(batches.size());
+  memset(str_desc.data(), 0, str_desc.size() * sizeof(prism_column));
+  const uint64_t mask = utf8 ? distinct_fp_mask() : ~0ULL;
+  for (size_t bi = 0; bi < batches.size(); ++bi) {
+    const HostBatch& b = batches[bi];
     if (b.n_rows == 0) continue;
     std::vector<DevColumn> keys;
     KeyArg K;
@@ -98,48 +156,87 @@ static std::unordered_map<uint64_t, int64_t> count_distinct(prism_ctx* ctx, cons
       keys.push_back(upload(ctx, b.cols[group_by[k]]));
       K.c[k] = keys.back().desc();
     }
-    DevColumn vc = upload(ctx, b.cols[column]);
-    hipLaunchKernelGGL(k_distinct_pairs, agrid(ctx, b.n_rows), dim3(kB), 0, st, K, vc.desc(),
-                       static_cast<uint64_t>(b.n_rows), cur.as<unsigned long long>(), g.as<uint64_t>(),
-                       v.as<uint64_t>());
+    vcols[bi] = upload(ctx, b.cols[column]);
+    if (utf8) str_desc[bi] = vcols[bi].desc();
+    hipLaunchKernelGGL(k_distinct_pairs, agrid(ctx, b.n_rows), dim3(kB), 0, st, K, vcols[bi].desc(),
+                       static_cast<uint64_t>(b.n_rows), static_cast<uint64_t>(bi), mask,
+                       cur.as<unsigned long long>(), v.as<uint64_t>(), gi.as<ulonglong2>());
     PRISM_CHECK_LAUNCH();
     PRISM_HIP(hipStreamSynchronize(st));
+    if (!utf8) vcols[bi] = DevColumn();
   }
   unsigned long long n = 0;
   PRISM_HIP(hipMemcpyAsync(&n, cur.ptr, 8, hipMemcpyDeviceToHost, st));
   PRISM_HIP(hipStreamSynchronize(st));
   std::unordered_map<uint64_t, int64_t> out;
   if (n == 0) return out;
-  // LSD order: by value, then stably by group -> sorted by (group, value).
-  DeviceBuffer g2(c, n * 8), v2(c, n * 8), flag(c, n * 4), ug(c, n * 8), uc(c, n * 8), nr(c, 8);
+  // LSD order: by word, then stably by group -> sorted by (group, word),
+  // rows ascending inside a run.
+  DeviceBuffer v2(c, n * 8), gi2(c, n * 16), g3(c, n * 8), g4(c, n * 8), vi3(c, n * 16), vi4(c, n * 16);
+  DeviceBuffer flag(c, n * 4), ug(c, n * 8), uc(c, n * 8), nr(c, 8), coll(c, 8);
   size_t t1 = 0, t2 = 0, t3 = 0;
   const int ni = static_cast<int>(n);
-  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, v.as<uint64_t>(), v2.as<uint64_t>(), g.as<uint64_t>(),
-                                               g2.as<uint64_t>(), ni, 0, 64, st));
-  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, g2.as<uint64_t>(), g.as<uint64_t>(), v2.as<uint64_t>(),
-                                               v.as<uint64_t>(), ni, 0, 64, st));
-  PRISM_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, t3, g.as<uint64_t>(), ug.as<uint64_t>(), flag.as<uint32_t>(),
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, v.as<uint64_t>(), v2.as<uint64_t>(), gi.as<ulonglong2>(),
+                                               gi2.as<ulonglong2>(), ni, 0, 64, st));
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, g3.as<uint64_t>(), g4.as<uint64_t>(),
+                                               vi3.as<ulonglong2>(), vi4.as<ulonglong2>(), ni, 0, 64, st));
+  PRISM_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, t3, g4.as<uint64_t>(), ug.as<uint64_t>(), flag.as<uint32_t>(),
                                               uc.as<uint64_t>(), nr.as<uint64_t>(), hipcub::Sum(), ni, st));
   size_t tmax = t1 > t2 ? t1 : t2;
   tmax = tmax > t3 ? tmax : t3;
   DeviceBuffer tmp(c, tmax);
-  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, t1, v.as<uint64_t>(), v2.as<uint64_t>(), g.as<uint64_t>(),
-                                               g2.as<uint64_t>(), ni, 0, 64, st));
-  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, t2, g2.as<uint64_t>(), g.as<uint64_t>(), v2.as<uint64_t>(),
-                                               v.as<uint64_t>(), ni, 0, 64, st));
-  hipLaunchKernelGGL(k_first_of_run, agrid(ctx, n), dim3(kB), 0, st, g.as<uint64_t>(), v.as<uint64_t>(), n,
-                     flag.as<uint32_t>());
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, t1, v.as<uint64_t>(), v2.as<uint64_t>(), gi.as<ulonglong2>(),
+                                               gi2.as<ulonglong2>(), ni, 0, 64, st));
+  hipLaunchKernelGGL(k_distinct_regroup, agrid(ctx, n), dim3(kB), 0, st, v2.as<uint64_t>(), gi2.as<ulonglong2>(), n,
+                     g3.as<uint64_t>(), vi3.as<ulonglong2>());
   PRISM_CHECK_LAUNCH();
-  PRISM_HIP(hipcub::DeviceReduce::ReduceByKey(tmp.ptr, t3, g.as<uint64_t>(), ug.as<uint64_t>(), flag.as<uint32_t>(),
+  PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, t2, g3.as<uint64_t>(), g4.as<uint64_t>(),
+                                               vi3.as<ulonglong2>(), vi4.as<ulonglong2>(), ni, 0, 64, st));
+  hipLaunchKernelGGL(k_fill_u64_zero, dim3(1), dim3(1), 0, st, coll.as<unsigned long long>());
+  DeviceBuffer d_str(c, str_desc.size() * sizeof(prism_column));
+  if (utf8) {
+    PRISM_HIP(hipMemcpyAsync(d_str.ptr, str_desc.data(), str_desc.size() * sizeof(prism_column),
+                             hipMemcpyHostToDevice, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+  }
+  hipLaunchKernelGGL(k_first_of_run, agrid(ctx, n), dim3(kB), 0, st, g4.as<uint64_t>(), vi4.as<ulonglong2>(), n,
+                     utf8 ? d_str.as<prism_column>() : nullptr, flag.as<uint32_t>(), coll.as<unsigned long long>());
+  PRISM_CHECK_LAUNCH();
+  PRISM_HIP(hipcub::DeviceReduce::ReduceByKey(tmp.ptr, t3, g4.as<uint64_t>(), ug.as<uint64_t>(), flag.as<uint32_t>(),
                                               uc.as<uint64_t>(), nr.as<uint64_t>(), hipcub::Sum(), ni, st));
   uint64_t runs = 0;
+  unsigned long long collisions = 0;
   PRISM_HIP(hipMemcpyAsync(&runs, nr.ptr, 8, hipMemcpyDeviceToHost, st));
+  PRISM_HIP(hipMemcpyAsync(&collisions, coll.ptr, 8, hipMemcpyDeviceToHost, st));
   PRISM_HIP(hipStreamSynchronize(st));
   std::vector<uint64_t> hg(runs), hc(runs);
   PRISM_HIP(hipMemcpyAsync(hg.data(), ug.ptr, runs * 8, hipMemcpyDeviceToHost, st));
   PRISM_HIP(hipMemcpyAsync(hc.data(), uc.ptr, runs * 8, hipMemcpyDeviceToHost, st));
   PRISM_HIP(hipStreamSynchronize(st));
   for (uint64_t i = 0; i < runs; ++i) out[hg[i]] = static_cast<int64_t>(hc[i]);
+  if (collisions) {
+    // Exact recount of the runs whose fingerprints collide: each counted
+    // once above; add the further distinct strings it holds.
+    std::vector<uint64_t> hgr(n);
+    std::vector<ulonglong2> hvi(n);
+    PRISM_HIP(hipMemcpyAsync(hgr.data(), g4.ptr, n * 8, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipMemcpyAsync(hvi.data(), vi4.ptr, n * 16, hipMemcpyDeviceToHost, st));
+    PRISM_HIP(hipStreamSynchronize(st));
+    auto str_of = [&](uint64_t id) {
+      const HostBatch& b = batches[id >> 40];
+      return host_str(b.cols[column], static_cast<int64_t>(id & ((1ULL << 40) - 1)));
+    };
+    for (uint64_t i = 0; i < n;) {
+      uint64_t j = i + 1;
+      while (j < n && hgr[j] == hgr[i] && hvi[j].x == hvi[i].x) ++j;
+      if (j - i > 1) {
+        std::unordered_set<std::string> seen;
+        for (uint64_t k = i; k < j; ++k) seen.insert(str_of(hvi[k].y));
+        out[hgr[i]] += static_cast<int64_t>(seen.size()) - 1;
+      }
+      i = j;
+    }
+  }
   return out;
 }
 
@@ -381,7 +478,7 @@ prism_status prism_hash_aggregate_batches(prism_ctx* ctx, const struct ArrowArra
                                           struct ArrowArray* out, struct ArrowSchema* out_schema) {
   return guarded([&] {
     if (!ctx) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context");
-    use_device(ctx);
+    Lane lane(ctx);
     std::vector<HostBatch> bs;
     for (uint32_t i = 0; i < n_batches; ++i) bs.push_back(import_batch(batches[i], schema));
     std::vector<HostColumn> sch = import_schema(schema);
@@ -566,7 +663,7 @@ prism_status prism_merge_partial_aggregates(prism_ctx* ctx, const struct ArrowAr
     if (!ctx || !schema || !out || !out_schema) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL argument");
     if ((n_partials && !partials) || (n_original && !original))
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL partial batches or aggregate list");
-    use_device(ctx);
+    Lane lane(ctx);
     Expanded e;
     expand(original, n_original, e);
     std::vector<HostColumn> sch = import_schema(schema);

@@ -838,7 +838,7 @@ prism_status prism_evaluate_predicate(prism_ctx* ctx, const struct ArrowArray* b
                                       struct ArrowSchema* out_schema) {
   return guarded([&] {
     if (!ctx || !out_mask || !out_schema) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/output");
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch hb = import_batch(batch, schema);
     EvalOut o;
     DevBatch db;
@@ -853,7 +853,7 @@ prism_status prism_filter_batch(prism_ctx* ctx, const struct ArrowArray* batch, 
   return guarded([&] {
     if (!ctx || !mask || !mask_schema || !out || !out_schema)
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/mask/output");
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch hb = import_batch(batch, schema);
     // The mask arrives as a one-column batch view of the BooleanArray.
     ArrowSchema wrap_schema;
@@ -918,7 +918,7 @@ prism_status prism_filter_and_project(prism_ctx* ctx, const struct ArrowArray* b
   return guarded([&] {
     if (!ctx || !out || !out_schema || (n_indices && !indices))
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/indices/output");
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch hb = import_batch(batch, schema);
     std::vector<uint32_t> keep;
     if (n_indices == 0) {
@@ -951,7 +951,7 @@ prism_status prism_evaluate_scalar_expr(prism_ctx* ctx, const struct ArrowArray*
                                         int32_t root, struct ArrowArray* out, struct ArrowSchema* out_schema) {
   return guarded([&] {
     if (!ctx || !out || !out_schema) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/output");
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch hb = import_batch(batch, schema);
     export_array(eval_expr(ctx, hb, exprs, n_exprs, preds, n_preds, root, "expr"), out, out_schema);
   });
@@ -965,7 +965,7 @@ prism_status prism_project_batch_with_exprs(prism_ctx* ctx, const struct ArrowAr
   return guarded([&] {
     if (!ctx || !out || !out_schema || (n_indices && !indices) || (n_roots && !roots))
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/indices/roots/output");
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch hb = import_batch(batch, schema);
     std::vector<OutColumn> cols;
     for (uint32_t k = 0; k < n_indices; ++k) {
@@ -1016,7 +1016,7 @@ prism_status prism_string_like(prism_ctx* ctx, const struct ArrowArray* array, c
   return guarded([&] {
     if (!ctx || !array || !schema || !out || !out_schema)
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/array/output");
-    use_device(ctx);
+    Lane lane(ctx);
     ArrowSchema wrap_schema;
     memset(&wrap_schema, 0, sizeof(wrap_schema));
     wrap_schema.format = "+s";

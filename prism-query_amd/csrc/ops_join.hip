@@ -400,6 +400,15 @@ static void check_join_type(int32_t jt) {
     fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown join type " + std::to_string(jt));
 }
 
+// Left pads unmatched probe rows with build row 0 (hash_join.rs:472-477); with
+// no build rows that index does not exist and arrow's take would refuse it
+// (the reference panics there).  Refused before anything is launched.
+static void check_left_pad(int32_t jt, const HostBatch& build, int64_t probe_rows) {
+  if (jt == PRISM_JOIN_LEFT && build.n_rows == 0 && probe_rows > 0)
+    fail(PRISM_ERR_ARROW, "Arrow error: left join pads unmatched probe rows with build row 0, but the build side "
+                          "is empty (take index 0 out of bounds for length 0)");
+}
+
 }  // namespace ajoin
 }  // namespace prism
 
@@ -417,7 +426,7 @@ prism_status prism_hash_table_build(prism_ctx* ctx, const struct ArrowArray* bat
   return guarded([&] {
     if (!ctx || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or output");
     *out = nullptr;
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch b = import_batch(batch, schema);
     auto h = std::make_unique<prism_hash_table>();
     h->t = build_table(ctx, b, key_cols, n_keys);
@@ -431,7 +440,7 @@ prism_status prism_hash_table_probe(prism_ctx* ctx, prism_hash_table* table, con
                                     struct ArrowArray* out_build_idx, struct ArrowSchema* out_build_schema) {
   return guarded([&] {
     if (!ctx || !table) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or table");
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch p = import_batch(probe, probe_schema);
     Pairs m = probe_table(ctx, *table->t, p, probe_keys, n_keys);
     OutColumn a = index_array(ctx, m.p, m.n, "probe_indices");
@@ -449,9 +458,10 @@ prism_status prism_hash_join(prism_ctx* ctx, const struct ArrowArray* probe, con
   return guarded([&] {
     if (!ctx || !cfg) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or config");
     check_join_type(cfg->join_type);
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch p = import_batch(probe, probe_schema);
     HostBatch b = import_batch(build, build_schema);
+    check_left_pad(cfg->join_type, b, p.n_rows);
     std::unique_ptr<Table> T = build_table(ctx, b, cfg->build_keys, cfg->n_build_keys);
     join_one(ctx, cfg->join_type, *T, p, b, cfg->probe_keys, cfg->n_probe_keys, out, out_schema);
   });
@@ -466,7 +476,7 @@ prism_status prism_probe_matches_chunked(prism_ctx* ctx, const struct ArrowArray
   return guarded([&] {
     if (!ctx || (n_batches && (!probe_batches || !out_probe_idx || !out_build_idx)))
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL inputs/outputs");
-    use_device(ctx);
+    Lane lane(ctx);
     HostBatch b = import_batch(build, build_schema);
     std::unique_ptr<Table> T = build_table(ctx, b, build_keys, n_keys);
     std::vector<HostBatch> ps;
@@ -505,9 +515,16 @@ prism_status prism_hash_join_probe_chunked(prism_ctx* ctx, const struct ArrowArr
     if (!ctx || !cfg || !n_out || (n_batches && (!probe_batches || !out_batches)))
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL inputs/outputs");
     check_join_type(cfg->join_type);
-    use_device(ctx);
+    Lane lane(ctx);
     *n_out = 0;
     HostBatch b = import_batch(build, build_schema);
+    std::vector<HostBatch> ps;
+    int64_t probe_rows = 0;
+    for (uint32_t i = 0; i < n_batches; ++i) {
+      ps.push_back(import_batch(probe_batches[i], probe_schema));
+      probe_rows += ps.back().n_rows;
+    }
+    check_left_pad(cfg->join_type, b, probe_rows);
     std::unique_ptr<Table> T = build_table(ctx, b, cfg->build_keys, cfg->n_build_keys);
     // Right/Full cannot be chunked and fall back to per-batch inner (hash_join.rs:261-270).
     const int32_t jt = (cfg->join_type == PRISM_JOIN_RIGHT || cfg->join_type == PRISM_JOIN_FULL) ? PRISM_JOIN_INNER
@@ -516,9 +533,8 @@ prism_status prism_hash_join_probe_chunked(prism_ctx* ctx, const struct ArrowArr
     bool have_schema = false;
     try {
       for (uint32_t i = 0; i < n_batches; ++i) {
-        HostBatch p = import_batch(probe_batches[i], probe_schema);
         ArrowSchema s;
-        join_one(ctx, jt, *T, p, b, cfg->probe_keys, cfg->n_probe_keys, &out_batches[k], &s);
+        join_one(ctx, jt, *T, ps[i], b, cfg->probe_keys, cfg->n_probe_keys, &out_batches[k], &s);
         if (out_batches[k].length == 0) {  // empty outputs are dropped (hash_join.rs:274-280)
           out_batches[k].release(&out_batches[k]);
           s.release(&s);

@@ -112,11 +112,13 @@ def test_chunked_probe_and_join():
     for m, p in zip(ms, probes):
         op, ob = orc.join_probe(build, [0], p, [0])
         assert m.probe_indices.to_pylist() == op.tolist() and m.build_indices.to_pylist() == ob.tolist()
-    outs = hash_join_probe_chunked(probes, build, HashJoinConfig(JoinType.Inner, [0], [0]))
-    nonempty = [len(orc.join_probe(build, [0], p, [0])[0]) for p in probes]
-    assert [o.num_rows for o in outs] == [n for n in nonempty if n]
-    outs_r = hash_join_probe_chunked(probes, build, HashJoinConfig(JoinType.Right, [0], [0]))
-    assert [o.num_rows for o in outs_r] == [o.num_rows for o in outs]
+    for jt in JoinType:
+        outs = hash_join_probe_chunked(probes, build, HashJoinConfig(jt, [0], [0]))
+        ref = orc.hash_join_probe_chunked(probes, build, int(jt), [0], [0])
+        assert len(outs) == len(ref), jt
+        for o, r in zip(outs, ref):
+            assert o.schema.equals(r.schema), jt
+            assert o.equals(r), jt
 
 
 def test_key_type_mismatch_is_a_join_error():
@@ -208,3 +210,113 @@ def test_empty_input_gives_empty_result():
     r = hash_aggregate(b, HashAggConfig([0], [AggExpr(1, AggFunc.Sum, "s"), AggExpr(1, AggFunc.Count, "c")]))
     assert r.num_rows == 0 and r.schema.names == ["k", "s", "c"]
     assert r.schema.field(2).type == pa.int64()
+
+
+def random_sides(seed, n_b, n_p, domain_b, domain_p, nulls=True):
+    rng = np.random.default_rng(seed)
+    bk = pa.array(rng.integers(0, domain_b, n_b), mask=(rng.random(n_b) < 0.05) if nulls else None)
+    pk = pa.array(rng.integers(0, domain_p, n_p), mask=(rng.random(n_p) < 0.05) if nulls else None)
+    bs = pa.array([None if i % 11 == 0 else f"b{i}" for i in rng.integers(0, 100, n_b)], pa.string())
+    ps = pa.array([None if i % 13 == 0 else f"p{i}" for i in rng.integers(0, 100, n_p)], pa.string())
+    build = pa.record_batch([bk, bs, pa.array(rng.standard_normal(n_b))], names=["k", "s", "x"])
+    probe = pa.record_batch([pk, ps, pa.array(rng.integers(-5, 5, n_p).astype(np.int32))], names=["k", "s", "y"])
+    return build, probe
+
+
+@pytest.mark.parametrize("jt", list(JoinType))
+def test_every_join_type_matches_oracle_on_random_inputs(jt):
+    say("arrow join type vs oracle", jt.name)
+    for seed, (nb, npr, db, dp) in enumerate([(3000, 9000, 1500, 2500), (1, 500, 3, 3), (2000, 0, 10, 10)]):
+        build, probe = random_sides(seed, nb, npr, db, dp)
+        got = hash_join(probe, build, HashJoinConfig(jt, [0], [0]))
+        ref = orc.hash_join(probe, build, int(jt), [0], [0])
+        assert got.schema.equals(ref.schema), (jt, seed)
+        assert got.equals(ref), (jt, seed)
+    # general (multi-key, Utf8 + Int64) path
+    build, probe = random_sides(7, 2500, 6000, 40, 50)
+    got = hash_join(probe, build, HashJoinConfig(jt, [1, 0], [1, 0]))
+    ref = orc.hash_join(probe, build, int(jt), [1, 0], [1, 0])
+    assert got.equals(ref) and got.schema.equals(ref.schema)
+
+
+def test_left_join_with_empty_build_is_refused():
+    say("arrow left join, empty build")
+    build = pa.record_batch([pa.array([], pa.int64()), pa.array([], pa.string())], names=["k", "s"])
+    probe = pa.record_batch([pa.array([1, 2], pa.int64())], names=["k"])
+    with pytest.raises(PrismError) as e:
+        hash_join(probe, build, HashJoinConfig(JoinType.Left, [0], [0]))
+    assert e.value.kind == "Arrow"
+    with pytest.raises(PrismError):
+        hash_join_probe_chunked([probe], build, HashJoinConfig(JoinType.Left, [0], [0]))
+    # nothing to pad: an empty probe side is fine, and so are the other types
+    empty_probe = probe.slice(0, 0)
+    assert hash_join(empty_probe, build, HashJoinConfig(JoinType.Left, [0], [0])).num_rows == 0
+    assert hash_join(probe, build, HashJoinConfig(JoinType.Full, [0], [0])).num_rows == 2
+
+
+def test_count_distinct_float_zero_and_nan_folding():
+    say("arrow distinct float +-0 / NaN")
+    rng = np.random.default_rng(12)
+    n = 20_000
+    nan2 = np.frombuffer(np.uint64(0x7ff8dead00000001).tobytes(), np.float64)[0]
+    pool = np.array([0.0, -0.0, np.nan, nan2, 1.5, -1.5, 2.0**60, 5e-324])
+    vals = pool[rng.integers(0, len(pool), n)]
+    b = pa.record_batch([pa.array(rng.integers(0, 50, n)), pa.array(vals, mask=rng.random(n) < 0.1)],
+                        names=["k", "f"])
+    r = hash_aggregate(b, HashAggConfig([0], [AggExpr(1, AggFunc.CountDistinct, "d")]))
+    ref = orc.hash_aggregate([b], [0], [(1, "COUNT_DISTINCT")])
+    assert r.column(1).to_pylist() == ref["i64"][0].tolist()
+    g = hash_aggregate(b, HashAggConfig([], [AggExpr(1, AggFunc.CountDistinct, "d")]))
+    assert g.column(0).to_pylist() == [6]  # {0, NaN, 1.5, -1.5, 2^60, 5e-324}
+
+
+@pytest.mark.parametrize("fp_bits", [64, 3, 0])
+def test_utf8_count_distinct_is_exact_under_fingerprint_collisions(fp_bits, monkeypatch):
+    say("arrow distinct utf8, fingerprint bits", fp_bits)
+    monkeypatch.setenv("PRISM_DISTINCT_FP_BITS", str(fp_bits))
+    rng = np.random.default_rng(13)
+    words = [f"w{i}" for i in range(300)] + ["", "a\x00", "a"]
+    batches = []
+    for n in (4000, 0, 2500, 3000):
+        ws = [words[i] for i in rng.integers(0, len(words), n)]
+        batches.append(pa.record_batch([pa.array(rng.integers(0, 9, n).astype(np.int32)),
+                                        pa.array(ws, pa.string(), mask=rng.random(n) < 0.05)], names=["k", "s"]))
+    r = hash_aggregate_batches(batches, HashAggConfig([0], [AggExpr(1, AggFunc.CountDistinct, "d")]))
+    ref = orc.hash_aggregate(batches, [0], [(1, "COUNT_DISTINCT")])
+    assert r.column(1).to_pylist() == ref["i64"][0].tolist()
+
+
+def test_concurrent_callers_on_one_context():
+    """Four threads run joins and aggregations through one process-wide
+    context at once (the reference's executor calls operators from rayon
+    par_iter); every result equals the oracle's."""
+    import threading
+    say("arrow concurrent callers")
+    errors = []
+
+    def worker(t):
+        try:
+            for it in range(6):
+                seed = 100 * t + it
+                build, probe = random_sides(seed, 4000, 30_000, 3000, 5000, nulls=False)
+                got = hash_join(probe, build, HashJoinConfig(JoinType.Inner, [0], [0]))
+                assert got.equals(orc.hash_join(probe, build, 0, [0], [0])), ("join", seed)
+                rng = np.random.default_rng(seed)
+                n = 50_000
+                b = pa.record_batch([pa.array(rng.integers(0, 500, n)), pa.array(rng.integers(-10**9, 10**9, n))],
+                                    names=["k", "v"])
+                r = hash_aggregate(b, HashAggConfig([0], [AggExpr(1, AggFunc.Sum, "s"),
+                                                          AggExpr(1, AggFunc.Count, "c")]))
+                ref = orc.hash_aggregate([b], [0], [(1, "SUM"), (1, "COUNT")])
+                assert r.column(0).to_pylist() == [b.column(0)[i].as_py() for i in ref["first_row"]], ("agg", seed)
+                assert r.column(1).to_pylist() == ref["f64"][0].tolist(), ("agg", seed)
+                assert r.column(2).to_pylist() == ref["i64"][1].tolist(), ("agg", seed)
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors, errors

@@ -98,11 +98,6 @@ def test_utf8_int32_keys_nulls_and_f64_values(ctx):
     assert np.array_equal(out["i64"][1], ref["i64"][1])
     assert np.array_equal(out["f64"][2], ref["f64"][2]) and np.array_equal(out["f64"][3], ref["f64"][3])
     # f64 sums: order-dependent; compare with the exactly rounded sum per group
-    groups = {}
-    hashes = [orc.group_hash(rb, [0, 1], r) for r in range(n)] if False else None
-    gid = np.searchsorted(np.array(ref["key_hash"]), 0)  # placeholder to keep numpy import used
-    for i, fr in enumerate(ref["first_row"]):
-        groups[int(ref["key_hash"][i])] = i
     ob = orc.Batch.from_arrow(rb)
     kh = np.array([orc.group_hash(ob, [0, 1], r) for r in range(n)], np.uint64)
     for i, h in enumerate(ref["key_hash"]):

@@ -3,6 +3,7 @@ with arrow's StreamReader): the library's decoder against pyarrow's writer, on
 the CPU (the host form needs no device)."""
 import ctypes
 import datetime
+import struct
 
 import numpy as np
 import pyarrow as pa
@@ -84,3 +85,29 @@ def test_truncated_and_compressed_streams_fail():
         pytest.skip("this pyarrow build writes no compressed IPC")
     st, msg = status_of(comp)
     assert st != 0 and "compression" in msg
+
+
+def forge(data: bytes, old: bytes, new: bytes) -> bytes:
+    assert data.count(old) >= 1
+    return data.replace(old, new)
+
+
+def test_forged_row_count_is_refused_without_overflow():
+    # 77 rows of one Int64 column without nulls; the batch length and the
+    # field node length both become 2^61, whose byte size wraps to 0 in 64 bits
+    b = pa.record_batch([pa.array(np.arange(1000, 1077, dtype=np.int64))], names=["k"])
+    data = stream([b])
+    forged = forge(data, struct.pack("<q", 77), struct.pack("<q", 1 << 61))
+    st, msg = status_of(forged)
+    assert st == 1 and "too short" in msg, msg  # PRISM_ERR_ARROW
+
+
+def test_forged_utf8_offsets_are_refused():
+    b = pa.record_batch([pa.array(["ab", "cd", "ef"])], names=["s"])
+    data = stream([b])
+    forged = forge(data, struct.pack("<4i", 0, 2, 4, 6), struct.pack("<4i", 0, 4, 2, 6))
+    st, msg = status_of(forged)
+    assert st == 1 and "decrease" in msg, msg
+    forged = forge(data, struct.pack("<4i", 0, 2, 4, 6), struct.pack("<4i", -4, 2, 4, 6))
+    st, msg = status_of(forged)
+    assert st == 1 and "outside" in msg, msg

@@ -94,9 +94,6 @@ def test_groupby_multibatch_chunk_merge_keeps_first_appearance():
         glob = [7_000 * fb + fr for fb, fr in zip(r["first_batch"], r["first_row"])]
         _, first = np.unique(d["key"], return_index=True)
         assert glob == sorted(first.tolist())
-        exp = {int(k): int(s) for k, s in zip(*np.unique(d["key"], return_counts=False, return_inverse=False),
-                                              )} if False else None
-        sums = np.bincount(d["key"], weights=None, minlength=1000)
         assert sum(r["sum128"][0]) == int(d["value"].sum())
 
 
@@ -223,3 +220,89 @@ def test_eager_join_agg_against_materialized_pairs():
     for a, v in zip(attr[bi].tolist(), pay[pi].tolist()):
         exp_s[a] += v
     assert sums == exp_s
+
+
+def _brute_pairs(pk, bk):
+    """Nested-loop equi-join in the reference's emission order (probe row
+    ascending, build row ascending)."""
+    return [(p, b) for p in range(len(pk)) for b in range(len(bk)) if pk[p] == bk[b]]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_join_types_against_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    nb, npr = int(rng.integers(0, 40)), int(rng.integers(0, 60))
+    bk, pk = rng.integers(0, 25, nb), rng.integers(0, 30, npr)
+    build = pa.record_batch([pa.array(bk), pa.array(np.arange(nb) * 10)], names=["k", "bv"])
+    probe = pa.record_batch([pa.array(pk), pa.array(np.arange(npr) * 7)], names=["k", "pv"])
+    pairs = _brute_pairs(pk.tolist(), bk.tolist())
+    mp = {p for p, _ in pairs}
+    mb = {b for _, b in pairs}
+    up = [p for p in range(npr) if p not in mp]
+    ub = [b for b in range(nb) if b not in mb]
+
+    def rows(r):
+        return list(zip(*[r.column(i).to_pylist() for i in range(r.num_columns)])) if r.num_rows else []
+
+    def expect(seq):
+        return [(None if p is None else int(pk[p]), None if p is None else 7 * p,
+                 None if b is None else int(bk[b]), None if b is None else 10 * b) for p, b in seq]
+
+    assert rows(orc.hash_join(probe, build, orc.INNER, [0], [0])) == expect(pairs)
+    if nb:  # Left pads with build row 0, which an empty build side lacks
+        assert rows(orc.hash_join(probe, build, orc.LEFT, [0], [0])) == expect(pairs + [(p, 0) for p in up])
+    assert rows(orc.hash_join(probe, build, orc.RIGHT, [0], [0])) == expect(pairs + [(None, b) for b in ub])
+    assert rows(orc.hash_join(probe, build, orc.FULL, [0], [0])) == expect(
+        pairs + [(p, None) for p in up] + [(None, b) for b in ub])
+    semi = orc.hash_join(probe, build, orc.LEFT_SEMI, [0], [0])
+    anti = orc.hash_join(probe, build, orc.LEFT_ANTI, [0], [0])
+    assert semi.column(1).to_pylist() == [7 * p for p in sorted(mp)]
+    assert anti.column(1).to_pylist() == [7 * p for p in up]
+    # Arrow C++'s outer joins hold the same rows (in another order)
+    tp, tb = pa.table({"k": pk, "p": np.arange(npr)}), pa.table({"k2": bk, "b": np.arange(nb)})
+    if npr and nb:
+        j = tp.join(tb, "k", "k2", join_type="full outer", use_threads=False)
+        ref = sorted(zip(j.column("p").to_pylist(), j.column("b").to_pylist()), key=str)
+        full = orc.hash_join(probe, build, orc.FULL, [0], [0])
+        got = [(None if v is None else v // 7, None if w is None else w // 10)
+               for v, w in zip(full.column(1).to_pylist(), full.column(3).to_pylist())]
+        assert sorted(got, key=str) == ref
+
+
+def test_join_chunked_drops_empty_and_runs_right_as_inner():
+    build = pa.record_batch([pa.array([1, 2, 2], pa.int64())], names=["k"])
+    probes = [pa.record_batch([pa.array(v, pa.int64())], names=["k"]) for v in ([2, 9], [9], [1])]
+    inner = orc.hash_join_probe_chunked(probes, build, orc.INNER, [0], [0])
+    assert [r.num_rows for r in inner] == [2, 1]
+    right = orc.hash_join_probe_chunked(probes, build, orc.RIGHT, [0], [0])
+    assert [r.to_pydict() for r in right] == [r.to_pydict() for r in inner]
+    left = orc.hash_join_probe_chunked(probes, build, orc.LEFT, [0], [0])
+    assert [r.column(1).to_pylist() for r in left] == [[2, 2, 1], [1], [1]]  # row 0 pads: key 1
+
+
+def _canon(v):
+    """DistinctValue of a Float64 (hash_aggregate.rs:466-474): +0 for -0, one NaN."""
+    if v != v:
+        return "nan"
+    return 0.0 if v == 0.0 else v
+
+
+def test_count_distinct_float_and_utf8_against_python_sets():
+    rng = np.random.default_rng(3)
+    n = 3000
+    pool = [0.0, -0.0, float("nan"), float(np.frombuffer(np.uint64(0x7ff8dead00000001).tobytes(), np.float64)[0]),
+            1.5, -1.5, 2.0**60, 5e-324]
+    vals = [pool[i] for i in rng.integers(0, len(pool), n)]
+    null = rng.random(n) < 0.1
+    words = ["", "a", "b", "ab", "ba", "a\x00", "abcdefghij", "abcdefghik"]
+    ss = [words[i] for i in rng.integers(0, len(words), n)]
+    keys = rng.integers(0, 7, n)
+    rb = pa.record_batch([pa.array(keys), pa.array(vals, mask=null), pa.array(ss, mask=rng.random(n) < 0.1)],
+                         names=["k", "f", "s"])
+    r = orc.hash_aggregate([rb], [0], [(1, "COUNT_DISTINCT"), (2, "COUNT_DISTINCT")])
+    fl, sl = rb.column(1).to_pylist(), rb.column(2).to_pylist()
+    for gi, fr in enumerate(r["first_row"]):
+        g = keys[fr]
+        rows = np.nonzero(keys == g)[0]
+        assert r["i64"][0][gi] == len({_canon(fl[i]) for i in rows if fl[i] is not None})
+        assert r["i64"][1][gi] == len({sl[i] for i in rows if sl[i] is not None})
