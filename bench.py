@@ -294,15 +294,19 @@ def bench_parquet(ctx, n: int, steps: int) -> dict:
             "pyarrow_read_table_ms": pa_wall * 1e3}
 
 
-def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
+def bench_service(ctx, n_requests: int, rates, seed: int = 7, conc_threads: int = 8) -> dict:
     """Served request stream: Poisson arrivals, rows log-uniform in [4096, 16M];
-    50% group-by (config 0 and config 2 shapes), 40% probes against 8 resident
-    build tables of 1M..32M unique keys, 10% LIKE '%abc%' filters over a Utf8
-    column of 8..64 random lowercase bytes.  Inputs are device-resident (slices
-    of generated pools).  Each mode runs the same schedule: "served" submits to
-    the operator service (continuous batching, persistent tile-queue kernels,
-    graph replay), "plain" runs each request with the one-request-per-launch
-    operators in arrival order.  Latency = result on the host - arrival."""
+    50% group-by (1M-row aggregation and fused filter + group-by shapes), 40%
+    probes against 8 resident build tables of 1M..32M unique keys, each
+    returning its (probe row, build row) pairs, 10% LIKE '%abc%' filters over
+    a Utf8 column of 8..64 random lowercase bytes, each returning its bitmap.
+    Inputs are device-resident (slices of generated pools); every request
+    writes its outputs to a region of its own.  The same schedule runs as
+    "served" (the operator service: continuous batching, persistent tile-queue
+    kernels, graph replay), "plain" (one request per launch sequence, in
+    arrival order, one thread and stream) and, at saturation, "plain_concurrent"
+    (the one-request-per-launch operators from `conc_threads` threads, each with
+    its own context and stream).  Latency = result on the host - arrival."""
     import threading
 
     import numpy as np
@@ -315,6 +319,7 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
     lib = ctx.lib
     S0 = dg.SEED
     NP = 16_000_000 + 4096
+    DOMAIN = 64_000_000
     I64, I32, UTF8 = L.TYPE_INT64, L.TYPE_INT32, L.TYPE_UTF8
     k0, v0 = D.empty(ctx, NP, np.int64), D.empty(ctx, NP, np.int64)
     L.check(lib.prism_datagen_uniform_i64(ctx.handle, k0.ptr, NP, S0, 1, 0, 1000))
@@ -328,7 +333,7 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
     L.check(lib.prism_datagen_flag_utf8(ctx.handle, fa.ptr, fao.ptr, NP, S0, 25, dg.FLAG_A, len(dg.FLAG_A)))
     L.check(lib.prism_datagen_flag_utf8(ctx.handle, fb.ptr, fbo.ptr, NP, S0, 26, dg.FLAG_B, len(dg.FLAG_B)))
     pk, pp = D.empty(ctx, NP, np.int64), D.empty(ctx, NP, np.int64)
-    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.ptr, NP, S0, 40, 0, 64_000_000))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.ptr, NP, S0, 40, 0, DOMAIN))
     L.check(lib.prism_datagen_bits_i64(ctx.handle, pp.ptr, NP, S0, 41))
     sb, so = D.empty(ctx, NP * 64, np.uint8), D.empty(ctx, NP + 1, np.int32)
     tb = ctypes.c_uint64()
@@ -361,39 +366,101 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
 
     rng = np.random.default_rng(seed)
     reqs = []
+    pair_total = mask_total = 0
     for _ in range(n_requests):
         n = int(np.exp(rng.uniform(np.log(4096), np.log(16_000_000))))
         off = int(rng.integers(0, NP - n))
         u = rng.random()
         kind = "gb0" if u < 0.25 else "gb2" if u < 0.5 else "probe" if u < 0.9 else "filter"
-        reqs.append((kind, n, off, int(rng.integers(0, len(sizes)))))
+        t = int(rng.integers(0, len(sizes)))
+        # output region of this request: pairs (room for 1.25x the expected
+        # matches; the result counts every match) or bitmap words
+        if kind == "probe":
+            cap = int(n * sizes[t] * 1e6 / DOMAIN * 1.25) + 4096
+            reqs.append((kind, n, off, t, pair_total, cap))
+            pair_total += cap
+        elif kind == "filter":
+            words = (n + 63) // 64
+            reqs.append((kind, n, off, t, mask_total, words))
+            mask_total += words
+        else:
+            reqs.append((kind, n, off, t, 0, 0))
     total_rows = sum(r[1] for r in reqs)
     gap = rng.exponential(1.0, n_requests)
+    out_p, out_b = D.empty(ctx, max(pair_total, 1), np.uint32), D.empty(ctx, max(pair_total, 1), np.uint32)
+    masks = D.empty(ctx, max(mask_total, 1), np.uint64)
 
-    # plain: device tables built once (resident, like the service's)
-    plain_tabs = [D.JoinTable(ctx, kk, payload=pl) for kk, pl in tables]
+    plain_tabs = [D.JoinTable(ctx, kk, payload=pl) for kk, pl in tables]  # resident, like the service's
+
+    class Lane:
+        """A plain caller: its own context (stream) and probe result buffer."""
+
+        def __init__(self, c):
+            self.ctx = c
+            self.res = D.empty(c, ctypes.sizeof(L.ProbeResult), np.uint8)
+
+        def run(self, req):
+            kind, n, off, t, o, cap = req
+            c = self.ctx
+            if kind in ("gb0", "gb2"):
+                g = D.GroupBy(c, expected_groups=1000 if kind == "gb0" else 8, **(GB0 if kind == "gb0" else GB2))
+                g.update(gb0_cols(off) if kind == "gb0" else gb2_cols(off), n, 0)
+                k = g.finish()
+                out = g.read(k)
+                g.close()
+                return (k, tuple(out["sum128"][0]), tuple(out["first_row"].tolist()))
+            if kind == "probe":
+                tab = plain_tabs[t]
+                L.check(lib.prism_join_probe_i64(c.handle, tab.handle, pk.ptr + 8 * off, n, 0, pp.ptr + 8 * off,
+                                                 None, out_p.ptr + 4 * o, out_b.ptr + 4 * o, cap, self.res.ptr))
+                c.sync()
+                r = L.ProbeResult.from_buffer_copy(self.res.numpy().tobytes())
+                return (int(r.matches), r.probe_sum, r.build_sum)
+            sel = filter_mask(c, LIKE, [UTF8], str_cols(off), n, masks[o:o + cap], strings=[b"%abc%"])
+            return (sel,)
+
+    main_lane = Lane(ctx)
 
     def run_plain(arrivals):
-        lat = []
+        lat, res = [], []
         t0 = time.perf_counter()
-        for (kind, n, off, t), a in zip(reqs, arrivals):
+        for req, a in zip(reqs, arrivals):
             now = time.perf_counter() - t0
             if now < a:
                 time.sleep(a - now)
-            if kind in ("gb0", "gb2"):
-                g = D.GroupBy(ctx, expected_groups=1000 if kind == "gb0" else 8, **(GB0 if kind == "gb0" else GB2))
-                g.update(gb0_cols(off) if kind == "gb0" else gb2_cols(off), n, 0)
-                k = g.finish()
-                g.read(k)
-                g.close()
-            elif kind == "probe":
-                plain_tabs[t].probe(pk[off:off + n], pp[off:off + n], None, want_pairs=False)
-            else:
-                filter_mask(ctx, LIKE, [UTF8], str_cols(off), n, None, strings=[b"%abc%"])
+            res.append(main_lane.run(req))
             lat.append((time.perf_counter() - t0 - a) * 1e3)
-        return lat, time.perf_counter() - t0
+        return lat, time.perf_counter() - t0, res
 
-    S = Service(ctx, hbm_budget=96 << 30)
+    lanes = [Lane(D.Context(0)) for _ in range(conc_threads)]
+
+    def run_plain_concurrent(arrivals):
+        lat = [0.0] * len(reqs)
+        res = [None] * len(reqs)
+        errors = []
+        t0 = time.perf_counter()
+
+        def worker(j):
+            try:
+                for i in range(j, len(reqs), conc_threads):
+                    now = time.perf_counter() - t0
+                    if now < arrivals[i]:
+                        time.sleep(arrivals[i] - now)
+                    res[i] = lanes[j].run(reqs[i])
+                    lat[i] = (time.perf_counter() - t0 - arrivals[i]) * 1e3
+            except Exception as e:  # reported below
+                errors.append(repr(e))
+
+        th = [threading.Thread(target=worker, args=(j,)) for j in range(conc_threads)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        if errors:
+            raise RuntimeError(errors[0])
+        return lat, time.perf_counter() - t0, res
+
+    S = Service(ctx, hbm_budget=96 << 30, arena_bytes=1 << 30)
     for i, (kk, pl) in enumerate(tables):
         S.register_table(i, kk.numpy(), pl.numpy())
     S.register_groupby(0, max_groups=1000, **GB0)
@@ -406,7 +473,7 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
         t0 = time.perf_counter()
 
         def submitter():
-            for i, ((kind, n, off, t), a) in enumerate(zip(reqs, arrivals)):
+            for i, ((kind, n, off, t, o, cap), a) in enumerate(zip(reqs, arrivals)):
                 now = time.perf_counter() - t0
                 if now < a:
                     time.sleep(a - now)
@@ -416,41 +483,65 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
                 elif kind == "gb2":
                     tickets[i] = S.submit_groupby(2, gb2_cols(off), n)
                 elif kind == "probe":
-                    tickets[i] = S.submit_probe(t, pk[off:off + n], pp[off:off + n])
+                    tickets[i] = S.submit_probe(t, pk[off:off + n], pp[off:off + n], probe_idx=out_p[o:o + cap],
+                                                build_idx=out_b[o:o + cap], capacity=cap)
                 else:
-                    tickets[i] = S.submit_filter(4, str_cols(off), n)
+                    tickets[i] = S.submit_filter(4, str_cols(off), n, masks[o:o + cap])
 
         th = threading.Thread(target=submitter)
         th.start()
         th.join()
         S.drain()
         wall = time.perf_counter() - t0
-        lat = []
+        lat, res = [], []
         for i, tk in enumerate(tickets):
             r = S.wait(tk)
             if r.status != 0:
                 raise RuntimeError(f"served request {i} failed with status {r.status}")
             lat.append(delay[i] + r.latency_ms)
+            kind = reqs[i][0]
+            if kind in ("gb0", "gb2"):
+                g = S.read_groups(tk, 0 if kind == "gb0" else 2, r.count)
+                res.append((int(r.count), tuple(g["sum128"][0]), tuple(g["first_row"].tolist())))
+            elif kind == "probe":
+                res.append((int(r.count), r.probe_sum, r.build_sum))
+            else:
+                res.append((int(r.count),))
             S.release(tk)
-        return lat, wall
+        return lat, wall, res
 
-    def summary(lat, wall):
+    def summary(lat, wall, res=None):
         a = np.asarray(lat)
         return {"rows_per_s": total_rows / wall, "requests_per_s": len(reqs) / wall, "p50_ms": float(np.percentile(a, 50)),
                 "p99_ms": float(np.percentile(a, 99)), "wall_s": wall}
 
-    run_served(np.zeros(len(reqs)))  # warm-up: builds tables, captures graphs, tunes tiles
-    # warm-up of the plain operators: the first request of each plan compiles
-    # its generated kernels
+    # warm-up: builds tables, captures graphs, tunes tiles, compiles the
+    # generated group-by kernels of both forms
+    _, _, served_res = run_served(np.zeros(len(reqs)))
     for kind, n in (("gb0", 4096), ("gb2", 4096)):
-        g = D.GroupBy(ctx, expected_groups=1000 if kind == "gb0" else 8, **(GB0 if kind == "gb0" else GB2))
-        g.update(gb0_cols(0) if kind == "gb0" else gb2_cols(0), n, 0)
-        g.read(g.finish())
-        g.close()
+        for ln in [main_lane] + lanes:
+            ln.run((kind, n, 0, 0, 0, 0))
+    # parity: every served result equals the plain operator's on the same input
+    _, _, plain_res = run_plain(np.zeros(len(reqs)))
+    mismatches = sum(1 for a, b in zip(served_res, plain_res) if a != b)
+    truncated = sum(1 for r, q in zip(served_res, reqs) if q[0] == "probe" and r[0] > q[5])
+    # closed form of a sample of probes: table t's keys are a permutation of
+    # 0..m-1, so key k matches iff k < m; its pairs in probe order
+    hk = pk.numpy()
+    checked = bad = 0
+    for i in [i for i, q in enumerate(reqs) if q[0] == "probe"][:40]:
+        kind, n, off, t, o, cap = reqs[i]
+        hit = np.nonzero(hk[off:off + n] < sizes[t] * 1_000_000)[0]
+        got = out_p[o:o + min(cap, len(hit))].numpy() if len(hit) else np.zeros(0, np.uint32)
+        bad += int(served_res[i][0] != len(hit) or not np.array_equal(got, hit[:len(got)].astype(np.uint32)))
+        checked += 1
     out = {"workload": f"{n_requests} requests, rows log-uniform [4096, 16M] (total {total_rows}), 50% group-by "
-                       "(config 0 / config 2 shapes), 40% probes of 8 resident tables (1M..32M unique keys), 10% "
-                       "LIKE '%abc%' over 8..64-byte lowercase Utf8; inputs resident in HBM",
-           "rates": {}}
+                       "(1M-row aggregation / fused filter + group-by shapes), 40% probes of 8 resident tables "
+                       "(1M..32M unique keys) returning their (probe row, build row) pairs, 10% LIKE '%abc%' over "
+                       "8..64-byte lowercase Utf8 returning the bitmap; inputs resident in HBM",
+           "served_equals_plain": mismatches == 0, "mismatched_requests": mismatches,
+           "probe_pairs_truncated": truncated, "probe_closed_form_checked": checked, "probe_closed_form_bad": bad,
+           "plain_concurrent_threads": conc_threads, "rates": {}}
     for rate in rates:
         arr = np.cumsum(gap) / rate
         arr -= arr[0]
@@ -458,22 +549,30 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7) -> dict:
         served = summary(*run_served(arr))
         served["service"] = {k: v for k, v in S.stats().items() if k.startswith("mean_") or k.startswith("max_")}
         out["rates"][str(rate)] = {"served": served, "plain": summary(*run_plain(arr))}
+        print(f"[bench] served stream at {rate} req/s done", file=sys.stderr, flush=True)
     # saturation: every request submitted at once; three runs of each form,
     # the run with the median served / plain ratio reported
     z = np.zeros(len(reqs))
     runs = []
     for _ in range(3):
-        sv, pl = summary(*run_served(z)), summary(*run_plain(z))
-        runs.append((sv["rows_per_s"] / pl["rows_per_s"], sv, pl))
+        sv, pl, pc = summary(*run_served(z)), summary(*run_plain(z)), summary(*run_plain_concurrent(z))
+        runs.append((sv["rows_per_s"] / pl["rows_per_s"], sv, pl, pc))
     runs.sort(key=lambda x: x[0])
-    sat = {"served": runs[1][1], "plain": runs[1][2], "throughput_ratio": runs[1][0],
+    mid = runs[1]
+    sat = {"served": mid[1], "plain": mid[2], "plain_concurrent": mid[3], "throughput_ratio": mid[0],
+           "throughput_ratio_vs_concurrent": mid[1]["rows_per_s"] / mid[3]["rows_per_s"],
            "throughput_ratios_all_runs": [r[0] for r in runs]}
     out["saturation"] = sat
     out["service_stats"] = S.stats()
     S.close()
     for t in plain_tabs:
         t.close()
-    for a in [k0, v0, date, price, disc, fa, fao, fb, fbo, pk, pp, sb, so] + [x for tp in tables for x in tp]:
+    for ln in lanes:
+        ln.res.free()
+        ln.ctx.close()
+    main_lane.res.free()
+    for a in [k0, v0, date, price, disc, fa, fao, fb, fbo, pk, pp, sb, so, out_p, out_b, masks] + \
+            [x for tp in tables for x in tp]:
         a.free()
     return out
 
@@ -494,7 +593,7 @@ def main() -> int:
     ap.add_argument("--no-service", action="store_true")
     ap.add_argument("--no-parquet", action="store_true")
     ap.add_argument("--parquet-rows", type=int, default=20_000_000)
-    ap.add_argument("--svc-requests", type=int, default=1000)
+    ap.add_argument("--svc-requests", type=int, default=10000)
     ap.add_argument("--svc-rates", type=str, default="200,500,1000")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="also run D2D copies of this many bytes")
     args = ap.parse_args()

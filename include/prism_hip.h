@@ -580,7 +580,7 @@ typedef struct {
 } prism_service_config;
 
 #define PRISM_REQ_GROUPBY 1     /* target = group-by plan id */
-#define PRISM_REQ_PROBE 2       /* target = table id: matches + int128 payload sums */
+#define PRISM_REQ_PROBE 2       /* target = table id: matches + int128 payload sums (+ pairs) */
 #define PRISM_REQ_FILTER 3      /* target = filter plan id: selected rows (+ optional bitmap) */
 typedef struct {
   int32_t kind;
@@ -590,6 +590,13 @@ typedef struct {
   const int64_t* probe_payload;   /* PROBE, may be NULL */
   uint64_t n_rows;
   unsigned long long* out_mask;   /* FILTER: ceil(n_rows / 64) device words, may be NULL */
+  /* PROBE, optional: HashTable::probe's output (hash_join.rs:143-200), the
+   * matched (probe row, build row) pairs in probe order, as
+   * prism_join_probe_i64 writes them; at most pair_capacity pairs are written
+   * (the result's count holds every match).  NULL = counts and sums only. */
+  uint32_t* probe_idx;
+  uint32_t* build_idx;
+  uint64_t pair_capacity;
 } prism_request;
 
 typedef struct {

@@ -82,6 +82,14 @@ struct ProbeDesc {
   const int64_t* pay;
   uint64_t n;
   unsigned long long* acc;  // kAccCopies x {matches, probe sum lo/hi, build sum lo/hi}
+  // pairs (optional): each tile writes its (probe row, build row) pairs in
+  // probe order to stage[tile start ...]; a scan over the request's tile
+  // counts and a move put them at their final offsets in out_p / out_b
+  uint2* stage;
+  uint32_t* out_p;
+  uint32_t* out_b;
+  uint64_t cap;
+  uint32_t t_first, t_count;  // the request's tiles in the batch's tile array
 };
 
 struct FiltDesc {
@@ -112,8 +120,13 @@ struct Control {
   ProbeDesc pr[kMaxReq];
   FiltDesc fd[kMaxReq];
   uint32_t gb_idx[kMaxReq];
+  uint32_t n_pair, pad2;
+  uint32_t pair_req[kMaxReq];  // probe descriptors that want pairs
   prism_column cols[kMaxReq][kMaxCols];
   Tile tiles[kMaxTiles];
+  // written on the device only (not part of the host -> device copy)
+  uint32_t tile_cnt[kMaxTiles];  // pairs of each probe tile
+  uint64_t tile_off[kMaxTiles];  // their first output offset
 };
 
 // ------------------------------------------------------------------ kernels
@@ -194,9 +207,14 @@ __device__ __forceinline__ void svc_probe_flush(unsigned long long* acc_base, un
 
 template <int KIND>
 __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
+  constexpr int U = 8;  // independent lookups in flight per thread
+  constexpr int kW = kSB / 64;
   __shared__ uint32_t s_t;
   __shared__ unsigned long long s_m[kSB / 64];
   __shared__ __int128 s_p[kSB / 64], s_b[kSB / 64];
+  __shared__ uint32_t s_cnt[U * kW], s_pre[U * kW], s_tot;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
   Queue* q = &C->q[v];
   const uint32_t n_tiles = q->n_tiles;
   uint32_t cur = 0xFFFFFFFFu;  // request whose sums the registers hold (uniform per workgroup)
@@ -213,7 +231,66 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
       if (cur != 0xFFFFFFFFu) svc_probe_flush(C->pr[cur].acc, m, ps, bs, s_m, s_p, s_b);
       cur = tl.req;
     }
-    constexpr int U = 8;  // independent lookups in flight per thread
+    if (d.stage) {
+      // Pairs: the tile's matches in row order.  Row c0 + u * kSB + tid has
+      // rank (u, wave, lane) in the chunk; per-(u, wave) ballot counts are
+      // scanned by one wave, so each hit knows its place.
+      uint32_t run = 0;
+      for (uint64_t c0 = tl.start; c0 < tl.end; c0 += static_cast<uint64_t>(kSB) * U) {
+        int64_t key[U], x[U];
+        bool hit[U];
+        uint32_t brow[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint64_t r = c0 + static_cast<uint64_t>(u) * kSB + threadIdx.x;
+          hit[u] = r < tl.end;
+          key[u] = hit[u] ? __builtin_nontemporal_load(d.keys + r) : 0;
+          x[u] = (hit[u] && d.pay) ? __builtin_nontemporal_load(d.pay + r) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          uint32_t val = 0, cnt;
+          int32_t pay = 0;
+          hit[u] = hit[u] && join::lookup<KIND, true>(d.T, key[u], val, cnt, pay);
+          brow[u] = val;
+          if (hit[u]) {
+            ++m;
+            ps += x[u];
+            if (d.T.has_payload) bs += pay;
+          }
+        }
+        unsigned long long bal[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) bal[u] = __ballot(hit[u]);
+        if (lane == 0) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) s_cnt[u * kW + wave] = static_cast<uint32_t>(__popcll(bal[u]));
+        }
+        __syncthreads();
+        if (wave == 0) {
+          const uint32_t own = lane < U * kW ? s_cnt[lane] : 0u;
+          uint32_t incl = own;
+          for (int k = 1; k < 64; k <<= 1) {
+            const uint32_t o = __shfl_up(incl, k, 64);
+            if (lane >= k) incl += o;
+          }
+          if (lane < U * kW) s_pre[lane] = incl - own;
+          if (lane == 63) s_tot = incl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (hit[u]) {
+            const uint32_t pos = run + s_pre[u * kW + wave] + static_cast<uint32_t>(__popcll(bal[u] & lt));
+            const uint64_t r = c0 + static_cast<uint64_t>(u) * kSB + threadIdx.x;
+            d.stage[tl.start + pos] = make_uint2(static_cast<uint32_t>(r), brow[u]);
+          }
+        run += s_tot;
+        __syncthreads();  // s_pre / s_tot are rewritten by the next chunk
+      }
+      if (threadIdx.x == 0) C->tile_cnt[q->offset + t] = run;
+      continue;
+    }
     for (uint64_t r0 = tl.start + threadIdx.x; r0 < tl.end; r0 += static_cast<uint64_t>(kSB) * U) {
       int64_t key[U], x[U];
       bool in[U];
@@ -238,6 +315,47 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
     }
   }
   if (cur != 0xFFFFFFFFu) svc_probe_flush(C->pr[cur].acc, m, ps, bs, s_m, s_p, s_b);
+}
+
+// Exclusive scan of each pairs request's tile counts (one workgroup per request).
+__global__ __launch_bounds__(kSB) void k_svc_pair_scan(Control* C) {
+  using BS = hipcub::BlockScan<uint64_t, kSB>;
+  __shared__ typename BS::TempStorage tmp;
+  const uint32_t n_pair = C->n_pair;
+  for (uint32_t k = blockIdx.x; k < n_pair; k += gridDim.x) {
+    const ProbeDesc& d = C->pr[C->pair_req[k]];
+    uint64_t carry = 0;
+    for (uint32_t b = 0; b < d.t_count; b += kSB) {
+      const uint32_t i = b + threadIdx.x;
+      const uint64_t v = i < d.t_count ? C->tile_cnt[d.t_first + i] : 0;
+      uint64_t ex, agg;
+      BS(tmp).ExclusiveSum(v, ex, agg);
+      if (i < d.t_count) C->tile_off[d.t_first + i] = carry + ex;
+      carry += agg;
+      __syncthreads();
+    }
+  }
+}
+
+// Staged pairs to their final offsets (the first `cap` pairs of a request).
+// The probe queues' tiles are adjacent in the tile array.
+__global__ __launch_bounds__(kSB) void k_svc_pair_move(Control* C) {
+  const uint32_t g0 = C->q[V_PROBE_DENSE].offset;
+  const uint32_t g1 = C->q[V_PROBE_HASHED].offset + C->q[V_PROBE_HASHED].n_tiles;
+  for (uint32_t g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
+    const Tile tl = C->tiles[g];
+    const ProbeDesc& d = C->pr[tl.req];
+    if (!d.stage) continue;
+    const uint32_t cnt = C->tile_cnt[g];
+    const uint64_t off = C->tile_off[g];
+    for (uint32_t i = threadIdx.x; i < cnt; i += kSB) {
+      const uint64_t o = off + i;
+      if (o >= d.cap) break;
+      const uint2 p = d.stage[tl.start + i];
+      d.out_p[o] = p.x;
+      d.out_b[o] = p.y;
+    }
+  }
 }
 
 // Filter tiles start on 64-row boundaries: every wave step covers whole words.
@@ -529,6 +647,7 @@ static void footprint(const Req* R, size_t& res_words, size_t& table_words) {
     }
     case PRISM_REQ_PROBE:
       res_words = 5 * kAccCopies;
+      if (R->r.probe_idx) table_words = R->r.n_rows + 32;  // staged pairs, 8 B per row
       break;
     default:
       res_words = kAccCopies + 1;
@@ -606,6 +725,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
   unsigned long long* tab_base = res_base + kResultsBytes / 8;
   size_t res_used = 0, tab_used = 0;
   uint32_t n_gb = 0, n_pr = 0, n_fd = 0;
+  std::vector<int> pair_variant;  // probe queue of each pairs request
   for (size_t i = 0; i < sl.reqs.size(); ++i) {
     Req* R = sl.reqs[i];
     size_t rw, tw;
@@ -685,13 +805,27 @@ static void launch_batch(prism_service* S, Slot& sl) {
         d.pay = R->r.probe_payload;
         d.n = R->r.n_rows;
         d.acc = R->d_out;
+        d.stage = nullptr;
+        d.out_p = d.out_b = nullptr;
+        d.cap = 0;
         InitDesc& a = H->init[H->n_init++];
         a.dst = R->d_out;
         a.n_words = 5 * kAccCopies;
         const int v = R->table->t->info.kind == PRISM_TABLE_DENSE ? V_PROBE_DENSE : V_PROBE_HASHED;
         const uint64_t tr = std::max<uint64_t>(kSB, tile_of[K_PROBE]);
+        d.t_first = static_cast<uint32_t>(vt[v].size());  // local to the queue until offsets are known
         for (uint64_t s0 = 0; s0 < R->r.n_rows; s0 += tr)
           vt[v].push_back(Tile{k, 0, s0, std::min<uint64_t>(R->r.n_rows, s0 + tr)});
+        d.t_count = static_cast<uint32_t>(vt[v].size()) - d.t_first;
+        if (R->r.probe_idx && R->r.n_rows) {
+          d.stage = reinterpret_cast<uint2*>(tab_base + tab_used);
+          tab_used += tw;
+          d.out_p = R->r.probe_idx;
+          d.out_b = R->r.build_idx;
+          d.cap = R->r.pair_capacity;
+          pair_variant.push_back(v);
+          H->pair_req[H->n_pair++] = k;
+        }
         vrows[v] += R->r.n_rows;
         break;
       }
@@ -753,6 +887,12 @@ static void launch_batch(prism_service* S, Slot& sl) {
                                                    std::max<uint64_t>(static_cast<uint64_t>(n_cu) * bpc,
                                                                       gb::kQueueLanes)));
   }
+  for (uint32_t i = 0; i < H->n_pair; ++i)  // tile indices of pairs requests, now global
+    H->pr[H->pair_req[i]].t_first += H->q[pair_variant[i]].offset;
+  const int pair_grid = H->n_pair ? static_cast<int>(pow2_grid(H->n_pair)) : 0;
+  const uint32_t probe_tiles = H->q[V_PROBE_DENSE].n_tiles + H->q[V_PROBE_HASHED].n_tiles;
+  const int move_grid =
+      H->n_pair ? static_cast<int>(std::min<uint64_t>(pow2_grid(probe_tiles), static_cast<uint64_t>(n_cu) * 8)) : 0;
   const size_t ctl_bytes = offsetof(Control, tiles) + off * sizeof(Tile);
   Control* dC = sl.control.as<Control>();
   hipStream_t st = sl.st;
@@ -800,11 +940,17 @@ static void launch_batch(prism_service* S, Slot& sl) {
       }
     }
     if (H->n_gb) hipLaunchKernelGGL(k_svc_gb_finish, dim3(fin_grid), dim3(gb::kFinishThreads), 0, st, dC);
+    if (pair_grid) {
+      hipLaunchKernelGGL(k_svc_pair_scan, dim3(pair_grid), dim3(kSB), 0, st, dC);
+      hipLaunchKernelGGL(k_svc_pair_move, dim3(move_grid), dim3(kSB), 0, st, dC);
+    }
   };
   // A launch sequence is captured into a graph the second time its shape
   // comes (a shape met once would pay capture + instantiation for nothing).
-  const std::string key = S->cfg.use_graphs && !timed ? shape_key(grid, lds, init_grid, fin_grid, jf, jgrid, jlds)
-                                                     : std::string();
+  const std::string key = S->cfg.use_graphs && !timed
+                              ? shape_key(grid, lds, init_grid, fin_grid, jf, jgrid, jlds) + "," +
+                                    std::to_string(pair_grid) + "," + std::to_string(move_grid)
+                              : std::string();
   const bool recurring = !key.empty() && (sl.graphs.count(key) || ++sl.shapes_seen[key] >= 2);
   if (recurring) {
     auto it = sl.graphs.find(key);
@@ -1008,14 +1154,30 @@ static void dispatcher(prism_service* S) {
           progress = true;
         }
       } catch (const std::exception& e) {
+        // End a capture the throw interrupted and let whatever was queued on
+        // the slot's stream finish before the requests' buffers go back to
+        // their callers.
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(sl.st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(sl.st, &g);
+          if (g) (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(sl.st);
+        const Error* pe = dynamic_cast<const Error*>(&e);
+        const prism_status code = pe ? pe->status : PRISM_ERR_INTERNAL;
         for (Req* R : sl.reqs) {
-          R->res.status = PRISM_ERR_EXECUTION;
+          R->res.status = code;
+          R->res.kind = R->r.kind;
           R->done = true;
           if (R->table) --R->table->pins;
           ++S->completed;
         }
         sl.reqs.clear();
-        S->fatal = e.what();
+        // only a failed HIP call stops the service; a batch the host refused
+        // (validation) fails alone
+        if (!pe || pe->status == PRISM_ERR_EXECUTION) S->fatal = e.what();
         S->cv_done.notify_all();
       }
     }
@@ -1235,6 +1397,8 @@ prism_status prism_service_submit(prism_service* s, const prism_request* req, ui
         if (!s->tables.count(req->target)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown table");
         if (req->n_rows && !req->probe_keys) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL probe keys");
         if (req->n_rows >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: probe rows exceed u32 row indices");
+        if ((req->probe_idx == nullptr) != (req->build_idx == nullptr))
+          fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: give both pair outputs or neither");
         break;
       case PRISM_REQ_FILTER: {
         auto it = s->filt_plans.find(req->target);

@@ -166,7 +166,8 @@ class ServiceConfig(Structure):
 
 class Request(Structure):
     _fields_ = [("kind", c_int32), ("target", c_uint32), ("columns", POINTER(Column)), ("probe_keys", c_void_p),
-                ("probe_payload", c_void_p), ("n_rows", c_uint64), ("out_mask", c_void_p)]
+                ("probe_payload", c_void_p), ("n_rows", c_uint64), ("out_mask", c_void_p), ("probe_idx", c_void_p),
+                ("build_idx", c_void_p), ("pair_capacity", c_uint64)]
 
 
 class RequestResult(Structure):

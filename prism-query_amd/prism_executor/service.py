@@ -59,9 +59,15 @@ class Service:
         cols = (L.Column * max(len(columns), 1))(*columns)
         return self._submit(L.Request(L.REQ["GROUPBY"], plan_id, cols, None, None, n_rows, None))
 
-    def submit_probe(self, table_id: int, keys, payload=None, n_rows: Optional[int] = None) -> int:
+    def submit_probe(self, table_id: int, keys, payload=None, n_rows: Optional[int] = None, probe_idx=None,
+                     build_idx=None, capacity: Optional[int] = None) -> int:
+        """HashTable::probe against a resident table: counts and exact payload
+        sums, and with probe_idx / build_idx (device u32 arrays) the matched
+        pairs in probe order, at most `capacity` of them (default: n_rows)."""
         n = keys.numel() if n_rows is None else n_rows
-        return self._submit(L.Request(L.REQ["PROBE"], table_id, None, ptr(keys), ptr(payload), n, None))
+        cap = n if capacity is None else capacity
+        return self._submit(L.Request(L.REQ["PROBE"], table_id, None, ptr(keys), ptr(payload), n, None,
+                                      ptr(probe_idx), ptr(build_idx), cap if probe_idx is not None else 0))
 
     def submit_filter(self, plan_id: int, columns: Sequence[L.Column], n_rows: int, out_mask=None) -> int:
         cols = (L.Column * max(len(columns), 1))(*columns)

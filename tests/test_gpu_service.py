@@ -187,14 +187,22 @@ def test_graph_replays_match_eager(ctx, world):
     for i, (k, p) in enumerate(tabs):
         S.register_table(i, k, p)
     got = []
-    for kind, n, off, t in reqs:
-        tk = S.submit_probe(t, P.pk[off:off + n], P.pp[off:off + n])
+    op, ob = D.empty(ctx, 6000, np.uint32), D.empty(ctx, 6000, np.uint32)
+    for i, (kind, n, off, t) in enumerate(reqs):
+        pairs = i % 2 == 1  # every other request also writes its pairs
+        tk = S.submit_probe(t, P.pk[off:off + n], P.pp[off:off + n], probe_idx=op if pairs else None,
+                            build_idx=ob if pairs else None, capacity=6000)
         r = S.wait(tk)
         got.append((int(r.count), r.probe_sum, r.build_sum))
         ctx.sync()  # eager work of this process between replays
-        _, _, rr = tabs_dev[0].probe(P.pk[off:off + n], P.pp[off:off + n], None, want_pairs=False)
+        pi, bi, rr = tabs_dev[0].probe(P.pk[off:off + n], P.pp[off:off + n], None, want_pairs=True)
         assert got[-1] == (int(rr.matches), rr.probe_sum, rr.build_sum)
+        if pairs:
+            m = int(rr.matches)
+            assert np.array_equal(op.numpy()[:m], pi.numpy()) and np.array_equal(ob.numpy()[:m], bi.numpy())
         S.release(tk)
+    op.free()
+    ob.free()
     assert S.stats()["graph_launches"] >= 20
     S.close()
 
