@@ -12,9 +12,16 @@ Prints one JSON line.  The roofline block covers the dominant kernel, the
 probe's lookup/compaction pass ("join_probe"): algorithmic bytes per launch =
 16 B per probe row (key + payload read) + 8 B per matched pair (two u32
 indices written), divided by that kernel's average duration, measured with
-HIP events that the library records on the launch stream around each launch.
-The CPU baseline times the oracle's restatement of the reference join on a
-bounded sample (1M build rows, 8M probe rows) on one host core.
+HIP events that the library records on the launch stream around each launch;
+roofline_whole_probe divides the same bytes by the whole probe call (lookup,
+tile scan and the move of the pairs to their final offsets).
+The CPU baseline times the restatement of the reference's CPU path
+(oracle/ref_cpu.c: its 64-partition build and batch-parallel probe) on the
+1M build rows and the first 16 batches of 1,048,576 probe rows, at every host
+core (at most 16) and at one core, 2 warm-ups + the median of 5 runs each.
+The other sections (1M-row aggregation, fused filter + group-by with its
+float64 variant, skewed join -> aggregation, Parquet scan, served request
+stream) carry their own roofline and CPU baseline.
 """
 import argparse
 import ctypes
@@ -42,34 +49,110 @@ def progress(section: str, t_start: float) -> None:
     print(f"[bench] {section} done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nb: int, sample: int) -> dict:
+CPU_LABEL = ("C restatement of the reference CPU path (oracle/ref_cpu.c): 64-partition HashMap<u64, Vec<u32>> "
+             "build, probe parallel over 1,048,576-row batches, aggregation in per-thread chunks merged in order")
+
+
+def _ref_cpu():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
+    import ref_cpu
+    return ref_cpu
 
-    import oracle as orc
-    from prism_executor import datagen as dg
 
-    d = dg.join_inputs(nb, sample)
-    b = orc.Batch([orc.Col(orc.INT64, d["build_key"])])
-    p = orc.Batch([orc.Col(orc.INT64, d["probe_key"])])
-    t0 = time.perf_counter()
-    pi, bi = orc.join_probe(b, [0], p, [0])
-    dt = time.perf_counter() - t0
-    assert len(pi) == int((d["probe_key"] < nb).sum())
-    return {"value": sample / dt, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"oracle C restatement of HashTable::build+probe, {nb} build rows + {sample} probe rows "
-                      f"(first {sample} rows of the same seeded probe stream), one thread, {dt:.2f} s"}
+def cpu_protocol(fn, rows: int, unit: str, sample: str, extra: dict = None) -> dict:
+    """Time fn(threads) at every host core and at one core: 2 warm-ups, then
+    the median of 5 runs each (the reference's scaling bench, scaling.rs:77-93)."""
+    rc = _ref_cpu()
+    t_all = rc.host_threads()
+    med_all, runs_all = rc.timed(lambda: fn(t_all))
+    med_one, runs_one = rc.timed(lambda: fn(1))
+    out = {"value": rows / med_all, "unit": unit, "cores": t_all, "kind": "port", "sample": sample,
+           "value_1core": rows / med_one, "median_s": med_all, "median_s_1core": med_one,
+           "runs_s": runs_all, "runs_s_1core": runs_one, "nproc": os.cpu_count(), "cpu_model": rc.cpu_model(),
+           "protocol": "2 warm-ups + median of 5, at every host core (at most 16) and at 1 core",
+           "label": CPU_LABEL}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def cpu_baseline(h_build, h_probe) -> dict:
+    """Config 1 on the CPU: HashTable::build over the 1M build keys, then
+    probe_matches_chunked over the first 16 batches of the probe stream."""
+    rc = _ref_cpu()
+    n_probe = len(h_probe)
+    res = {}
+
+    def run(threads):
+        t = rc.JoinTable(h_build, threads)
+        res["pairs"] = t.probe_chunked(h_probe, threads)[0]
+        t.close()
+
+    out = cpu_protocol(run, n_probe, "rows/s",
+                       f"build {len(h_build)} rows + probe the first {n_probe} rows of the same seeded probe stream "
+                       f"({(n_probe + rc.BATCH_ROWS - 1) // rc.BATCH_ROWS} batches of {rc.BATCH_ROWS} rows)")
+    out["pairs"] = res["pairs"]
+    return out
 
 
 FILTER_LE_2470 = [("LOAD_I", 0), ("CONST_I", 0, 2470), ("CMP_I", 3)]
 REVENUE_EXACT = [("LOAD_I", 1), ("CONST_I", 0, 100), ("LOAD_I", 2), ("SUB_I",), ("MUL_I",)]
 
 
+REVENUE_F64 = [("LOAD_I", 1), ("I2F",), ("CONST_F", 0, 0, 0.01), ("MUL_F",), ("CONST_F", 0, 0, 1.0), ("LOAD_I", 2),
+               ("I2F",), ("CONST_F", 0, 0, 0.01), ("MUL_F",), ("SUB_F",), ("MUL_F",)]
+
+
+def groupby_host_check(date, price, disc, fa, fb, n: int, chunk: int = 8_000_000) -> dict:
+    """Per group (flag A, flag B) over the rows with date <= 2470, from the
+    device columns copied back chunk by chunk: the exact integer SUM of
+    price * (100 - discount) and COUNT (per-chunk f64 bincounts stay below
+    2^53, so they are exact), the reference's own SUM of that product in f64,
+    added left to right in row order (hash_aggregate.rs:135-137, one batch),
+    and the SUM of price * 0.01 * (1 - discount * 0.01) in long double (the
+    f64 variant's check value)."""
+    import numpy as np
+
+    exact = [0] * 6
+    count = [0] * 6
+    l2r = [0.0] * 6
+    frac = [np.longdouble(0)] * 6
+    amap = np.zeros(256, np.int64)
+    bmap = np.zeros(256, np.int64)
+    for i, c in enumerate(b"ANR"):
+        amap[c] = i
+    for i, c in enumerate(b"FO"):
+        bmap[c] = i
+    for lo in range(0, n, chunk):
+        hi = min(n, lo + chunk)
+        keep = date[lo:hi].numpy() <= 2470
+        p = price[lo:hi].numpy()[keep]
+        d = disc[lo:hi].numpy()[keep]
+        gid = amap[fa[lo:hi].numpy()[keep]] * 2 + bmap[fb[lo:hi].numpy()[keep]]
+        rev = p * (100 - d)
+        s = np.bincount(gid, weights=rev.astype(np.float64), minlength=6)
+        c = np.bincount(gid, minlength=6)
+        order = np.argsort(gid, kind="stable")
+        seg = np.searchsorted(gid[order], np.arange(7))
+        rev_f = rev.astype(np.float64)[order]
+        fv = ((p.astype(np.float64) * 0.01) * (1.0 - d.astype(np.float64) * 0.01))[order]
+        for g in range(6):
+            exact[g] += int(s[g])
+            count[g] += int(c[g])
+            part = rev_f[seg[g]:seg[g + 1]]
+            if len(part):
+                l2r[g] = float(np.cumsum(np.concatenate([[l2r[g]], part]))[-1])
+            frac[g] += np.sum(fv[seg[g]:seg[g + 1]], dtype=np.longdouble)
+    return {"exact": exact, "count": count, "l2r": l2r, "frac": [float(x) for x in frac]}
+
+
 def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
     """Fused filter + group-by: date int32 <= 2470, group by (flag A, flag B)
-    (one-byte Utf8), SUM(price * (100 - discount)) exact int128, COUNT, AVG.
-    Algorithmic bytes: 4 (date) + 8 (price) + 8 (discount) + 2 x (4 offset + 1
-    byte) (flags) = 30 B per row."""
+    (one-byte Utf8), SUM(price * (100 - discount)) exact int128, COUNT, AVG;
+    and the float64 variant, SUM(price * 0.01 * (1 - discount * 0.01)), COUNT,
+    AVG in f64.  Algorithmic bytes: 4 (date) + 8 (price) + 8 (discount) + 2 x
+    (4 offset + 1 byte) (flags) = 30 B per row.  Both results are checked at
+    full size against host sums of the same columns."""
     import numpy as np
 
     from prism_executor import _lib as L
@@ -91,57 +174,144 @@ def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
     ctx.sync()
     types = [L.TYPE_INT32, L.TYPE_INT64, L.TYPE_INT64, L.TYPE_UTF8, L.TYPE_UTF8]
     cols = [D.col_i32(date), D.col_i64(price), D.col_i64(disc), D.col_utf8(fa, fa_off), D.col_utf8(fb, fb_off)]
-    g = D.GroupBy(ctx, types, [3, 4], [REVENUE_EXACT], [("SUM", 0), ("COUNT", 0), ("AVG", 0)],
-                  filter_prog=FILTER_LE_2470, expected_groups=8)
-    for _ in range(2):
-        g.reset()
-        g.update(cols, n, 0)
-        g.finish()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        g.reset()
-        g.update(cols, n, 0)
-        k = g.finish()
-    wall = (time.perf_counter() - t0) / steps
-    out = g.read(k)
-    ctx.kernel_timing(True)
-    for _ in range(steps):
-        L.check(lib.prism_groupby_update(g.handle, (L.Column * len(cols))(*cols), n, 0))
-    tot, cnt = ctx.kernel_time("groupby_update")
-    ctx.kernel_timing(False)
-    upd_ms = tot / max(cnt, 1)
+
+    def run(value_prog):
+        g = D.GroupBy(ctx, types, [3, 4], [value_prog], [("SUM", 0), ("COUNT", 0), ("AVG", 0)],
+                      filter_prog=FILTER_LE_2470, expected_groups=8)
+        for _ in range(2):
+            g.reset()
+            g.update(cols, n, 0)
+            g.finish()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.reset()
+            g.update(cols, n, 0)
+            k = g.finish()
+        wall = (time.perf_counter() - t0) / steps
+        out = g.read(k)
+        ctx.kernel_timing(True)
+        for _ in range(steps):
+            L.check(lib.prism_groupby_update(g.handle, (L.Column * len(cols))(*cols), n, 0))
+        tot, cnt = ctx.kernel_time("groupby_update")
+        ctx.kernel_timing(False)
+        g.close()
+        upd_ms = tot / max(cnt, 1)
+        ach = n * 30 / (upd_ms * 1e-3) / 1e9
+        return out, k, wall, upd_ms, {"bound": "hbm", "kernel": "groupby_update", "achieved": ach,
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+
+    out, k, wall, upd_ms, roof = run(REVENUE_EXACT)
+    roof["traffic"] = TRAFFIC.get("groupby_update") if n == DEFAULT_SIZES[2] else None
+    out_f, k_f, wall_f, upd_f, roof_f = run(REVENUE_F64)
+    hc = groupby_host_check(date, price, disc, fa, fb, n)
+    # groups come in first-appearance order; the host sums are indexed by
+    # (flag A, flag B) -> map each device group through its first row
+    hfa, hfb = fa.numpy(), fb.numpy()
+    gidx = ["ANR".index(chr(hfa[r])) * 2 + "FO".index(chr(hfb[r])) for r in out["first_row"].tolist()]
+    gidx_f = ["ANR".index(chr(hfa[r])) * 2 + "FO".index(chr(hfb[r])) for r in out_f["first_row"].tolist()]
+    int_ok = (sorted(gidx) == list(range(6)) and
+              all(out["sum128"][0][i] == hc["exact"][g] for i, g in enumerate(gidx)) and
+              all(int(out["i64"][1][i]) == hc["count"][g] for i, g in enumerate(gidx)))
+    f64_err = max(abs(out_f["f64"][0][i] - hc["frac"][g]) / abs(hc["frac"][g]) for i, g in enumerate(gidx_f))
+    ref_dev = [abs(hc["l2r"][g] - hc["exact"][g]) / hc["exact"][g] for g in range(6)]
     rows = int(out["row_count"].sum())
     res = {"workload": f"fused filter + group-by, {n} rows: date<=2470, group (flagA, flagB), "
                        "SUM(price*(100-discount)) int128, COUNT, AVG",
            "rows": n, "groups": k, "rows_passing_filter": rows, "ms_per_step": wall * 1e3,
-           "rows_per_s": n / wall, "update_ms": upd_ms,
-           "roofline": {"bound": "hbm", "kernel": "groupby_update", "achieved": n * 30 / (upd_ms * 1e-3) / 1e9,
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": n * 30 / (upd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                        "traffic": TRAFFIC.get("groupby_update") if n == DEFAULT_SIZES[2] else None}}
+           "rows_per_s": n / wall, "update_ms": upd_ms, "roofline": roof,
+           "check": {"int128_sums_and_counts_exact": bool(int_ok),
+                     "reference_f64_sum_max_rel_deviation": max(ref_dev),
+                     "reference_f64_sum_rel_deviation_per_group": ref_dev,
+                     "note": "the reference sums price*(100-discount) in f64 left to right; per-group exact sums "
+                             "pass 2^53, so its result deviates by the listed relative amounts; the device's SUM "
+                             "is the exact int128 value"},
+           "f64_variant": {"workload": f"the same filter and groups, SUM(price*0.01*(1-discount*0.01)) in f64, COUNT, "
+                                       "AVG", "groups": k_f, "ms_per_step": wall_f * 1e3, "rows_per_s": n / wall_f,
+                           "update_ms": upd_f, "roofline": roof_f,
+                           "check": {"max_rel_error_vs_long_double_sum": f64_err, "tolerance": 1e-12,
+                                     "ok": bool(f64_err <= 1e-12 and k_f == 6)}}}
     if cpu_sample:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyarrow as pa
+        rc = _ref_cpu()
+        m = min(cpu_sample, n)
+        hd, hp, hdi = date[:m].numpy(), price[:m].numpy(), disc[:m].numpy()
+        ha = (fa[:m].numpy(), fa_off[:m + 1].numpy())
+        hb = (fb[:m].numpy(), fb_off[:m + 1].numpy())
+        got = {}
+        for variant, key, expr in ((0, None, "price * (100 - discount)"),
+                                   (1, "f64_variant", "price * 0.01 * (1 - discount * 0.01)")):
+            def run(threads):
+                got["r"] = rc.filter_project_aggregate(hd, hp, hdi, ha, hb, 2470, threads, variant=variant)
 
-        import oracle as orc
-        d = dg.filter_groupby(cpu_sample)
-        keep = d["date"] <= 2470
-        t0 = time.perf_counter()
-        rev = d["price"][keep] * (100 - d["discount"][keep])
-        a_b, a_o = d["flag_a"]
-        b_b, b_o = d["flag_b"]
-        a_sel, b_sel = a_b[keep], b_b[keep]
-        m = int(keep.sum())
-        batch = orc.Batch([orc.Col(orc.UTF8, a_sel, offsets=np.arange(m + 1, dtype=np.int32)),
-                           orc.Col(orc.UTF8, b_sel, offsets=np.arange(m + 1, dtype=np.int32)),
-                           orc.Col(orc.INT64, rev)])
-        orc.hash_aggregate([batch], [0, 1], [(2, "SUM"), (2, "COUNT"), (2, "AVG")])
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": cpu_sample / dt, "unit": "rows/s", "cores": 1, "kind": "port",
-                               "sample": f"numpy filter+project then the oracle's hash_aggregate restatement on "
-                                         f"{cpu_sample} rows, one thread, {dt:.2f} s"}
+            cb = cpu_protocol(
+                run, m, "rows/s", f"Filter(date <= 2470) -> Project({expr}) -> Aggregate by (flag A, flag B) over "
+                f"the first {m} rows ({(m + rc.BATCH_ROWS - 1) // rc.BATCH_ROWS} batches of {rc.BATCH_ROWS} rows), "
+                "f64 arithmetic as the reference")
+            cb["groups"] = got["r"]["n_groups"]
+            (res if key is None else res[key])["cpu_baseline"] = cb
     g.close()
     for a in (date, price, disc, fa, fa_off, fb, fb_off):
         a.free()
+    return res
+
+
+def bench_groupby_1m(ctx, steps: int, cpu: bool) -> dict:
+    """Plain hash aggregation of one 1M-row batch: key int64 U[0, 1000), value
+    int64 U[0, 10^6); SUM, COUNT, MIN, MAX by key, checked bit for bit against
+    the oracle.  Launch-bound at this size (16 MB read)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from prism_executor import _lib as L
+    from prism_executor import datagen as dg
+    from prism_executor import device as D
+
+    n = 1_000_000
+    lib = ctx.lib
+    k, v = D.empty(ctx, n, np.int64), D.empty(ctx, n, np.int64)
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, k.ptr, n, dg.SEED, 1, 0, 1000))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, v.ptr, n, dg.SEED, 2, 0, 1_000_000))
+    ctx.sync()
+    spec = dict(column_types=[L.TYPE_INT64, L.TYPE_INT64], keys=[0], values=[[("LOAD_I", 1)]],
+                aggs=[("SUM", 0), ("COUNT", 0), ("MIN", 0), ("MAX", 0)])
+    cols = [D.col_i64(k), D.col_i64(v)]
+    g = D.GroupBy(ctx, expected_groups=1000, **spec)
+
+    def step():
+        g.reset()
+        g.update(cols, n, 0)
+        return g.read(g.finish())
+
+    for _ in range(3):
+        out = step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = step()
+    wall = (time.perf_counter() - t0) / steps
+    ctx.kernel_timing(True)
+    for _ in range(steps):
+        L.check(lib.prism_groupby_update(g.handle, (L.Column * 2)(*cols), n, 0))
+    tot, cnt = ctx.kernel_time("groupby_update")
+    ctx.kernel_timing(False)
+    g.close()
+    hk, hv = k.numpy(), v.numpy()
+    ref = orc.hash_aggregate([orc.Batch([orc.Col(orc.INT64, hk), orc.Col(orc.INT64, hv)])], [0],
+                             [(1, "SUM"), (1, "COUNT"), (1, "MIN"), (1, "MAX")])
+    ok = (out["first_row"].tolist() == ref["first_row"].tolist() and out["key_hash"].tolist() == ref["key_hash"].tolist()
+          and all(np.array_equal(out["f64"][a], ref["f64"][a]) for a in (0, 2, 3))
+          and np.array_equal(out["i64"][1], ref["i64"][1]))
+    upd_ms = tot / max(cnt, 1)
+    res = {"workload": "hash aggregation, one batch of 1000000 rows: key int64 U[0,1000), value int64 U[0,1e6); "
+                       "SUM/COUNT/MIN/MAX by key (launch-bound at this size; parity config)",
+           "groups": int(len(out["first_row"])), "bit_exact_vs_oracle": bool(ok), "ms_per_step": wall * 1e3,
+           "rows_per_s": n / wall, "update_ms": upd_ms, "update_GBps": n * 16 / (upd_ms * 1e-3) / 1e9}
+    if cpu:
+        rc = _ref_cpu()
+        res["cpu_baseline"] = cpu_protocol(lambda threads: rc.groupby_i64(hk, hv, threads), n, "rows/s",
+                                           "hash_aggregate over the same 1000000-row batch (one batch: the "
+                                           "reference's chunking gives it one core at any thread count)")
+    k.free()
+    v.free()
     return res
 
 
@@ -213,16 +383,31 @@ def bench_join_agg(ctx, n_build: int, n_probe: int, n_values: int, n_attr: int, 
            "roofline": {"bound": "hbm", "kernel": "whole query", "achieved": alg / wall / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": alg / wall / 1e9 / HBM_PEAK_GBS, "alg_bytes": alg}}
     if cpu_sample:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as orc
-        m = min(cpu_sample, n_probe)
-        hp = pp.numpy()[:m]
-        t0 = time.perf_counter()
-        orc.join_agg_eager(hb, ba.numpy(), hk[:m], hp, n_attr)
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": m / dt, "unit": "probe rows/s", "cores": 1, "kind": "port",
-                               "sample": f"oracle eager join-aggregate restatement, all {n_build} build rows + first "
-                                         f"{m} probe rows, one thread, {dt:.2f} s"}
+        # The reference's fused path materialises the join output per probe
+        # batch (executor.rs:333-458): ~3e5 pairs per probe row here, so the
+        # sample is the prefix of the probe stream holding at most
+        # `cpu_sample` pairs.  One batch: the reference parallelises only
+        # across batches, so every thread count runs it on one core.
+        rc = _ref_cpu()
+        bcount = np.bincount(hb, minlength=n_values)
+        per_row = bcount[hk[:1_000_000]]
+        k = int(np.searchsorted(np.cumsum(per_row), cpu_sample, side="right"))
+        k = max(k, 1)
+        hp = pp[:k].numpy()
+        hba = ba.numpy()
+        got = {}
+
+        def run(threads):
+            t = rc.JoinTable(hb, threads)
+            got["pairs"] = t.join_aggregate(hba, hk[:k], hp, n_attr, threads)[0]
+            t.close()
+
+        res["cpu_baseline"] = cpu_protocol(
+            run, k, "probe rows/s", f"build all {n_build} rows (64-partition table) + the first {k} probe rows "
+            f"({int(per_row[:k].sum())} join pairs, materialised per probe batch then aggregated, as the "
+            "reference's fused join -> aggregate runs)")
+        res["cpu_baseline"]["pairs"] = got["pairs"]
+        res["cpu_baseline"]["pairs_per_s"] = got["pairs"] / res["cpu_baseline"]["median_s"]
     for a in (cdf, bk, ba, pk, pp):
         a.free()
     return res
@@ -584,7 +769,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--build-rows", type=int, default=1_000_000)
     ap.add_argument("--probe-rows", type=int, default=64_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=8_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=16 * 1_048_576)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--groupby-rows", type=int, default=200_000_000)
     ap.add_argument("--no-groupby", action="store_true")
@@ -707,6 +892,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": "join_probe", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": TRAFFIC.get("join_probe") if (nb, npr) == DEFAULT_SIZES[:2] else None},
+        "roofline_whole_probe": {"bound": "hbm", "kernel": "join_probe + scan + join_probe_move (one probe call)",
+                                 "achieved": alg_bytes / (probe_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": alg_bytes / (probe_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernels": kernels,
         "probe_ms": probe_ms,
         "probe_GBps": alg_bytes / (probe_ms * 1e-3) / 1e9,
@@ -716,18 +904,20 @@ def main() -> int:
     }
     progress("join", t_start)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(nb, args.cpu_sample)
+        line["cpu_baseline"] = cpu_baseline(bk.numpy(), pk[:args.cpu_sample].numpy())
         progress("join cpu baseline", t_start)
     for a in (op, ob, pk, pp):
         a.free()
     if not args.no_groupby:
+        line["groupby_1m"] = bench_groupby_1m(ctx, args.steps, not args.no_cpu_baseline)
+        progress("group-by 1M", t_start)
         line["groupby_filter"] = bench_groupby(ctx, args.groupby_rows, args.steps,
-                                               0 if args.no_cpu_baseline else 10_000_000)
+                                               0 if args.no_cpu_baseline else 16 * 1_048_576)
         progress("group-by", t_start)
     if not args.no_join_agg:
         line["join_aggregate"] = bench_join_agg(ctx, 16_000_000, args.join_agg_probe_rows, 8_000_000, 1_000_000,
                                                 max(2, min(args.steps, 5)),
-                                                0 if args.no_cpu_baseline else 8_000_000)
+                                                0 if args.no_cpu_baseline else 20_000_000)
         progress("join-aggregate", t_start)
     if not args.no_parquet:
         line["parquet_scan"] = bench_parquet(ctx, args.parquet_rows, max(2, min(args.steps, 5)))
