@@ -248,7 +248,6 @@ def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
                 "f64 arithmetic as the reference")
             cb["groups"] = got["r"]["n_groups"]
             (res if key is None else res[key])["cpu_baseline"] = cb
-    g.close()
     for a in (date, price, disc, fa, fa_off, fb, fb_off):
         a.free()
     return res
