@@ -270,6 +270,23 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t,
                                   uint32_t* d_probe_idx, uint32_t* d_build_idx, uint64_t capacity,
                                   prism_probe_result* d_result);
 
+/* The same probe without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The table is built
+ * beforehand; every output is the caller's.  Its scratch is a fixed 128 MiB
+ * device array per GPU, held until the process ends, which limits one probe
+ * to 16,752,640 rows with pairs (any row count without pairs); a larger
+ * probe fails with PRISM_ERR_INVALID_ARGUMENT before anything is launched.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several contexts or streams) race, and must be ordered by the
+ * caller.  Results equal prism_join_probe_i64's. */
+prism_status prism_join_probe_i64_noalloc(prism_ctx* ctx, const prism_join_table* t,
+                                          const int64_t* d_probe_keys, uint64_t n_probe,
+                                          uint32_t probe_row_base,
+                                          const int64_t* d_probe_payload, const int32_t* d_build_payload,
+                                          uint32_t* d_probe_idx, uint32_t* d_build_idx, uint64_t capacity,
+                                          prism_probe_result* d_result);
+
 /* ------------------------------------------------------------------------- */
 /* Hash aggregation, device layer (hash_aggregate.rs:182-339).                */
 /* ------------------------------------------------------------------------- */

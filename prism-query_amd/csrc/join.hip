@@ -2360,103 +2360,183 @@ void prism_join_table_free(prism_join_table* t) {
   delete t;
 }
 
+}  // extern "C"
+
+namespace prism {
+namespace join {
+
+static uint64_t align256(uint64_t b) { return (b + 255) & ~255ULL; }
+
+// Scratch of one probe of n_probe rows: per-tile int128 payload partials,
+// then either the unique path's tile counts / offsets / slice totals / staged
+// pairs or the duplicate path's look-back status words.  Every word the
+// kernels read is written by them first (k_probe_clear clears the status
+// words), so the scratch needs no clearing.
+static uint64_t probe_scratch_bytes(const prism_join_table* t, uint64_t n_probe, bool pairs) {
+  const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
+  const uint64_t sums_bytes = align256(n_tiles * 32 + 32);
+  if (t->info.unique) {
+    const uint64_t n_slices_max = ceil_div(n_tiles, kSliceTiles) + 1;
+    const uint64_t cnt_bytes = align256(n_tiles * 4 + 4), off_bytes = align256(n_tiles * 8 + 8);
+    const uint64_t slice_bytes = align256(n_slices_max * 8) + align256(n_slices_max * 32);
+    const uint64_t stage_bytes = pairs ? n_tiles * kTileRows * sizeof(uint2) : 0;
+    return sums_bytes + cnt_bytes + off_bytes + slice_bytes + stage_bytes;
+  }
+  return sums_bytes + align256((n_tiles + 1) * 8);
+}
+
+static void probe_checks(prism_ctx* ctx, const prism_join_table* t, const int64_t* d_probe_keys, uint64_t n_probe,
+                         uint32_t probe_row_base, uint32_t* d_probe_idx, uint32_t* d_build_idx,
+                         prism_probe_result* d_result) {
+  if (!ctx || !t || !d_result) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/table/result");
+  if ((d_probe_idx == nullptr) != (d_build_idx == nullptr))
+    fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: give both index outputs or neither");
+  if (n_probe && !d_probe_keys) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL probe keys");
+  if (n_probe + probe_row_base > 0xFFFFFFFFULL)
+    fail(PRISM_ERR_JOIN, "Join error: probe rows exceed u32 row indices (UInt32Array, hash_join.rs:68-69)");
+}
+
+// The probe's launches over `base` (probe_scratch_bytes of scratch); the
+// caller holds ctx->mu.  `timed`: bracket the main kernels with the context's
+// per-kernel timers.
+static void probe_launch(prism_ctx* ctx, const prism_join_table* t, const int64_t* d_probe_keys, uint64_t n_probe,
+                         uint32_t probe_row_base, const int64_t* d_probe_payload, const int32_t* d_build_payload,
+                         uint32_t* d_probe_idx, uint32_t* d_build_idx, uint64_t capacity, prism_probe_result* d_result,
+                         char* base, bool timed) {
+  const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
+  const bool vec = (reinterpret_cast<uintptr_t>(d_probe_keys) & 15) == 0;
+  const TableView& T = t->view;
+  dim3 g(static_cast<unsigned>(n_tiles ? n_tiles : 1));
+  const uint32_t nt = static_cast<uint32_t>(n_tiles);
+  const bool pairs = d_probe_idx != nullptr;
+  const uint64_t sums_bytes = align256(n_tiles * 32 + 32);
+  __int128* tile_sums = reinterpret_cast<__int128*>(base);
+  if (t->info.unique) {
+    const uint64_t n_slices_max = ceil_div(n_tiles, kSliceTiles) + 1;
+    const uint64_t cnt_bytes = align256(n_tiles * 4 + 4), off_bytes = align256(n_tiles * 8 + 8);
+    const uint64_t slice_bytes = align256(n_slices_max * 8) + align256(n_slices_max * 32);
+    char* p = base + sums_bytes;
+    uint32_t* tile_count = reinterpret_cast<uint32_t*>(p);
+    uint64_t* tile_off = reinterpret_cast<uint64_t*>(p + cnt_bytes);
+    uint64_t* slice_total = reinterpret_cast<uint64_t*>(p + cnt_bytes + off_bytes);
+    __int128* slice_sums = reinterpret_cast<__int128*>(p + cnt_bytes + off_bytes + align256(n_slices_max * 8));
+    uint2* staged = pairs ? reinterpret_cast<uint2*>(p + cnt_bytes + off_bytes + slice_bytes) : nullptr;
+    hipLaunchKernelGGL(k_probe_clear, dim3(1), dim3(64), 0, ctx->stream, nullptr, 0, d_result);
+    PRISM_CHECK_LAUNCH();
+    if (n_probe == 0) return;
+#define PRISM_PROBE1(KIND_, VEC_)                                                                          \
+  hipLaunchKernelGGL((k_probe_unique<KIND_, VEC_>), g, dim3(kPB), 0, ctx->stream, T, d_probe_keys, n_probe, \
+                     probe_row_base, d_probe_payload, d_build_payload, staged, tile_count, tile_sums)
+    {
+      std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe") : nullptr);
+      if (t->info.kind == PRISM_TABLE_DENSE) {
+        if (vec) PRISM_PROBE1(PRISM_TABLE_DENSE, true);
+        else PRISM_PROBE1(PRISM_TABLE_DENSE, false);
+      } else {
+        if (vec) PRISM_PROBE1(PRISM_TABLE_HASHED, true);
+        else PRISM_PROBE1(PRISM_TABLE_HASHED, false);
+      }
+    }
+#undef PRISM_PROBE1
+    PRISM_CHECK_LAUNCH();
+    const bool want_sums = d_probe_payload || d_build_payload || t->view.has_payload;
+    const uint64_t n_slices = ceil_div(n_tiles, kSliceTiles);
+    hipLaunchKernelGGL(k_probe_slices, dim3(static_cast<unsigned>(n_slices)), dim3(kSliceThreads), 0, ctx->stream,
+                       tile_count, pairs ? tile_off : nullptr, want_sums ? tile_sums : nullptr, n_tiles, slice_total,
+                       slice_sums);
+    PRISM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_probe_finish, dim3(1), dim3(kFinishThreads), 0, ctx->stream, slice_total,
+                       want_sums ? slice_sums : nullptr, n_slices, d_result);
+    PRISM_CHECK_LAUNCH();
+    if (pairs) {
+      std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe_move") : nullptr);
+      hipLaunchKernelGGL(k_probe_move, g, dim3(kPB), 0, ctx->stream, staged, tile_count, tile_off, slice_total,
+                         log2_pow2(kSliceTiles), d_probe_idx, d_build_idx, capacity);
+      PRISM_CHECK_LAUNCH();
+    }
+    return;
+  }
+  uint64_t* status = reinterpret_cast<uint64_t*>(base + sums_bytes);
+  hipLaunchKernelGGL(k_probe_clear, grid_for(ctx, n_tiles + 1), dim3(kBuildBlock), 0, ctx->stream, status, n_tiles,
+                     d_result);
+  PRISM_CHECK_LAUNCH();
+  if (n_probe == 0) return;
+  {
+    std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe") : nullptr);
+    if (t->info.kind == PRISM_TABLE_DENSE)
+      launch_probe_vec<PRISM_TABLE_DENSE, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
+                                                 d_probe_payload, d_build_payload, d_probe_idx, d_build_idx, capacity,
+                                                 status, tile_sums, d_result);
+    else
+      launch_probe_vec<PRISM_TABLE_HASHED, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
+                                                  d_probe_payload, d_build_payload, d_probe_idx, d_build_idx, capacity,
+                                                  status, tile_sums, d_result);
+  }
+  PRISM_CHECK_LAUNCH();
+  if (d_probe_payload || d_build_payload || t->view.has_payload) {
+    hipLaunchKernelGGL(k_probe_sums, dim3(1), dim3(1024), 0, ctx->stream, tile_sums, n_tiles, d_result);
+    PRISM_CHECK_LAUNCH();
+  }
+}
+
+// Scratch of prism_join_probe_i64_noalloc: a fixed device array, one copy per
+// GPU, held until the process ends.  Host code reaches it only through the
+// address hipGetSymbolAddress returns on each device.
+constexpr uint64_t kNoAllocScratchBytes = 128ULL << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::map<int, char*> g_scratch_ptr;  // device -> address of `scratch` there
+
+// Address of `scratch` on ctx's device (looked up once per device, with that
+// device current).
+static char* noalloc_scratch(prism_ctx* ctx) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(ctx->device);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* ptr = nullptr;
+  PRISM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) fail(PRISM_ERR_EXECUTION, "Execution error: no device address for the probe scratch array");
+  g_scratch_ptr[ctx->device] = static_cast<char*>(ptr);
+  return static_cast<char*>(ptr);
+}
+
+}  // namespace join
+}  // namespace prism
+
+extern "C" {
+
 prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, const int64_t* d_probe_keys,
                                   uint64_t n_probe, uint32_t probe_row_base, const int64_t* d_probe_payload,
                                   const int32_t* d_build_payload, uint32_t* d_probe_idx, uint32_t* d_build_idx,
                                   uint64_t capacity, prism_probe_result* d_result) {
   return guarded([&] {
-    if (!ctx || !t || !d_result) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context/table/result");
-    if ((d_probe_idx == nullptr) != (d_build_idx == nullptr))
-      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: give both index outputs or neither");
-    if (n_probe && !d_probe_keys) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL probe keys");
-    if (n_probe + probe_row_base > 0xFFFFFFFFULL)
-      fail(PRISM_ERR_JOIN, "Join error: probe rows exceed u32 row indices (UInt32Array, hash_join.rs:68-69)");
+    probe_checks(ctx, t, d_probe_keys, n_probe, probe_row_base, d_probe_idx, d_build_idx, d_result);
     use_device(ctx);
     Ctx* c = as_ctx(ctx);
     std::lock_guard<std::mutex> lock(ctx->mu);
-    const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
-    const bool vec = (reinterpret_cast<uintptr_t>(d_probe_keys) & 15) == 0;
-    const TableView& T = t->view;
-    dim3 g(static_cast<unsigned>(n_tiles ? n_tiles : 1));
-    const uint32_t nt = static_cast<uint32_t>(n_tiles);
-    const bool pairs = d_probe_idx != nullptr;
-    auto align = [](uint64_t b) { return (b + 255) & ~255ULL; };
-    // scratch: per-tile int128 payload partials, then either the unique
-    // path's counts / offsets / staged pairs or the duplicate path's
-    // look-back status words
-    const uint64_t sums_bytes = align(n_tiles * 32 + 32);
-    __int128* tile_sums = c->scratch.as<__int128>();
-    if (t->info.unique) {
-      const uint64_t n_slices_max = ceil_div(n_tiles, kSliceTiles) + 1;
-      const uint64_t cnt_bytes = align(n_tiles * 4 + 4), off_bytes = align(n_tiles * 8 + 8);
-      const uint64_t slice_bytes = align(n_slices_max * 8) + align(n_slices_max * 32);
-      const uint64_t stage_bytes = pairs ? n_tiles * kTileRows * sizeof(uint2) : 0;
-      c->scratch.reserve(c, sums_bytes + cnt_bytes + off_bytes + slice_bytes + stage_bytes);
-      tile_sums = c->scratch.as<__int128>();
-      char* p = c->scratch.as<char>() + sums_bytes;
-      uint32_t* tile_count = reinterpret_cast<uint32_t*>(p);
-      uint64_t* tile_off = reinterpret_cast<uint64_t*>(p + cnt_bytes);
-      uint64_t* slice_total = reinterpret_cast<uint64_t*>(p + cnt_bytes + off_bytes);
-      __int128* slice_sums = reinterpret_cast<__int128*>(p + cnt_bytes + off_bytes + align(n_slices_max * 8));
-      uint2* staged = pairs ? reinterpret_cast<uint2*>(p + cnt_bytes + off_bytes + slice_bytes) : nullptr;
-      hipLaunchKernelGGL(k_probe_clear, dim3(1), dim3(64), 0, ctx->stream, nullptr, 0, d_result);
-      PRISM_CHECK_LAUNCH();
-      if (n_probe == 0) return;
-#define PRISM_PROBE1(KIND_, VEC_)                                                                          \
-  hipLaunchKernelGGL((k_probe_unique<KIND_, VEC_>), g, dim3(kPB), 0, ctx->stream, T, d_probe_keys, n_probe, \
-                     probe_row_base, d_probe_payload, d_build_payload, staged, tile_count, tile_sums)
-      {
-        KernelTimer kt(ctx, "join_probe");
-        if (t->info.kind == PRISM_TABLE_DENSE) {
-          if (vec) PRISM_PROBE1(PRISM_TABLE_DENSE, true);
-          else PRISM_PROBE1(PRISM_TABLE_DENSE, false);
-        } else {
-          if (vec) PRISM_PROBE1(PRISM_TABLE_HASHED, true);
-          else PRISM_PROBE1(PRISM_TABLE_HASHED, false);
-        }
-      }
-#undef PRISM_PROBE1
-      PRISM_CHECK_LAUNCH();
-      const bool want_sums = d_probe_payload || d_build_payload || t->view.has_payload;
-      const uint64_t n_slices = ceil_div(n_tiles, kSliceTiles);
-      hipLaunchKernelGGL(k_probe_slices, dim3(static_cast<unsigned>(n_slices)), dim3(kSliceThreads), 0, ctx->stream,
-                         tile_count, pairs ? tile_off : nullptr, want_sums ? tile_sums : nullptr, n_tiles,
-                         slice_total, slice_sums);
-      PRISM_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_probe_finish, dim3(1), dim3(kFinishThreads), 0, ctx->stream, slice_total,
-                         want_sums ? slice_sums : nullptr, n_slices, d_result);
-      PRISM_CHECK_LAUNCH();
-      if (pairs) {
-        KernelTimer kt(ctx, "join_probe_move");
-        hipLaunchKernelGGL(k_probe_move, g, dim3(kPB), 0, ctx->stream, staged, tile_count, tile_off, slice_total,
-                           log2_pow2(kSliceTiles), d_probe_idx, d_build_idx, capacity);
-        PRISM_CHECK_LAUNCH();
-      }
-      return;
-    } else {
-      const uint64_t status_bytes = align((n_tiles + 1) * 8);
-      c->scratch.reserve(c, sums_bytes + status_bytes);
-      tile_sums = c->scratch.as<__int128>();
-      uint64_t* status = reinterpret_cast<uint64_t*>(c->scratch.as<char>() + sums_bytes);
-      hipLaunchKernelGGL(k_probe_clear, grid_for(ctx, n_tiles + 1), dim3(kBuildBlock), 0, ctx->stream, status,
-                         n_tiles, d_result);
-      PRISM_CHECK_LAUNCH();
-      if (n_probe == 0) return;
-      KernelTimer kt(ctx, "join_probe");
-      if (t->info.kind == PRISM_TABLE_DENSE)
-        launch_probe_vec<PRISM_TABLE_DENSE, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
-                                                   d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                   capacity, status, tile_sums, d_result);
-      else
-        launch_probe_vec<PRISM_TABLE_HASHED, false>(ctx, vec, g, T, d_probe_keys, n_probe, nt, probe_row_base,
-                                                    d_probe_payload, d_build_payload, d_probe_idx, d_build_idx,
-                                                    capacity, status, tile_sums, d_result);
-    }
-    PRISM_CHECK_LAUNCH();
-    if (d_probe_payload || d_build_payload || t->view.has_payload) {
-      hipLaunchKernelGGL(k_probe_sums, dim3(1), dim3(1024), 0, ctx->stream, tile_sums, n_tiles, d_result);
-      PRISM_CHECK_LAUNCH();
-    }
+    c->scratch.reserve(c, probe_scratch_bytes(t, n_probe, d_probe_idx != nullptr));
+    probe_launch(ctx, t, d_probe_keys, n_probe, probe_row_base, d_probe_payload, d_build_payload, d_probe_idx,
+                 d_build_idx, capacity, d_result, c->scratch.as<char>(), true);
+  });
+}
+
+prism_status prism_join_probe_i64_noalloc(prism_ctx* ctx, const prism_join_table* t, const int64_t* d_probe_keys,
+                                          uint64_t n_probe, uint32_t probe_row_base, const int64_t* d_probe_payload,
+                                          const int32_t* d_build_payload, uint32_t* d_probe_idx,
+                                          uint32_t* d_build_idx, uint64_t capacity, prism_probe_result* d_result) {
+  return guarded([&] {
+    probe_checks(ctx, t, d_probe_keys, n_probe, probe_row_base, d_probe_idx, d_build_idx, d_result);
+    const uint64_t need = probe_scratch_bytes(t, n_probe, d_probe_idx != nullptr);
+    if (need > kNoAllocScratchBytes)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: probe of " + std::to_string(n_probe) + " rows needs " +
+                                           std::to_string(need) + " bytes of scratch; the allocation-free probe has " +
+                                           std::to_string(kNoAllocScratchBytes));
+    use_device(ctx);
+    char* base = noalloc_scratch(ctx);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    probe_launch(ctx, t, d_probe_keys, n_probe, probe_row_base, d_probe_payload, d_build_payload, d_probe_idx,
+                 d_build_idx, capacity, d_result, base, false);
   });
 }
 

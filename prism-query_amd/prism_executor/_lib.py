@@ -229,6 +229,8 @@ _SIGNATURES = [
     ("prism_join_table_free", None, [c_void_p]),
     ("prism_join_probe_i64", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_void_p,
                                        c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
+    ("prism_join_probe_i64_noalloc", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_void_p,
+                                               c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),
     ("prism_groupby_create", c_int32, [c_void_p, POINTER(GroupbyPlan), c_uint64, POINTER(c_void_p)]),
     ("prism_groupby_update", c_int32, [c_void_p, POINTER(Column), c_uint64, c_uint64]),
     ("prism_groupby_finish", c_int32, [c_void_p, POINTER(c_uint64)]),
