@@ -37,9 +37,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each dominant kernel on the default workloads, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script (each counter
 # scaled by its reading for a plain 1 GiB device copy: FETCH 2.0, WRITE 1.0),
-# recorded in profiles/bench_default_hbm_traffic.md.  Measured offline, not by
-# this run: they hold for the default sizes only.
-TRAFFIC = {"join_probe": 3_494_300_000, "groupby_update": 6_118_900_000}
+# recorded in profiles/join_probe_1Mx64M.md and profiles/groupby_filter_200M.md.
+# Measured offline, not by this run: they hold for the default sizes only.
+TRAFFIC = {"join_probe": 3_494_600_000, "groupby_update": 6_133_500_000}
 DEFAULT_SIZES = (1_000_000, 64_000_000, 200_000_000)
 METRIC = "rows/s & %HBM peak (join probe, group-by); p99 latency batched vs plain"
 
@@ -146,7 +146,7 @@ def groupby_host_check(date, price, disc, fa, fb, n: int, chunk: int = 8_000_000
     return {"exact": exact, "count": count, "l2r": l2r, "frac": [float(x) for x in frac]}
 
 
-def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
+def bench_groupby(ctx, n: int, steps: int, cpu_sample: int, check: bool = True) -> dict:
     """Fused filter + group-by: date int32 <= 2470, group by (flag A, flag B)
     (one-byte Utf8), SUM(price * (100 - discount)) exact int128, COUNT, AVG;
     and the float64 variant, SUM(price * 0.01 * (1 - discount * 0.01)), COUNT,
@@ -203,6 +203,11 @@ def bench_groupby(ctx, n: int, steps: int, cpu_sample: int) -> dict:
     out, k, wall, upd_ms, roof = run(REVENUE_EXACT)
     roof["traffic"] = TRAFFIC.get("groupby_update") if n == DEFAULT_SIZES[2] else None
     out_f, k_f, wall_f, upd_f, roof_f = run(REVENUE_F64)
+    if not check:  # profiling runs: the timed work only
+        for a in (date, price, disc, fa, fa_off, fb, fb_off):
+            a.free()
+        return {"workload": "fused filter + group-by (unchecked profiling run)", "update_ms": upd_ms,
+                "roofline": roof, "f64_variant": {"update_ms": upd_f, "roofline": roof_f}}
     hc = groupby_host_check(date, price, disc, fa, fb, n)
     # groups come in first-appearance order; the host sums are indexed by
     # (flag A, flag B) -> map each device group through its first row
@@ -780,6 +785,8 @@ def main() -> int:
     ap.add_argument("--svc-requests", type=int, default=10000)
     ap.add_argument("--svc-rates", type=str, default="200,500,1000")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="also run D2D copies of this many bytes")
+    ap.add_argument("--skip-host-check", action="store_true",
+                    help="skip the host recomputation of the 200M-row group-by sums (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU sharding is a design note, "
@@ -911,7 +918,8 @@ def main() -> int:
         line["groupby_1m"] = bench_groupby_1m(ctx, args.steps, not args.no_cpu_baseline)
         progress("group-by 1M", t_start)
         line["groupby_filter"] = bench_groupby(ctx, args.groupby_rows, args.steps,
-                                               0 if args.no_cpu_baseline else 16 * 1_048_576)
+                                               0 if args.no_cpu_baseline else 16 * 1_048_576,
+                                               check=not args.skip_host_check)
         progress("group-by", t_start)
     if not args.no_join_agg:
         line["join_aggregate"] = bench_join_agg(ctx, 16_000_000, args.join_agg_probe_rows, 8_000_000, 1_000_000,
