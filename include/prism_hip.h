@@ -218,6 +218,12 @@ typedef struct prism_join_table prism_join_table;
 #define PRISM_TABLE_HASHED 1 /* open addressing on hash_i64_fast(key), linear probing */
 #define PRISM_TABLE_DENSE 2  /* direct-address array over [min_key, max_key] */
 
+/* Probe forms of a table (prism_join_build_opts.probe). */
+#define PRISM_PROBE_DIRECT 0       /* every probe row looks its key up in the table in HBM */
+#define PRISM_PROBE_PARTITIONED 1  /* radix-partitioned: probe rows are grouped by the hash bits that pick the
+                                      table's partition, each partition's slots are loaded into LDS once and
+                                      probed there, and the pairs are put back in probe order */
+
 typedef struct {
   uint64_t n_rows;        /* build rows */
   uint64_t n_distinct;    /* distinct keys */
@@ -227,6 +233,9 @@ typedef struct {
   int32_t unique;         /* 1 when every key occurs once */
   uint64_t slots;         /* slots (HASHED) or key range (DENSE) */
   uint64_t device_bytes;  /* HBM held by the table */
+  uint64_t partition_slots; /* HASHED: slots of one partition (= slots when unpartitioned) */
+  uint32_t partitions;    /* HASHED: slots / partition_slots */
+  int32_t probe;          /* PRISM_PROBE_* */
 } prism_join_table_info;
 
 /* HashTable::build (hash_join.rs:87-141) on a device-resident int64 key column.
@@ -240,6 +249,41 @@ prism_status prism_join_build_i64(prism_ctx* ctx, const int64_t* d_keys, uint64_
  * with the row id instead of gathering it. */
 prism_status prism_join_build_i64_payload(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload,
                                           uint64_t n_rows, int32_t kind, prism_join_table** out);
+/* Table layout and probe form, for the build with options (no counterpart in
+ * the reference, whose HashTable::build has no knobs; hash_join.rs:41-45 fixes
+ * 64 partitions).  A HASHED table of `slots` = next power of two of
+ * n / load_factor is cut into slots / partition_slots partitions by the top
+ * bits of hash_i64_fast(key); a probe run wraps inside its partition, so a
+ * partition of partition_slots x 16 B (4096 slots: 64 KB) is an
+ * open-addressing table that fits LDS.  The build refuses a layout where a
+ * partition would hold partition_slots keys or more. */
+typedef struct {
+  int32_t kind;             /* PRISM_TABLE_* (PARTITIONED probes need PRISM_TABLE_HASHED) */
+  int32_t probe;            /* PRISM_PROBE_*; PARTITIONED needs unique keys and 256..8192 partition_slots */
+  uint64_t partition_slots; /* HASHED: slots per partition, a power of two; 0 = one partition */
+  double load_factor;       /* HASHED: keys per slot in (0, 0.9]; 0 = 0.5 */
+} prism_join_build_opts;
+prism_status prism_join_build_i64_opts(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload,
+                                       uint64_t n_rows, const prism_join_build_opts* opts, prism_join_table** out);
+
+/* Autotuner of the table layout and probe form (no counterpart in the
+ * reference): builds a table of the keys (and payload) for each candidate --
+ * the AUTO table probed directly, and HASHED tables of load factor 0.5 / 0.7 cut
+ * into partitions of 1024..8192 slots, probed directly and partitioned -- times
+ * one probe of the sample with pairs and payload sums on the context stream,
+ * and writes the fastest candidate's options to *best.  cands (room for cap
+ * entries, may be NULL) receives every candidate tried, in order. */
+typedef struct {
+  prism_join_build_opts opts;
+  double build_ms;  /* device time of the build */
+  double probe_ms;  /* device time of one probe of the sample */
+  int32_t status;   /* PRISM_OK, or why the candidate was not built */
+  int32_t reserved;
+} prism_join_tune_candidate;
+prism_status prism_join_tune(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload, uint64_t n_rows,
+                             const int64_t* d_probe_sample, const int64_t* d_probe_payload, uint64_t n_sample,
+                             prism_join_build_opts* best, prism_join_tune_candidate* cands, uint32_t cap,
+                             uint32_t* n_cands);
 prism_status prism_join_table_info_get(const prism_join_table* t, prism_join_table_info* out);
 void prism_join_table_free(prism_join_table* t);
 

@@ -214,6 +214,7 @@ prism_status prism_ctx_create(int device, prism_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
       c->num_cus = prop.multiProcessorCount;
+      if (prop.sharedMemPerBlock) c->lds_per_block = prop.sharedMemPerBlock;
     }
     hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {

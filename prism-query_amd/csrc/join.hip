@@ -173,9 +173,15 @@ __global__ __launch_bounds__(256) void k_dense_count(const int64_t* __restrict__
     if (sk[i] != kNone) atomicAdd(&cnt[sk[i]], sc[i]);
 }
 
+// Rows per partition of a partitioned HASHED table (partition = top hash bits).
+__global__ void k_pt_count(const int64_t* __restrict__ keys, uint64_t n, int pshift, uint32_t* __restrict__ cnt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[hash_i64_fast(keys[i]) >> pshift], 1u);
+}
+
 // HASHED: claim a slot per distinct key (CAS on the key word), count rows.
 __global__ void k_hashed_insert(const int64_t* __restrict__ keys, uint64_t n, Slot* slots, uint64_t mask,
-                                int shift, uint32_t* slot_of_row) {
+                                uint64_t pmask, int shift, uint32_t* slot_of_row) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     int64_t k = keys[i];
     uint64_t s;
@@ -188,7 +194,7 @@ __global__ void k_hashed_insert(const int64_t* __restrict__ keys, uint64_t n, Sl
                                            static_cast<unsigned long long>(kEmpty),
                                            static_cast<unsigned long long>(k));
         if (old == static_cast<unsigned long long>(kEmpty) || old == static_cast<unsigned long long>(k)) break;
-        s = (s + 1) & mask;
+        s = (s & ~pmask) | ((s + 1) & pmask);  // wraps inside the slot's partition
       }
     }
     slot_of_row[i] = static_cast<uint32_t>(s);
@@ -675,6 +681,225 @@ __global__ __launch_bounds__(kPB) void k_probe_move(const uint2* __restrict__ st
 }
 
 
+// ---------------------------------------------------------------------------
+// Partitioned probe of a unique HASHED table cut into LDS-sized partitions.
+//   A  k_pt_hist     rows of each 4096-row tile per partition (partition-major)
+//   -  scan          record offset of each (partition, tile) segment
+//   C  k_pt_scatter  (key, row in tile) records into their segments
+//   D  k_pt_join     per partition: its slots into LDS, its records looked up
+//                    there, (build row, payload) or none per record
+//   E  k_pt_restore  per tile: the results of its segments back in row order,
+//                    then ranked and staged as the direct probe stages them
+// followed by the direct probe's tile scan and move.  One partition (the
+// whole table in LDS) skips A-D: k_pt_restore<true> looks rows up itself.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPtChunk = 16384;    // records a join workgroup takes at a time
+constexpr uint32_t kPtMaxParts = 2048;  // partitions (segment table = partitions x tiles)
+
+__global__ __launch_bounds__(kPB) void k_pt_hist(const int64_t* __restrict__ keys, uint64_t n, int pshift, uint32_t P,
+                                                 uint32_t n_tiles, uint32_t* __restrict__ hist) {
+  extern __shared__ uint32_t s_h[];
+  for (uint32_t i = threadIdx.x; i < P; i += kPB) s_h[i] = 0;
+  __syncthreads();
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTileRows;
+  for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(kTileRows); i += kPB) {
+    const uint64_t r = base + i;
+    if (r < n) atomicAdd(&s_h[hash_i64_fast(__builtin_nontemporal_load(keys + r)) >> pshift], 1u);
+  }
+  __syncthreads();
+  for (uint32_t p = threadIdx.x; p < P; p += kPB) hist[static_cast<uint64_t>(p) * n_tiles + blockIdx.x] = s_h[p];
+}
+
+__global__ __launch_bounds__(kPB) void k_pt_scatter(const int64_t* __restrict__ keys, uint64_t n, int pshift,
+                                                    uint32_t P, uint32_t n_tiles, const uint32_t* __restrict__ offs,
+                                                    int64_t* __restrict__ rec_key, uint16_t* __restrict__ rec_row) {
+  extern __shared__ uint32_t s_c[];
+  for (uint32_t p = threadIdx.x; p < P; p += kPB) s_c[p] = offs[static_cast<uint64_t>(p) * n_tiles + blockIdx.x];
+  __syncthreads();
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kTileRows;
+  for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(kTileRows); i += kPB) {
+    const uint64_t r = base + i;
+    if (r >= n) continue;
+    const int64_t k = __builtin_nontemporal_load(keys + r);
+    const uint32_t at = atomicAdd(&s_c[hash_i64_fast(k) >> pshift], 1u);
+    rec_key[at] = k;
+    rec_row[at] = static_cast<uint16_t>(i);
+  }
+}
+
+// (build row, slot payload word) of key k in a partition's slots held in LDS;
+// false when absent.
+__device__ __forceinline__ bool pt_lookup(const TableView& T, const Slot* __restrict__ s_t, int64_t k, uint2& out) {
+  if (k == kEmpty) {
+    if (!T.sentinel_present) return false;
+    const Slot e = T.slots[T.mask + 1];
+    out = make_uint2(e.val, e.cnt);
+    return true;
+  }
+  uint64_t s = (hash_i64_fast(k) >> T.shift) & T.pmask;
+  while (true) {  // the build left an empty slot in every partition
+    const Slot e = s_t[s];
+    if (e.key == k) {
+      out = make_uint2(e.val, e.cnt);
+      return true;
+    }
+    if (e.key == kEmpty) return false;
+    s = (s + 1) & T.pmask;
+  }
+}
+
+// D: workgroup (p, m) of M per partition loads partition p's slots and looks
+// up records m, m + M, ... chunks of that partition's segment range.
+__global__ __launch_bounds__(kPB) void k_pt_join(TableView T, const int64_t* __restrict__ rec_key,
+                                                 const uint32_t* __restrict__ offs, uint32_t n_tiles, uint32_t M,
+                                                 uint2* __restrict__ result) {
+  extern __shared__ Slot s_t[];
+  const uint64_t PS = T.pmask + 1;
+  const uint32_t p = blockIdx.x / M, m = blockIdx.x % M;
+  const uint32_t start = offs[static_cast<uint64_t>(p) * n_tiles];
+  const uint32_t end = offs[static_cast<uint64_t>(p + 1) * n_tiles];
+  if (static_cast<uint64_t>(start) + static_cast<uint64_t>(m) * kPtChunk >= end) return;
+  const Slot* src = T.slots + static_cast<uint64_t>(p) * PS;
+  for (uint64_t i = threadIdx.x; i < PS; i += kPB) s_t[i] = src[i];
+  __syncthreads();
+  for (uint64_t c0 = start + static_cast<uint64_t>(m) * kPtChunk; c0 < end; c0 += static_cast<uint64_t>(M) * kPtChunk) {
+    const uint64_t c1 = c0 + kPtChunk < end ? c0 + kPtChunk : end;
+    for (uint64_t i = c0 + threadIdx.x; i < c1; i += kPB) {
+      uint2 out = make_uint2(kNone, 0);
+      pt_lookup(T, s_t, __builtin_nontemporal_load(rec_key + i), out);
+      result[i] = out;
+    }
+  }
+}
+
+// E: one tile's matches in row order, ranked and staged exactly as
+// k_probe_unique stages them.  DIRECT: the table is one partition held in LDS
+// and the workgroup looks its tiles' keys up itself (tiles blockIdx.x,
+// + gridDim.x, ...); otherwise one tile per workgroup, gathered from the
+// tile's segment of every partition.
+template <bool DIRECT>
+__global__ __launch_bounds__(kPB) void k_pt_restore(TableView T, const int64_t* __restrict__ keys, uint64_t n,
+                                                    uint32_t n_tiles, const uint32_t* __restrict__ offs, uint32_t P,
+                                                    const uint16_t* __restrict__ rec_row,
+                                                    const uint2* __restrict__ result, uint32_t probe_base,
+                                                    const int64_t* __restrict__ ppay, const int32_t* __restrict__ bpay,
+                                                    uint2* __restrict__ staged, uint32_t* __restrict__ tile_count,
+                                                    __int128* __restrict__ tile_sums) {
+  extern __shared__ Slot s_t[];  // DIRECT: the table's slots
+  __shared__ uint2 s_m[kTileRows];  // (build row, payload) per row of the tile, then its staged pairs
+  __shared__ uint32_t s_wave_tot[kPairs][kWaves];
+  __shared__ __int128 s_red[2][kWaves];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const bool table_pay = T.has_payload && bpay == nullptr;
+  const bool sums = ppay || bpay || table_pay;
+  if (DIRECT) {
+    for (uint64_t i = t; i <= T.pmask; i += kPB) s_t[i] = T.slots[i];
+    __syncthreads();
+  }
+  for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += DIRECT ? gridDim.x : n_tiles) {
+    const uint64_t base = static_cast<uint64_t>(tile) * kTileRows;
+    if (!DIRECT) {
+      for (uint32_t i = t; i < static_cast<uint32_t>(kTileRows); i += kPB) s_m[i] = make_uint2(kNone, 0);
+      __syncthreads();
+      for (uint32_t p = t; p < P; p += kPB) {
+        const uint64_t g = static_cast<uint64_t>(p) * n_tiles + tile;
+        const uint32_t a = offs[g], b = offs[g + 1];
+        for (uint32_t i = a; i < b; ++i) s_m[rec_row[i]] = result[i];
+      }
+      __syncthreads();
+    }
+    uint2 val[kPairs][2];
+    bool hit[kPairs][2];
+    __int128 psum = 0, bsum = 0;
+#pragma unroll
+    for (int j = 0; j < kPairs; ++j) {
+      const uint32_t l0 = static_cast<uint32_t>(j) * kChunk + 2 * t;
+      const uint64_t r0 = base + l0;
+      const bool ok0 = r0 < n, ok1 = r0 + 1 < n;
+      int64_t p0 = 0, p1 = 0;
+      if (ppay) {
+        p0 = ok0 ? ppay[r0] : 0;
+        p1 = ok1 ? ppay[r0 + 1] : 0;
+      }
+      uint2 v0 = make_uint2(kNone, 0), v1 = make_uint2(kNone, 0);
+      if (DIRECT) {
+        if (ok0) pt_lookup(T, s_t, keys[r0], v0);
+        if (ok1) pt_lookup(T, s_t, keys[r0 + 1], v1);
+      } else {
+        if (ok0) v0 = s_m[l0];
+        if (ok1) v1 = s_m[l0 + 1];
+      }
+      hit[j][0] = v0.x != kNone;
+      hit[j][1] = v1.x != kNone;
+      val[j][0] = v0;
+      val[j][1] = v1;
+      if (hit[j][0]) {
+        psum += p0;
+        bsum += bpay ? bpay[v0.x] : (table_pay ? static_cast<int32_t>(v0.y) : 0);
+      }
+      if (hit[j][1]) {
+        psum += p1;
+        bsum += bpay ? bpay[v1.x] : (table_pay ? static_cast<int32_t>(v1.y) : 0);
+      }
+    }
+    if (sums) {
+      wave_reduce_i128(psum);
+      wave_reduce_i128(bsum);
+      if (lane == 0) {
+        s_red[0][wave] = psum;
+        s_red[1][wave] = bsum;
+      }
+    }
+    uint32_t local[kPairs];
+#pragma unroll
+    for (int j = 0; j < kPairs; ++j) {
+      const uint64_t b0 = ballot64(hit[j][0]), b1 = ballot64(hit[j][1]);
+      local[j] = lanes_below(b0) + lanes_below(b1);
+      if (lane == 0) s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
+    }
+    __syncthreads();  // also: every s_m read above is done before the pairs overwrite it
+    uint32_t tile_total = 0;
+#pragma unroll
+    for (int j = 0; j < kPairs; ++j) {
+      uint32_t before = tile_total;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t v = s_wave_tot[j][w];
+        before += (w < wave) ? v : 0;
+        tile_total += v;
+      }
+      local[j] += before;
+    }
+    if (t == 0) {
+      tile_count[tile] = tile_total;
+      if (sums) {
+        __int128 pp = 0, bb = 0;
+        for (int w = 0; w < kWaves; ++w) {
+          pp += s_red[0][w];
+          bb += s_red[1][w];
+        }
+        tile_sums[2 * tile] = pp;
+        tile_sums[2 * tile + 1] = bb;
+      }
+    }
+    if (staged) {
+#pragma unroll
+      for (int j = 0; j < kPairs; ++j) {
+        const uint32_t row0 = static_cast<uint32_t>(base + static_cast<uint64_t>(j) * kChunk + 2 * t) + probe_base;
+        if (hit[j][0]) s_m[local[j]] = make_uint2(row0, val[j][0].x);
+        if (hit[j][1]) s_m[local[j] + (hit[j][0] ? 1 : 0)] = make_uint2(row0 + 1, val[j][1].x);
+      }
+      __syncthreads();
+      uint2* dst = staged + base;
+      for (uint32_t i = t; i < tile_total; i += kPB) dst[i] = s_m[i];
+    }
+    __syncthreads();  // s_m, s_wave_tot and s_red are reused by the next tile
+  }
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
 template <int KIND, bool UNIQUE, bool VEC>
 __global__ __launch_bounds__(kPB, 2) void k_probe(TableView T, const int64_t* __restrict__ keys, uint64_t n,
                                                uint32_t n_tiles, uint32_t probe_base,
@@ -1048,7 +1273,7 @@ __device__ __forceinline__ int64_t slot_of(const TableView& T, int64_t k, bool u
       const int64_t key = T.slots[s].key;
       if (key == k) return static_cast<int64_t>(s);
       if (key == kEmpty) return -1;
-      s = (s + 1) & T.mask;
+      s = (s & ~T.pmask) | ((s + 1) & T.pmask);
     }
   }
 }
@@ -2092,10 +2317,30 @@ using namespace prism::join;
 extern "C" {
 
 static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload, uint64_t n_rows,
-                               int32_t kind, prism_join_table** out, bool keys_only = false) {
+                               int32_t kind, prism_join_table** out, bool keys_only = false,
+                               const prism_join_build_opts* opts = nullptr) {
   return guarded([&] {
     if (!ctx || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or output");
     *out = nullptr;
+    uint64_t part_slots = 0;
+    double load = 0.5;
+    int32_t probe_form = PRISM_PROBE_DIRECT;
+    if (opts) {
+      part_slots = opts->partition_slots;
+      if (opts->load_factor != 0.0) load = opts->load_factor;
+      probe_form = opts->probe;
+      if (!(load > 0.0 && load <= 0.9))
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: load factor must lie in (0, 0.9]");
+      if (part_slots && (part_slots & (part_slots - 1)))
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: slots per partition must be a power of two");
+      if (probe_form != PRISM_PROBE_DIRECT && probe_form != PRISM_PROBE_PARTITIONED)
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: unknown probe form");
+      if (probe_form == PRISM_PROBE_PARTITIONED && kind != PRISM_TABLE_HASHED)
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: the partitioned probe needs a HASHED table");
+      if (probe_form == PRISM_PROBE_PARTITIONED && (part_slots < 256 || part_slots > kPtMaxSlots))
+        fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: the partitioned probe needs 256.." +
+                                             std::to_string(kPtMaxSlots) + " slots per partition");
+    }
     if (n_rows >= 0xFFFFFFFFULL)
       fail(PRISM_ERR_JOIN, "Join error: build side exceeds u32 row indices (UInt32Array, hash_join.rs:68-69)");
     if (n_rows && !d_keys) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL build keys");
@@ -2270,15 +2515,37 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       T->view.range = range;
     } else {
       T->info.kind = PRISM_TABLE_HASHED;
-      uint64_t S = next_pow2(n_rows * 2 > 64 ? n_rows * 2 : 64);
+      const uint64_t want = static_cast<uint64_t>(static_cast<double>(n_rows) / load) + 1;
+      uint64_t S = next_pow2(want > 64 ? want : 64);
+      if (part_slots && S < part_slots) S = part_slots;
+      const uint64_t PS = part_slots ? part_slots : S;  // slots per partition
       T->info.slots = S;
+      T->info.partition_slots = PS;
+      T->info.partitions = static_cast<uint32_t>(S / PS);
       const int shift = 64 - log2_pow2(S);
+      if (S / PS > 1 && n_rows) {
+        // every partition must keep an empty slot, or an insert run there
+        // never ends: rows per partition (an upper bound of its keys) < PS
+        const uint64_t P = S / PS;
+        DeviceBuffer pc(c, P * 4);
+        hipLaunchKernelGGL(k_fill_u32, grid_for(ctx, P), dim3(kBuildBlock), 0, st, pc.as<uint32_t>(), P, 0u);
+        hipLaunchKernelGGL(k_pt_count, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows,
+                           64 - log2_pow2(P), pc.as<uint32_t>());
+        PRISM_CHECK_LAUNCH();
+        std::vector<uint32_t> h(P);
+        PRISM_HIP(hipMemcpyAsync(h.data(), pc.ptr, P * 4, hipMemcpyDeviceToHost, st));
+        PRISM_HIP(hipStreamSynchronize(st));
+        const uint32_t mx = *std::max_element(h.begin(), h.end());
+        if (mx >= PS)
+          fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: a partition of " + std::to_string(PS) + " slots would hold " +
+                                               std::to_string(mx) + " keys; use fewer, larger partitions");
+      }
       T->slots.alloc(c, (S + 1) * sizeof(Slot));
       Slot* slots = T->slots.as<Slot>();
       hipLaunchKernelGGL(k_init_slots, grid_for(ctx, S + 1), dim3(kBuildBlock), 0, st, slots, S + 1);
       if (n_rows)
         hipLaunchKernelGGL(k_hashed_insert, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows,
-                           slots, S - 1, shift, slot_of_row.as<uint32_t>());
+                           slots, S - 1, PS - 1, shift, slot_of_row.as<uint32_t>());
       DeviceBuffer cnt(c, (S + 1) * 4);
       hipLaunchKernelGGL(k_slot_counts, grid_for(ctx, S + 1), dim3(kBuildBlock), 0, st, slots, S + 1,
                          cnt.as<uint32_t>());
@@ -2308,8 +2575,27 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       }
       T->view.slots = slots;
       T->view.mask = S - 1;
+      T->view.pmask = PS - 1;
       T->view.shift = shift;
       T->view.sentinel_present = hs.sentinel != 0;
+      if (probe_form == PRISM_PROBE_PARTITIONED) {
+        if (!T->info.unique || keys_only)
+          fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: the partitioned probe needs unique build keys");
+        const uint64_t P = S / PS;
+        // one partition is looked up in LDS beside a 32 KB tile buffer; more
+        // are joined one partition per workgroup
+        const uint64_t lds_need = PS * sizeof(Slot) + (P == 1 ? kTileRows * sizeof(uint2) + 4096 : 0);
+        if (lds_need > ctx->lds_per_block)
+          fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: a partition of " + std::to_string(PS) +
+                                               " slots does not fit the " + std::to_string(ctx->lds_per_block) +
+                                               " bytes of LDS a workgroup may use" +
+                                               (P == 1 ? " beside the tile buffer" : ""));
+        if (P > kPtMaxParts)
+          fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: the partitioned probe takes at most " +
+                                               std::to_string(kPtMaxParts) + " partitions (" + std::to_string(P) +
+                                               " asked); use larger partitions");
+        T->info.probe = PRISM_PROBE_PARTITIONED;
+      }
     }
     if (d_payload && n_rows && !fast) {  // carry the build payload in the table entries
       T->view.has_payload = 1;
@@ -2347,6 +2633,83 @@ prism_status prism_join_build_i64_payload(prism_ctx* ctx, const int64_t* d_keys,
   return build_impl(ctx, d_keys, d_payload, n_rows, kind, out);
 }
 
+prism_status prism_join_build_i64_opts(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload,
+                                       uint64_t n_rows, const prism_join_build_opts* opts, prism_join_table** out) {
+  if (!opts) return build_impl(ctx, d_keys, d_payload, n_rows, PRISM_TABLE_AUTO, out);
+  return build_impl(ctx, d_keys, d_payload, n_rows, opts->kind, out, false, opts);
+}
+
+prism_status prism_join_tune(prism_ctx* ctx, const int64_t* d_keys, const int32_t* d_payload, uint64_t n_rows,
+                             const int64_t* d_probe_sample, const int64_t* d_probe_payload, uint64_t n_sample,
+                             prism_join_build_opts* best, prism_join_tune_candidate* cands, uint32_t cap,
+                             uint32_t* n_cands) {
+  return guarded([&] {
+    if (!ctx || !best) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL context or output");
+    if (n_sample && !d_probe_sample) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL probe sample");
+    use_device(ctx);
+    Ctx* c = as_ctx(ctx);
+    std::vector<prism_join_build_opts> list;
+    list.push_back({PRISM_TABLE_AUTO, PRISM_PROBE_DIRECT, 0, 0.0});
+    for (double lf : {0.5, 0.7}) list.push_back({PRISM_TABLE_HASHED, PRISM_PROBE_DIRECT, 0, lf});
+    for (double lf : {0.5, 0.7})
+      for (uint64_t ps : {1024ULL, 2048ULL, 4096ULL, 8192ULL}) {
+        const uint64_t want = static_cast<uint64_t>(static_cast<double>(n_rows) / lf) + 1;
+        const uint64_t S = std::max<uint64_t>(next_pow2(want > 64 ? want : 64), ps);
+        const uint64_t P = S / ps;
+        if (P > kPtMaxParts || (P == 1 && ps > 4096)) continue;
+        list.push_back({PRISM_TABLE_HASHED, PRISM_PROBE_PARTITIONED, ps, lf});
+      }
+    const uint64_t room = n_sample ? n_sample : 1;
+    DeviceBuffer op(c, room * 4), ob(c, room * 4), res(c, sizeof(prism_probe_result));
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    PRISM_HIP(hipEventCreate(&e0));
+    PRISM_HIP(hipEventCreate(&e1));
+    PRISM_HIP(hipEventCreate(&e2));
+    struct Events {
+      hipEvent_t* e[3];
+      ~Events() {
+        for (hipEvent_t* x : e) (void)hipEventDestroy(*x);
+      }
+    } guard_ev{{&e0, &e1, &e2}};
+    double best_ms = -1.0;
+    uint32_t k = 0;
+    for (const prism_join_build_opts& o : list) {
+      prism_join_tune_candidate cd{};
+      cd.opts = o;
+      prism_join_table* t = nullptr;
+      PRISM_HIP(hipEventRecord(e0, ctx->stream));
+      cd.status = build_impl(ctx, d_keys, d_payload, n_rows, o.kind, &t, false, &o);
+      if (cd.status == PRISM_OK) {
+        PRISM_HIP(hipEventRecord(e1, ctx->stream));
+        prism_status s = PRISM_OK;
+        for (int rep = 0; rep < 2 && s == PRISM_OK; ++rep) {  // a warm-up, then the timed probe
+          if (rep == 1) PRISM_HIP(hipEventRecord(e1, ctx->stream));
+          s = prism_join_probe_i64(ctx, t, d_probe_sample, n_sample, 0, d_probe_payload, nullptr,
+                                   op.as<uint32_t>(), ob.as<uint32_t>(), room, res.as<prism_probe_result>());
+        }
+        PRISM_HIP(hipEventRecord(e2, ctx->stream));
+        PRISM_HIP(hipEventSynchronize(e2));
+        float b = 0, pr = 0;
+        PRISM_HIP(hipEventElapsedTime(&b, e0, e1));
+        PRISM_HIP(hipEventElapsedTime(&pr, e1, e2));
+        cd.build_ms = b;
+        cd.probe_ms = pr;
+        cd.status = s;
+        if (s == PRISM_OK && (best_ms < 0 || pr < best_ms)) {
+          best_ms = pr;
+          *best = o;
+        }
+        prism_join_table_free(t);
+      }
+      if (cands && k < cap) cands[k] = cd;
+      ++k;
+    }
+    clear_last_error();
+    if (n_cands) *n_cands = k < cap || !cands ? k : cap;
+    if (best_ms < 0) fail(PRISM_ERR_EXECUTION, "Execution error: no table layout could be built for these keys");
+  });
+}
+
 prism_status prism_join_table_info_get(const prism_join_table* t, prism_join_table_info* out) {
   return guarded([&] {
     if (!t || !out) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL table or output");
@@ -2372,6 +2735,45 @@ static uint64_t align256(uint64_t b) { return (b + 255) & ~255ULL; }
 // pairs or the duplicate path's look-back status words.  Every word the
 // kernels read is written by them first (k_probe_clear clears the status
 // words), so the scratch needs no clearing.
+// Scratch of the partitioned probe's passes A-D (after the direct probe's):
+// segment counts and offsets (partitions x tiles + 1), the scan's temporary
+// storage, the records' keys and rows in tile, the per-record results.
+struct PtPlan {
+  bool on = false;   // the table is probed partitioned
+  bool direct = false;  // one partition: looked up in LDS without passes A-D
+  uint32_t P = 1, n_tiles = 0, M = 1;
+  uint64_t hist = 0, offs = 0, tmp = 0, tmp_bytes = 0, rkey = 0, rrow = 0, res = 0, bytes = 0;
+};
+
+static PtPlan pt_plan(const prism_join_table* t, uint64_t n_probe) {
+  PtPlan q;
+  if (t->info.probe != PRISM_PROBE_PARTITIONED || !t->info.unique || t->info.kind != PRISM_TABLE_HASHED) return q;
+  q.on = true;
+  q.P = t->info.partitions;
+  q.direct = q.P == 1;
+  q.n_tiles = static_cast<uint32_t>(ceil_div(n_probe, kTileRows));
+  if (q.direct || n_probe == 0) return q;
+  const uint64_t cells = static_cast<uint64_t>(q.P) * q.n_tiles;
+  if (cells >= (1ULL << 31)) {  // the segment table is scanned with int counts: probe directly
+    q.on = false;
+    return q;
+  }
+  q.M = static_cast<uint32_t>(ceil_div(ceil_div(n_probe, q.P), kPtChunk));
+  if (q.M < 1) q.M = 1;
+  size_t tb = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, static_cast<const uint32_t*>(nullptr),
+                                         static_cast<uint32_t*>(nullptr), static_cast<int>(cells));
+  q.tmp_bytes = tb;
+  q.hist = 0;
+  q.offs = q.hist + align256(cells * 4);
+  q.tmp = q.offs + align256((cells + 1) * 4);
+  q.rkey = q.tmp + align256(tb);
+  q.rrow = q.rkey + align256(n_probe * 8);
+  q.res = q.rrow + align256(n_probe * 2);
+  q.bytes = q.res + align256(n_probe * 8);
+  return q;
+}
+
 static uint64_t probe_scratch_bytes(const prism_join_table* t, uint64_t n_probe, bool pairs) {
   const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
   const uint64_t sums_bytes = align256(n_tiles * 32 + 32);
@@ -2379,10 +2781,48 @@ static uint64_t probe_scratch_bytes(const prism_join_table* t, uint64_t n_probe,
     const uint64_t n_slices_max = ceil_div(n_tiles, kSliceTiles) + 1;
     const uint64_t cnt_bytes = align256(n_tiles * 4 + 4), off_bytes = align256(n_tiles * 8 + 8);
     const uint64_t slice_bytes = align256(n_slices_max * 8) + align256(n_slices_max * 32);
-    const uint64_t stage_bytes = pairs ? n_tiles * kTileRows * sizeof(uint2) : 0;
-    return sums_bytes + cnt_bytes + off_bytes + slice_bytes + stage_bytes;
+    const uint64_t stage_bytes = align256(pairs ? n_tiles * kTileRows * sizeof(uint2) : 0);
+    return sums_bytes + cnt_bytes + off_bytes + slice_bytes + stage_bytes + pt_plan(t, n_probe).bytes;
   }
   return sums_bytes + align256((n_tiles + 1) * 8);
+}
+
+// Passes A-E of the partitioned probe (see k_pt_restore), writing what
+// k_probe_unique writes: staged pairs, tile counts, tile payload partials.
+static void pt_probe(prism_ctx* ctx, const prism_join_table* t, const PtPlan& q, const int64_t* keys, uint64_t n,
+                     uint32_t probe_base, const int64_t* ppay, const int32_t* bpay, uint2* staged,
+                     uint32_t* tile_count, __int128* tile_sums, char* pt) {
+  hipStream_t st = ctx->stream;
+  const TableView& T = t->view;
+  const uint64_t PS = T.pmask + 1;
+  if (q.direct) {
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(q.n_tiles, static_cast<uint64_t>(ctx->num_cus) * 2));
+    hipLaunchKernelGGL(k_pt_restore<true>, dim3(grid), dim3(kPB), PS * sizeof(Slot), st, T, keys, n, q.n_tiles,
+                       nullptr, 1u, nullptr, nullptr, probe_base, ppay, bpay, staged, tile_count, tile_sums);
+    PRISM_CHECK_LAUNCH();
+    return;
+  }
+  uint32_t* hist = reinterpret_cast<uint32_t*>(pt + q.hist);
+  uint32_t* offs = reinterpret_cast<uint32_t*>(pt + q.offs);
+  int64_t* rkey = reinterpret_cast<int64_t*>(pt + q.rkey);
+  uint16_t* rrow = reinterpret_cast<uint16_t*>(pt + q.rrow);
+  uint2* res = reinterpret_cast<uint2*>(pt + q.res);
+  const int pshift = 64 - log2_pow2(q.P);
+  const uint64_t cells = static_cast<uint64_t>(q.P) * q.n_tiles;
+  hipLaunchKernelGGL(k_pt_hist, dim3(q.n_tiles), dim3(kPB), q.P * 4, st, keys, n, pshift, q.P, q.n_tiles, hist);
+  PRISM_CHECK_LAUNCH();
+  size_t tb = q.tmp_bytes;
+  PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(pt + q.tmp, tb, static_cast<const uint32_t*>(hist), offs,
+                                             static_cast<int>(cells), st));
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, offs + cells, static_cast<uint32_t>(n));
+  hipLaunchKernelGGL(k_pt_scatter, dim3(q.n_tiles), dim3(kPB), q.P * 4, st, keys, n, pshift, q.P, q.n_tiles,
+                     static_cast<const uint32_t*>(offs), rkey, rrow);
+  hipLaunchKernelGGL(k_pt_join, dim3(q.P * q.M), dim3(kPB), PS * sizeof(Slot), st, T,
+                     static_cast<const int64_t*>(rkey), static_cast<const uint32_t*>(offs), q.n_tiles, q.M, res);
+  hipLaunchKernelGGL(k_pt_restore<false>, dim3(q.n_tiles), dim3(kPB), 0, st, T, keys, n, q.n_tiles,
+                     static_cast<const uint32_t*>(offs), q.P, static_cast<const uint16_t*>(rrow),
+                     static_cast<const uint2*>(res), probe_base, ppay, bpay, staged, tile_count, tile_sums);
+  PRISM_CHECK_LAUNCH();
 }
 
 static void probe_checks(prism_ctx* ctx, const prism_join_table* t, const int64_t* d_probe_keys, uint64_t n_probe,
@@ -2421,13 +2861,19 @@ static void probe_launch(prism_ctx* ctx, const prism_join_table* t, const int64_
     uint64_t* slice_total = reinterpret_cast<uint64_t*>(p + cnt_bytes + off_bytes);
     __int128* slice_sums = reinterpret_cast<__int128*>(p + cnt_bytes + off_bytes + align256(n_slices_max * 8));
     uint2* staged = pairs ? reinterpret_cast<uint2*>(p + cnt_bytes + off_bytes + slice_bytes) : nullptr;
+    const uint64_t stage_bytes = align256(pairs ? n_tiles * kTileRows * sizeof(uint2) : 0);
     hipLaunchKernelGGL(k_probe_clear, dim3(1), dim3(64), 0, ctx->stream, nullptr, 0, d_result);
     PRISM_CHECK_LAUNCH();
     if (n_probe == 0) return;
 #define PRISM_PROBE1(KIND_, VEC_)                                                                          \
   hipLaunchKernelGGL((k_probe_unique<KIND_, VEC_>), g, dim3(kPB), 0, ctx->stream, T, d_probe_keys, n_probe, \
                      probe_row_base, d_probe_payload, d_build_payload, staged, tile_count, tile_sums)
-    {
+    const PtPlan q = pt_plan(t, n_probe);
+    if (q.on) {
+      std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe") : nullptr);
+      pt_probe(ctx, t, q, d_probe_keys, n_probe, probe_row_base, d_probe_payload, d_build_payload, staged,
+               tile_count, tile_sums, p + cnt_bytes + off_bytes + slice_bytes + stage_bytes);
+    } else {
       std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe") : nullptr);
       if (t->info.kind == PRISM_TABLE_DENSE) {
         if (vec) PRISM_PROBE1(PRISM_TABLE_DENSE, true);

@@ -9,6 +9,8 @@ namespace join {
 
 constexpr int64_t kEmpty = static_cast<int64_t>(0x8000000000000000ULL);  // INT64_MIN
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// Slots of one partition the partitioned probe loads into LDS (16 B each).
+constexpr uint64_t kPtMaxSlots = 8192;
 
 struct Slot {
   int64_t key;
@@ -21,6 +23,11 @@ struct TableView {
   const Slot* slots;        // HASHED: mask + 2 entries; entry mask+1 holds the INT64_MIN key
   uint64_t mask;
   int shift;
+  // HASHED: slots of one partition - 1.  The slot array is cut into equal
+  // partitions by the hash's top bits and a probe run wraps inside its own
+  // partition, so each partition is an open-addressing table of its own (one
+  // partition, pmask = mask, is the plain table).
+  uint64_t pmask;
   const uint32_t* dense_u;  // DENSE unique: row per key offset, kNone = absent
   const uint2* dense_nu;    // DENSE duplicates: {start, cnt}
   int64_t min_key;
@@ -92,7 +99,7 @@ __device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& 
         return true;
       }
       if (e.key == kEmpty) return false;
-      s = (s + 1) & T.mask;
+      s = (s & ~T.pmask) | ((s + 1) & T.pmask);
     }
   }
 }

@@ -62,7 +62,20 @@ class PString(Structure):
 class JoinTableInfo(Structure):
     _fields_ = [("n_rows", c_uint64), ("n_distinct", c_uint64), ("max_dup", c_uint64),
                 ("min_key", c_int64), ("max_key", c_int64), ("kind", c_int32), ("unique", c_int32),
-                ("slots", c_uint64), ("device_bytes", c_uint64)]
+                ("slots", c_uint64), ("device_bytes", c_uint64), ("partition_slots", c_uint64),
+                ("partitions", c_uint32), ("probe", c_int32)]
+
+
+PROBE_DIRECT, PROBE_PARTITIONED = 0, 1
+
+
+class BuildOpts(Structure):
+    _fields_ = [("kind", c_int32), ("probe", c_int32), ("partition_slots", c_uint64), ("load_factor", c_double)]
+
+
+class TuneCandidate(Structure):
+    _fields_ = [("opts", BuildOpts), ("build_ms", c_double), ("probe_ms", c_double), ("status", c_int32),
+                ("reserved", c_int32)]
 
 
 class ProbeResult(Structure):
@@ -226,6 +239,10 @@ _SIGNATURES = [
     ("prism_join_build_i64_payload", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_int32,
                                                POINTER(c_void_p)]),
     ("prism_join_table_info_get", c_int32, [c_void_p, POINTER(JoinTableInfo)]),
+    ("prism_join_build_i64_opts", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, POINTER(BuildOpts),
+                                            POINTER(c_void_p)]),
+    ("prism_join_tune", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_void_p, c_void_p, c_uint64,
+                                  POINTER(BuildOpts), POINTER(TuneCandidate), c_uint32, POINTER(c_uint32)]),
     ("prism_join_table_free", None, [c_void_p]),
     ("prism_join_probe_i64", c_int32, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint32, c_void_p,
                                        c_void_p, c_void_p, c_void_p, c_uint64, c_void_p]),

@@ -172,11 +172,14 @@ def col_utf8(data, offsets, validity=None):
 class JoinTable:
     """HashTable::build on a device-resident int64 key column (hash_join.rs:87-141)."""
 
-    def __init__(self, ctx: Context, keys, kind: int = L.TABLE_AUTO, payload=None):
+    def __init__(self, ctx: Context, keys, kind: int = L.TABLE_AUTO, payload=None, opts: Optional[L.BuildOpts] = None):
         self.ctx = ctx
         self.lib = ctx.lib
         self.handle = c_void_p()
-        if payload is None:
+        if opts is not None:  # layout and probe form chosen by the caller (or by tune())
+            L.check(self.lib.prism_join_build_i64_opts(ctx.handle, ptr(keys), ptr(payload), keys.numel(), byref(opts),
+                                                       byref(self.handle)))
+        elif payload is None:
             L.check(self.lib.prism_join_build_i64(ctx.handle, ptr(keys), keys.numel(), kind, byref(self.handle)))
         else:
             L.check(self.lib.prism_join_build_i64_payload(ctx.handle, ptr(keys), ptr(payload), keys.numel(), kind,
@@ -215,6 +218,18 @@ class JoinTable:
             if want_pairs:
                 return op[: int(r.matches)], ob[: int(r.matches)], r
             return None, None, r
+
+    @staticmethod
+    def tune(ctx: Context, keys, payload=None, sample=None, sample_payload=None):
+        """prism_join_tune: (fastest BuildOpts, [TuneCandidate, ...]) for these
+        build keys, timed on one probe of `sample`."""
+        best = L.BuildOpts()
+        cands = (L.TuneCandidate * 32)()
+        n = ctypes.c_uint32()
+        L.check(ctx.lib.prism_join_tune(ctx.handle, ptr(keys), ptr(payload), keys.numel(), ptr(sample),
+                                        ptr(sample_payload), sample.numel() if sample is not None else 0,
+                                        byref(best), cands, 32, byref(n)))
+        return best, [cands[i] for i in range(n.value)]
 
     def close(self) -> None:
         if self.handle:
