@@ -258,6 +258,91 @@ def bench_groupby(ctx, n: int, steps: int, cpu_sample: int, check: bool = True) 
     return res
 
 
+def bench_join_layouts(ctx, steps: int, bk, bp, pk, pp, label: str, sample: int = 8_000_000) -> dict:
+    """The join-layout autotuner on one build side: every candidate layout
+    (AUTO table probed directly; HASHED tables of load factor 0.5 / 0.7 probed
+    directly; HASHED tables cut into 1024..8192-slot partitions probed
+    radix-partitioned through LDS) timed on a probe sample, then the full
+    probe (pairs + exact payload sums) timed with the tuned layout and with
+    the direct HASHED table beside it.  Inputs resident in HBM."""
+    import numpy as np
+
+    from prism_executor import _lib as L
+    from prism_executor import device as D
+
+    n_probe = pk.numel()
+    m = min(sample, n_probe)
+    best, cands = D.JoinTable.tune(ctx, bk, bp, pk[:m], pp[:m])
+    res = D.empty(ctx, ctypes.sizeof(L.ProbeResult), np.uint8)
+    op, ob = D.empty(ctx, n_probe, np.uint32), D.empty(ctx, n_probe, np.uint32)
+
+    def timed_probe(opts):
+        t = D.JoinTable(ctx, bk, payload=bp, opts=opts)
+        t.probe_async(pk, res, pp, None, op, ob, capacity=n_probe)
+        ctx.timer_begin()
+        for _ in range(steps):
+            t.probe_async(pk, res, pp, None, op, ob, capacity=n_probe)
+        ms = ctx.timer_end() / steps
+        r = L.ProbeResult.from_buffer_copy(res.numpy().tobytes())
+        info = t.info
+        t.close()
+        return ms, r, info
+
+    tuned_ms, r_t, info_t = timed_probe(best)
+    direct_ms, r_d, _ = timed_probe(L.BuildOpts(L.TABLE_HASHED, L.PROBE_DIRECT, 0, 0.5))
+    alg = n_probe * 16 + int(r_t.matches) * 8
+    same = (r_t.matches, r_t.probe_sum, r_t.build_sum) == (r_d.matches, r_d.probe_sum, r_d.build_sum)
+    for a in (res, op, ob):
+        a.free()
+    form = {0: "direct", 1: "partitioned"}
+    return {"workload": label, "matches": int(r_t.matches), "tuned_equals_direct_hashed": bool(same),
+            "tuned": {"kind": {0: "auto", 1: "hashed", 2: "dense"}[best.kind], "probe": form[best.probe],
+                      "load_factor": best.load_factor or 0.5, "partition_slots": int(best.partition_slots),
+                      "partitions": int(info_t.partitions), "table_kind": "dense" if info_t.kind == 2 else "hashed"},
+            "tuned_probe_ms": tuned_ms, "tuned_GBps": alg / (tuned_ms * 1e-3) / 1e9,
+            "tuned_hbm_frac": alg / (tuned_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "direct_hashed_probe_ms": direct_ms, "speedup_vs_direct_hashed": direct_ms / tuned_ms,
+            "tuner_sample_rows": m,
+            "candidates": [{"kind": c.opts.kind, "probe": form[c.opts.probe], "load_factor": c.opts.load_factor,
+                            "partition_slots": int(c.opts.partition_slots), "status": int(c.status),
+                            "build_ms": c.build_ms, "probe_ms": c.probe_ms} for c in cands]}
+
+
+def bench_large_hashed_join(ctx, steps: int) -> dict:
+    """A build side as large as the Infinity Cache: 8M unique keys spread
+    over the whole int64 range (a seeded permutation of 0..8M-1 multiplied by
+    an odd constant: sparse, so HASHED; 256 MB of slots at load factor 0.5),
+    int32 payload; 64M probe rows, half of them build keys; the layout tuner
+    against the direct HASHED table."""
+    import numpy as np
+
+    from prism_executor import _lib as L
+    from prism_executor import datagen as dg
+    from prism_executor import device as D
+
+    lib = ctx.lib
+    nb, npr = 8_000_000, 64_000_000
+    odd = 0x9E3779B97F4A7C15 - (1 << 64)  # an odd multiplier as int64
+    bk, bp = D.empty(ctx, nb, np.int64), D.empty(ctx, nb, np.int32)
+    pk, pp = D.empty(ctx, npr, np.int64), D.empty(ctx, npr, np.int64)
+    L.check(lib.prism_datagen_permutation_i64(ctx.handle, bk.ptr, nb, dg.SEED, 90))
+    L.check(lib.prism_datagen_uniform_i32(ctx.handle, bp.ptr, nb, dg.SEED, 91, -(1 << 31), 1 << 32))
+    L.check(lib.prism_datagen_uniform_i64(ctx.handle, pk.ptr, npr, dg.SEED, 92, 0, 2 * nb))
+    L.check(lib.prism_datagen_bits_i64(ctx.handle, pp.ptr, npr, dg.SEED, 93))
+    ctx.sync()
+    # sparse keys: multiply on the host side of the device buffers
+    hb, hp = bk.numpy(), pk.numpy()
+    with np.errstate(over="ignore"):
+        bk2 = D.to_device(ctx, hb * np.int64(odd))
+        pk2 = D.to_device(ctx, hp * np.int64(odd))
+    out = bench_join_layouts(ctx, steps, bk2, bp, pk2, pp,
+                             f"inner equi-join, {nb} sparse unique int64 build keys (int32 payload) x {npr} probe rows "
+                             "(half match, int64 payload): pairs + exact payload sums")
+    for a in (bk, bp, pk, pp, bk2, pk2):
+        a.free()
+    return out
+
+
 def bench_groupby_1m(ctx, steps: int, cpu: bool) -> dict:
     """Plain hash aggregation of one 1M-row batch: key int64 U[0, 1000), value
     int64 U[0, 10^6); SUM, COUNT, MIN, MAX by key, checked bit for bit against
@@ -777,6 +862,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--groupby-rows", type=int, default=200_000_000)
     ap.add_argument("--no-groupby", action="store_true")
+    ap.add_argument("--no-layouts", action="store_true", help="skip the join layout tuner sections")
     ap.add_argument("--no-join-agg", action="store_true")
     ap.add_argument("--join-agg-probe-rows", type=int, default=256_000_000)
     ap.add_argument("--no-service", action="store_true")
@@ -909,6 +995,11 @@ def main() -> int:
         "matches": int(r.matches),
     }
     progress("join", t_start)
+    if not args.no_layouts:
+        line["join_layouts"] = bench_join_layouts(
+            ctx, min(args.steps, 5), bk, bp, pk, pp, f"config above: build {nb} keys (permutation), probe {npr} rows")
+        line["join_large_hashed"] = bench_large_hashed_join(ctx, min(args.steps, 5))
+        progress("join layouts", t_start)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(bk.numpy(), pk[:args.cpu_sample].numpy())
         progress("join cpu baseline", t_start)
