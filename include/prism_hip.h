@@ -268,8 +268,9 @@ prism_status prism_join_build_i64_opts(prism_ctx* ctx, const int64_t* d_keys, co
 
 /* Autotuner of the table layout and probe form (no counterpart in the
  * reference): builds a table of the keys (and payload) for each candidate --
- * the AUTO table probed directly, and HASHED tables of load factor 0.5 / 0.7 cut
- * into partitions of 1024..8192 slots, probed directly and partitioned -- times
+ * the AUTO table probed directly, HASHED tables of load factor 0.5 / 0.7 probed
+ * directly, and the same cut into partitions of 1024..8192 slots and probed
+ * partitioned, where the partitions fit the probe's limits -- times
  * one probe of the sample with pairs and payload sums on the context stream,
  * and writes the fastest candidate's options to *best.  cands (room for cap
  * entries, may be NULL) receives every candidate tried, in order. */
