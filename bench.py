@@ -828,6 +828,7 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7, conc_threads: int 
     # the run with the median served / plain ratio reported
     z = np.zeros(len(reqs))
     runs = []
+    S.reset_stats()
     for _ in range(3):
         sv, pl, pc = summary(*run_served(z)), summary(*run_plain(z)), summary(*run_plain_concurrent(z))
         runs.append((sv["rows_per_s"] / pl["rows_per_s"], sv, pl, pc))
@@ -836,6 +837,10 @@ def bench_service(ctx, n_requests: int, rates, seed: int = 7, conc_threads: int 
     sat = {"served": mid[1], "plain": mid[2], "plain_concurrent": mid[3], "throughput_ratio": mid[0],
            "throughput_ratio_vs_concurrent": mid[1]["rows_per_s"] / mid[3]["rows_per_s"],
            "throughput_ratios_all_runs": [r[0] for r in runs]}
+    st = S.stats()
+    sat["service"] = {k: st[k] for k in ("batches", "mean_batch_requests", "mean_batch_rows", "mean_batch_gpu_ms",
+                                         "max_batch_gpu_ms", "mean_batch_kernel_ms", "mean_batch_launch_ms",
+                                         "mean_batch_detect_ms", "graph_launches") if k in st}
     out["saturation"] = sat
     out["service_stats"] = S.stats()
     S.close()
@@ -1021,7 +1026,8 @@ def main() -> int:
         line["parquet_scan"] = bench_parquet(ctx, args.parquet_rows, max(2, min(args.steps, 5)))
         progress("parquet scan", t_start)
     if not args.no_service:
-        line["served_stream"] = bench_service(ctx, args.svc_requests, [int(x) for x in args.svc_rates.split(",")])
+        line["served_stream"] = bench_service(ctx, args.svc_requests,
+                                              [int(x) for x in args.svc_rates.split(",") if x.strip()])
         progress("served stream", t_start)
     if args.calibrate_copy:
         src = D.empty(ctx, args.calibrate_copy, np.uint8)

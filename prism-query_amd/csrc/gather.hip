@@ -152,7 +152,8 @@ __global__ void k_concat_u32(uint32_t* dst, const uint32_t* __restrict__ src, ui
 // ---------------------------------------------------------------------------
 // Host wrappers.
 // ---------------------------------------------------------------------------
-DevColumn take(prism_ctx* ctx, const DevColumn& src, const uint32_t* idx, const uint8_t* idx_valid, uint64_t n) {
+DevColumn take(prism_ctx* ctx, const DevColumn& src, const uint32_t* idx, const uint8_t* idx_valid, uint64_t n,
+               uint32_t* err_word) {
   Ctx* c = as_ctx(ctx);
   hipStream_t st = ctx->stream;
   DevColumn d;
@@ -163,10 +164,17 @@ DevColumn take(prism_ctx* ctx, const DevColumn& src, const uint32_t* idx, const 
   d.dev_offset = 0;
   const dim3 g = ggrid(ctx, n), b(kGB);
   const uint64_t sl = src.length > 0 ? static_cast<uint64_t>(src.length) : 0;
-  DeviceBuffer err(c, 4);
-  fill_bytes(ctx, err.as<uint8_t>(), 4, 0);
-  uint32_t* e = err.as<uint32_t>();
+  // With the caller's error word the kernels only set it; else an own word is
+  // read back here (a synchronisation).
+  DeviceBuffer err;
+  uint32_t* e = err_word;
+  if (!e) {
+    err.alloc(c, 4);
+    fill_bytes(ctx, err.as<uint8_t>(), 4, 0);
+    e = err.as<uint32_t>();
+  }
   auto check_err = [&] {
+    if (err_word) return;
     uint32_t h = 0;
     PRISM_HIP(hipMemcpyAsync(&h, e, 4, hipMemcpyDeviceToHost, st));
     PRISM_HIP(hipStreamSynchronize(st));

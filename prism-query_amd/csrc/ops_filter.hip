@@ -31,6 +31,7 @@
 
 #include "arrow_io.hpp"
 #include "gather.hpp"
+#include "like_scan.hpp"
 #include "program.hpp"
 
 namespace prism {
@@ -44,6 +45,13 @@ struct DevProgram {
 };
 
 enum OutKind : int32_t { OUT_BITS = 0, OUT_W8 = 1, OUT_W4 = 2 };
+
+// A program that is one LIKE '%literal%' of a Utf8 column (col >= 0, from
+// like_scan_prepare): a bitmap of it runs as the byte scan.
+struct LikeScan {
+  int col = -1;
+  LikeArg arg;
+};
 
 constexpr int kEB = 256;  // threads per workgroup
 
@@ -91,6 +99,25 @@ __global__ __launch_bounds__(kEB) void k_eval(const DevProgram* __restrict__ P, 
   // one atomic per wave (wave-uniform value)
   if (lane == 0 && nulls) atomicAdd(null_count, nulls);
   if (lane == 0 && selected) atomicAdd(null_count + 2, selected);
+}
+
+// A program that is one LIKE '%literal%' of a Utf8 column: the byte scan of
+// like_scan.hpp over sub-tiles of kLsRows rows.
+__global__ __launch_bounds__(kLsThreads) void k_like_scan(const LikeArg* __restrict__ A,
+                                                          const prism_column* __restrict__ cols, int col, uint64_t n,
+                                                          unsigned long long* __restrict__ out,
+                                                          unsigned long long* __restrict__ out_valid,
+                                                          unsigned long long* __restrict__ counters) {
+  __shared__ uint32_t s_bits[kLsBitWords];
+  const prism_column c = cols[col];
+  unsigned long long nulls = 0, selected = 0;
+  for (uint64_t r0 = blockIdx.x * static_cast<uint64_t>(kLsRows); r0 < n;
+       r0 += static_cast<uint64_t>(gridDim.x) * kLsRows)
+    like_scan_tile(c, r0, r0 + kLsRows < n ? r0 + kLsRows : n, *A, s_bits, out, out_valid, nulls, selected);
+  if ((threadIdx.x & 63) == 0) {
+    if (nulls) atomicAdd(counters, nulls);
+    if (selected) atomicAdd(counters + 2, selected);
+  }
 }
 
 // Mask words from an Arrow BooleanArray (values AND validity), bit offset < 8.
@@ -475,7 +502,7 @@ struct Compiler {
   }
 
   // Check the program and upload it.
-  ValType finish(prism_ctx* ctx, DeviceBuffer& dev, int* depth) {
+  ValType finish(prism_ctx* ctx, DeviceBuffer& dev, int* depth, LikeScan* like = nullptr) {
     std::vector<int32_t> types(cols.size());
     for (size_t i = 0; i < cols.size(); ++i)
       types[i] = cols[i].type == DType::UInt32 || cols[i].type == DType::Decimal128 ? 0 : static_cast<int32_t>(cols[i].type);
@@ -493,6 +520,7 @@ struct Compiler {
       ps[i].len = static_cast<uint32_t>(strings[i].size());
     }
     build_str_table(ps.data(), static_cast<uint32_t>(ps.size()), P->strs);
+    if (like) like->col = like_scan_prepare(code.data(), static_cast<uint32_t>(code.size()), P->strs, like->arg);
     dev.alloc(as_ctx(ctx), sizeof(DevProgram));
     PRISM_HIP(hipMemcpyAsync(dev.ptr, P.get(), sizeof(DevProgram), hipMemcpyHostToDevice, ctx->stream));
     PRISM_HIP(hipStreamSynchronize(ctx->stream));  // P is a host temporary
@@ -542,7 +570,7 @@ struct EvalOut {
 };
 
 static void run_eval(prism_ctx* ctx, const DevProgram* P, int depth, const prism_column* cols, uint64_t n,
-                     OutKind kind, EvalOut& o, void* out_values = nullptr) {
+                     OutKind kind, EvalOut& o, void* out_values = nullptr, const LikeScan* like = nullptr) {
   Ctx* c = as_ctx(ctx);
   hipStream_t st = ctx->stream;
   const uint64_t nw = ceil_div(n, 64);
@@ -551,7 +579,17 @@ static void run_eval(prism_ctx* ctx, const DevProgram* P, int depth, const prism
   o.valid.alloc(c, (nw ? nw : 1) * 8);
   o.counters.alloc(c, 32);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, o.counters.as<unsigned long long>(), 3);
-  if (n) {
+  DeviceBuffer la;
+  if (n && like && like->col >= 0 && kind == OUT_BITS) {
+    la.alloc(c, sizeof(LikeArg));
+    PRISM_HIP(hipMemcpyAsync(la.ptr, &like->arg, sizeof(LikeArg), hipMemcpyHostToDevice, st));
+    const uint64_t tiles = ceil_div(n, static_cast<uint64_t>(kLsRows));
+    const uint64_t cap = static_cast<uint64_t>(ctx->num_cus) * 8;
+    KernelTimer kt(ctx, "filter_like_scan");
+    hipLaunchKernelGGL(k_like_scan, dim3(static_cast<unsigned>(tiles < cap ? tiles : cap)), dim3(kLsThreads), 0, st,
+                       la.as<LikeArg>(), cols, like->col, n, static_cast<unsigned long long*>(vals),
+                       o.valid.as<unsigned long long>(), o.counters.as<unsigned long long>());
+  } else if (n) {
     KernelTimer kt(ctx, "filter_eval");
     if (depth <= 4)
       hipLaunchKernelGGL((k_eval<4, 4>), egrid(ctx, nw, 4), dim3(kEB), 0, st, P, cols, n, static_cast<int>(kind),
@@ -581,8 +619,10 @@ static void predicate_words(prism_ctx* ctx, const HostBatch& hb, const prism_pre
   upload_cols(ctx, hb, used, db);
   DeviceBuffer prog;
   int depth = 1;
-  C.finish(ctx, prog, &depth);
-  run_eval(ctx, prog.as<DevProgram>(), depth, db.desc.as<prism_column>(), static_cast<uint64_t>(hb.n_rows), OUT_BITS, o);
+  LikeScan like;
+  C.finish(ctx, prog, &depth, &like);
+  run_eval(ctx, prog.as<DevProgram>(), depth, db.desc.as<prism_column>(), static_cast<uint64_t>(hb.n_rows), OUT_BITS, o,
+           nullptr, &like);
 }
 
 static OutColumn bits_column(prism_ctx* ctx, const EvalOut& o, uint64_t n, const std::string& name, DType type) {
@@ -999,13 +1039,15 @@ prism_status prism_filter_mask(prism_ctx* ctx, const prism_instr* code, uint32_t
     P->len = static_cast<int32_t>(n_code);
     memcpy(P->code, code, n_code * sizeof(prism_instr));
     build_str_table(strings, n_strings, P->strs);
+    LikeScan like;
+    like.col = like_scan_prepare(code, n_code, P->strs, like.arg);
     Ctx* c = as_ctx(ctx);
     DeviceBuffer prog(c, sizeof(DevProgram)), desc(c, (n_columns ? n_columns : 1) * sizeof(prism_column));
     PRISM_HIP(hipMemcpyAsync(prog.ptr, P.get(), sizeof(DevProgram), hipMemcpyHostToDevice, ctx->stream));
     if (n_columns)
       PRISM_HIP(hipMemcpyAsync(desc.ptr, columns, n_columns * sizeof(prism_column), hipMemcpyHostToDevice, ctx->stream));
     EvalOut o;
-    run_eval(ctx, prog.as<DevProgram>(), depth, desc.as<prism_column>(), n_rows, OUT_BITS, o, d_mask);
+    run_eval(ctx, prog.as<DevProgram>(), depth, desc.as<prism_column>(), n_rows, OUT_BITS, o, d_mask, &like);
     *selected = o.selected;
   });
 }

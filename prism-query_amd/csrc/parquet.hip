@@ -985,7 +985,7 @@ static DevColumn decode_device(prism_ctx* ctx, HostChunk& H, uint32_t* err) {
   DevColumn out;
   {
     KernelTimer kt(ctx, "parquet_gather");
-    out = take(ctx, S, idx.as<uint32_t>(), H.null_count ? valid.as<uint8_t>() : nullptr, rows);
+    out = take(ctx, S, idx.as<uint32_t>(), H.null_count ? valid.as<uint8_t>() : nullptr, rows, err);
   }
   out.name = H.leaf.name;
   out.nullable = H.leaf.optional;

@@ -38,6 +38,7 @@
 #include "groupby.hpp"
 #include "jit.hpp"
 #include "join.hpp"
+#include "like_scan.hpp"
 #include "program.hpp"
 
 namespace prism {
@@ -69,9 +70,10 @@ static Kind kind_of(int v) { return v <= V_GB_288S ? K_GB : v <= V_PROBE_HASHED 
 
 struct FilterProg {
   int32_t len;
-  int32_t pad;
+  int32_t like_col;  // >= 0: the program is one LIKE '%literal%' of this column, run as the byte scan
   prism_instr code[kMaxCode];
   StrTable strs;
+  LikeArg like;
 };
 
 using GbDesc = gb::TileReq;
@@ -361,7 +363,9 @@ __global__ __launch_bounds__(kSB) void k_svc_pair_move(Control* C) {
 // Filter tiles start on 64-row boundaries: every wave step covers whole words.
 template <int R, int DEPTH>
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(4))) void k_svc_filter(Control* C, int v) {
+  static_assert(kSB == kLsThreads, "the LIKE scan uses the filter workgroup");
   __shared__ uint32_t s_t;
+  __shared__ uint32_t s_bits[kLsBitWords];
   Queue* q = &C->q[v];
   const uint32_t n_tiles = q->n_tiles;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -382,6 +386,13 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (cur != 0xFFFFFFFFu && lane == 0 && sel) atomicAdd(C->fd[cur].count + copy, sel);
       sel = 0;
       cur = tl.req;
+    }
+    if (d.P->like_col >= 0) {  // uniform: one LIKE '%literal%', as a byte scan
+      unsigned long long nulls = 0;
+      for (uint64_t r0 = tl.start; r0 < tl.end; r0 += kLsRows)
+        like_scan_tile(d.cols[d.P->like_col], r0, r0 + kLsRows < tl.end ? r0 + kLsRows : tl.end, d.P->like, s_bits,
+                       d.mask, nullptr, nulls, sel);
+      continue;
     }
     const uint64_t w_end = ceil_div(tl.end, 64);
     for (uint64_t w0 = tl.start / 64 + static_cast<uint64_t>(wave) * R; w0 < w_end; w0 += (kSB / 64) * R) {
@@ -1359,6 +1370,7 @@ prism_status prism_service_register_filter(prism_service* s, uint32_t plan_id, c
     P->len = static_cast<int32_t>(n_code);
     memcpy(P->code, code, n_code * sizeof(prism_instr));
     build_str_table(strings, n_strings, P->strs);
+    P->like_col = like_scan_prepare(code, n_code, P->strs, P->like);
     use_device(s->ctx);
     FiltPlan F;
     F.prog.alloc(s->ctx, sizeof(FilterProg));

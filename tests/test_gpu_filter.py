@@ -220,3 +220,25 @@ def test_like_shapes_on_long_strings(ctx, pattern, ci):
         v, m = fo.eval_pred(pa.record_batch([arr], names=["s"]), (fp.ILike if ci else fp.Like)(0, pattern))
         got = (so.string_ilike if ci else so.string_like)(arr, pattern)
         assert got.to_pylist() == fo.mask_array(v, m).to_pylist()
+
+
+@pytest.mark.parametrize("ci", [False, True])
+def test_like_contains_random_literals(ctx, ci):
+    """CONTAINS patterns of 1..12 literal bytes over strings of 0..100 bytes
+    from a small alphabet (frequent partial matches, letters of both cases
+    beside the bytes that differ from a letter by 0x20, a two-byte UTF-8
+    character), every alignment of string starts, matches across 16-byte
+    blocks and at both string ends, against the filter oracle."""
+    say(f"like contains random literals ci={ci}")
+    rng = np.random.default_rng(29 + ci)
+    alpha = ["a", "b", "A", "B", "@", "[", "`", "{", "\u00e9"]
+    strs = ["".join(rng.choice(alpha, int(rng.integers(0, 101)))) for _ in range(6000)]
+    arr = pa.array(strs, pa.string())
+    for lit_len in range(1, 13):
+        for _ in range(2):
+            lit = "".join(rng.choice(alpha[:8], lit_len))
+            pattern = "%" + lit + "%"
+            for a in (arr, arr.slice(5, 3001)):
+                v, m = fo.eval_pred(pa.record_batch([a], names=["s"]), (fp.ILike if ci else fp.Like)(0, pattern))
+                got = (so.string_ilike if ci else so.string_like)(a, pattern)
+                assert got.to_pylist() == fo.mask_array(v, m).to_pylist(), pattern
