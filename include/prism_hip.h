@@ -544,6 +544,18 @@ prism_status prism_ipc_column(const prism_ipc_stream* s, uint32_t batch, uint32_
                               prism_column* out);
 void prism_ipc_stream_free(prism_ipc_stream* s);
 
+/* Arrow IPC stream writer of prism-jni's reply (prism-jni/src/lib.rs:111-121,
+ * arrow's StreamWriter over the plan's output batches): the schema message,
+ * one record batch message per batch (struct arrays sharing `schema`, sliced
+ * inputs written from their offset), then the end-of-stream marker, into a
+ * host buffer the library allocates (free it with prism_ipc_buffer_free).
+ * With no batches the output is empty, as the reference writes no stream
+ * when the plan produced none.  Types: the operator types (Int32, Int64,
+ * UInt32, Float64, Utf8, Boolean, Date32, Decimal128(38,0)).  No device needed. */
+prism_status prism_ipc_write(const struct ArrowArray* batches, uint32_t n_batches, const struct ArrowSchema* schema,
+                             uint8_t** out_bytes, uint64_t* out_len);
+void prism_ipc_buffer_free(uint8_t* bytes);
+
 /* ------------------------------------------------------------------------- */
 /* Parquet decode: prism-executor's parquet_scan reads each file's footer,    */
 /* skips row groups whose statistics rule the predicate out and decodes the  */
@@ -697,6 +709,22 @@ prism_status prism_service_register_table(prism_service* s, uint32_t table_id, c
                                           const int32_t* h_payload, uint64_t n);
 /* Group-by plan (same plan as prism_groupby_create) holding at most max_groups
  * (1..1024) groups per request. */
+/* Resident build tables under store keys, as prism-flight's PartitionStore
+ * keeps batches under string keys (shuffle_writer.rs:42-86; prism-bench names
+ * a table by its `store_key`, handler.rs:800-843).  put: append the rows to the
+ * key's table, creating it (with a fresh id >= 2^31, written to *table_id) on
+ * the first put; the table is rebuilt and its keys must stay unique (a batch
+ * that repeats a key is refused and the key keeps its earlier rows).
+ * table_id: the id of a key's table, for prism_request.target.  clear: drop a
+ * key's table (absent keys are a no-op).  drop_exchange: drop every table
+ * whose key starts with "exchange/<exchange_id>/" (PartitionStore::drop_exchange).
+ * A table that requests wait for or run on is not changed: the call fails and
+ * the caller retries after they complete. */
+prism_status prism_service_put_table(prism_service* s, const char* key, const int64_t* h_keys, const int32_t* h_payload,
+                                     uint64_t n, uint32_t* table_id);
+prism_status prism_service_table_id(prism_service* s, const char* key, uint32_t* table_id);
+prism_status prism_service_clear_table(prism_service* s, const char* key);
+prism_status prism_service_drop_exchange(prism_service* s, const char* exchange_id);
 prism_status prism_service_register_groupby(prism_service* s, uint32_t plan_id, const prism_groupby_plan* plan,
                                             uint64_t max_groups);
 /* Filter plan: a boolean row program (the compiled form of a Predicate). */

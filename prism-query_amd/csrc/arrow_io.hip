@@ -42,8 +42,11 @@ int dtype_width(DType t) {
   }
 }
 
-static DType parse_format(const char* f) {
+// `outputs`: also the types only operators produce (Decimal128(38,0) sums),
+// for writers of operator results.
+static DType parse_format(const char* f, bool outputs = false) {
   std::string s = f ? f : "";
+  if (outputs && s == "d:38,0") return DType::Decimal128;
   if (s == "i") return DType::Int32;
   if (s == "l") return DType::Int64;
   if (s == "g") return DType::Float64;
@@ -54,9 +57,10 @@ static DType parse_format(const char* f) {
   fail(PRISM_ERR_ARROW, "Arrow error: unsupported Arrow type (format '" + s + "')");
 }
 
-static HostColumn import_child(const ArrowArray* ca, const ArrowSchema* cs, int64_t length, int64_t parent_off) {
+static HostColumn import_child(const ArrowArray* ca, const ArrowSchema* cs, int64_t length, int64_t parent_off,
+                               bool outputs) {
   HostColumn c;
-  c.type = parse_format(cs->format);
+  c.type = parse_format(cs->format, outputs);
   c.name = cs->name ? cs->name : "";
   c.nullable = (cs->flags & ARROW_FLAG_NULLABLE) != 0;
   c.length = length;
@@ -77,14 +81,14 @@ static HostColumn import_child(const ArrowArray* ca, const ArrowSchema* cs, int6
   return c;
 }
 
-HostBatch import_batch(const ArrowArray* a, const ArrowSchema* s) {
+HostBatch import_batch(const ArrowArray* a, const ArrowSchema* s, bool outputs) {
   if (!a || !s) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL ArrowArray/ArrowSchema");
   if (!s->format || std::string(s->format) != "+s")
     fail(PRISM_ERR_ARROW, "Arrow error: expected a RecordBatch exported as a struct array");
   if (a->n_children != s->n_children) fail(PRISM_ERR_ARROW, "Arrow error: array/schema child count mismatch");
   HostBatch b;
   b.n_rows = a->length;
-  for (int64_t i = 0; i < a->n_children; ++i) b.cols.push_back(import_child(a->children[i], s->children[i], a->length, a->offset));
+  for (int64_t i = 0; i < a->n_children; ++i) b.cols.push_back(import_child(a->children[i], s->children[i], a->length, a->offset, outputs));
   return b;
 }
 

@@ -38,8 +38,9 @@ struct HostBatch {
 };
 
 // Import a RecordBatch exported as a struct array ("+s").  The producer keeps
-// ownership; nothing is released here.
-HostBatch import_batch(const ArrowArray* array, const ArrowSchema* schema);
+// ownership; nothing is released here.  `outputs`: also accept the types only
+// operators produce (Decimal128(38,0)), for writers of operator results.
+HostBatch import_batch(const ArrowArray* array, const ArrowSchema* schema, bool outputs = false);
 // Schema only (for empty batch lists).
 std::vector<HostColumn> import_schema(const ArrowSchema* schema);
 

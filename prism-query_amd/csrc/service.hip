@@ -30,6 +30,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -540,6 +541,8 @@ struct prism_service {
   std::deque<prism::svc::Req*> pending;
   std::unordered_map<uint64_t, prism::svc::Req*> reqs;
   std::map<uint32_t, prism::svc::Table> tables;
+  std::map<std::string, uint32_t> table_keys;  // store keys (PartitionStore style) -> table ids
+  uint32_t next_keyed_id = 0x80000000u;        // ids given to keyed tables
   std::map<uint32_t, prism::svc::GbPlan> gb_plans;
   std::map<uint32_t, prism::svc::FiltPlan> filt_plans;
   prism::svc::Tuner tuner[3];
@@ -1293,10 +1296,32 @@ void prism_service_destroy(prism_service* s) {
   prism_ctx_destroy(own);
 }
 
+// A table no request waits for or runs on (callers hold s->mu).
+static void check_unused(prism_service* s, uint32_t id) {
+  auto it = s->tables.find(id);
+  if (it != s->tables.end() && it->second.pins)
+    fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: table in use by requests in flight");
+  for (const prism::svc::Req* R : s->pending)
+    if (R->r.kind == PRISM_REQ_PROBE && R->r.target == id)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: table in use by requests waiting to run");
+}
+
+static void drop_table(prism_service* s, uint32_t id) {
+  auto it = s->tables.find(id);
+  if (it == s->tables.end()) return;
+  if (it->second.t) {
+    s->resident_bytes -= it->second.bytes;
+    prism_join_table_free(it->second.t);
+  }
+  s->tables.erase(it);
+}
+
 prism_status prism_service_register_table(prism_service* s, uint32_t table_id, const int64_t* h_keys,
                                           const int32_t* h_payload, uint64_t n) {
   return guarded([&] {
     if (!s || (n && !h_keys)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service or keys");
+    if (table_id >= 0x80000000u)
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: table ids from 2^31 up belong to keyed tables");
     if (n >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: build side exceeds u32 row indices");
     std::lock_guard<std::mutex> lock(s->mu);
     Table& T = s->tables[table_id];
@@ -1312,6 +1337,87 @@ prism_status prism_service_register_table(prism_service* s, uint32_t table_id, c
     if (h_payload) T.payload.assign(h_payload, h_payload + n);
     use_device(s->ctx);
     make_resident(s, T);
+  });
+}
+
+prism_status prism_service_put_table(prism_service* s, const char* key, const int64_t* h_keys, const int32_t* h_payload,
+                                     uint64_t n, uint32_t* table_id) {
+  return guarded([&] {
+    if (!s || !key || (n && !h_keys)) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service, key or keys");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto kt = s->table_keys.find(key);
+    const bool fresh = kt == s->table_keys.end();
+    uint32_t id = 0;
+    if (fresh) {
+      if (s->next_keyed_id == 0xFFFFFFFFu) fail(PRISM_ERR_RESOURCE, "Resource error: keyed table ids exhausted");
+      id = s->next_keyed_id++;
+    } else {
+      id = kt->second;
+      check_unused(s, id);
+    }
+    prism::svc::Table& T = s->tables[id];
+    if (!fresh && !T.keys.empty() && T.payload.empty() != (h_payload == nullptr))
+      fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: a batch put under a key must match the key's payload column");
+    if (T.keys.size() + n >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: build side exceeds u32 row indices");
+    const size_t old_n = T.keys.size();
+    if (T.t) {
+      s->resident_bytes -= T.bytes;
+      prism_join_table_free(T.t);
+      T.t = nullptr;
+      T.bytes = 0;
+    }
+    T.id = id;
+    T.keys.insert(T.keys.end(), h_keys, h_keys + n);
+    if (h_payload) T.payload.insert(T.payload.end(), h_payload, h_payload + n);
+    use_device(s->ctx);
+    try {
+      make_resident(s, T);
+    } catch (...) {  // e.g. the batch repeats a key: the key keeps its earlier rows
+      T.keys.resize(old_n);
+      if (h_payload) T.payload.resize(old_n);
+      if (fresh) s->tables.erase(id);
+      throw;
+    }
+    if (fresh) s->table_keys[key] = id;
+    if (table_id) *table_id = id;
+  });
+}
+
+prism_status prism_service_table_id(prism_service* s, const char* key, uint32_t* table_id) {
+  return guarded([&] {
+    if (!s || !key || !table_id) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service, key or output");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto kt = s->table_keys.find(key);
+    if (kt == s->table_keys.end()) fail(PRISM_ERR_INVALID_ARGUMENT, std::string("Invalid argument: no table under key '") + key + "'");
+    *table_id = kt->second;
+  });
+}
+
+prism_status prism_service_clear_table(prism_service* s, const char* key) {
+  return guarded([&] {
+    if (!s || !key) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service or key");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto kt = s->table_keys.find(key);
+    if (kt == s->table_keys.end()) return;  // clearing an absent key is a no-op
+    check_unused(s, kt->second);
+    drop_table(s, kt->second);
+    s->table_keys.erase(kt);
+  });
+}
+
+prism_status prism_service_drop_exchange(prism_service* s, const char* exchange_id) {
+  return guarded([&] {
+    if (!s || !exchange_id) fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: NULL service or exchange id");
+    std::lock_guard<std::mutex> lock(s->mu);
+    const std::string prefix = std::string("exchange/") + exchange_id + "/";
+    std::vector<std::map<std::string, uint32_t>::iterator> hit;
+    for (auto it = s->table_keys.lower_bound(prefix); it != s->table_keys.end() && it->first.compare(0, prefix.size(), prefix) == 0; ++it)
+      hit.push_back(it);
+    for (auto& it : hit) check_unused(s, it->second);  // all or nothing
+    for (auto& it : hit) {
+      drop_table(s, it->second);
+      s->table_keys.erase(it);
+    }
   });
 }
 

@@ -280,6 +280,8 @@ _SIGNATURES = [
     ("prism_ipc_field", c_int32, [c_void_p, c_uint32, POINTER(c_char_p), POINTER(c_int32), POINTER(c_int32)]),
     ("prism_ipc_column", c_int32, [c_void_p, c_uint32, c_uint32, POINTER(c_uint64), POINTER(Column)]),
     ("prism_ipc_stream_free", None, [c_void_p]),
+    ("prism_ipc_write", c_int32, [c_void_p, c_uint32, c_void_p, POINTER(c_void_p), POINTER(c_uint64)]),
+    ("prism_ipc_buffer_free", None, [c_void_p]),
     ("prism_parquet_open", c_int32, [c_void_p, c_uint64, POINTER(c_void_p)]),
     ("prism_parquet_file_info", c_int32, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int64)]),
     ("prism_parquet_field", c_int32, [c_void_p, c_uint32, POINTER(c_char_p), POINTER(c_int32), POINTER(c_int32),
@@ -306,6 +308,10 @@ _SIGNATURES = [
     ("prism_service_create", c_int32, [c_void_p, POINTER(ServiceConfig), POINTER(c_void_p)]),
     ("prism_service_destroy", None, [c_void_p]),
     ("prism_service_register_table", c_int32, [c_void_p, c_uint32, c_void_p, c_void_p, c_uint64]),
+    ("prism_service_put_table", c_int32, [c_void_p, c_char_p, c_void_p, c_void_p, c_uint64, POINTER(c_uint32)]),
+    ("prism_service_table_id", c_int32, [c_void_p, c_char_p, POINTER(c_uint32)]),
+    ("prism_service_clear_table", c_int32, [c_void_p, c_char_p]),
+    ("prism_service_drop_exchange", c_int32, [c_void_p, c_char_p]),
     ("prism_service_register_groupby", c_int32, [c_void_p, c_uint32, POINTER(GroupbyPlan), c_uint64]),
     ("prism_service_register_filter", c_int32, [c_void_p, c_uint32, POINTER(Instr), c_uint32, POINTER(PString),
                                                 c_uint32, POINTER(c_int32), c_uint32]),

@@ -1,10 +1,11 @@
-"""Arrow IPC stream ingest (the JNI byte arrays prism-jni decodes with arrow's
-StreamReader, native/prism-jni/src/lib.rs:96-121) through the C ABI: as Arrow
-C Data arrays over the caller's bytes (no copy) or straight into HBM."""
+"""Arrow IPC streams at the JNI boundary through the C ABI: ingest (the byte
+arrays prism-jni decodes with arrow's StreamReader, native/prism-jni/src/lib.rs:96-121)
+as Arrow C Data arrays over the caller's bytes (no copy) or straight into HBM,
+and the reply stream prism-jni writes with arrow's StreamWriter (lib.rs:111-121)."""
 from __future__ import annotations
 
 import ctypes
-from typing import List, Tuple
+from typing import List, Sequence, Tuple
 
 import pyarrow as pa
 
@@ -67,3 +68,33 @@ class IpcStream:
         if self.handle:
             self.ctx.lib.prism_ipc_stream_free(self.handle)
             self.handle = None
+
+
+def write_ipc(batches: Sequence[pa.RecordBatch]) -> bytes:
+    """The Arrow IPC stream of the batches (one schema, the batches, the
+    end-of-stream marker); empty when there are no batches, as the reference's
+    reply is (prism-jni/src/lib.rs:111-121)."""
+    lib = L.load()  # no device needed
+    n = len(batches)
+    arrays = (L.ArrowArray * max(n, 1))()
+    schema = L.ArrowSchema()
+    exported = []
+    try:
+        if n:
+            batches[0].schema._export_to_c(ctypes.addressof(schema))
+            exported.append(schema)
+            for i, b in enumerate(batches):
+                if not b.schema.equals(batches[0].schema):
+                    raise ValueError("record batches of one stream share one schema")
+                b._export_to_c(ctypes.addressof(arrays[i]))
+                exported.append(arrays[i])
+        out, size = ctypes.c_void_p(), ctypes.c_uint64()
+        L.check(lib.prism_ipc_write(arrays, n, ctypes.addressof(schema), ctypes.byref(out), ctypes.byref(size)))
+        try:
+            return ctypes.string_at(out, size.value) if size.value else b""
+        finally:
+            lib.prism_ipc_buffer_free(out)
+    finally:
+        for s in exported:
+            if s.release:
+                ctypes.CFUNCTYPE(None, ctypes.c_void_p)(s.release)(ctypes.addressof(s))

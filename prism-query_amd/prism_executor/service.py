@@ -35,6 +35,30 @@ class Service:
         L.check(self.lib.prism_service_register_table(self.handle, table_id, k.ctypes.data,
                                                       None if p is None else p.ctypes.data, len(k)))
 
+    def put_table(self, key: str, keys: np.ndarray, payload: Optional[np.ndarray] = None) -> int:
+        """Append rows to the resident table under a store key (PartitionStore::put,
+        prism-flight shuffle_writer.rs:52-55); returns the table id for probes."""
+        k = np.ascontiguousarray(keys, np.int64)
+        p = None if payload is None else np.ascontiguousarray(payload, np.int32)
+        tid = ctypes.c_uint32()
+        L.check(self.lib.prism_service_put_table(self.handle, key.encode(), k.ctypes.data,
+                                                 None if p is None else p.ctypes.data, len(k), byref(tid)))
+        return tid.value
+
+    def table_id(self, key: str) -> int:
+        tid = ctypes.c_uint32()
+        L.check(self.lib.prism_service_table_id(self.handle, key.encode(), byref(tid)))
+        return tid.value
+
+    def clear_table(self, key: str) -> None:
+        """PartitionStore::clear (shuffle_writer.rs:62-65)."""
+        L.check(self.lib.prism_service_clear_table(self.handle, key.encode()))
+
+    def drop_exchange(self, exchange_id: str) -> None:
+        """PartitionStore::drop_exchange (shuffle_writer.rs:77-85): every table
+        under "exchange/<exchange_id>/"."""
+        L.check(self.lib.prism_service_drop_exchange(self.handle, exchange_id.encode()))
+
     def register_groupby(self, plan_id: int, column_types, keys, values, aggs, filter_prog=None, strings=(),
                          max_groups: int = 1024) -> None:
         plan, keep = build_plan(column_types, keys, values, aggs, filter_prog, strings)

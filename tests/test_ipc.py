@@ -1,6 +1,8 @@
-"""Arrow IPC stream ingest (prism-jni lib.rs:96-121 decodes the JNI byte array
-with arrow's StreamReader): the library's decoder against pyarrow's writer, on
-the CPU (the host form needs no device)."""
+"""Arrow IPC streams at the JNI boundary, on the CPU (the host forms need no
+device): ingest (prism-jni lib.rs:96-121 decodes the JNI byte array with
+arrow's StreamReader), the library's decoder against pyarrow's writer; the
+reply (lib.rs:111-121 writes the output with arrow's StreamWriter), the
+library's writer against pyarrow's reader and the library's own decoder."""
 import ctypes
 import datetime
 import struct
@@ -10,7 +12,7 @@ import pyarrow as pa
 import pytest
 
 from prism_executor import _lib as L
-from prism_executor.ipc import read_ipc_host
+from prism_executor.ipc import read_ipc_host, write_ipc
 
 
 def stream(batches, schema=None, **opts):
@@ -111,3 +113,47 @@ def test_forged_utf8_offsets_are_refused():
     forged = forge(data, struct.pack("<4i", 0, 2, 4, 6), struct.pack("<4i", -4, 2, 4, 6))
     st, msg = status_of(forged)
     assert st == 1 and "outside" in msg, msg
+
+
+def reply_sample(n, seed):
+    """Every type the operators output: the ingest types plus UInt32 row
+    indices and exact Decimal128(38,0) sums."""
+    b = sample(n, seed)
+    rng = np.random.default_rng(seed + 100)
+    dec = [None if rng.random() < 0.1 else int(rng.integers(-2**62, 2**62)) * 2**40 + int(rng.integers(0, 2**40))
+           for _ in range(n)]
+    return pa.record_batch(list(b.columns) + [
+        pa.array(rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32), pa.uint32()),
+        pa.array(dec, pa.decimal128(38, 0)),
+        pa.array(rng.random(n) < 0.3, pa.bool_(), mask=rng.random(n) < 0.2),
+    ], names=b.schema.names + ["u32", "dec", "bn"])
+
+
+def test_reply_stream_reads_back_in_pyarrow():
+    bs = [reply_sample(1000, 5), reply_sample(0, 6), reply_sample(3001, 7), reply_sample(777, 8).slice(13, 500),
+          reply_sample(64, 9).slice(3, 0)]
+    data = write_ipc(bs)
+    assert len(data) % 8 == 0
+    got = pa.ipc.open_stream(data).read_all()
+    assert got.schema.equals(bs[0].schema)
+    assert got.equals(pa.Table.from_batches(bs))
+    assert [b.num_rows for b in pa.ipc.open_stream(data)] == [b.num_rows for b in bs]
+
+
+def test_reply_stream_reads_back_in_the_library():
+    bs = [sample(2000, 11), sample(999, 12).slice(7, 900)]
+    back, _ = read_ipc_host(write_ipc(bs))
+    assert len(back) == 2
+    for a, b in zip(back, bs):
+        assert a.equals(b)
+
+
+def test_reply_stream_edge_cases():
+    assert write_ipc([]) == b""  # no output batch: no stream, as the reference
+    schema = pa.schema([pa.field("k", pa.int64(), nullable=False), pa.field("s", pa.utf8())])
+    b = pa.record_batch([pa.array([1, 2, 3]), pa.array([None, None, None], pa.utf8())], schema=schema)
+    got = pa.ipc.open_stream(write_ipc([b])).read_all()
+    assert got.schema.equals(schema) and got.to_pylist() == b.to_pylist()
+    assert got.schema.field("k").nullable is False and got.column("s").null_count == 3
+    with pytest.raises(ValueError):
+        write_ipc([b, sample(3, 1)])
