@@ -3212,7 +3212,19 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
       PRISM_CHECK_LAUNCH();
       return;
     }
-    uint64_t chunk = env_u64("PRISM_JA_CHUNK", 32ULL << 20);
+    // Larger passes amortise each pass's fixed work (range totals, scans, the
+    // staging tails): 128M rows per pass (6 GB of scratch) measured 3.95 ms for
+    // 256M rows against 4.43 ms in passes of 32M.  A device with less free
+    // memory gets passes whose scratch fits a quarter of it.
+    uint64_t chunk = env_u64("PRISM_JA_CHUNK", 0);
+    if (chunk == 0) {
+      chunk = 128ULL << 20;
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const uint64_t fit = static_cast<uint64_t>(free_b) / 4 / 48;
+        if (fit < chunk) chunk = fit > (1ULL << 20) ? fit : (1ULL << 20);
+      }
+    }
     if (chunk > n_probe) chunk = n_probe;
     const uint64_t n_ranges = static_cast<uint64_t>(sh.n_p1) * sh.fan;
     // twice a partition's share of the pass, plus the unused tails of the
