@@ -1856,15 +1856,31 @@ __global__ __launch_bounds__(kJaPartBlock) void k_ja_count2(JaPart P) {
     P.blk2[static_cast<uint64_t>(blockIdx.x) * P.fan + i] = hist[i];
 }
 
-// Range totals over the repartition blocks (one thread per range).
-__global__ void k_ja_tot2(JaPart P, uint32_t n_ranges) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_ranges) return;
-  const uint32_t fan = static_cast<uint32_t>(P.fan);
-  const uint32_t p = r / fan, g = r % fan;
-  uint32_t t = 0;
-  for (uint32_t k = 0; k < P.nb; ++k) t += static_cast<uint32_t>(P.blk2[(static_cast<uint64_t>(p) * P.nb + k) * fan + g]);
-  P.cnt2[r] = t;
+// Per level-1 partition (one workgroup): the blocks' record counts of each
+// range become exclusive offsets within the range (in place), and the range
+// totals go to cnt2.  Thread (chunk, range) sums a run of blocks, the runs are
+// scanned in LDS, then each thread rewrites its run as offsets.
+__global__ __launch_bounds__(kJaPartBlock) void k_ja_blk_scan(JaPart P) {
+  __shared__ unsigned long long part[kJaPartBlock];
+  const uint32_t fan = static_cast<uint32_t>(P.fan);  // a power of two <= kJaFanMax, so it divides the block
+  const uint32_t chunks = kJaPartBlock / fan;
+  const uint32_t g = threadIdx.x % fan, ch = threadIdx.x / fan;
+  const uint32_t per = (P.nb + chunks - 1) / chunks;
+  const uint32_t k0 = ch * per < P.nb ? ch * per : P.nb;
+  const uint32_t k1 = k0 + per < P.nb ? k0 + per : P.nb;
+  unsigned long long* b = P.blk2 + static_cast<uint64_t>(blockIdx.x) * P.nb * fan;
+  unsigned long long s = 0;
+  for (uint32_t k = k0; k < k1; ++k) s += b[static_cast<uint64_t>(k) * fan + g];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  unsigned long long run = 0;
+  for (uint32_t c = 0; c < ch; ++c) run += part[c * fan + g];
+  if (ch == chunks - 1) P.cnt2[static_cast<uint64_t>(blockIdx.x) * fan + g] = static_cast<uint32_t>(run + s);
+  for (uint32_t k = k0; k < k1; ++k) {
+    const unsigned long long v = b[static_cast<uint64_t>(k) * fan + g];
+    b[static_cast<uint64_t>(k) * fan + g] = run;
+    run += v;
+  }
 }
 
 // Exclusive scan of the range totals: each range's first place in l2 (one
@@ -1908,10 +1924,8 @@ __global__ __launch_bounds__(kJaPartBlock) void k_ja_scatter2(JaPart P) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t i = threadIdx.x; i < fan; i += blockDim.x) {
     // this block's first place in range i: after the earlier blocks' records
-    unsigned long long at = P.off2[static_cast<uint64_t>(p) * fan + i];
-    for (uint32_t k = 0; k < blk; ++k) at += P.blk2[(static_cast<uint64_t>(p) * P.nb + k) * fan + i];
     st_n[i] = 0;
-    cur[i] = at;
+    cur[i] = P.off2[static_cast<uint64_t>(p) * fan + i] + P.blk2[(static_cast<uint64_t>(p) * P.nb + blk) * fan + i];
   }
   __syncthreads();
   const uint64_t r0 = static_cast<uint64_t>(p) * P.cap1;
@@ -3276,7 +3290,7 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
                            st, T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), J->hot.as<uint32_t>() + 16, cpart);
       const dim3 gp(static_cast<unsigned>(sh.n_p1) * nb);
       hipLaunchKernelGGL(k_ja_count2, gp, dim3(kJaPartBlock), 0, st, P);
-      hipLaunchKernelGGL(k_ja_tot2, grid_for(c, n_ranges), dim3(kBuildBlock), 0, st, P, static_cast<uint32_t>(n_ranges));
+      hipLaunchKernelGGL(k_ja_blk_scan, dim3(static_cast<unsigned>(sh.n_p1)), dim3(kJaPartBlock), 0, st, P);
       hipLaunchKernelGGL(k_ja_scan2, dim3(1), dim3(kJaPartBlock), 0, st, P, static_cast<uint32_t>(n_ranges));
       hipLaunchKernelGGL(k_ja_scatter2, gp, dim3(kJaPartBlock), 0, st, P);
       hipLaunchKernelGGL(k_ja_agg2, dim3(static_cast<unsigned>(n_ranges)), dim3(kJaPartBlock), 0, st, P, A);
