@@ -1346,6 +1346,16 @@ __device__ __forceinline__ void ja_cache_init(JaCache& C, const uint32_t* __rest
   }
 }
 
+// One bucket's four ways in one 16-byte LDS load.  The pointer is cast to the
+// LDS address space explicitly: a volatile access through a generic pointer
+// would be a flat load, which counts as a global load too, so waiting for it
+// would wait for every stream load in flight.  Volatile: ways claimed by other
+// threads since the last look must be seen.
+typedef __attribute__((address_space(3))) v4u32 lds_v4u32;
+__device__ __forceinline__ v4u32 lds_load_ways(const uint32_t* p) {
+  return *(const volatile lds_v4u32*)(p);
+}
+
 // Entry of slot s: a slot placed at create time, or a free way of either
 // bucket claimed now (the first slots a workgroup meets, which covers a probe
 // side skewed on keys the build side does not rank); -1 when both buckets
@@ -1355,7 +1365,7 @@ __device__ __forceinline__ int ja_cache_entry(JaCache& C, uint32_t s) {
   for (int t = 0; t < 2; ++t) {
     const uint32_t b = ja_bucket(s, t);
     for (int attempt = 0; attempt < 2; ++attempt) {
-      const v4u32 e = *reinterpret_cast<const volatile v4u32*>(&C.key[kJaWays * b]);
+      const v4u32 e = lds_load_ways(&C.key[kJaWays * b]);
       if (e.x == s) return kJaWays * b;
       if (e.y == s) return kJaWays * b + 1;
       if (e.z == s) return kJaWays * b + 2;
@@ -1624,26 +1634,59 @@ __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4)
   }
   __syncthreads();
   constexpr uint64_t kRound = static_cast<uint64_t>(kJaBlock1) * kJaRows1;
-  for (uint64_t base = blockIdx.x * kRound; base < n; base += static_cast<uint64_t>(gridDim.x) * kRound) {
+  const uint64_t step = static_cast<uint64_t>(gridDim.x) * kRound;
+  // A round's keys and payloads are loaded during the round before it, so the
+  // stream overlaps the LDS work.  Every load is unconditional (rows past the
+  // end read row n - 1 and are masked where used), so the compiler can count
+  // the loads in flight and wait for exactly the ones a use needs.  Streamed
+  // once: non-temporal, so the presence bits and slot records stay in L2.
+  const int64_t* pay_src = pay ? pay : keys;
+  auto load_round = [&](uint64_t b, int64_t (&kk)[kJaRows1], int64_t (&xx)[kJaRows1]) {
+#pragma unroll
+    for (int j = 0; j < kJaRows1; ++j) {
+      const uint64_t r = b + static_cast<uint64_t>(j) * kJaBlock1 + threadIdx.x;
+      const uint64_t rc = r < n ? r : n - 1;
+      kk[j] = __builtin_nontemporal_load(keys + rc);
+      xx[j] = __builtin_nontemporal_load(pay_src + rc);
+    }
+  };
+  const bool hoist = KIND == PRISM_TABLE_DENSE && present != nullptr;
+  int64_t k[kJaRows1], x[kJaRows1];
+  load_round(blockIdx.x * kRound, k, x);
+  for (uint64_t base = blockIdx.x * kRound; base < n; base += step) {
+    // DENSE: the rows' presence words are requested before the next round's
+    // loads, so waiting for them does not wait for those (loads return in order)
+    uint32_t pw[kJaRows1];
+    if (hoist) {
+#pragma unroll
+      for (int j = 0; j < kJaRows1; ++j) {
+        const uint64_t r = base + static_cast<uint64_t>(j) * kJaBlock1 + threadIdx.x;
+        const uint64_t d = static_cast<uint64_t>(k[j]) - static_cast<uint64_t>(T.min_key);
+        const bool ok = r < n && d < T.range;
+        const uint32_t w = present[(ok ? d : 0) >> 5];
+        pw[j] = ok ? w : 0u;
+      }
+    }
+    int64_t kn[kJaRows1], xn[kJaRows1];
+    load_round(base + step, kn, xn);
     // a partition whose block may run out in this round's flush gets its
-    // next block now; the reply is read after the round's loads are issued
+    // next block now; the reply is read at the end of the round
     const bool want = static_cast<int>(threadIdx.x) < P.n_p1 && S.nxt[threadIdx.x] == ~0ULL &&
                       S.end[threadIdx.x] - S.cur[threadIdx.x] < static_cast<unsigned long long>(2 * kJaStage);
     unsigned long long ahead = 0;
     if (want) ahead = atomicAdd(&P.fill1[threadIdx.x], static_cast<unsigned long long>(kJaBlk));
-    int64_t k[kJaRows1], x[kJaRows1];
-#pragma unroll
-    for (int j = 0; j < kJaRows1; ++j) {
-      const uint64_t r = base + static_cast<uint64_t>(j) * kJaBlock1 + threadIdx.x;
-      // streamed once: non-temporal, so the presence bits and slot records stay in L2
-      k[j] = r < n ? __builtin_nontemporal_load(keys + r) : 0;
-      x[j] = (pay && r < n) ? __builtin_nontemporal_load(pay + r) : 0;
-    }
 #pragma unroll
     for (int j = 0; j < kJaRows1; ++j) {
       const uint64_t r = base + static_cast<uint64_t>(j) * kJaBlock1 + threadIdx.x;
       if (r >= n) continue;
-      const int64_t s = probe_slot<KIND>(T, k[j], unique, present);
+      if (!pay) x[j] = 0;
+      int64_t s;
+      if (hoist) {
+        const uint64_t d = static_cast<uint64_t>(k[j]) - static_cast<uint64_t>(T.min_key);
+        s = ((pw[j] >> (d & 31)) & 1u) ? static_cast<int64_t>(d) : -1;
+      } else {
+        s = probe_slot<KIND>(T, k[j], unique, present);
+      }
       if (s < 0) continue;
       const unsigned long long row = row_base + r;
       const uint32_t su = static_cast<uint32_t>(s);
@@ -1667,6 +1710,11 @@ __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4)
     lds_barrier();
     ja_stage1_flush(S, P, A, kJaFlushAt);
     lds_barrier();
+#pragma unroll
+    for (int j = 0; j < kJaRows1; ++j) {
+      k[j] = kn[j];
+      x[j] = xn[j];
+    }
   }
   ja_stage1_flush(S, P, A, 1);
   lds_barrier();
