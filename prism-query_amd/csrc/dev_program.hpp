@@ -41,6 +41,12 @@ struct StrTable {
 
 __device__ __forceinline__ bool bit_at(const uint8_t* bm, uint64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
 
+// Column pointers of the generated kernels are typed as global memory: loads
+// through a generic pointer become flat loads, which also count against the
+// LDS counter, so every wait for an LDS result would wait for them too.
+#define PRISM_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ bool bit_at(const PRISM_GLOBAL uint8_t* bm, uint64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
+
 __device__ __forceinline__ bool col_valid(const prism_column& c, uint64_t row) {
   return c.validity == nullptr || bit_at(c.validity, c.offset + row);
 }
@@ -232,6 +238,17 @@ __device__ __forceinline__ void str_head_load(const uint8_t* s, int32_t len, con
   const uintptr_t a = reinterpret_cast<uintptr_t>(s);
   const uint32_t* w = len > 0 ? reinterpret_cast<const uint32_t*>(a & ~static_cast<uintptr_t>(3))
                               : static_cast<const uint32_t*>(fallback);
+  const int32_t last = len > 0 ? static_cast<int32_t>(((a + static_cast<uintptr_t>(len) - 1) >> 2) - (a >> 2)) : 0;
+  w0 = w[0];
+  w1 = w[last < 1 ? last : 1];
+  w2 = w[last < 2 ? last : 2];
+}
+// str_head_load over a column in global memory (global, not flat, loads).
+__device__ __forceinline__ void str_head_load(const PRISM_GLOBAL uint8_t* s, int32_t len, const PRISM_GLOBAL void* fallback,
+                                              uint32_t& w0, uint32_t& w1, uint32_t& w2) {
+  const uintptr_t a = (uintptr_t)(s);
+  const PRISM_GLOBAL uint32_t* w = len > 0 ? (const PRISM_GLOBAL uint32_t*)(a & ~static_cast<uintptr_t>(3))
+                                           : static_cast<const PRISM_GLOBAL uint32_t*>(fallback);
   const int32_t last = len > 0 ? static_cast<int32_t>(((a + static_cast<uintptr_t>(len) - 1) >> 2) - (a >> 2)) : 0;
   w0 = w[0];
   w1 = w[last < 1 ? last : 1];

@@ -170,27 +170,29 @@ class Gen {
       const std::string C = std::to_string(c);
       switch (type(c)) {
         case PRISM_TYPE_INT64:
-          o_ << "  const int64_t* __restrict__ d" << C << " = static_cast<const int64_t*>(cols[" << C << "].data) + cols["
-             << C << "].offset;\n";
+          o_ << "  const PRISM_GLOBAL int64_t* __restrict__ d" << C << " = (const PRISM_GLOBAL int64_t*)(static_cast<const int64_t*>(cols["
+             << C << "].data) + cols[" << C << "].offset);\n";
           break;
         case PRISM_TYPE_FLOAT64:
-          o_ << "  const double* __restrict__ d" << C << " = static_cast<const double*>(cols[" << C << "].data) + cols["
-             << C << "].offset;\n";
+          o_ << "  const PRISM_GLOBAL double* __restrict__ d" << C << " = (const PRISM_GLOBAL double*)(static_cast<const double*>(cols["
+             << C << "].data) + cols[" << C << "].offset);\n";
           break;
         case PRISM_TYPE_INT32:
         case PRISM_TYPE_DATE32:
-          o_ << "  const int32_t* __restrict__ d" << C << " = static_cast<const int32_t*>(cols[" << C << "].data) + cols["
-             << C << "].offset;\n";
+          o_ << "  const PRISM_GLOBAL int32_t* __restrict__ d" << C << " = (const PRISM_GLOBAL int32_t*)(static_cast<const int32_t*>(cols["
+             << C << "].data) + cols[" << C << "].offset);\n";
           break;
         case PRISM_TYPE_BOOL:
-          o_ << "  const uint8_t* __restrict__ d" << C << " = static_cast<const uint8_t*>(cols[" << C << "].data);\n";
+          o_ << "  const PRISM_GLOBAL uint8_t* __restrict__ d" << C << " = (const PRISM_GLOBAL uint8_t*)(cols[" << C << "].data);\n";
           break;
         default:  // UTF8
-          o_ << "  const int32_t* __restrict__ of" << C << " = cols[" << C << "].offsets + cols[" << C << "].offset;\n"
-             << "  const uint8_t* __restrict__ d" << C << " = static_cast<const uint8_t*>(cols[" << C << "].data);\n";
+          o_ << "  const PRISM_GLOBAL int32_t* __restrict__ of" << C << " = (const PRISM_GLOBAL int32_t*)(cols[" << C
+             << "].offsets + cols[" << C << "].offset);\n"
+             << "  const PRISM_GLOBAL uint8_t* __restrict__ d" << C << " = (const PRISM_GLOBAL uint8_t*)(cols[" << C << "].data);\n";
       }
       if (type(c) == PRISM_TYPE_BOOL || nullable(c)) o_ << "  const uint64_t o" << C << " = cols[" << C << "].offset;\n";
-      if (nullable(c)) o_ << "  const uint8_t* __restrict__ n" << C << " = cols[" << C << "].validity;\n";
+      if (nullable(c))
+        o_ << "  const PRISM_GLOBAL uint8_t* __restrict__ n" << C << " = (const PRISM_GLOBAL uint8_t*)(cols[" << C << "].validity);\n";
     }
   }
 
@@ -355,7 +357,7 @@ class Gen {
           const std::string n = fresh();
           o_ << "      int64_t " << n << " = 0;\n      bool " << n << "v = false;\n"
              << "      if (in[r]) {\n        const int32_t sb = of" << C << "[row[r]], se = of" << C << "[row[r] + 1];\n"
-             << "        const uint8_t* s = d" << C << " + sb;\n        " << n << " = " << e << ";\n        " << n
+             << "        const uint8_t* s = (const uint8_t*)(d" << C << " + sb);\n        " << n << " = " << e << ";\n        " << n
              << "v = " << valid(c, "row[r]") << ";\n      }\n";
           st.push_back({n, n + "v", 'I'});
           break;
@@ -459,7 +461,7 @@ class Gen {
         }
         o_ << "      hh = fx_step(hh, x" << C << "v[r] ? " << w << " : 0);\n";
       } else {
-        const std::string s = "d" + C + " + sb" + C + "[r]";
+        const std::string s = "(const uint8_t*)(d" + C + " + sb" + C + "[r])";
         const std::string hashed = "hash_str_head(" + s + ", sl" + C + "[r], str_head_word(" + s + ", sl" + C +
                                    "[r], hw" + C + "[r][0], hw" + C + "[r][1], hw" + C + "[r][2]), hh)";
         if (nullable(c))
