@@ -75,10 +75,10 @@ __device__ __forceinline__ uint32_t ls_zero_bytes(uint32_t y) {
 // 16-byte units from `base`; `last` = the last block holding a byte of the
 // range, repeated past it).  Bit i = the literal's first min(m, 8) bytes start
 // at byte i of the block.
-__device__ __forceinline__ uint32_t ls_block_bits(const uint4* __restrict__ base, int64_t j, int64_t last,
+__device__ __forceinline__ uint32_t ls_block_bits(const PRISM_GLOBAL v4u32* __restrict__ base, int64_t j, int64_t last,
                                                   const LikeArg& A, int mh) {
-  const uint4 v = base[j];
-  const uint4 u = base[j + 1 <= last ? j + 1 : last];
+  const v4u32 v = base[j];
+  const v4u32 u = base[j + 1 <= last ? j + 1 : last];
   const uint32_t W[8] = {v.x, v.y, v.z, v.w, u.x, u.y, u.z, u.w};
   uint32_t bits = 0;
 #pragma unroll
@@ -109,12 +109,15 @@ __device__ __forceinline__ void like_scan_tile(const prism_column& c, uint64_t r
                                                unsigned long long* out_valid, unsigned long long& nulls,
                                                unsigned long long& selected) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int32_t* off = c.offsets + c.offset;
-  const uint8_t* bytes = static_cast<const uint8_t*>(c.data);
+  // global-typed pointers: generic ones would make every load a flat load,
+  // whose waits also drain the LDS bitmap traffic
+  const PRISM_GLOBAL int32_t* off = (const PRISM_GLOBAL int32_t*)(c.offsets + c.offset);
+  const PRISM_GLOBAL uint8_t* bytes = (const PRISM_GLOBAL uint8_t*)(c.data);
+  const PRISM_GLOBAL uint8_t* vbits = (const PRISM_GLOBAL uint8_t*)(c.validity);
   const int32_t b0 = off[r0], b1 = off[r1];
-  const uintptr_t a0 = reinterpret_cast<uintptr_t>(bytes + b0);
+  const uintptr_t a0 = (uintptr_t)(bytes + b0);
   const int32_t lead = static_cast<int32_t>(a0 & 15);
-  const uint4* base = reinterpret_cast<const uint4*>(a0 - lead);
+  const PRISM_GLOBAL v4u32* base = (const PRISM_GLOBAL v4u32*)(a0 - lead);
   const int64_t span = static_cast<int64_t>(b1 - b0) + lead;  // byte positions of the sub-tile
   const int m = A.m;
   const bool in_lds = span <= 32LL * kLsBitWords;
@@ -131,7 +134,7 @@ __device__ __forceinline__ void like_scan_tile(const prism_column& c, uint64_t r
           cand &= cand - 1;
           const int64_t p = 16 * j + i;
           bool ok = p + m <= span;
-          const uint8_t* s = reinterpret_cast<const uint8_t*>(base) + p;
+          const PRISM_GLOBAL uint8_t* s = (const PRISM_GLOBAL uint8_t*)(base) + p;
           for (int k = 8; ok && k < m; ++k) ok = (A.ci ? fold_ascii(s[k]) : s[k]) == A.lit[k];
           if (!ok) bits &= ~(1u << i);
         }
@@ -147,7 +150,7 @@ __device__ __forceinline__ void like_scan_tile(const prism_column& c, uint64_t r
     const bool in = r < r1;
     bool v = false, res = false;
     if (in) {
-      v = col_valid(c, r);
+      v = vbits == nullptr || bit_at(vbits, static_cast<uint64_t>(c.offset) + r);
       const int32_t rb = off[r], re = off[r + 1];
       if (m == 0) {
         res = true;
@@ -165,7 +168,7 @@ __device__ __forceinline__ void like_scan_tile(const prism_column& c, uint64_t r
           }
           res = x != 0;
         } else {
-          res = contains_lit(bytes + rb, re - rb, A.lit, m, A.ci != 0);
+          res = contains_lit((const uint8_t*)(bytes + rb), re - rb, A.lit, m, A.ci != 0);
         }
       }
     }
