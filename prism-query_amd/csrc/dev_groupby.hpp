@@ -502,12 +502,21 @@ __device__ __forceinline__ uint32_t order_groups(const GTable& G, unsigned long 
   const uint64_t slots = G.cap + 1;
   if (threadIdx.x == 0) *s_n = 0;
   __syncthreads();
-  for (uint64_t s = threadIdx.x; s < slots; s += kFinishThreads) {
-    const unsigned long long f = G.w[s * stride + kW_FIRST];
-    if (f != kNoRow) {
+  // 4 slots' first rows in flight per thread (their loads do not wait for
+  // each other's LDS appends)
+  for (uint64_t s0 = threadIdx.x; s0 < slots; s0 += 4 * kFinishThreads) {
+    unsigned long long f[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t s = s0 + static_cast<uint64_t>(u) * kFinishThreads;
+      f[u] = s < slots ? G.w[s * stride + kW_FIRST] : kNoRow;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (f[u] == kNoRow) continue;
       const uint32_t i = atomicAdd(s_n, 1u);
-      s_first[i] = f;
-      s_slot[i] = static_cast<uint32_t>(s);
+      s_first[i] = f[u];
+      s_slot[i] = static_cast<uint32_t>(s0 + static_cast<uint64_t>(u) * kFinishThreads);
     }
   }
   __syncthreads();
@@ -537,9 +546,23 @@ __device__ __forceinline__ uint32_t order_groups(const GTable& G, unsigned long 
       }
       __syncthreads();
     }
-  for (uint64_t e = threadIdx.x; e < static_cast<uint64_t>(n) * stride; e += kFinishThreads) {
-    const uint64_t i = e / stride, j = e % stride;
-    rows[e] = G.w[static_cast<uint64_t>(s_slot[i]) * stride + j];
+  // the ordered slots' words, 8 in flight per thread
+  const uint64_t total = static_cast<uint64_t>(n) * stride;
+  for (uint64_t e0 = threadIdx.x; e0 < total; e0 += 8 * kFinishThreads) {
+    unsigned long long v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint64_t e = e0 + static_cast<uint64_t>(u) * kFinishThreads;
+      if (e < total) {
+        const uint64_t i = e / stride, j = e % stride;
+        v[u] = G.w[static_cast<uint64_t>(s_slot[i]) * stride + j];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint64_t e = e0 + static_cast<uint64_t>(u) * kFinishThreads;
+      if (e < total) rows[e] = v[u];
+    }
   }
   return n;
 }
