@@ -39,6 +39,30 @@ __host__ __device__ __forceinline__ double f64_unorder(uint64_t o) {
 typedef long long v2i64 __attribute__((ext_vector_type(2)));
 typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 
+// Pointers typed as global memory.  A load through a generic pointer (any
+// pointer read from memory, such as a column descriptor's buffers) is a flat
+// load, which also counts against the LDS counter, so every wait for an LDS
+// result waits for it too; the same load through a global pointer does not.
+#define PRISM_GLOBAL __attribute__((address_space(1)))
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+template <int N> struct U32Vec;
+template <> struct U32Vec<2> { typedef unsigned int type __attribute__((ext_vector_type(2))); };
+template <> struct U32Vec<4> { typedef v4u32 type; };
+template <class T>
+__device__ __forceinline__ T gload(const void* p, uint64_t i) {
+  if constexpr (__is_scalar(T)) {
+    return ((const PRISM_GLOBAL T*)(p))[i];
+  } else {  // a struct of 8 or 16 bytes: one vector load (no copy constructor for a global lvalue)
+    static_assert(sizeof(T) == 8 || sizeof(T) == 16, "8- or 16-byte records");
+    typedef typename U32Vec<sizeof(T) / 4>::type W;
+    const W w = ((const PRISM_GLOBAL W*)(p))[i];
+    T t;
+    __builtin_memcpy(&t, &w, sizeof(T));
+    return t;
+  }
+}
+#endif
+
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 // Number of set bits of `mask` in lanes below this one.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {

@@ -41,29 +41,29 @@ struct StrTable {
 
 __device__ __forceinline__ bool bit_at(const uint8_t* bm, uint64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
 
-// Column pointers of the generated kernels are typed as global memory: loads
-// through a generic pointer become flat loads, which also count against the
-// LDS counter, so every wait for an LDS result would wait for them too.
-#define PRISM_GLOBAL __attribute__((address_space(1)))
 __device__ __forceinline__ bool bit_at(const PRISM_GLOBAL uint8_t* bm, uint64_t i) { return (bm[i >> 3] >> (i & 7)) & 1; }
 
+__device__ __forceinline__ bool gbit(const void* bm, uint64_t i) {
+  return bit_at((const PRISM_GLOBAL uint8_t*)(bm), i);
+}
+
 __device__ __forceinline__ bool col_valid(const prism_column& c, uint64_t row) {
-  return c.validity == nullptr || bit_at(c.validity, c.offset + row);
+  return c.validity == nullptr || gbit(c.validity, c.offset + row);
 }
 
 __device__ __forceinline__ int64_t col_load_i(const prism_column& c, uint64_t row) {
   const uint64_t i = c.offset + row;
   switch (c.type) {
     case PRISM_TYPE_INT64:
-      return static_cast<const int64_t*>(c.data)[i];
+      return gload<int64_t>(c.data, i);
     case PRISM_TYPE_BOOL:
-      return bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0;
+      return gbit(c.data, i) ? 1 : 0;
     default:  // INT32, DATE32
-      return static_cast<const int32_t*>(c.data)[i];
+      return gload<int32_t>(c.data, i);
   }
 }
 __device__ __forceinline__ double col_load_f(const prism_column& c, uint64_t row) {
-  return static_cast<const double*>(c.data)[c.offset + row];
+  return gload<double>(c.data, c.offset + row);
 }
 // Raw 64-bit word of a fixed-width column: int64 / f64 bits as stored, int32 /
 // date32 sign-extended, bool as 0 / 1.
@@ -72,11 +72,11 @@ __device__ __forceinline__ uint64_t col_load_raw(const prism_column& c, uint64_t
   switch (c.type) {
     case PRISM_TYPE_INT64:
     case PRISM_TYPE_FLOAT64:
-      return static_cast<const uint64_t*>(c.data)[i];
+      return gload<uint64_t>(c.data, i);
     case PRISM_TYPE_BOOL:
-      return bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0;
+      return gbit(c.data, i) ? 1 : 0;
     default:
-      return static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[i]));
+      return static_cast<uint64_t>(static_cast<int64_t>(gload<int32_t>(c.data, i)));
   }
 }
 // FxHash word of a non-null fixed-width key from its raw word (the same words
@@ -287,18 +287,17 @@ __device__ __forceinline__ uint64_t hash_key(const prism_column& c, uint64_t row
   const uint64_t i = c.offset + row;
   switch (c.type) {
     case PRISM_TYPE_INT64:
-      return fx_step(state, static_cast<uint64_t>(static_cast<const int64_t*>(c.data)[i]));
+      return fx_step(state, static_cast<uint64_t>(gload<int64_t>(c.data, i)));
     case PRISM_TYPE_INT32:  // write_i32 -> write_u64(i as u64): sign extension
-      return fx_step(state, static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(c.data)[i])));
+      return fx_step(state, static_cast<uint64_t>(static_cast<int64_t>(gload<int32_t>(c.data, i))));
     case PRISM_TYPE_FLOAT64:  // to_bits().hash()
-      return fx_step(state, static_cast<const uint64_t*>(c.data)[i]);
+      return fx_step(state, gload<uint64_t>(c.data, i));
     case PRISM_TYPE_DATE32:  // value word kept away from the null word 0
-      return fx_step(state, static_cast<uint64_t>(static_cast<uint32_t>(static_cast<const int32_t*>(c.data)[i])) |
-                                (1ULL << 32));
+      return fx_step(state, static_cast<uint64_t>(static_cast<uint32_t>(gload<int32_t>(c.data, i))) | (1ULL << 32));
     case PRISM_TYPE_BOOL:
-      return fx_step(state, 2 + (bit_at(static_cast<const uint8_t*>(c.data), i) ? 1 : 0));
+      return fx_step(state, 2 + (gbit(c.data, i) ? 1 : 0));
     default: {  // UTF8
-      const int32_t b = c.offsets[i], e = c.offsets[i + 1];
+      const int32_t b = gload<int32_t>(c.offsets, i), e = gload<int32_t>(c.offsets, i + 1);
       return hash_str(static_cast<const uint8_t*>(c.data) + b, e - b, state);
     }
   }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "dev_common.hpp"
 
 namespace prism {
 namespace join {
@@ -63,18 +64,18 @@ __device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& 
     if (d >= T.range) return false;
     if constexpr (UNIQUE) {
       if (T.has_payload) {
-        const uint2 e = T.dense_up[d];
+        const uint2 e = gload<uint2>(T.dense_up, d);
         val = e.x;
         pay = static_cast<int32_t>(e.y);
         cnt = 1;
         return e.x != kNone;
       }
-      uint32_t v = T.dense_u[d];
+      uint32_t v = gload<uint32_t>(T.dense_u, d);
       val = v;
       cnt = 1;
       return v != kNone;
     } else {
-      uint2 e = T.dense_nu[d];
+      uint2 e = gload<uint2>(T.dense_nu, d);
       val = e.x;
       cnt = e.y;
       return e.y != 0;
@@ -83,7 +84,7 @@ __device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& 
     uint64_t s;
     if (k == kEmpty) {
       if (!T.sentinel_present) return false;
-      const Slot e = T.slots[T.mask + 1];
+      const Slot e = gload<Slot>(T.slots, T.mask + 1);
       val = e.val;
       cnt = (UNIQUE && T.has_payload) ? 1u : e.cnt;
       pay = static_cast<int32_t>(e.cnt);
@@ -91,7 +92,7 @@ __device__ __forceinline__ bool lookup(const TableView& T, int64_t k, uint32_t& 
     }
     s = hash_i64_fast(k) >> T.shift;
     while (true) {  // load factor <= 1/2: an empty slot always ends the run
-      const Slot e = T.slots[s];
+      const Slot e = gload<Slot>(T.slots, s);
       if (e.key == k) {
         val = e.val;
         cnt = (UNIQUE && T.has_payload) ? 1u : e.cnt;

@@ -166,7 +166,7 @@ __device__ __forceinline__ bool src_pick(const StrSrcs* S, const uint32_t* idx, 
   }
   if (!col_valid(s.col, r)) return false;
   const uint64_t i = s.col.offset + r;
-  const int32_t b = s.col.offsets[i], e = s.col.offsets[i + 1];
+  const int32_t b = gload<int32_t>(s.col.offsets, i), e = gload<int32_t>(s.col.offsets, i + 1);
   p = static_cast<const uint8_t*>(s.col.data) + b;
   len = e - b;
   return true;

@@ -177,7 +177,7 @@ __device__ __forceinline__ void run_program(const prism_instr* __restrict__ code
           bool res = false;
           if (in[r]) {
             const uint64_t i = c.offset + row[r];
-            const int32_t b = c.offsets[i], e = c.offsets[i + 1];
+            const int32_t b = gload<int32_t>(c.offsets, i), e = gload<int32_t>(c.offsets, i + 1);
             const uint8_t* s = static_cast<const uint8_t*>(c.data) + b;
             if (op == PRISM_OP_CMP_STR) {
               int k = str_cmp(s, e - b, lit, nlit);

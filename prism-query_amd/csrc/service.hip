@@ -229,11 +229,16 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
     const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t, first));
     if (t >= n_tiles) break;
     const Tile tl = C->tiles[q->offset + t];
-    const ProbeDesc& d = C->pr[tl.req];
+    // the descriptor in registers (a reference into the control block would be
+    // re-read after every store), its buffers as global pointers
+    const ProbeDesc d = C->pr[tl.req];
     if (tl.req != cur) {
       if (cur != 0xFFFFFFFFu) svc_probe_flush(C->pr[cur].acc, m, ps, bs, s_m, s_p, s_b);
       cur = tl.req;
     }
+    const PRISM_GLOBAL int64_t* keys = (const PRISM_GLOBAL int64_t*)(d.keys);
+    const PRISM_GLOBAL int64_t* pay_src = (const PRISM_GLOBAL int64_t*)(d.pay ? d.pay : d.keys);
+    PRISM_GLOBAL uint64_t* stage = (PRISM_GLOBAL uint64_t*)(d.stage);  // {probe row, build row} pairs
     if (d.stage) {
       // Pairs: the tile's matches in row order.  Row c0 + u * kSB + tid has
       // rank (u, wave, lane) in the chunk; per-(u, wave) ballot counts are
@@ -243,18 +248,22 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
         int64_t key[U], x[U];
         bool hit[U];
         uint32_t brow[U];
+        // unconditional loads (rows past the tile read its last row and are
+        // masked), so the compiler waits for exactly the loads each use needs
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const uint64_t r = c0 + static_cast<uint64_t>(u) * kSB + threadIdx.x;
+          const uint64_t rc = r < tl.end ? r : tl.end - 1;
           hit[u] = r < tl.end;
-          key[u] = hit[u] ? __builtin_nontemporal_load(d.keys + r) : 0;
-          x[u] = (hit[u] && d.pay) ? __builtin_nontemporal_load(d.pay + r) : 0;
+          key[u] = __builtin_nontemporal_load(keys + rc);
+          x[u] = __builtin_nontemporal_load(pay_src + rc);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           uint32_t val = 0, cnt;
           int32_t pay = 0;
-          hit[u] = hit[u] && join::lookup<KIND, true>(d.T, key[u], val, cnt, pay);
+          if (!d.pay) x[u] = 0;
+          hit[u] = join::lookup<KIND, true>(d.T, key[u], val, cnt, pay) && hit[u];
           brow[u] = val;
           if (hit[u]) {
             ++m;
@@ -286,7 +295,7 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
           if (hit[u]) {
             const uint32_t pos = run + s_pre[u * kW + wave] + static_cast<uint32_t>(__popcll(bal[u] & lt));
             const uint64_t r = c0 + static_cast<uint64_t>(u) * kSB + threadIdx.x;
-            d.stage[tl.start + pos] = make_uint2(static_cast<uint32_t>(r), brow[u]);
+            stage[tl.start + pos] = static_cast<uint32_t>(r) | (static_cast<uint64_t>(brow[u]) << 32);
           }
         run += s_tot;
         __syncthreads();  // s_pre / s_tot are rewritten by the next chunk
@@ -300,16 +309,19 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const uint64_t r = r0 + static_cast<uint64_t>(u) * kSB;
+        const uint64_t rc = r < tl.end ? r : tl.end - 1;
         in[u] = r < tl.end;
-        // streamed once: non-temporal, so the table's lines keep their place in L2 / MALL
-        key[u] = in[u] ? __builtin_nontemporal_load(d.keys + r) : 0;
-        x[u] = (in[u] && d.pay) ? __builtin_nontemporal_load(d.pay + r) : 0;  // with the key: no dependent load
+        // streamed once: non-temporal, so the table's lines keep their place in
+        // L2 / MALL; unconditional (masked) loads, with the key: no dependent load
+        key[u] = __builtin_nontemporal_load(keys + rc);
+        x[u] = __builtin_nontemporal_load(pay_src + rc);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         uint32_t val, cnt;
         int32_t pay = 0;
-        if (in[u] && join::lookup<KIND, true>(d.T, key[u], val, cnt, pay)) {
+        if (!d.pay) x[u] = 0;
+        if (join::lookup<KIND, true>(d.T, key[u], val, cnt, pay) && in[u]) {
           ++m;
           ps += x[u];
           if (d.T.has_payload) bs += pay;
@@ -597,13 +609,6 @@ static int cand_index(prism_service* S, Kind k) {
   if (T.best >= 0) return T.best;
   if (!S->cfg.autotune) return 0;
   return static_cast<int>(T.next % T.cands.size());
-}
-
-static bool tuning(prism_service* S) {
-  if (!S->cfg.autotune) return false;
-  for (int k = 0; k < 3; ++k)
-    if (S->tuner[k].best < 0) return true;
-  return false;
 }
 
 // Make a table resident: evict least-recently-used unpinned tables until it fits.
@@ -918,7 +923,15 @@ static void launch_batch(prism_service* S, Slot& sl) {
   PRISM_HIP(hipMemcpyAsync(dC, H, ctl_bytes, hipMemcpyHostToDevice, st));
   const int init_grid = static_cast<int>(std::min<uint64_t>(pow2_grid(H->n_init), static_cast<uint64_t>(n_cu) * 4));
   const int fin_grid = static_cast<int>(std::min<uint64_t>(pow2_grid(H->n_gb), static_cast<uint64_t>(n_cu) * 2));
-  const bool timed = tuning(S);
+  // kinds launched by this batch; only a batch holding a kind still being
+  // tuned is timed per launch (and kept out of graphs)
+  unsigned kinds = jf.empty() ? 0u : 1u << K_GB;
+  for (int v = 0; v < kVariants; ++v)
+    if (grid[v]) kinds |= 1u << kind_of(v);
+  bool timed = false;
+  if (S->cfg.autotune)
+    for (int k = 0; k < 3; ++k)
+      if (((kinds >> k) & 1) && S->tuner[k].best < 0) timed = true;
   auto issue = [&](bool with_events) {
     hipLaunchKernelGGL(k_svc_init, dim3(init_grid), dim3(kSB), 0, st, dC);
     for (int v = 0; v < kVariants; ++v) {
@@ -994,7 +1007,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
   sl.t_launched = clk::now();
   if (S->cfg.autotune)
     for (int k = 0; k < 3; ++k)
-      if (S->tuner[k].best < 0) ++S->tuner[k].next;
+      if (((kinds >> k) & 1) && S->tuner[k].best < 0) ++S->tuner[k].next;
 }
 
 // Finish a completed batch (called with the service lock held).
