@@ -965,9 +965,11 @@ def main() -> int:
     kern_ms = kernels["join_probe"]["avg_us"] * 1e-3
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     kernels["join_probe"]["alg_bytes"] = alg_bytes
-    if kernels["join_probe_move"]["launches"]:  # the staged probe form moves pairs in a second kernel
-        kernels["join_probe_move"]["alg_bytes"] = r.matches * 16
-        kernels["join_probe_move"]["GBps"] = r.matches * 16 / (kernels["join_probe_move"]["avg_us"] * 1e-6) / 1e9
+    if kernels["join_probe_move"]["launches"]:  # the staged probe form moves the matches in a second kernel
+        # staged build rows (4 B per match) and match ballots (512 B per 4096-row tile) read, pairs written
+        move_bytes = r.matches * 12 + -(-npr // 4096) * 512
+        kernels["join_probe_move"]["alg_bytes"] = move_bytes
+        kernels["join_probe_move"]["GBps"] = move_bytes / (kernels["join_probe_move"]["avg_us"] * 1e-6) / 1e9
     else:
         del kernels["join_probe_move"]
     line = {

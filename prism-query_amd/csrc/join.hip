@@ -538,6 +538,24 @@ __global__ void k_probe_clear(uint64_t* status, uint64_t n_tiles, prism_probe_re
   }
 }
 
+// Staging of the probe's matches between the lookup pass and the move pass:
+// the build row of each match, in row order at tile * kTileRows, and the
+// tile's match ballots, two 64-bit words (element 0, element 1 of each lane's
+// row pair) per (chunk, wave), from which the move pass recovers the probe
+// rows: 4 B per match + 512 B per tile instead of an 8 B pair per match.
+constexpr int kTileBallots = kPairs * kWaves * 2;
+struct PairStage {
+  uint32_t* brow;
+  unsigned long long* bits;
+};
+
+__device__ __forceinline__ void stage_ballots(const PairStage& S, uint32_t tile, int j, int wave, uint64_t b0,
+                                              uint64_t b1) {
+  unsigned long long* w = S.bits + static_cast<uint64_t>(tile) * kTileBallots + (j * kWaves + wave) * 2;
+  w[0] = b0;
+  w[1] = b1;
+}
+
 // Unique-key probe, phase 1: each tile looks its keys up, ranks its matches in
 // LDS (row order), writes them contiguously at tile * kTileRows of a staging
 // buffer and records its match count and payload partials.  No tile waits on
@@ -545,11 +563,11 @@ __global__ void k_probe_clear(uint64_t* status, uint64_t n_tiles, prism_probe_re
 // final offset with coalesced copies.
 template <int KIND, bool VEC>
 __global__ __launch_bounds__(kPB) void k_probe_unique(TableView T, const int64_t* __restrict__ keys, uint64_t n,
-                                                      uint32_t probe_base, const int64_t* __restrict__ ppay,
-                                                      const int32_t* __restrict__ bpay, uint2* __restrict__ staged,
+                                                      const int64_t* __restrict__ ppay,
+                                                      const int32_t* __restrict__ bpay, PairStage stage,
                                                       uint32_t* __restrict__ tile_count,
                                                       __int128* __restrict__ tile_sums) {
-  __shared__ uint2 s_pair[kTileRows];
+  __shared__ uint32_t s_b[kTileRows];
   __shared__ uint32_t s_wave_tot[kPairs][kWaves];
   __shared__ __int128 s_red[2][kWaves];
 
@@ -617,7 +635,10 @@ __global__ __launch_bounds__(kPB) void k_probe_unique(TableView T, const int64_t
   for (int j = 0; j < kPairs; ++j) {
     const uint64_t b0 = ballot64(hit[j][0]), b1 = ballot64(hit[j][1]);
     local[j] = lanes_below(b0) + lanes_below(b1);
-    if (lane == 0) s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
+    if (lane == 0) {
+      s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
+      if (stage.bits) stage_ballots(stage, tile, j, wave, b0, b1);
+    }
   }
   __syncthreads();
   uint32_t tile_total = 0;
@@ -644,38 +665,259 @@ __global__ __launch_bounds__(kPB) void k_probe_unique(TableView T, const int64_t
       tile_sums[2 * tile + 1] = b;
     }
   }
-  if (!staged) return;
+  if (!stage.brow) return;
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    if (hit[j][0]) s_b[local[j]] = val[j][0];
+    if (hit[j][1]) s_b[local[j] + (hit[j][0] ? 1 : 0)] = val[j][1];
+  }
+  __syncthreads();
+  uint32_t* dst = stage.brow + base;
+  // Plain stores: the move pass reads the staged rows back from the
+  // Infinity Cache (non-temporal stores made it slower).
+  for (uint32_t i = t; i < tile_total; i += kPB) dst[i] = s_b[i];
+}
+
+// Look-back of one wave over the status words of the tiles before `tile`
+// (lane i reads tile - 1 - i, 64 predecessors per memory round trip).
+// Returns the exclusive match prefix of `tile`.  A predecessor still
+// unpublished after kStallPolls polls is counted by this wave and its count
+// published, so the scan ends whatever order the workgroups run in.
+template <int KIND>
+__device__ uint64_t wave_lookback(uint64_t* status, uint32_t tile, const TableView& T,
+                                  const int64_t* __restrict__ keys, uint64_t n, uint32_t stall_polls) {
+  const uint32_t lane = lane_id();
+  uint64_t excl = 0;
+  int64_t base = static_cast<int64_t>(tile) - 1;
+  uint32_t polls = 0;
+  while (true) {
+    const int64_t idx = base - static_cast<int64_t>(lane);
+    const uint64_t v = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : kStPrefix;  // before tile 0: an inclusive prefix of 0
+    const uint64_t st = v >> 62;
+    const uint64_t pm = ballot64(st == 2), zm = ballot64(st == 0);
+    const uint32_t fp = pm ? static_cast<uint32_t>(__builtin_ctzll(pm)) : 64u;  // nearest inclusive prefix
+    const uint64_t upto = fp >= 63 ? ~0ULL : ((2ULL << fp) - 1);               // lanes 0..fp
+    const uint64_t waiting = zm & upto;
+    if (waiting) {
+      if (++polls < stall_polls) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      const int64_t q = base - static_cast<int64_t>(__builtin_ctzll(waiting));
+      const uint64_t c = tile_count<KIND, true>(T, keys, n, static_cast<uint64_t>(q));
+      if (lane == 0) {
+        unsigned long long expect = 0;
+        __hip_atomic_compare_exchange_strong(reinterpret_cast<unsigned long long*>(&status[q]), &expect,
+                                             static_cast<unsigned long long>(kStAgg | c), __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      polls = 0;
+      continue;
+    }
+    uint64_t mine = (lane <= fp) ? (v & kStMask) : 0;
+    for (int d = 32; d > 0; d >>= 1) mine += __shfl_xor(mine, d, 64);
+    excl += mine;
+    if (fp < 64) return excl;
+    base -= 64;
+  }
+}
+
+// Unique-key probe in one pass: as k_probe_unique up to the tile's ranked
+// pairs in LDS, then the tile publishes its match count, one wave looks back
+// over the predecessors' counts for the tile's output offset, and the
+// workgroup writes its pairs there with coalesced stores.  The pairs are
+// written once, with no staging copy and no move pass.
+template <int KIND, bool VEC>
+__global__ __launch_bounds__(kPB) void k_probe_unique_lb(TableView T, const int64_t* __restrict__ keys, uint64_t n,
+                                                         uint32_t n_tiles, uint32_t probe_base,
+                                                         const int64_t* __restrict__ ppay,
+                                                         const int32_t* __restrict__ bpay, uint32_t* __restrict__ out_p,
+                                                         uint32_t* __restrict__ out_b, uint64_t capacity,
+                                                         uint64_t* status, __int128* __restrict__ tile_sums,
+                                                         prism_probe_result* res, uint32_t stall_polls) {
+  __shared__ uint32_t s_p[kTileRows];
+  __shared__ uint32_t s_b[kTileRows];
+  __shared__ uint32_t s_wave_tot[kPairs][kWaves];
+  __shared__ __int128 s_red[2][kWaves];
+  __shared__ uint64_t s_excl;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const uint32_t tile = blockIdx.x;
+  const uint64_t base = static_cast<uint64_t>(tile) * kTileRows;
+  const bool table_pay = T.has_payload && bpay == nullptr;
+  const bool sums = ppay || bpay || table_pay;
+
+  uint32_t val[kPairs][2];
+  bool hit[kPairs][2];
+  __int128 psum = 0, bsum = 0;
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    const uint64_t r0 = base + static_cast<uint64_t>(j) * kChunk + 2 * t;
+    int64_t k0, k1, p0 = 0, p1 = 0;
+    bool ok0, ok1;
+    if (VEC && r0 + 1 < n) {
+      const v2i64 v = __builtin_nontemporal_load(reinterpret_cast<const v2i64*>(keys + r0));
+      k0 = v.x;
+      k1 = v.y;
+      ok0 = ok1 = true;
+      if (ppay) {
+        const v2i64 p = __builtin_nontemporal_load(reinterpret_cast<const v2i64*>(ppay + r0));
+        p0 = p.x;
+        p1 = p.y;
+      }
+    } else {
+      ok0 = r0 < n;
+      ok1 = r0 + 1 < n;
+      k0 = ok0 ? keys[r0] : 0;
+      k1 = ok1 ? keys[r0 + 1] : 0;
+      if (ppay) {
+        p0 = ok0 ? ppay[r0] : 0;
+        p1 = ok1 ? ppay[r0 + 1] : 0;
+      }
+    }
+    uint32_t v0 = 0, v1 = 0, c;
+    int32_t y0 = 0, y1 = 0;
+    hit[j][0] = ok0 && lookup<KIND, true>(T, k0, v0, c, y0);
+    hit[j][1] = ok1 && lookup<KIND, true>(T, k1, v1, c, y1);
+    val[j][0] = v0;
+    val[j][1] = v1;
+    if (hit[j][0]) {
+      psum += p0;
+      bsum += bpay ? bpay[v0] : (table_pay ? y0 : 0);
+    }
+    if (hit[j][1]) {
+      psum += p1;
+      bsum += bpay ? bpay[v1] : (table_pay ? y1 : 0);
+    }
+  }
+  // Rank matches inside the tile: row order is (chunk, thread, element).
+  uint32_t local[kPairs];
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    const uint64_t b0 = ballot64(hit[j][0]), b1 = ballot64(hit[j][1]);
+    local[j] = lanes_below(b0) + lanes_below(b1);
+    if (lane == 0) s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
+  }
+  if (sums) {
+    wave_reduce_i128(psum);
+    wave_reduce_i128(bsum);
+    if (lane == 0) {
+      s_red[0][wave] = psum;
+      s_red[1][wave] = bsum;
+    }
+  }
+  __syncthreads();
+  uint32_t tile_total = 0;
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    uint32_t before = tile_total;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t v = s_wave_tot[j][w];
+      before += (w < wave) ? v : 0;
+      tile_total += v;
+    }
+    local[j] += before;
+  }
+  // Publish the count as early as possible: successors are looking back.
+  if (t == 0)
+    __hip_atomic_store(&status[tile], (tile == 0 ? kStPrefix : kStAgg) | tile_total, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
   for (int j = 0; j < kPairs; ++j) {
     const uint32_t row0 = static_cast<uint32_t>(base + static_cast<uint64_t>(j) * kChunk + 2 * t) + probe_base;
-    if (hit[j][0]) s_pair[local[j]] = make_uint2(row0, val[j][0]);
-    if (hit[j][1]) s_pair[local[j] + (hit[j][0] ? 1 : 0)] = make_uint2(row0 + 1, val[j][1]);
+    if (hit[j][0]) {
+      s_p[local[j]] = row0;
+      s_b[local[j]] = val[j][0];
+    }
+    if (hit[j][1]) {
+      const uint32_t q = local[j] + (hit[j][0] ? 1 : 0);
+      s_p[q] = row0 + 1;
+      s_b[q] = val[j][1];
+    }
+  }
+  if (wave == 0) {
+    const uint64_t excl = tile == 0 ? 0 : wave_lookback<KIND>(status, tile, T, keys, n, stall_polls);
+    if (lane == 0) {
+      if (tile != 0)
+        __hip_atomic_store(&status[tile], kStPrefix | (excl + tile_total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      s_excl = excl;
+      if (tile == n_tiles - 1) res->matches = excl + tile_total;
+      if (sums) {
+        __int128 p = 0, b = 0;
+        for (int w = 0; w < kWaves; ++w) {
+          p += s_red[0][w];
+          b += s_red[1][w];
+        }
+        tile_sums[2 * tile] = p;
+        tile_sums[2 * tile + 1] = b;
+      }
+    }
   }
   __syncthreads();
-  uint2* dst = staged + base;
-  // Plain stores: the move pass reads the staged pairs back from the
-  // Infinity Cache (non-temporal stores made it slower).
-  for (uint32_t i = t; i < tile_total; i += kPB) dst[i] = s_pair[i];
+  if (!out_p) return;
+  const uint64_t gbase = s_excl;
+  for (uint32_t i = t; i < tile_total; i += kPB) {
+    const uint64_t pos = gbase + i;
+    if (pos < capacity) {
+      __builtin_nontemporal_store(s_p[i], out_p + pos);
+      __builtin_nontemporal_store(s_b[i], out_b + pos);
+    }
+  }
 }
 
-// Phase 3: move each tile's staged pairs to their final offset (one workgroup
-// per tile; consecutive threads copy consecutive pairs).
-__global__ __launch_bounds__(kPB) void k_probe_move(const uint2* __restrict__ staged,
-                                                    const uint32_t* __restrict__ tile_count,
+// Phase 3: move each tile's matches to their final offset.  The ballots give
+// every match's probe row and rank in the tile (as the lookup pass ranked
+// them); the probe rows are ranked into LDS, then both index arrays are
+// written with coalesced stores, the build rows read in rank order.
+__global__ __launch_bounds__(kPB) void k_probe_move(PairStage S, const uint32_t* __restrict__ tile_count,
                                                     const uint64_t* __restrict__ tile_off,
                                                     const uint64_t* __restrict__ slice_off, int slice_shift,
-                                                    uint32_t* __restrict__ out_p, uint32_t* __restrict__ out_b,
-                                                    uint64_t capacity) {
+                                                    uint32_t probe_base, uint32_t* __restrict__ out_p,
+                                                    uint32_t* __restrict__ out_b, uint64_t capacity) {
+  __shared__ uint32_t s_p[kTileRows];
+  __shared__ unsigned long long s_bits[kTileBallots];
+  __shared__ uint32_t s_pre[kTileBallots / 2];  // matches before each (chunk, wave) group
+  static_assert(kTileBallots == 64, "one wave scans the tile's groups");
   const uint32_t tile = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t cnt = tile_count[tile];
   const uint64_t off = slice_off[tile >> slice_shift] + tile_off[tile];
-  const uint2* src = staged + static_cast<uint64_t>(tile) * kTileRows;
-  for (uint32_t i = threadIdx.x; i < cnt; i += kPB) {
+  const uint64_t base = static_cast<uint64_t>(tile) * kTileRows;
+  if (wave == 0) {
+    const unsigned long long w = S.bits[static_cast<uint64_t>(tile) * kTileBallots + lane];
+    s_bits[lane] = w;
+    // group g = lane / 2 holds words 2g, 2g + 1: the exclusive scan of the
+    // words' match counts at word 2g is the matches before group g
+    const uint32_t c = __popcll(w);
+    uint32_t inc = c;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += o;
+    }
+    if ((lane & 1) == 0) s_pre[lane >> 1] = inc - c;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPairs; ++j) {
+    const int g = j * kWaves + wave;
+    const uint64_t b0 = s_bits[2 * g], b1 = s_bits[2 * g + 1];
+    const uint32_t r0 = s_pre[g] + lanes_below(b0) + lanes_below(b1);
+    const bool h0 = (b0 >> lane) & 1, h1 = (b1 >> lane) & 1;
+    const uint32_t row0 = static_cast<uint32_t>(base + static_cast<uint64_t>(j) * kChunk + 2 * t) + probe_base;
+    if (h0) s_p[r0] = row0;
+    if (h1) s_p[r0 + (h0 ? 1 : 0)] = row0 + 1;
+  }
+  __syncthreads();
+  const uint32_t* src = S.brow + base;
+  for (uint32_t i = t; i < cnt; i += kPB) {
     const uint64_t pos = off + i;
     if (pos < capacity) {
-      const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(src) + i);
-      __builtin_nontemporal_store(static_cast<uint32_t>(v), out_p + pos);
-      __builtin_nontemporal_store(static_cast<uint32_t>(v >> 32), out_b + pos);
+      __builtin_nontemporal_store(s_p[i], out_p + pos);
+      __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), out_b + pos);
     }
   }
 }
@@ -781,12 +1023,12 @@ template <bool DIRECT>
 __global__ __launch_bounds__(kPB) void k_pt_restore(TableView T, const int64_t* __restrict__ keys, uint64_t n,
                                                     uint32_t n_tiles, const uint32_t* __restrict__ offs, uint32_t P,
                                                     const uint16_t* __restrict__ rec_row,
-                                                    const uint2* __restrict__ result, uint32_t probe_base,
+                                                    const uint2* __restrict__ result,
                                                     const int64_t* __restrict__ ppay, const int32_t* __restrict__ bpay,
-                                                    uint2* __restrict__ staged, uint32_t* __restrict__ tile_count,
+                                                    PairStage stage, uint32_t* __restrict__ tile_count,
                                                     __int128* __restrict__ tile_sums) {
   extern __shared__ Slot s_t[];  // DIRECT: the table's slots
-  __shared__ uint2 s_m[kTileRows];  // (build row, payload) per row of the tile, then its staged pairs
+  __shared__ uint2 s_m[kTileRows];  // (build row, payload) per row of the tile, then its staged build rows
   __shared__ uint32_t s_wave_tot[kPairs][kWaves];
   __shared__ __int128 s_red[2][kWaves];
   const int t = threadIdx.x;
@@ -856,9 +1098,12 @@ __global__ __launch_bounds__(kPB) void k_pt_restore(TableView T, const int64_t* 
     for (int j = 0; j < kPairs; ++j) {
       const uint64_t b0 = ballot64(hit[j][0]), b1 = ballot64(hit[j][1]);
       local[j] = lanes_below(b0) + lanes_below(b1);
-      if (lane == 0) s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
+      if (lane == 0) {
+        s_wave_tot[j][wave] = __popcll(b0) + __popcll(b1);
+        if (stage.bits) stage_ballots(stage, tile, j, wave, b0, b1);
+      }
     }
-    __syncthreads();  // also: every s_m read above is done before the pairs overwrite it
+    __syncthreads();  // also: every s_m read above is done before the build rows overwrite it
     uint32_t tile_total = 0;
 #pragma unroll
     for (int j = 0; j < kPairs; ++j) {
@@ -883,16 +1128,16 @@ __global__ __launch_bounds__(kPB) void k_pt_restore(TableView T, const int64_t* 
         tile_sums[2 * tile + 1] = bb;
       }
     }
-    if (staged) {
+    if (stage.brow) {
+      uint32_t* s_mb = reinterpret_cast<uint32_t*>(s_m);
 #pragma unroll
       for (int j = 0; j < kPairs; ++j) {
-        const uint32_t row0 = static_cast<uint32_t>(base + static_cast<uint64_t>(j) * kChunk + 2 * t) + probe_base;
-        if (hit[j][0]) s_m[local[j]] = make_uint2(row0, val[j][0].x);
-        if (hit[j][1]) s_m[local[j] + (hit[j][0] ? 1 : 0)] = make_uint2(row0 + 1, val[j][1].x);
+        if (hit[j][0]) s_mb[local[j]] = val[j][0].x;
+        if (hit[j][1]) s_mb[local[j] + (hit[j][0] ? 1 : 0)] = val[j][1].x;
       }
       __syncthreads();
-      uint2* dst = staged + base;
-      for (uint32_t i = t; i < tile_total; i += kPB) dst[i] = s_m[i];
+      uint32_t* dst = stage.brow + base;
+      for (uint32_t i = t; i < tile_total; i += kPB) dst[i] = s_mb[i];
     }
     __syncthreads();  // s_m, s_wave_tot and s_red are reused by the next tile
   }
@@ -2791,10 +3036,20 @@ namespace prism {
 namespace join {
 
 static uint64_t align256(uint64_t b) { return (b + 255) & ~255ULL; }
+static uint64_t stage_rows_bytes(uint64_t n_tiles) { return align256(n_tiles * kTileRows * 4); }
+static uint64_t stage_bits_bytes(uint64_t n_tiles) { return align256(n_tiles * kTileBallots * 8); }
+
+static uint64_t env_u64(const char* name, uint64_t dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  const unsigned long long x = std::strtoull(v, nullptr, 10);
+  return x ? x : dflt;
+}
 
 // Scratch of one probe of n_probe rows: per-tile int128 payload partials,
 // then either the unique path's tile counts / offsets / slice totals / staged
-// pairs or the duplicate path's look-back status words.  Every word the
+// build rows and match ballots (PairStage) or the look-back status words of
+// the duplicate path and the one-pass probe.  Every word the
 // kernels read is written by them first (k_probe_clear clears the status
 // words), so the scratch needs no clearing.
 // Scratch of the partitioned probe's passes A-D (after the direct probe's):
@@ -2836,31 +3091,45 @@ static PtPlan pt_plan(const prism_join_table* t, uint64_t n_probe) {
   return q;
 }
 
-static uint64_t probe_scratch_bytes(const prism_join_table* t, uint64_t n_probe, bool pairs) {
+// Probe form of a unique table probed directly: staged (lookup pass, tile
+// scan, move pass; 8 B of staging per probe row) by default, the faster form
+// on MI355X; one-pass (k_probe_unique_lb: pairs written once at offsets found
+// by looking back over the tiles' counts; 40 B of scratch per 4096-row tile)
+// for the allocation-free probe, whose fixed scratch the staging would bound,
+// and when PRISM_PROBE_ONE_PASS=1.  The partitioned probe always stages.
+static bool probe_one_pass(const prism_join_table* t, uint64_t n_probe, bool noalloc) {
+  if (!t->info.unique || pt_plan(t, n_probe).on) return false;
+  if (noalloc) return true;
+  const char* v = std::getenv("PRISM_PROBE_ONE_PASS");
+  return v && v[0] == '1';
+}
+
+static uint64_t probe_scratch_bytes(const prism_join_table* t, uint64_t n_probe, bool pairs, bool one_pass) {
   const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
   const uint64_t sums_bytes = align256(n_tiles * 32 + 32);
-  if (t->info.unique) {
+  if (t->info.unique && !one_pass) {
     const uint64_t n_slices_max = ceil_div(n_tiles, kSliceTiles) + 1;
     const uint64_t cnt_bytes = align256(n_tiles * 4 + 4), off_bytes = align256(n_tiles * 8 + 8);
     const uint64_t slice_bytes = align256(n_slices_max * 8) + align256(n_slices_max * 32);
-    const uint64_t stage_bytes = align256(pairs ? n_tiles * kTileRows * sizeof(uint2) : 0);
+    const uint64_t stage_bytes = pairs ? stage_rows_bytes(n_tiles) + stage_bits_bytes(n_tiles) : 0;
     return sums_bytes + cnt_bytes + off_bytes + slice_bytes + stage_bytes + pt_plan(t, n_probe).bytes;
   }
   return sums_bytes + align256((n_tiles + 1) * 8);
 }
 
 // Passes A-E of the partitioned probe (see k_pt_restore), writing what
-// k_probe_unique writes: staged pairs, tile counts, tile payload partials.
+// k_probe_unique writes: staged build rows and ballots, tile counts, tile
+// payload partials.
 static void pt_probe(prism_ctx* ctx, const prism_join_table* t, const PtPlan& q, const int64_t* keys, uint64_t n,
-                     uint32_t probe_base, const int64_t* ppay, const int32_t* bpay, uint2* staged,
-                     uint32_t* tile_count, __int128* tile_sums, char* pt) {
+                     const int64_t* ppay, const int32_t* bpay, PairStage stage, uint32_t* tile_count,
+                     __int128* tile_sums, char* pt) {
   hipStream_t st = ctx->stream;
   const TableView& T = t->view;
   const uint64_t PS = T.pmask + 1;
   if (q.direct) {
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(q.n_tiles, static_cast<uint64_t>(ctx->num_cus) * 2));
     hipLaunchKernelGGL(k_pt_restore<true>, dim3(grid), dim3(kPB), PS * sizeof(Slot), st, T, keys, n, q.n_tiles,
-                       nullptr, 1u, nullptr, nullptr, probe_base, ppay, bpay, staged, tile_count, tile_sums);
+                       nullptr, 1u, nullptr, nullptr, ppay, bpay, stage, tile_count, tile_sums);
     PRISM_CHECK_LAUNCH();
     return;
   }
@@ -2883,7 +3152,7 @@ static void pt_probe(prism_ctx* ctx, const prism_join_table* t, const PtPlan& q,
                      static_cast<const int64_t*>(rkey), static_cast<const uint32_t*>(offs), q.n_tiles, q.M, res);
   hipLaunchKernelGGL(k_pt_restore<false>, dim3(q.n_tiles), dim3(kPB), 0, st, T, keys, n, q.n_tiles,
                      static_cast<const uint32_t*>(offs), q.P, static_cast<const uint16_t*>(rrow),
-                     static_cast<const uint2*>(res), probe_base, ppay, bpay, staged, tile_count, tile_sums);
+                     static_cast<const uint2*>(res), ppay, bpay, stage, tile_count, tile_sums);
   PRISM_CHECK_LAUNCH();
 }
 
@@ -2904,7 +3173,7 @@ static void probe_checks(prism_ctx* ctx, const prism_join_table* t, const int64_
 static void probe_launch(prism_ctx* ctx, const prism_join_table* t, const int64_t* d_probe_keys, uint64_t n_probe,
                          uint32_t probe_row_base, const int64_t* d_probe_payload, const int32_t* d_build_payload,
                          uint32_t* d_probe_idx, uint32_t* d_build_idx, uint64_t capacity, prism_probe_result* d_result,
-                         char* base, bool timed) {
+                         char* base, bool timed, bool one_pass) {
   const uint64_t n_tiles = ceil_div(n_probe, kTileRows);
   const bool vec = (reinterpret_cast<uintptr_t>(d_probe_keys) & 15) == 0;
   const TableView& T = t->view;
@@ -2913,6 +3182,37 @@ static void probe_launch(prism_ctx* ctx, const prism_join_table* t, const int64_
   const bool pairs = d_probe_idx != nullptr;
   const uint64_t sums_bytes = align256(n_tiles * 32 + 32);
   __int128* tile_sums = reinterpret_cast<__int128*>(base);
+  if (one_pass) {
+    uint64_t* status = reinterpret_cast<uint64_t*>(base + sums_bytes);
+    // PRISM_LOOKBACK_POLLS (tests): polls before a tile counts an unpublished
+    // predecessor itself; 1 sends nearly every look-back down that path.
+    const uint32_t stall_polls = static_cast<uint32_t>(std::max<uint64_t>(1, env_u64("PRISM_LOOKBACK_POLLS", kStallPolls)));
+    hipLaunchKernelGGL(k_probe_clear, grid_for(ctx, n_tiles + 1), dim3(kBuildBlock), 0, ctx->stream, status,
+                       n_tiles, d_result);
+    PRISM_CHECK_LAUNCH();
+    if (n_probe == 0) return;
+    {
+      std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe") : nullptr);
+#define PRISM_PROBE_LB(KIND_, VEC_)                                                                             \
+  hipLaunchKernelGGL((k_probe_unique_lb<KIND_, VEC_>), g, dim3(kPB), 0, ctx->stream, T, d_probe_keys, n_probe, \
+                     nt, probe_row_base, d_probe_payload, d_build_payload, d_probe_idx, d_build_idx, capacity,  \
+                     status, tile_sums, d_result, stall_polls)
+      if (t->info.kind == PRISM_TABLE_DENSE) {
+        if (vec) PRISM_PROBE_LB(PRISM_TABLE_DENSE, true);
+        else PRISM_PROBE_LB(PRISM_TABLE_DENSE, false);
+      } else {
+        if (vec) PRISM_PROBE_LB(PRISM_TABLE_HASHED, true);
+        else PRISM_PROBE_LB(PRISM_TABLE_HASHED, false);
+      }
+#undef PRISM_PROBE_LB
+    }
+    PRISM_CHECK_LAUNCH();
+    if (d_probe_payload || d_build_payload || t->view.has_payload) {
+      hipLaunchKernelGGL(k_probe_sums, dim3(1), dim3(1024), 0, ctx->stream, tile_sums, n_tiles, d_result);
+      PRISM_CHECK_LAUNCH();
+    }
+    return;
+  }
   if (t->info.unique) {
     const uint64_t n_slices_max = ceil_div(n_tiles, kSliceTiles) + 1;
     const uint64_t cnt_bytes = align256(n_tiles * 4 + 4), off_bytes = align256(n_tiles * 8 + 8);
@@ -2922,19 +3222,24 @@ static void probe_launch(prism_ctx* ctx, const prism_join_table* t, const int64_
     uint64_t* tile_off = reinterpret_cast<uint64_t*>(p + cnt_bytes);
     uint64_t* slice_total = reinterpret_cast<uint64_t*>(p + cnt_bytes + off_bytes);
     __int128* slice_sums = reinterpret_cast<__int128*>(p + cnt_bytes + off_bytes + align256(n_slices_max * 8));
-    uint2* staged = pairs ? reinterpret_cast<uint2*>(p + cnt_bytes + off_bytes + slice_bytes) : nullptr;
-    const uint64_t stage_bytes = align256(pairs ? n_tiles * kTileRows * sizeof(uint2) : 0);
+    char* sp = p + cnt_bytes + off_bytes + slice_bytes;
+    PairStage stage{nullptr, nullptr};
+    if (pairs) {
+      stage.brow = reinterpret_cast<uint32_t*>(sp);
+      stage.bits = reinterpret_cast<unsigned long long*>(sp + stage_rows_bytes(n_tiles));
+    }
+    const uint64_t stage_bytes = pairs ? stage_rows_bytes(n_tiles) + stage_bits_bytes(n_tiles) : 0;
     hipLaunchKernelGGL(k_probe_clear, dim3(1), dim3(64), 0, ctx->stream, nullptr, 0, d_result);
     PRISM_CHECK_LAUNCH();
     if (n_probe == 0) return;
 #define PRISM_PROBE1(KIND_, VEC_)                                                                          \
   hipLaunchKernelGGL((k_probe_unique<KIND_, VEC_>), g, dim3(kPB), 0, ctx->stream, T, d_probe_keys, n_probe, \
-                     probe_row_base, d_probe_payload, d_build_payload, staged, tile_count, tile_sums)
+                     d_probe_payload, d_build_payload, stage, tile_count, tile_sums)
     const PtPlan q = pt_plan(t, n_probe);
     if (q.on) {
       std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe") : nullptr);
-      pt_probe(ctx, t, q, d_probe_keys, n_probe, probe_row_base, d_probe_payload, d_build_payload, staged,
-               tile_count, tile_sums, p + cnt_bytes + off_bytes + slice_bytes + stage_bytes);
+      pt_probe(ctx, t, q, d_probe_keys, n_probe, d_probe_payload, d_build_payload, stage, tile_count, tile_sums,
+               sp + stage_bytes);
     } else {
       std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe") : nullptr);
       if (t->info.kind == PRISM_TABLE_DENSE) {
@@ -2958,8 +3263,8 @@ static void probe_launch(prism_ctx* ctx, const prism_join_table* t, const int64_
     PRISM_CHECK_LAUNCH();
     if (pairs) {
       std::unique_ptr<KernelTimer> kt(timed ? new KernelTimer(ctx, "join_probe_move") : nullptr);
-      hipLaunchKernelGGL(k_probe_move, g, dim3(kPB), 0, ctx->stream, staged, tile_count, tile_off, slice_total,
-                         log2_pow2(kSliceTiles), d_probe_idx, d_build_idx, capacity);
+      hipLaunchKernelGGL(k_probe_move, g, dim3(kPB), 0, ctx->stream, stage, tile_count, tile_off, slice_total,
+                         log2_pow2(kSliceTiles), probe_row_base, d_probe_idx, d_build_idx, capacity);
       PRISM_CHECK_LAUNCH();
     }
     return;
@@ -3023,9 +3328,10 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t, con
     use_device(ctx);
     Ctx* c = as_ctx(ctx);
     std::lock_guard<std::mutex> lock(ctx->mu);
-    c->scratch.reserve(c, probe_scratch_bytes(t, n_probe, d_probe_idx != nullptr));
+    const bool one_pass = probe_one_pass(t, n_probe, false);
+    c->scratch.reserve(c, probe_scratch_bytes(t, n_probe, d_probe_idx != nullptr, one_pass));
     probe_launch(ctx, t, d_probe_keys, n_probe, probe_row_base, d_probe_payload, d_build_payload, d_probe_idx,
-                 d_build_idx, capacity, d_result, c->scratch.as<char>(), true);
+                 d_build_idx, capacity, d_result, c->scratch.as<char>(), true, one_pass);
   });
 }
 
@@ -3035,7 +3341,8 @@ prism_status prism_join_probe_i64_noalloc(prism_ctx* ctx, const prism_join_table
                                           uint32_t* d_build_idx, uint64_t capacity, prism_probe_result* d_result) {
   return guarded([&] {
     probe_checks(ctx, t, d_probe_keys, n_probe, probe_row_base, d_probe_idx, d_build_idx, d_result);
-    const uint64_t need = probe_scratch_bytes(t, n_probe, d_probe_idx != nullptr);
+    const bool one_pass = probe_one_pass(t, n_probe, true);
+    const uint64_t need = probe_scratch_bytes(t, n_probe, d_probe_idx != nullptr, one_pass);
     if (need > kNoAllocScratchBytes)
       fail(PRISM_ERR_INVALID_ARGUMENT, "Invalid argument: probe of " + std::to_string(n_probe) + " rows needs " +
                                            std::to_string(need) + " bytes of scratch; the allocation-free probe has " +
@@ -3044,7 +3351,7 @@ prism_status prism_join_probe_i64_noalloc(prism_ctx* ctx, const prism_join_table
     char* base = noalloc_scratch(ctx);
     std::lock_guard<std::mutex> lock(ctx->mu);
     probe_launch(ctx, t, d_probe_keys, n_probe, probe_row_base, d_probe_payload, d_build_payload, d_probe_idx,
-                 d_build_idx, capacity, d_result, base, false);
+                 d_build_idx, capacity, d_result, base, false, one_pass);
   });
 }
 
@@ -3226,12 +3533,6 @@ static bool ja_shape(uint64_t n_slots, JaShape& sh) {
   return true;
 }
 
-static uint64_t env_u64(const char* name, uint64_t dflt) {
-  const char* v = std::getenv(name);
-  if (!v || !*v) return dflt;
-  const unsigned long long x = std::strtoull(v, nullptr, 10);
-  return x ? x : dflt;
-}
 
 }  // namespace join
 }  // namespace prism

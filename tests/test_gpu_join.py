@@ -153,3 +153,31 @@ def test_full_size_probe_properties(ctx):
     assert np.array_equal(hk[gbi], hpk[gpi])
     assert r.probe_sum == exact_sum(hpp[mask])
     assert r.build_sum == int(hbp[gbi].astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("form", ["one_pass", "staged", "stall"])
+@pytest.mark.parametrize("kind", [L.TABLE_DENSE, L.TABLE_HASHED])
+def test_unique_probe_forms(ctx, monkeypatch, form, kind):
+    """Unique tables probed directly: the staged form (lookup pass, scan, move
+    pass), the one-pass probe (pairs written at offsets found by looking back
+    over the tiles' counts), and the one-pass probe with a look-back that
+    counts an unpublished predecessor after a single poll; every form equals
+    the oracle, ragged last tile and payload table included."""
+    say("unique probe form", form, kind)
+    if form in ("one_pass", "stall"):
+        monkeypatch.setenv("PRISM_PROBE_ONE_PASS", "1")
+    if form == "stall":
+        monkeypatch.setenv("PRISM_LOOKBACK_POLLS", "1")
+    d = dg.join_inputs(200_000, 3_000_001)
+    bk, bp = d["build_key"], d["build_payload"]
+    pk, pp = d["probe_key"], d["probe_payload"]
+    check_pairs(ctx, bk, pk, kind, pp, bp)
+    t = D.JoinTable(ctx, to_dev(bk), kind, payload=to_dev(bp))  # payload carried in the table
+    pi, bi, r = t.probe(to_dev(pk), to_dev(pp))
+    opi, obi = orc.join_probe(i64_batch(bk), [0], i64_batch(pk), [0])
+    assert np.array_equal(pi.numpy().view(np.uint32), opi) and np.array_equal(bi.numpy().view(np.uint32), obi)
+    assert r.matches == len(opi) and r.probe_sum == exact_sum(pp[opi])
+    assert r.build_sum == exact_sum(bp[obi].astype(np.int64))
+    _, _, r2 = t.probe(to_dev(pk), to_dev(pp), want_pairs=False)
+    assert r2.matches == len(opi) and r2.probe_sum == r.probe_sum and r2.build_sum == r.build_sum
+    t.close()
