@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # scaled by its reading for a plain 1 GiB device copy: FETCH 2.0, WRITE 1.0),
 # recorded in profiles/join_probe_1Mx64M.md and profiles/groupby_filter_200M.md.
 # Measured offline, not by this run: they hold for the default sizes only.
-TRAFFIC = {"join_probe": 3_494_600_000, "groupby_update": 6_133_500_000}
+TRAFFIC = {"join_probe": 3_319_100_000, "groupby_update": 6_133_500_000}
 DEFAULT_SIZES = (1_000_000, 64_000_000, 200_000_000)
 METRIC = "rows/s & %HBM peak (join probe, group-by); p99 latency batched vs plain"
 

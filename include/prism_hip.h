@@ -319,9 +319,14 @@ prism_status prism_join_probe_i64(prism_ctx* ctx, const prism_join_table* t,
  * reference; for callers that cannot allocate (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  The table is built
  * beforehand; every output is the caller's.  Its scratch is a fixed 128 MiB
- * device array per GPU, held until the process ends, which limits one probe
- * to 16,752,640 rows with pairs (any row count without pairs); a larger
- * probe fails with PRISM_ERR_INVALID_ARGUMENT before anything is launched.
+ * device array per GPU, held until the process ends.  A table probed
+ * directly takes the one-pass probe here (pairs written at offsets found by
+ * looking back over the tiles' match counts, no pair staging): 40 bytes per
+ * 4,096 probe rows, so any u32 row count fits.  A partitioned table
+ * (PRISM_PROBE_PARTITIONED) stages about 22 bytes per probe row plus 8 bytes
+ * per (partition, 4,096-row tile), about 5.9M rows at 256 partitions; a
+ * larger probe fails with PRISM_ERR_INVALID_ARGUMENT before anything is
+ * launched.
  * Every call on one GPU shares that array: calls that overlap on one GPU
  * (from several contexts or streams) race, and must be ordered by the
  * caller.  Results equal prism_join_probe_i64's. */
