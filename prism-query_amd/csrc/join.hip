@@ -2445,6 +2445,179 @@ __global__ void k_ja_fold(const uint32_t* __restrict__ sattr, const unsigned lon
 
 // Groups holding rows, compacted without atomics: each block counts the
 // groups of its span, then writes them after the earlier blocks' groups.
+// Dense attribute groups without sorting the build rows: the rows are
+// partitioned by attribute bucket (kFoldGroups consecutive attribute values;
+// a counting pass per block of rows, one scan, a scatter with LDS cursors),
+// then one workgroup per bucket adds each row's key totals to its group's
+// accumulator in LDS and writes the bucket's groups once.  Sums are exact and
+// the first pair is a minimum, so the order of the rows inside a bucket does
+// not matter.
+constexpr int kFoldGroups = 2048;           // groups per bucket (LDS: 32 B each)
+constexpr int kFoldBlockRows = 4096;        // build rows per partition block (ordered by bucket in LDS)
+constexpr int kFoldScatterThreads = 1024;
+constexpr uint32_t kFoldMaxBuckets = 4096;  // a block's bucket counters in LDS (16 KB)
+constexpr int kFoldThreads = 512;
+constexpr int kFoldAhead = 4;               // rows (and slot records) in flight per thread
+
+__global__ __launch_bounds__(1024) void k_fold_hist(const int32_t* __restrict__ attr, uint64_t n, int64_t attr_min,
+                                                    uint32_t n_buckets, uint32_t n_blocks, uint32_t* __restrict__ hist) {
+  extern __shared__ uint32_t s_h[];
+  for (uint32_t i = threadIdx.x; i < n_buckets; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kFoldBlockRows;
+  for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(kFoldBlockRows); i += blockDim.x) {
+    const uint64_t r = base + i;
+    if (r < n) atomicAdd(&s_h[static_cast<uint32_t>(static_cast<int64_t>(attr[r]) - attr_min) / kFoldGroups], 1u);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < n_buckets; b += blockDim.x)
+    hist[static_cast<uint64_t>(b) * n_blocks + blockIdx.x] = s_h[b];
+}
+
+// Each row's (attribute offset, slot << 32 | row) to its bucket's place
+// (slot ~0: the key has no slot; the fold skips it).  The block's rows are
+// first ordered by bucket in LDS, so that consecutive threads then write
+// consecutive places of each bucket's run.
+template <int KIND>
+__global__ __launch_bounds__(kFoldScatterThreads) void k_fold_scatter(
+    TableView T, bool unique, const int64_t* __restrict__ bkeys, const int32_t* __restrict__ attr, uint64_t n,
+    int64_t attr_min, uint32_t n_buckets, uint32_t n_blocks, const uint32_t* __restrict__ offs,
+    uint32_t* __restrict__ sattr, unsigned long long* __restrict__ sval) {
+  extern __shared__ uint32_t s_dyn[];
+  uint32_t* s_cnt = s_dyn;                 // rows per bucket in this block, then their start in the block
+  uint32_t* s_a = s_dyn + ((n_buckets + 1) & ~1u);  // block rows ordered by bucket: attribute offset
+  uint32_t* s_b = s_a + kFoldBlockRows;    //   its bucket
+  unsigned long long* s_v = reinterpret_cast<unsigned long long*>(s_b + kFoldBlockRows);  // slot << 32 | row
+  __shared__ uint32_t s_wave[kFoldScatterThreads / 64];
+  for (uint32_t i = threadIdx.x; i < n_buckets; i += kFoldScatterThreads) s_cnt[i] = 0;
+  __syncthreads();
+  constexpr int kPer = kFoldBlockRows / kFoldScatterThreads;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kFoldBlockRows;
+  uint32_t a[kPer], bk[kPer], rank[kPer];
+  unsigned long long v[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint64_t r = base + static_cast<uint64_t>(j) * kFoldScatterThreads + threadIdx.x;
+    bk[j] = ~0u;
+    if (r < n) {
+      a[j] = static_cast<uint32_t>(static_cast<int64_t>(attr[r]) - attr_min);
+      const int64_t sl = slot_of<KIND>(T, bkeys[r], unique);
+      v[j] = ((sl < 0 ? 0xFFFFFFFFULL : static_cast<unsigned long long>(sl)) << 32) | r;
+      bk[j] = a[j] / kFoldGroups;
+      rank[j] = atomicAdd(&s_cnt[bk[j]], 1u);
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the bucket counts in place (thread t: buckets t*q ..)
+  const uint32_t q = (n_buckets + kFoldScatterThreads - 1) / kFoldScatterThreads;
+  uint32_t run = 0;
+  for (uint32_t i = 0; i < q; ++i) {
+    const uint32_t b = threadIdx.x * q + i;
+    if (b < n_buckets) run += s_cnt[b];
+  }
+  uint32_t inc = run;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  uint32_t before = inc - run;
+  for (int w = 0; w < wave; ++w) before += s_wave[w];
+  for (uint32_t i = 0; i < q; ++i) {
+    const uint32_t b = threadIdx.x * q + i;
+    if (b < n_buckets) {
+      const uint32_t c = s_cnt[b];
+      s_cnt[b] = before;
+      before += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    if (bk[j] == ~0u) continue;
+    const uint32_t at = s_cnt[bk[j]] + rank[j];
+    s_a[at] = a[j];
+    s_b[at] = bk[j];
+    s_v[at] = v[j];
+  }
+  __syncthreads();
+  const uint32_t rows = n - base < static_cast<uint64_t>(kFoldBlockRows) ? static_cast<uint32_t>(n - base)
+                                                                         : static_cast<uint32_t>(kFoldBlockRows);
+  for (uint32_t i = threadIdx.x; i < rows; i += kFoldScatterThreads) {
+    const uint32_t b = s_b[i];
+    const uint32_t at = offs[static_cast<uint64_t>(b) * n_blocks + blockIdx.x] + (i - s_cnt[b]);
+    sattr[at] = s_a[i];
+    sval[at] = s_v[i];
+  }
+}
+
+// One workgroup per bucket: every row adds its key slot's totals (count,
+// int128 sum, first probe row) to its group in LDS; the bucket's groups are
+// then written with plain stores (no other workgroup has them).
+__global__ __launch_bounds__(kFoldThreads) void k_fold_lds(const uint32_t* __restrict__ sattr,
+                                                           const unsigned long long* __restrict__ sval,
+                                                           const uint32_t* __restrict__ offs, uint32_t n_buckets,
+                                                           uint32_t n_blocks, uint64_t n, JaAcc A, JaGroups G) {
+  __shared__ unsigned long long s_cnt[kFoldGroups], s_lo[kFoldGroups], s_hi[kFoldGroups], s_f[kFoldGroups];
+  const uint32_t b = blockIdx.x;
+  for (int g = threadIdx.x; g < kFoldGroups; g += kFoldThreads) {
+    s_cnt[g] = 0;
+    s_lo[g] = 0;
+    s_hi[g] = 0;
+    s_f[g] = ~0ULL;
+  }
+  __syncthreads();
+  const uint64_t beg = offs[static_cast<uint64_t>(b) * n_blocks];
+  const uint64_t end = b + 1 < n_buckets ? offs[static_cast<uint64_t>(b + 1) * n_blocks] : n;
+  const uint32_t g0 = b * static_cast<uint32_t>(kFoldGroups);
+  const uint4* rec = reinterpret_cast<const uint4*>(A.w);
+  for (uint64_t i0 = beg + threadIdx.x; i0 < end; i0 += static_cast<uint64_t>(kFoldThreads) * kFoldAhead) {
+    uint32_t at[kFoldAhead];
+    unsigned long long v[kFoldAhead];
+    uint4 r0[kFoldAhead], r1[kFoldAhead];
+#pragma unroll
+    for (int j = 0; j < kFoldAhead; ++j) {
+      const uint64_t i = i0 + static_cast<uint64_t>(j) * kFoldThreads;
+      const uint64_t ic = i < end ? i : end - 1;
+      at[j] = sattr[ic];
+      v[j] = sval[ic];
+    }
+#pragma unroll
+    for (int j = 0; j < kFoldAhead; ++j) {
+      const uint32_t s = static_cast<uint32_t>(v[j] >> 32);
+      const uint64_t q = s == 0xFFFFFFFFu ? 0 : 2 * static_cast<uint64_t>(s);
+      r0[j] = rec[q];
+      r1[j] = rec[q + 1];
+    }
+#pragma unroll
+    for (int j = 0; j < kFoldAhead; ++j) {
+      const uint64_t i = i0 + static_cast<uint64_t>(j) * kFoldThreads;
+      if (i >= end || static_cast<uint32_t>(v[j] >> 32) == 0xFFFFFFFFu) continue;
+      const unsigned long long cc = (static_cast<unsigned long long>(r0[j].y) << 32) | r0[j].x;
+      if (!cc) continue;
+      const uint32_t g = at[j] - g0;
+      atomicAdd(&s_cnt[g], cc);
+      const uint64_t l = (static_cast<uint64_t>(r0[j].w) << 32) | r0[j].z;
+      const uint64_t h = (static_cast<uint64_t>(r1[j].y) << 32) | r1[j].x;
+      const unsigned long long old = atomicAdd(&s_lo[g], static_cast<unsigned long long>(l));
+      const uint64_t hh = h + (old + l < old ? 1u : 0u);
+      if (hh) atomicAdd(&s_hi[g], static_cast<unsigned long long>(hh));
+      atomicMin(&s_f[g], (static_cast<unsigned long long>(r1[j].z) << 32) | (v[j] & 0xFFFFFFFFULL));
+    }
+  }
+  __syncthreads();
+  for (int g = threadIdx.x; g < kFoldGroups; g += kFoldThreads) {
+    const uint64_t gg = static_cast<uint64_t>(g0) + g;
+    if (gg >= G.n || !s_cnt[g]) continue;
+    G.cnt[gg] = s_cnt[g];
+    G.sum[2 * gg] = s_lo[g];
+    G.sum[2 * gg + 1] = s_hi[g];
+    G.first[gg] = s_f[g];
+  }
+}
+
 constexpr int kCollectBlocks = 1024;
 
 __device__ __forceinline__ void ja_span(const JaGroups& G, uint64_t& lo, uint64_t& hi) {
@@ -2583,6 +2756,11 @@ struct prism_join_agg {
     return reinterpret_cast<unsigned long long*>(fold.as<char>() + ((n_build * 4 + 15) & ~15ULL));
   }
   uint64_t part_chunk = 0;  // probe rows per partitioned pass the scratch is sized for
+  // dense groups folded by bucket: the scan of the (bucket, block) row counts,
+  // which also gives each bucket's first row in `fold`; 0 buckets = `fold`
+  // holds the rows sorted by attribute instead
+  prism::DeviceBuffer fold_offs;
+  uint32_t fold_buckets = 0, fold_blocks = 0;
 
   prism::join::JaAcc accs() {
     prism::join::JaAcc A;
@@ -3477,7 +3655,40 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
       J->dense_groups = true;
       J->n_groups_cap = range;
       J->groups.alloc(c, range * 4 * 8);
-      if (n_build) {
+      const uint64_t n_buckets = ceil_div(range, kFoldGroups);
+      if (n_build && n_buckets <= kFoldMaxBuckets) {
+        // build rows partitioned by attribute bucket, for the LDS fold
+        J->fold.alloc(c, n_build * 12 + 16);
+        const uint32_t nbk = static_cast<uint32_t>(n_buckets);
+        const uint32_t nbl = static_cast<uint32_t>(ceil_div(n_build, kFoldBlockRows));
+        const uint64_t cells = static_cast<uint64_t>(nbk) * nbl;
+        J->fold_offs.alloc(c, cells * 4 + 16);
+        DeviceBuffer hist(c, cells * 4 + 16);
+        hipLaunchKernelGGL(k_fold_hist, dim3(nbl), dim3(1024), nbk * 4, st, d_build_attr, n_build, lo, nbk, nbl,
+                           hist.as<uint32_t>());
+        PRISM_CHECK_LAUNCH();
+        size_t tmp = 0;
+        PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, hist.as<uint32_t>(), J->fold_offs.as<uint32_t>(),
+                                                   static_cast<int>(cells), st));
+        DeviceBuffer t2(c, tmp);
+        PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(t2.ptr, tmp, hist.as<uint32_t>(), J->fold_offs.as<uint32_t>(),
+                                                   static_cast<int>(cells), st));
+        const size_t scatter_lds = ((nbk + 1) & ~1u) * 4 + static_cast<size_t>(kFoldBlockRows) * 16;
+        if (t_kind_dense(J->table))
+          hipLaunchKernelGGL(k_fold_scatter<PRISM_TABLE_DENSE>, dim3(nbl), dim3(kFoldScatterThreads), scatter_lds, st,
+                             J->table->view,
+                             J->table->info.unique, d_build_keys, d_build_attr, n_build, lo, nbk, nbl,
+                             J->fold_offs.as<uint32_t>(), J->fold.as<uint32_t>(), J->fold_vals());
+        else
+          hipLaunchKernelGGL(k_fold_scatter<PRISM_TABLE_HASHED>, dim3(nbl), dim3(kFoldScatterThreads), scatter_lds, st,
+                             J->table->view,
+                             J->table->info.unique, d_build_keys, d_build_attr, n_build, lo, nbk, nbl,
+                             J->fold_offs.as<uint32_t>(), J->fold.as<uint32_t>(), J->fold_vals());
+        PRISM_CHECK_LAUNCH();
+        J->fold_buckets = nbk;
+        J->fold_blocks = nbl;
+        PRISM_HIP(hipStreamSynchronize(st));
+      } else if (n_build) {
         // build rows sorted by attribute once, for the atomic-free fold
         J->fold.alloc(c, n_build * 12 + 16);
         uint32_t* sattr = J->fold.as<uint32_t>();
@@ -3668,7 +3879,13 @@ prism_status prism_join_agg_finish(prism_join_agg* J, uint64_t* n_groups) {
     const TableView& T = J->table->view;
     const bool unique = J->table->info.unique;
     JaGroups GV = J->groups_view();
-    if (J->n_build && J->dense_groups) {
+    if (J->n_build && J->dense_groups && J->fold_buckets) {
+      KernelTimer kt(c, "join_agg_build");
+      hipLaunchKernelGGL(k_fold_lds, dim3(J->fold_buckets), dim3(kFoldThreads), 0, st, J->fold.as<uint32_t>(),
+                         J->fold_vals(), J->fold_offs.as<uint32_t>(), J->fold_buckets, J->fold_blocks, J->n_build,
+                         J->accs(), GV);
+      PRISM_CHECK_LAUNCH();
+    } else if (J->n_build && J->dense_groups) {
       KernelTimer kt(c, "join_agg_build");
       const uint64_t threads = ceil_div(J->n_build, kFoldRows);
       hipLaunchKernelGGL(k_ja_fold, dim3(static_cast<unsigned>(ceil_div(threads, kBuildBlock))), dim3(kBuildBlock), 0,

@@ -194,3 +194,54 @@ def test_unique_build_keys_zipf_probe(ctx, monkeypatch, chunk):
     pay = np.random.default_rng(33).integers(-10**9, 10**9, 2_000_000)
     out = run(ctx, bk, attr, [(pk, pay, 0)])
     eager_check(out, bk, attr, pk, pay, n_attr)
+
+
+def first_order(bk, attr, pk):
+    """Groups of the join in first-appearance order (probe row ascending,
+    build row ascending within it) with each group's first pair, vectorised."""
+    uk, fi = np.unique(pk, return_index=True)
+    pos = np.minimum(np.searchsorted(uk, bk), len(uk) - 1)
+    rows = np.nonzero(uk[pos] == bk)[0]
+    key = fi[pos[rows]].astype(np.int64) * (1 << 32) + rows
+    a = attr[rows]
+    o = np.lexsort((key, a))
+    a_s, k_s = a[o], key[o]
+    head = np.r_[True, a_s[1:] != a_s[:-1]]
+    ga, gk = a_s[head], k_s[head]
+    order = np.argsort(gk, kind="stable")
+    return ga[order], gk[order] >> 32, gk[order] & 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("shape", ["buckets", "sorted"])
+def test_dense_group_folds(ctx, shape):
+    """Dense attribute groups are folded per 2,048-value bucket in LDS when the
+    range needs at most 4,096 buckets, else over build rows sorted by
+    attribute; both give every group's count, exact sum and first pair, in
+    first-appearance order."""
+    say(f"join-agg dense group fold {shape}")
+    rng = np.random.default_rng(41)
+    if shape == "buckets":  # 12,000 attribute values, negative ones included: six buckets
+        nb, lo_a, hi_a = 400_000, -3_000, 9_000
+    else:  # a 9.5M attribute range: more than 4,096 buckets
+        nb, lo_a, hi_a = 2_500_000, 0, 9_500_000
+    cdf = dg.zipf_cdf(200_000, 1.1)
+    bk = dg.zipf_i64(nb, 41, 1, cdf, 2)
+    attr = rng.integers(lo_a, hi_a, nb).astype(np.int32)
+    attr[0], attr[1] = lo_a, hi_a - 1
+    pk = dg.zipf_i64(3_000_000, 41, 3, cdf, 2)
+    pay = rng.integers(0, 1_000_000, 3_000_000)
+    out = run(ctx, bk, attr, [(pk, pay, 0)])
+    ga, gr, gb = first_order(bk, attr, pk)
+    assert np.array_equal(out["attr"], ga)
+    assert np.array_equal(out["first_probe_row"], gr) and np.array_equal(out["first_build_row"], gb)
+    uk, inv, kc = np.unique(pk, return_inverse=True, return_counts=True)
+    ks = np.zeros(len(uk), np.int64)
+    np.add.at(ks, inv, pay)
+    pos = np.minimum(np.searchsorted(uk, bk), len(uk) - 1)
+    hit = uk[pos] == bk
+    gc = np.zeros(hi_a - lo_a, np.int64)
+    gs = np.zeros(hi_a - lo_a, np.int64)
+    np.add.at(gc, attr[hit] - lo_a, kc[pos[hit]])
+    np.add.at(gs, attr[hit] - lo_a, ks[pos[hit]])
+    assert np.array_equal(np.asarray(out["count"], np.int64), gc[ga - lo_a])
+    assert np.array_equal(np.asarray(out["sum"], np.int64), gs[ga - lo_a])
