@@ -1568,21 +1568,19 @@ constexpr int kJaBuckets = kJaCache / kJaWays;
 static_assert(kJaBuckets == 256, "two 8-bit bucket indices");
 
 __host__ __device__ __forceinline__ uint32_t ja_bucket(uint32_t s, int t) {
-  const uint64_t h = static_cast<uint64_t>(s) * 0x9E3779B97F4A7C15ULL;
-  return static_cast<uint32_t>(h >> (t ? 48 : 56)) & (kJaBuckets - 1);
+  const uint32_t h = s * 0x9E3779B1u;  // one 32-bit multiply: the top two bytes pick the buckets
+  return (h >> (t ? 16 : 24)) & (kJaBuckets - 1);
 }
 
-constexpr int kJaRegHot = 4;                    // slots summed in registers (the hottest)
-constexpr int kJaCacheAll = kJaCache + kJaRegHot;  // accumulators: cache entries, then register slots
 
 struct JaCache {
   uint32_t key[kJaCache];
-  unsigned long long cnt[kJaCacheAll], lo[kJaCacheAll], first[kJaCacheAll];
-  long long hi[kJaCacheAll];
+  unsigned long long cnt[kJaCache], lo[kJaCache], first[kJaCache];
+  long long hi[kJaCache];
 };
 
 __device__ __forceinline__ void ja_cache_init(JaCache& C, const uint32_t* __restrict__ keys) {
-  for (int i = threadIdx.x; i < kJaCacheAll; i += blockDim.x) {
+  for (int i = threadIdx.x; i < kJaCache; i += blockDim.x) {
     if (i < kJaCache) C.key[i] = keys[i];
     C.cnt[i] = 0;
     C.lo[i] = 0;
@@ -1785,91 +1783,20 @@ __device__ __forceinline__ void ja_stage1_flush(JaStage1& S, const JaPart& P, co
   }
 }
 
-// K1: probe rows -> LDS cache, misses staged by level-1 partition.
-// The hottest slots (by build-side row count) are summed in registers: a
-// Zipf head would otherwise serialise on one LDS address.
-
-struct JaReg {
-  uint32_t cnt, first;
-  uint64_t lo;
-  int64_t hi;
-  __device__ __forceinline__ void clear() {
-    cnt = 0;
-    first = 0xFFFFFFFFu;
-    lo = 0;
-    hi = 0;
-  }
-  // branch-free: every row updates all register slots under its match flag
-  __device__ __forceinline__ void add(bool m, int64_t x, uint32_t row) {
-    cnt += m ? 1u : 0u;
-    lo += m ? static_cast<uint32_t>(static_cast<uint64_t>(x)) : 0u;
-    hi += m ? (x >> 32) : 0;
-    first = (m && row < first) ? row : first;
-  }
-  // wave totals added to the workgroup's accumulators of register entry j
-  __device__ __forceinline__ void to_lds(JaCache& C, int j) {
-    unsigned long long c = cnt, l = lo;
-    long long h = hi;
-    uint32_t f = first;
-    for (int d = 32; d > 0; d >>= 1) {
-      c += __shfl_xor(c, d, 64);
-      l += __shfl_xor(l, d, 64);
-      h += __shfl_xor(h, d, 64);
-      const uint32_t o = __shfl_xor(f, d, 64);
-      f = o < f ? o : f;
-    }
-    if (lane_id() == 0 && c) {
-      const int e = kJaCache + j;
-      atomicAdd(&C.cnt[e], c);
-      atomicAdd(&C.lo[e], l);
-      atomicAdd(reinterpret_cast<unsigned long long*>(&C.hi[e]), static_cast<unsigned long long>(h));
-      atomicMin(&C.first[e], static_cast<unsigned long long>(f));
-    }
-  }
-  // wave totals, added to slot s's record by lane 0
-  __device__ __forceinline__ void flush(uint32_t s, const JaAcc& A) {
-    unsigned long long c = cnt, l = lo;
-    long long h = hi;
-    uint32_t f = first;
-    for (int d = 32; d > 0; d >>= 1) {
-      c += __shfl_xor(c, d, 64);
-      l += __shfl_xor(l, d, 64);
-      h += __shfl_xor(h, d, 64);
-      const uint32_t o = __shfl_xor(f, d, 64);
-      f = o < f ? o : f;
-    }
-    if (lane_id() == 0 && c) {
-      atomicAdd(&A.w[4 * static_cast<uint64_t>(s)], c);
-      Split128 sp;
-      sp.lo32 = l;
-      sp.hi32 = h;
-      uint64_t wl;
-      int64_t wh;
-      sp.to_i128(wl, wh);
-      if (wl | static_cast<uint64_t>(wh))
-        atomic_add_i128(&A.w[4 * static_cast<uint64_t>(s) + 1], &A.w[4 * static_cast<uint64_t>(s) + 2], wl, wh);
-      atomicMin(&A.w[4 * static_cast<uint64_t>(s) + 3], static_cast<unsigned long long>(f));
-    }
-  }
-};
-
+// K1: probe rows -> LDS cache, misses staged by level-1 partition.  The
+// hottest slots go through the cache like the others: LDS atomics of a Zipf
+// head on one address cost less than summing the top slots in registers,
+// which every row paid for (probe of 256M rows: 3.15 ms against 3.36 ms with
+// four register slots).
 template <int KIND>
 __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ja_probe_part(TableView T, bool unique, const int64_t* __restrict__ keys,
                                                              const int64_t* __restrict__ pay, uint64_t n,
                                                              uint64_t row_base, JaAcc A,
                                                              const uint32_t* __restrict__ present, JaPart P,
-                                                             const uint32_t* __restrict__ hot_slots,
                                                              const uint32_t* __restrict__ cache_keys,
                                                              unsigned long long* __restrict__ cache_part) {
   __shared__ JaCache C;
   __shared__ JaStage1 S;
-  static_assert(kJaRegHot == 4, "four register slots");
-  const uint32_t h0 = hot_slots[0], h1 = hot_slots[1], h2 = hot_slots[2], h3 = hot_slots[3];
-  JaReg r0, r1, r2, r3;
-  r0.clear();
-  r1.clear();
-  r2.clear();
-  r3.clear();
   ja_cache_init(C, cache_keys);
   for (int p = threadIdx.x; p < kJaP1Max; p += blockDim.x) {
     S.n[p] = 0;
@@ -1935,12 +1862,6 @@ __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4)
       if (s < 0) continue;
       const unsigned long long row = row_base + r;
       const uint32_t su = static_cast<uint32_t>(s);
-      const bool m0 = su == h0, m1 = su == h1, m2 = su == h2, m3 = su == h3;
-      r0.add(m0, x[j], static_cast<uint32_t>(row));
-      r1.add(m1, x[j], static_cast<uint32_t>(row));
-      r2.add(m2, x[j], static_cast<uint32_t>(row));
-      r3.add(m3, x[j], static_cast<uint32_t>(row));
-      if (m0 | m1 | m2 | m3) continue;
       if (ja_cache_add(C, static_cast<uint32_t>(s), x[j], row)) continue;
       const uint32_t p = static_cast<uint32_t>(static_cast<uint64_t>(s) >> P.shift1);
       const uint32_t i = atomicAdd(&S.n[p], 1u);
@@ -1976,22 +1897,17 @@ __global__ __launch_bounds__(kJaBlock1) __attribute__((amdgpu_waves_per_eu(4, 4)
         P.l1_sr[static_cast<uint64_t>(p) * P.cap1 + at] = kJaHole;
     }
   }
-  // this workgroup's totals of the slots placed at create time and of the
-  // register slots are added to its own row of per-workgroup partials (the
-  // same row in every pass of the probe call; one reduction after the last
-  // pass adds them to the slot records); entries claimed at run time hold
-  // slots of this workgroup only and go to the slot records directly
+  // this workgroup's totals of the slots placed at create time are added to
+  // its own row of per-workgroup partials (the same row in every pass of the
+  // probe call; one reduction after the last pass adds them to the slot
+  // records); entries claimed at run time hold slots of this workgroup only
+  // and go to the slot records directly
   __syncthreads();
-  r0.to_lds(C, 0);
-  r1.to_lds(C, 1);
-  r2.to_lds(C, 2);
-  r3.to_lds(C, 3);
-  __syncthreads();
-  unsigned long long* row = cache_part + static_cast<uint64_t>(blockIdx.x) * kJaCacheAll * 4;
-  for (int c = threadIdx.x; c < kJaCacheAll; c += blockDim.x) {
+  unsigned long long* row = cache_part + static_cast<uint64_t>(blockIdx.x) * kJaCache * 4;
+  for (int c = threadIdx.x; c < kJaCache; c += blockDim.x) {
     const unsigned long long cnt = C.cnt[c];
     if (!cnt) continue;
-    if (c < kJaCache && C.key[c] != cache_keys[c]) {  // claimed by this workgroup: its own slot
+    if (C.key[c] != cache_keys[c]) {  // claimed by this workgroup: its own slot
       const uint64_t s = C.key[c];
       atomicAdd(&A.w[4 * s], cnt);
       Split128 sp;
@@ -2023,9 +1939,9 @@ __global__ __launch_bounds__(256) void k_ja_cache_reduce(const unsigned long lon
   const uint32_t c = blockIdx.x * kJaRedEntries + ce;
   unsigned long long cnt = 0, lo = 0, f = ~0ULL;
   long long hi = 0;
-  if (c < static_cast<uint32_t>(kJaCacheAll))
+  if (c < static_cast<uint32_t>(kJaCache))
     for (uint32_t w = g; w < n_wg; w += 8) {
-      const unsigned long long* e = cache_part + (static_cast<uint64_t>(w) * kJaCacheAll + c) * 4;
+      const unsigned long long* e = cache_part + (static_cast<uint64_t>(w) * kJaCache + c) * 4;
       cnt += e[0];
       lo += e[1];
       hi += static_cast<long long>(e[2]);
@@ -2036,14 +1952,14 @@ __global__ __launch_bounds__(256) void k_ja_cache_reduce(const unsigned long lon
   r_hi[g][ce] = hi;
   r_f[g][ce] = f;
   __syncthreads();
-  if (g != 0 || c >= static_cast<uint32_t>(kJaCacheAll)) return;
+  if (g != 0 || c >= static_cast<uint32_t>(kJaCache)) return;
   for (int k = 1; k < 8; ++k) {
     cnt += r_cnt[k][ce];
     lo += r_lo[k][ce];
     hi += r_hi[k][ce];
     f = r_f[k][ce] < f ? r_f[k][ce] : f;
   }
-  const uint32_t s = c < static_cast<uint32_t>(kJaCache) ? hot[16 + c] : hot[c - kJaCache];
+  const uint32_t s = hot[c];
   if (s == kJaEmpty || !cnt) return;
   atomicAdd(&A.w[4 * static_cast<uint64_t>(s)], cnt);
   Split128 sp;
@@ -2747,7 +2663,7 @@ struct prism_join_agg {
   // partitioned tail: level-1 regions (slot, row, payload arrays), level-2
   // records, per-partition and per-range counters
   prism::DeviceBuffer part_l1, part_l2, part_ctl;
-  prism::DeviceBuffer hot;  // the slots with the most build rows: register slots, then LDS cache keys (u32)
+  prism::DeviceBuffer hot;  // the slots with the most build rows: the LDS cache keys (u32)
   prism::DeviceBuffer cache_part;  // per K1 workgroup: cache and register totals (4 words per entry)
   // dense groups: build rows sorted by attribute: attribute offsets (u32), then
   // (slot << 32 | row) values (u64, 16-byte aligned)
@@ -3535,10 +3451,9 @@ prism_status prism_join_probe_i64_noalloc(prism_ctx* ctx, const prism_join_table
 
 static bool t_kind_dense(const prism_join_table* t) { return t->info.kind == PRISM_TABLE_DENSE; }
 
-// The slots with the most build rows: the first kJaRegHot go to registers,
-// the next ones into the LDS cache's buckets (first free way of either
-// bucket, in rank order).  hot buffer: u32 [0, 16) register slots, then the
-// kJaCache cache keys.
+// The slots with the most build rows go into the LDS cache's buckets (first
+// free way of either bucket, in rank order).  hot buffer: the kJaCache cache
+// keys (u32, kJaEmpty = free way).
 static void ja_pick_hot(prism_join_agg* J, hipStream_t st) {
   Ctx* c = J->ctx;
   prism_join_table* t = J->table;
@@ -3562,7 +3477,7 @@ static void ja_pick_hot(prism_join_agg* J, hipStream_t st) {
   int min_bin = kJaHotBins;
   uint64_t taken = 0;
   while (min_bin > 0 && taken + h[min_bin - 1] <= kJaHotListMax) taken += h[--min_bin];
-  std::vector<uint32_t> hot(16 + kJaCache, kJaEmpty);
+  std::vector<uint32_t> hot(kJaCache, kJaEmpty);
   if (taken) {
     if (dense)
       hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_DENSE>, g, dim3(256), 0, st, t->view, unique, n_slots, min_bin, list,
@@ -3579,10 +3494,8 @@ static void ja_pick_hot(prism_join_agg* J, hipStream_t st) {
       const uint32_t ca = static_cast<uint32_t>(a >> 32), cb = static_cast<uint32_t>(b >> 32);
       return ca != cb ? ca > cb : static_cast<uint32_t>(a) < static_cast<uint32_t>(b);
     });
-    size_t i = 0;
-    for (int j = 0; j < kJaRegHot && i < l.size(); ++j, ++i) hot[j] = static_cast<uint32_t>(l[i]);
-    uint32_t* keys = hot.data() + 16;
-    for (; i < l.size(); ++i) {
+    uint32_t* keys = hot.data();
+    for (size_t i = 0; i < l.size(); ++i) {
       const uint32_t sl = static_cast<uint32_t>(l[i]);
       bool placed = false;
       for (int tb = 0; tb < 2 && !placed; ++tb) {
@@ -3778,21 +3691,22 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
       if (dense)
         hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_DENSE>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0, st,
                            T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(), present,
-                           J->hot.as<uint32_t>() + 16);
+                           J->hot.as<uint32_t>());
       else
         hipLaunchKernelGGL(k_ja_probe<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(blocks)), dim3(kJaBlock), 0, st,
                            T, unique, d_probe_keys, d_probe_payload, n_probe, probe_row_base, J->accs(), present,
-                           J->hot.as<uint32_t>() + 16);
+                           J->hot.as<uint32_t>());
       PRISM_CHECK_LAUNCH();
       return;
     }
     // Larger passes amortise each pass's fixed work (range totals, scans, the
-    // staging tails): 128M rows per pass (6 GB of scratch) measured 3.95 ms for
-    // 256M rows against 4.43 ms in passes of 32M.  A device with less free
-    // memory gets passes whose scratch fits a quarter of it.
+    // staging tails): for 256M probe rows, one pass of 256M rows (12 GB of
+    // scratch) measured 3.36 ms, passes of 128M 3.46 ms, of 64M 3.60 ms, of
+    // 32M 4.43 ms.  A device with less free memory gets passes whose scratch
+    // fits a quarter of it.
     uint64_t chunk = env_u64("PRISM_JA_CHUNK", 0);
     if (chunk == 0) {
-      chunk = 128ULL << 20;
+      chunk = 256ULL << 20;
       size_t free_b = 0, total_b = 0;
       if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
         const uint64_t fit = static_cast<uint64_t>(free_b) / 4 / 48;
@@ -3831,10 +3745,10 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     P.nb = nb;
     P.l2 = J->part_l2.as<uint4>();
     const JaAcc A = J->accs();
-    J->cache_part.reserve(c, g1max * kJaCacheAll * 32);
+    J->cache_part.reserve(c, g1max * kJaCache * 32);
     unsigned long long* cpart = J->cache_part.as<unsigned long long>();
-    hipLaunchKernelGGL(k_fill_acc, grid_for(c, g1max * kJaCacheAll), dim3(kBuildBlock), 0, st, cpart,
-                       g1max * kJaCacheAll);
+    hipLaunchKernelGGL(k_fill_acc, grid_for(c, g1max * kJaCache), dim3(kBuildBlock), 0, st, cpart,
+                       g1max * kJaCache);
     for (uint64_t at = 0; at < n_probe; at += chunk) {
       const uint64_t m = n_probe - at < chunk ? n_probe - at : chunk;
       hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, st, P.fill1, static_cast<uint64_t>(sh.n_p1), 0ULL);
@@ -3844,10 +3758,10 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
       const int64_t* pp = d_probe_payload ? d_probe_payload + at : nullptr;
       if (dense)
         hipLaunchKernelGGL(k_ja_probe_part<PRISM_TABLE_DENSE>, dim3(static_cast<unsigned>(g1)), dim3(kJaBlock1), 0, st,
-                           T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), J->hot.as<uint32_t>() + 16, cpart);
+                           T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), cpart);
       else
         hipLaunchKernelGGL(k_ja_probe_part<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(g1)), dim3(kJaBlock1), 0,
-                           st, T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), J->hot.as<uint32_t>() + 16, cpart);
+                           st, T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), cpart);
       const dim3 gp(static_cast<unsigned>(sh.n_p1) * nb);
       hipLaunchKernelGGL(k_ja_count2, gp, dim3(kJaPartBlock), 0, st, P);
       hipLaunchKernelGGL(k_ja_blk_scan, dim3(static_cast<unsigned>(sh.n_p1)), dim3(kJaPartBlock), 0, st, P);
@@ -3856,7 +3770,7 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
       hipLaunchKernelGGL(k_ja_agg2, dim3(static_cast<unsigned>(n_ranges)), dim3(kJaPartBlock), 0, st, P, A);
       PRISM_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_ja_cache_reduce, dim3(static_cast<unsigned>(ceil_div(kJaCacheAll, kJaRedEntries))), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_ja_cache_reduce, dim3(static_cast<unsigned>(ceil_div(kJaCache, kJaRedEntries))), dim3(256), 0, st,
                        cpart, static_cast<uint32_t>(g1max), J->hot.as<uint32_t>(), A);
     PRISM_CHECK_LAUNCH();
   });
