@@ -2196,7 +2196,7 @@ __global__ __launch_bounds__(kJaPartBlock) void k_ja_agg2(JaPart P, JaAcc A) {
     if (r.z) atomicAdd(&a_lo[r.x], static_cast<unsigned long long>(r.z));
     if (r.w) atomicAdd(reinterpret_cast<unsigned long long*>(&a_hi[r.x]),
                        static_cast<unsigned long long>(static_cast<long long>(static_cast<int32_t>(r.w))));
-    if (r.y < a_first[r.x]) atomicMin(&a_first[r.x], r.y);
+    atomicMin(&a_first[r.x], r.y);  // no return: nothing waits on it
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kJaSub; i += blockDim.x) {
