@@ -534,8 +534,6 @@ struct Slot {
   };
   std::vector<Timed> timed;
   hipEvent_t ev_kern = nullptr;       // after the kernels, before the results copy
-  std::atomic<int> signalled{0};      // set by the stream's host callback
-  prism_service* owner = nullptr;
   Slot() = default;
   Slot(Slot&&) noexcept {}
 };
@@ -577,14 +575,6 @@ void prism_service::wake() {
 
 namespace prism {
 namespace svc {
-
-// Runs on the runtime's callback thread when a batch's results are on the
-// host: flag the slot and wake the dispatcher (no HIP calls here).
-void on_batch_done(void* p) {
-  Slot* sl = static_cast<Slot*>(p);
-  sl->signalled.store(1);
-  sl->owner->wake();
-}
 
 static double ms_between(clk::time_point a, clk::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -703,7 +693,6 @@ static void launch_variant(int v, dim3 grid, size_t lds, hipStream_t st, Control
   }
 }
 
-void on_batch_done(void* p);
 
 // Build the batch's control block, launch it on the slot's stream.
 static void launch_batch(prism_service* S, Slot& sl) {
@@ -1002,8 +991,6 @@ static void launch_batch(prism_service* S, Slot& sl) {
   sl.results_words = res_used;
   sl.table_words = tab_used;
   PRISM_HIP(hipEventRecord(sl.ev, st));
-  sl.signalled.store(0);
-  PRISM_HIP(hipLaunchHostFunc(st, on_batch_done, &sl));
   sl.t_launched = clk::now();
   if (S->cfg.autotune)
     for (int k = 0; k < 3; ++k)
@@ -1163,7 +1150,7 @@ static void dispatcher(prism_service* S) {
   while (true) {
     bool progress = false;
     for (Slot& sl : S->slots)
-      if (sl.busy && (sl.signalled.load() || hipEventQuery(sl.ev) == hipSuccess)) {
+      if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
         PRISM_HIP(hipEventSynchronize(sl.ev));
         finalize(S, sl);
         progress = true;
@@ -1212,12 +1199,28 @@ static void dispatcher(prism_service* S) {
     for (Slot& sl : S->slots) any_busy = any_busy || sl.busy;
     if (S->stop && S->pending.empty() && !any_busy) break;
     if (!progress) {
-      // woken by submit() or by a batch's completion callback
       lock.unlock();
-      {
+      if (any_busy) {
+        // Batches in flight: poll their events (0.1 us a query) and the submit
+        // flag, yielding between polls, for up to 200 us.  A completion
+        // callback on the stream (hipLaunchHostFunc) cost 13-24 us of the
+        // dispatcher's time per batch and woke it no sooner.
+        const auto until = clk::now() + std::chrono::microseconds(200);
+        bool ready = false;
+        while (!ready && clk::now() < until) {
+          {
+            std::lock_guard<std::mutex> wl(S->wake_mu);
+            ready = S->woke;
+            S->woke = false;
+          }
+          for (Slot& sl : S->slots)  // busy and ev are the dispatcher's own
+            ready = ready || (sl.busy && hipEventQuery(sl.ev) == hipSuccess);
+          if (!ready) std::this_thread::yield();
+        }
+      } else {
+        // idle: woken by submit()
         std::unique_lock<std::mutex> wl(S->wake_mu);
-        S->cv_work.wait_for(wl, any_busy ? std::chrono::microseconds(200) : std::chrono::milliseconds(5),
-                            [&] { return S->woke; });
+        S->cv_work.wait_for(wl, std::chrono::milliseconds(5), [&] { return S->woke; });
         S->woke = false;
       }
       lock.lock();
@@ -1259,7 +1262,6 @@ prism_status prism_service_create(prism_ctx* ctx, const prism_service_config* cf
       PRISM_HIP(hipEventCreate(&sl.ev));
       PRISM_HIP(hipEventCreate(&sl.ev_start));
       PRISM_HIP(hipEventCreate(&sl.ev_kern));
-      sl.owner = S.get();
       sl.arena.alloc(S->ctx, c.arena_bytes);
       sl.control.alloc(S->ctx, sizeof(Control));
       PRISM_HIP(hipHostMalloc(reinterpret_cast<void**>(&sl.h_control), sizeof(Control), hipHostMallocDefault));
