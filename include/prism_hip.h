@@ -654,7 +654,9 @@ typedef struct {
   uint32_t max_batch_requests;  /* requests per batch (0 or > 256: 256) */
   uint64_t max_batch_rows;      /* rows per batch (0: 256 Mi) */
   uint64_t arena_bytes;         /* working HBM per in-flight batch (0: 256 MiB) */
-  int32_t use_graphs;           /* replay captured launch sequences for recurring batch shapes */
+  int32_t use_graphs;           /* replay captured launch sequences for recurring batch shapes: 0 never, 1 batches
+                                   of 10 or more kernel launches (a graph launch costs the host about as much as
+                                   ten kernel launches), 2 every recurring shape */
   int32_t autotune;             /* pick tile rows / workgroups per CU per kind from the first batches */
 } prism_service_config;
 

@@ -60,6 +60,7 @@ constexpr int kMaxTiles = 32768;       // tiles per batch
 using gb::kFinishCap;                  // group-table slots a finish workgroup orders in LDS
 constexpr int kVariants = 10;          // persistent-kernel variants (see Variant)
 constexpr int kJitQueues = 16;         // queues of generated group-by tile kernels per batch
+constexpr int kGraphMinLaunches = 10;  // launches per batch from which a graph replay pays
 constexpr size_t kResultsBytes = 48u << 20;
 
 enum Variant : int {
@@ -961,9 +962,15 @@ static void launch_batch(prism_service* S, Slot& sl) {
       hipLaunchKernelGGL(k_svc_pair_move, dim3(move_grid), dim3(kSB), 0, st, dC);
     }
   };
-  // A launch sequence is captured into a graph the second time its shape
-  // comes (a shape met once would pay capture + instantiation for nothing).
-  const std::string key = S->cfg.use_graphs && !timed
+  // A launch sequence of at least kGraphMinLaunches kernels is captured into
+  // a graph the second time its shape comes (a shape met once would pay
+  // capture + instantiation for nothing).  Fewer launches go directly: a
+  // graph launch costs the host ~29 us, a kernel launch ~2.6 us (median HIP
+  // API times of the served stream), so a batch of one or two requests
+  // (2-4 kernels) launched from a graph waited longer.
+  int n_launch = 1 + static_cast<int>(jf.size()) + (H->n_gb ? 1 : 0) + (pair_grid ? 2 : 0);
+  for (int v = 0; v < kVariants; ++v) n_launch += grid[v] ? 1 : 0;
+  const std::string key = S->cfg.use_graphs && !timed && (S->cfg.use_graphs >= 2 || n_launch >= kGraphMinLaunches)
                               ? shape_key(grid, lds, init_grid, fin_grid, jf, jgrid, jlds) + "," +
                                     std::to_string(pair_grid) + "," + std::to_string(move_grid)
                               : std::string();

@@ -19,11 +19,11 @@ from .device import Context, build_plan, instrs, ptr, read_groups
 
 class Service:
     def __init__(self, ctx: Context, hbm_budget: int = 0, max_inflight: int = 3, max_batch_requests: int = 256,
-                 max_batch_rows: int = 0, arena_bytes: int = 0, use_graphs: bool = True, autotune: bool = True):
+                 max_batch_rows: int = 0, arena_bytes: int = 0, use_graphs: int = 1, autotune: bool = True):
         self.ctx = ctx
         self.lib = ctx.lib
         cfg = L.ServiceConfig(hbm_budget, max_inflight, max_batch_requests, max_batch_rows, arena_bytes,
-                              1 if use_graphs else 0, 1 if autotune else 0)
+                              int(use_graphs), 1 if autotune else 0)
         self.handle = c_void_p()
         L.check(self.lib.prism_service_create(ctx.handle, byref(cfg), byref(self.handle)))
         self._n_aggs = {}

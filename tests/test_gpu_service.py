@@ -181,9 +181,11 @@ def test_mix_without_graphs_or_tuning(ctx, world):
 def test_graph_replays_match_eager(ctx, world):
     say("service graph replays")
     P, tabs, tabs_dev = world
-    # one batch shape over and over: small probes against table 0 only
+    # one batch shape over and over: small probes against table 0 only, each
+    # batch's few launches replayed from a graph (use_graphs=2: every
+    # recurring shape, not only batches of 10 or more launches)
     reqs = [("probe", 5000 + 7 * i, 1000 * i, 0) for i in range(30)]
-    S = Service(ctx, autotune=False, max_inflight=1, max_batch_requests=2)
+    S = Service(ctx, autotune=False, max_inflight=1, max_batch_requests=2, use_graphs=2)
     for i, (k, p) in enumerate(tabs):
         S.register_table(i, k, p)
     got = []
