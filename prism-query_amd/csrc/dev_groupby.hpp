@@ -453,7 +453,9 @@ struct RowTile {
   uint64_t start, end;
 };
 
-// One group-by request of a batch.
+// One group-by request of a batch.  A batch's requests sit kTileReqStride
+// bytes apart (each request's record also holds its other descriptors and
+// column descriptors), the group-by descriptor first.
 struct TileReq {
   const DevPlan* P;
   const prism_column* cols;
@@ -464,6 +466,12 @@ struct TileReq {
   uint32_t* flags;
   uint32_t pad;
 };
+constexpr uint32_t kTileReqStride = 1024;
+
+__device__ __forceinline__ const TileReq& tile_req(const TileReq* base, uint32_t i) {
+  return *reinterpret_cast<const TileReq*>(reinterpret_cast<const char*>(base) +
+                                           static_cast<uint64_t>(i) * kTileReqStride);
+}
 
 // Next tile index of the workgroup.  Claims on one device-wide counter
 // serialise (about 0.1 us each), so the first tile of a workgroup is its own

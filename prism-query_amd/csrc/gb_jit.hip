@@ -109,7 +109,7 @@ class Gen {
           "    const uint32_t t = next_tile(q, &s_t, first);\n"
           "    if (t >= n_tiles) break;\n"
           "    const RowTile tl = tiles[q->offset + t];\n"
-          "    const TileReq& d = reqs[tl.req];\n"
+          "    const TileReq& d = tile_req(reqs, tl.req);\n"
           "    gb_init<false>(d.P, lds, d.lds_slots);\n"
           "    gb_body(d.P, d.cols, tl.start, 256 * R_, tl.end, 0, d.G, lds, d.lds_slots, d.lds_shift, d.flags);\n"
           "    gb_flush<false>(d.P, d.G, lds, d.lds_slots, d.flags);\n"

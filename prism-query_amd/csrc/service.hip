@@ -115,31 +115,53 @@ struct InitDesc {
   uint32_t pad;
 };
 
+// One request of a batch: its descriptor (by kind), the words its batch
+// clears or fills before the tile kernels run, and its column descriptors.
+struct ReqRec {
+  union {
+    GbDesc gb;  // first: the generated group-by tile kernels read it at gb::kTileReqStride steps
+    ProbeDesc pr;
+    FiltDesc fd;
+  };
+  InitDesc init[2];  // n_words 0: unused
+  prism_column cols[kMaxCols];
+};
+static_assert(sizeof(ReqRec) <= gb::kTileReqStride, "a request record fits its stride");
+static_assert(offsetof(ReqRec, gb) == 0, "the group-by descriptor leads the record");
+
+// The records of a batch's n requests, then its tiles, packed from the start
+// of the pool: tile j of the batch is tile (first_tile(n) + j) of the pool
+// seen as a tile array, so one host -> device copy of the header and the used
+// part of the pool carries the whole batch.
+constexpr uint32_t first_tile(uint32_t n_req) {
+  return static_cast<uint32_t>((static_cast<uint64_t>(n_req) * gb::kTileReqStride + sizeof(Tile) - 1) / sizeof(Tile));
+}
+constexpr uint32_t kPoolTiles = first_tile(kMaxReq) + kMaxTiles;
+
 // Control block (device mirror of a pinned host staging block).
 struct Control {
   Queue q[kVariants + kJitQueues];  // prebuilt variants, then generated group-by kernels
-  uint32_t n_init, n_gb, pad0, pad1;
-  InitDesc init[2 * kMaxReq];
-  GbDesc gb[kMaxReq];
-  ProbeDesc pr[kMaxReq];
-  FiltDesc fd[kMaxReq];
-  uint32_t gb_idx[kMaxReq];
-  uint32_t n_pair, pad2;
-  uint32_t pair_req[kMaxReq];  // probe descriptors that want pairs
-  prism_column cols[kMaxReq][kMaxCols];
-  Tile tiles[kMaxTiles];
+  uint32_t n_req, n_gb, n_pair, pad0;
+  uint32_t gb_idx[kMaxReq];    // group-by requests
+  uint32_t pair_req[kMaxReq];  // probe requests that want pairs
+  alignas(256) unsigned char pool[static_cast<size_t>(kPoolTiles) * sizeof(Tile)];
   // written on the device only (not part of the host -> device copy)
-  uint32_t tile_cnt[kMaxTiles];  // pairs of each probe tile
-  uint64_t tile_off[kMaxTiles];  // their first output offset
+  uint32_t tile_cnt[kPoolTiles];  // pairs of each probe tile
+  uint64_t tile_off[kPoolTiles];  // their first output offset
+
+  __host__ __device__ ReqRec& req(uint32_t i) {
+    return *reinterpret_cast<ReqRec*>(pool + static_cast<size_t>(i) * gb::kTileReqStride);
+  }
+  __host__ __device__ Tile* tiles() { return reinterpret_cast<Tile*>(pool); }
 };
 
 // ------------------------------------------------------------------ kernels
 using gb::next_tile;
 
 __global__ __launch_bounds__(kSB) void k_svc_init(Control* C) {
-  const uint32_t n = C->n_init;
+  const uint32_t n = 2 * C->n_req;
   for (uint32_t d = blockIdx.x; d < n; d += gridDim.x) {
-    const InitDesc D = C->init[d];
+    const InitDesc D = C->req(d >> 1).init[d & 1];
     for (uint64_t i = threadIdx.x; i < D.n_words; i += kSB) D.dst[i] = D.pattern ? D.pattern[i % D.period] : 0ULL;
   }
 }
@@ -155,8 +177,8 @@ __global__ __launch_bounds__(kSB) void k_svc_groupby(Control* C, int v) {
     // uniform per workgroup: scalar loads of the tile and its request descriptor
     const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t, first));
     if (t >= n_tiles) break;
-    const Tile tl = C->tiles[q->offset + t];
-    const GbDesc& d = C->gb[tl.req];
+    const Tile tl = C->tiles()[q->offset + t];
+    const GbDesc& d = C->req(tl.req).gb;
     gb::gb_init<kPriv>(d.P, lds, d.lds_slots);
     gb::gb_rows<kRows, kDepth, kPref, kPriv>(d.P, d.cols, tl.start, static_cast<uint64_t>(kSB) * kRows, tl.end, 0,
                                              d.G, lds, d.lds_slots, d.lds_shift, d.flags);
@@ -229,12 +251,12 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
     // uniform per workgroup: scalar loads of the tile and its request descriptor
     const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t, first));
     if (t >= n_tiles) break;
-    const Tile tl = C->tiles[q->offset + t];
+    const Tile tl = C->tiles()[q->offset + t];
     // the descriptor in registers (a reference into the control block would be
     // re-read after every store), its buffers as global pointers
-    const ProbeDesc d = C->pr[tl.req];
+    const ProbeDesc d = C->req(tl.req).pr;
     if (tl.req != cur) {
-      if (cur != 0xFFFFFFFFu) svc_probe_flush(C->pr[cur].acc, m, ps, bs, s_m, s_p, s_b);
+      if (cur != 0xFFFFFFFFu) svc_probe_flush(C->req(cur).pr.acc, m, ps, bs, s_m, s_p, s_b);
       cur = tl.req;
     }
     const PRISM_GLOBAL int64_t* keys = (const PRISM_GLOBAL int64_t*)(d.keys);
@@ -330,7 +352,7 @@ __global__ __launch_bounds__(kSB) void k_svc_probe(Control* C, int v) {
       }
     }
   }
-  if (cur != 0xFFFFFFFFu) svc_probe_flush(C->pr[cur].acc, m, ps, bs, s_m, s_p, s_b);
+  if (cur != 0xFFFFFFFFu) svc_probe_flush(C->req(cur).pr.acc, m, ps, bs, s_m, s_p, s_b);
 }
 
 // Exclusive scan of each pairs request's tile counts (one workgroup per request).
@@ -339,7 +361,7 @@ __global__ __launch_bounds__(kSB) void k_svc_pair_scan(Control* C) {
   __shared__ typename BS::TempStorage tmp;
   const uint32_t n_pair = C->n_pair;
   for (uint32_t k = blockIdx.x; k < n_pair; k += gridDim.x) {
-    const ProbeDesc& d = C->pr[C->pair_req[k]];
+    const ProbeDesc& d = C->req(C->pair_req[k]).pr;
     uint64_t carry = 0;
     for (uint32_t b = 0; b < d.t_count; b += kSB) {
       const uint32_t i = b + threadIdx.x;
@@ -359,8 +381,8 @@ __global__ __launch_bounds__(kSB) void k_svc_pair_move(Control* C) {
   const uint32_t g0 = C->q[V_PROBE_DENSE].offset;
   const uint32_t g1 = C->q[V_PROBE_HASHED].offset + C->q[V_PROBE_HASHED].n_tiles;
   for (uint32_t g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
-    const Tile tl = C->tiles[g];
-    const ProbeDesc& d = C->pr[tl.req];
+    const Tile tl = C->tiles()[g];
+    const ProbeDesc& d = C->req(tl.req).pr;
     if (!d.stage) continue;
     const uint32_t cnt = C->tile_cnt[g];
     const uint64_t off = C->tile_off[g];
@@ -394,10 +416,10 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(4))) void k
     // uniform per workgroup: scalar loads of the tile and its request descriptor
     const uint32_t t = __builtin_amdgcn_readfirstlane(next_tile(q, &s_t, first));
     if (t >= n_tiles) break;
-    const Tile tl = C->tiles[q->offset + t];
-    const FiltDesc& d = C->fd[tl.req];
+    const Tile tl = C->tiles()[q->offset + t];
+    const FiltDesc& d = C->req(tl.req).fd;
     if (tl.req != cur) {
-      if (cur != 0xFFFFFFFFu && lane == 0 && sel) atomicAdd(C->fd[cur].count + copy, sel);
+      if (cur != 0xFFFFFFFFu && lane == 0 && sel) atomicAdd(C->req(cur).fd.count + copy, sel);
       sel = 0;
       cur = tl.req;
     }
@@ -430,7 +452,7 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
     }
   }
-  if (cur != 0xFFFFFFFFu && lane == 0 && sel) atomicAdd(C->fd[cur].count + copy, sel);
+  if (cur != 0xFFFFFFFFu && lane == 0 && sel) atomicAdd(C->req(cur).fd.count + copy, sel);
 }
 
 // Group-by finish: one workgroup per request collects the occupied slots,
@@ -441,7 +463,7 @@ __global__ __launch_bounds__(gb::kFinishThreads) void k_svc_gb_finish(Control* C
   __shared__ uint32_t s_n;
   const uint32_t n_gb = C->n_gb;
   for (uint32_t k = blockIdx.x; k < n_gb; k += gridDim.x) {
-    const GbDesc& d = C->gb[C->gb_idx[k]];
+    const GbDesc& d = C->req(C->gb_idx[k]).gb;
     const uint32_t n = gb::order_groups(d.G, d.out + 2, s_first, s_slot, &s_n);
     if (threadIdx.x == 0) {
       d.out[0] = n;
@@ -698,7 +720,8 @@ static void launch_variant(int v, dim3 grid, size_t lds, hipStream_t st, Control
 // Build the batch's control block, launch it on the slot's stream.
 static void launch_batch(prism_service* S, Slot& sl) {
   Control* H = sl.h_control;
-  memset(H, 0, offsetof(Control, tiles));
+  Control* dC = sl.control.as<Control>();
+  memset(H, 0, offsetof(Control, pool));
   const int n_cu = S->ctx->num_cus;
   std::vector<Tile> vt[kVariants];
   uint64_t vrows[kVariants] = {};
@@ -733,27 +756,31 @@ static void launch_batch(prism_service* S, Slot& sl) {
   unsigned long long* res_base = sl.arena.as<unsigned long long>();
   unsigned long long* tab_base = res_base + kResultsBytes / 8;
   size_t res_used = 0, tab_used = 0;
-  uint32_t n_gb = 0, n_pr = 0, n_fd = 0;
   std::vector<int> pair_variant;  // probe queue of each pairs request
-  for (size_t i = 0; i < sl.reqs.size(); ++i) {
-    Req* R = sl.reqs[i];
+  const uint32_t n_req = static_cast<uint32_t>(sl.reqs.size());
+  H->n_req = n_req;
+  for (uint32_t k = 0; k < n_req; ++k) {
+    Req* R = sl.reqs[k];
     size_t rw, tw;
     footprint(R, rw, tw);
     R->res_off = res_used;
     R->d_out = res_base + res_used;
     R->out_words = rw;
     res_used += rw;
+    ReqRec& rec = H->req(k);
+    memset(&rec, 0, sizeof(ReqRec));
+    // device address of the record's column descriptors
+    const prism_column* d_cols = dC->req(k).cols;
     switch (R->r.kind) {
       case PRISM_REQ_GROUPBY: {
         const GbPlan& gp = *R->gbp;
-        const uint32_t k = n_gb++;
-        GbDesc& d = H->gb[k];
+        GbDesc& d = rec.gb;
         const int stride = gp.g->plan.stride;
         unsigned long long* table = tab_base + tab_used;
         tab_used += tw;
-        memcpy(H->cols[k], R->cols.data(), R->cols.size() * sizeof(prism_column));
+        memcpy(rec.cols, R->cols.data(), R->cols.size() * sizeof(prism_column));
         d.P = gp.g->d_plan.as<gb::DevPlan>();
-        d.cols = &reinterpret_cast<Control*>(sl.control.ptr)->cols[k][0];
+        d.cols = d_cols;
         d.n = R->r.n_rows;
         d.G.w = table;
         d.G.cap = gp.cap;
@@ -783,12 +810,12 @@ static void launch_batch(prism_service* S, Slot& sl) {
         d.lds_slots = qi >= 0 ? gp.jit_slots : gp.lds_slots;
         d.lds_shift = qi >= 0 ? gp.jit_shift : gp.lds_shift;
         H->gb_idx[H->n_gb++] = k;
-        InitDesc& a = H->init[H->n_init++];
+        InitDesc& a = rec.init[0];
         a.dst = table;
         a.n_words = (gp.cap + 1) * stride;
         a.pattern = reinterpret_cast<const unsigned long long*>(gp.g->d_plan.as<char>() + offsetof(gb::DevPlan, tmpl));
         a.period = static_cast<uint32_t>(stride);
-        InitDesc& b = H->init[H->n_init++];
+        InitDesc& b = rec.init[1];
         b.dst = table + (gp.cap + 1) * stride;
         b.n_words = 2;
         const uint64_t step =
@@ -807,8 +834,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
         break;
       }
       case PRISM_REQ_PROBE: {
-        const uint32_t k = n_pr++;
-        ProbeDesc& d = H->pr[k];
+        ProbeDesc& d = rec.pr;
         d.T = R->table->t->view;
         d.keys = R->r.probe_keys;
         d.pay = R->r.probe_payload;
@@ -817,7 +843,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
         d.stage = nullptr;
         d.out_p = d.out_b = nullptr;
         d.cap = 0;
-        InitDesc& a = H->init[H->n_init++];
+        InitDesc& a = rec.init[0];
         a.dst = R->d_out;
         a.n_words = 5 * kAccCopies;
         const int v = R->table->t->info.kind == PRISM_TABLE_DENSE ? V_PROBE_DENSE : V_PROBE_HASHED;
@@ -840,15 +866,14 @@ static void launch_batch(prism_service* S, Slot& sl) {
       }
       default: {
         const FiltPlan& fp = S->filt_plans[R->r.target];
-        const uint32_t k = n_fd++;
-        FiltDesc& d = H->fd[k];
-        memcpy(H->cols[kMaxReq - 1 - k], R->cols.data(), R->cols.size() * sizeof(prism_column));
+        FiltDesc& d = rec.fd;
+        memcpy(rec.cols, R->cols.data(), R->cols.size() * sizeof(prism_column));
         d.P = fp.prog.as<FilterProg>();
-        d.cols = &reinterpret_cast<Control*>(sl.control.ptr)->cols[kMaxReq - 1 - k][0];
+        d.cols = d_cols;
         d.n = R->r.n_rows;
         d.mask = R->r.out_mask;
         d.count = R->d_out;
-        InitDesc& a = H->init[H->n_init++];
+        InitDesc& a = rec.init[0];
         a.dst = R->d_out;
         a.n_words = kAccCopies + 1;
         const uint64_t tr = std::max<uint64_t>(64 * 64, tile_of[K_FILT] / 64 * 64);
@@ -858,16 +883,18 @@ static void launch_batch(prism_service* S, Slot& sl) {
       }
     }
   }
-  uint32_t off = 0;
+  // tiles follow the request records in the pool
+  const uint32_t t0 = first_tile(n_req);
+  uint32_t off = t0;
   int grid[kVariants] = {};
   for (int v = 0; v < kVariants; ++v) {
     H->q[v].n_tiles = static_cast<uint32_t>(vt[v].size());
     memset(H->q[v].next, 0, sizeof(H->q[v].next));
     H->q[v].offset = off;
     if (!vt[v].empty()) {
-      if (off + vt[v].size() > static_cast<size_t>(kMaxTiles))
+      if (off - t0 + vt[v].size() > static_cast<size_t>(kMaxTiles))
         fail(PRISM_ERR_INTERNAL, "Internal error: batch tile overflow");
-      memcpy(&H->tiles[off], vt[v].data(), vt[v].size() * sizeof(Tile));
+      memcpy(&H->tiles()[off], vt[v].data(), vt[v].size() * sizeof(Tile));
       off += static_cast<uint32_t>(vt[v].size());
       const Cand cd = cand_of(S, kind_of(v));
       uint32_t bpc = cd.bpc;
@@ -885,8 +912,9 @@ static void launch_batch(prism_service* S, Slot& sl) {
     q.n_tiles = static_cast<uint32_t>(jt[i].size());
     memset(q.next, 0, sizeof(q.next));
     q.offset = off;
-    if (off + jt[i].size() > static_cast<size_t>(kMaxTiles)) fail(PRISM_ERR_INTERNAL, "Internal error: batch tile overflow");
-    memcpy(&H->tiles[off], jt[i].data(), jt[i].size() * sizeof(Tile));
+    if (off - t0 + jt[i].size() > static_cast<size_t>(kMaxTiles))
+      fail(PRISM_ERR_INTERNAL, "Internal error: batch tile overflow");
+    memcpy(&H->tiles()[off], jt[i].data(), jt[i].size() * sizeof(Tile));
     off += static_cast<uint32_t>(jt[i].size());
     const Cand cd = cand_of(S, K_GB);
     uint32_t bpc = cd.bpc;
@@ -897,13 +925,13 @@ static void launch_batch(prism_service* S, Slot& sl) {
                                                                       gb::kQueueLanes)));
   }
   for (uint32_t i = 0; i < H->n_pair; ++i)  // tile indices of pairs requests, now global
-    H->pr[H->pair_req[i]].t_first += H->q[pair_variant[i]].offset;
+    H->req(H->pair_req[i]).pr.t_first += H->q[pair_variant[i]].offset;
   const int pair_grid = H->n_pair ? static_cast<int>(pow2_grid(H->n_pair)) : 0;
   const uint32_t probe_tiles = H->q[V_PROBE_DENSE].n_tiles + H->q[V_PROBE_HASHED].n_tiles;
   const int move_grid =
       H->n_pair ? static_cast<int>(std::min<uint64_t>(pow2_grid(probe_tiles), static_cast<uint64_t>(n_cu) * 8)) : 0;
-  const size_t ctl_bytes = offsetof(Control, tiles) + off * sizeof(Tile);
-  Control* dC = sl.control.as<Control>();
+  // the header, the request records and the tiles after them
+  const size_t ctl_bytes = offsetof(Control, pool) + static_cast<size_t>(off) * sizeof(Tile);
   hipStream_t st = sl.st;
   uint64_t brows = 0;
   for (int v = 0; v < kVariants; ++v) brows += vrows[v];
@@ -911,7 +939,7 @@ static void launch_batch(prism_service* S, Slot& sl) {
   sl.batch_rows = brows;
   PRISM_HIP(hipEventRecord(sl.ev_start, st));
   PRISM_HIP(hipMemcpyAsync(dC, H, ctl_bytes, hipMemcpyHostToDevice, st));
-  const int init_grid = static_cast<int>(std::min<uint64_t>(pow2_grid(H->n_init), static_cast<uint64_t>(n_cu) * 4));
+  const int init_grid = static_cast<int>(std::min<uint64_t>(pow2_grid(2 * n_req), static_cast<uint64_t>(n_cu) * 4));
   const int fin_grid = static_cast<int>(std::min<uint64_t>(pow2_grid(H->n_gb), static_cast<uint64_t>(n_cu) * 2));
   // kinds launched by this batch; only a batch holding a kind still being
   // tuned is timed per launch (and kept out of graphs)
@@ -938,8 +966,8 @@ static void launch_batch(prism_service* S, Slot& sl) {
         sl.timed.push_back(Slot::Timed{v, a, b, vrows[v], cand_index(S, kind_of(v))});
       }
     }
-    Tile* d_tiles = &dC->tiles[0];
-    GbDesc* d_gb = &dC->gb[0];
+    Tile* d_tiles = dC->tiles();
+    GbDesc* d_gb = &dC->req(0).gb;
     for (size_t i = 0; i < jf.size(); ++i) {
       hipEvent_t a = nullptr, b = nullptr;
       if (with_events) {
