@@ -136,16 +136,20 @@ __global__ void k_init_slots(Slot* s, uint64_t n) {
 // DENSE: count rows per key offset; remember each row's offset.  Counts
 // gather in a workgroup LDS table first (skewed keys would otherwise
 // serialise on one counter); offsets it cannot place are counted in HBM.
+// With `sample_shift` > 0 only rows i << sample_shift are counted (and no
+// row offsets are written).
 constexpr int kCountSlots = 4096, kCountProbes = 4;
 __global__ __launch_bounds__(256) void k_dense_count(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key,
-                                                     uint32_t* cnt, uint32_t* slot_of_row) {
+                                                     uint32_t* cnt, uint32_t* slot_of_row, int sample_shift = 0) {
   __shared__ uint32_t sk[kCountSlots], sc[kCountSlots];
   for (int i = threadIdx.x; i < kCountSlots; i += blockDim.x) {
     sk[i] = kNone;
     sc[i] = 0;
   }
   __syncthreads();
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+  const uint64_t ns = sample_shift ? ((n - 1) >> sample_shift) + 1 : n;
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < ns; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = j << sample_shift;
     const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
     if (slot_of_row) slot_of_row[i] = d;
     uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(d) * 0x9E3779B97F4A7C15ULL) >> 52);  // 12 bits
@@ -171,6 +175,14 @@ __global__ __launch_bounds__(256) void k_dense_count(const int64_t* __restrict__
   __syncthreads();
   for (int i = threadIdx.x; i < kCountSlots; i += blockDim.x)
     if (sk[i] != kNone) atomicAdd(&cnt[sk[i]], sc[i]);
+}
+
+// DENSE key marks: cnt[d] = 1 for every key offset d of a build row (plain
+// stores: every writer stores the same value).
+__global__ __launch_bounds__(256) void k_dense_mark(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key,
+                                                    uint32_t* cnt) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    cnt[static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key)] = 1u;
 }
 
 // Rows per partition of a partitioned HASHED table (partition = top hash bits).
@@ -2033,6 +2045,35 @@ __global__ __launch_bounds__(256) void k_ja_hot_list(TableView T, bool unique, u
   }
 }
 
+// The hot-slot candidates in placement order - most build rows first, then
+// the lower slot - as keys (~rows << 32 | slot) sorted ascending by a bitonic
+// sort in LDS (one workgroup).
+constexpr int kJaHotSortThreads = 1024;
+__global__ __launch_bounds__(kJaHotSortThreads) void k_ja_hot_sort(unsigned long long* list, uint32_t n) {
+  __shared__ unsigned long long v[kJaHotListMax];
+  for (uint32_t i = threadIdx.x; i < kJaHotListMax; i += kJaHotSortThreads) {
+    const unsigned long long e = i < n ? list[i] : 0ULL;
+    v[i] = i < n ? (static_cast<unsigned long long>(~static_cast<uint32_t>(e >> 32)) << 32) | static_cast<uint32_t>(e)
+                 : ~0ULL;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= kJaHotListMax; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < kJaHotListMax; i += kJaHotSortThreads) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const unsigned long long a = v[i], b = v[l];
+          if ((a > b) == ((i & k) == 0)) {
+            v[i] = b;
+            v[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (uint32_t i = threadIdx.x; i < n; i += kJaHotSortThreads) list[i] = v[i];
+}
+
 // Level-1 records of repartition workgroup b: partition b / nb, records [lo, hi).
 __device__ __forceinline__ bool ja_part_block(const JaPart& P, uint32_t& p, uint32_t& blk, uint64_t& lo, uint64_t& hi) {
   p = blockIdx.x / P.nb;
@@ -2878,6 +2919,32 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       T->info.slots = range;
       DeviceBuffer cnt(c, range * 4);
       hipLaunchKernelGGL(k_fill_u32, grid_for(ctx, range), dim3(kBuildBlock), 0, st, cnt.as<uint32_t>(), range, 0u);
+      if (keys_only) {
+        // Key presence only (the join aggregate): dense_u[d] = 1 + the rows of
+        // key d among every 2^shift-th build row, 0 = absent.  Presence is
+        // exact; the multiplicity is an estimate, used to pick hot keys.
+        // Exact counts of a skewed key set cost device atomics for every key
+        // outside a workgroup's LDS table (~350 us for 16M Zipf keys).
+        const int shift = n_rows >= (4ULL << 20) ? 3 : 0;
+        if (n_rows) {
+          hipLaunchKernelGGL(k_dense_mark, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, hs.min_key,
+                             cnt.as<uint32_t>());
+          hipLaunchKernelGGL(k_dense_count, grid_for(ctx, n_rows >> shift), dim3(kBuildBlock), 0, st, d_keys, n_rows,
+                             hs.min_key, cnt.as<uint32_t>(), nullptr, shift);
+        }
+        PRISM_CHECK_LAUNCH();
+        T->info.n_distinct = 0;  // not counted
+        T->info.max_dup = 0;
+        T->info.unique = false;
+        T->dense = std::move(cnt);
+        T->view.dense_u = T->dense.as<uint32_t>();
+        T->view.presence_only = 1;
+        T->view.min_key = T->info.min_key;
+        T->view.range = range;
+        T->info.device_bytes = T->slots.bytes + T->dense.bytes + T->rows.bytes + T->payload.bytes;
+        *out = guard.release();
+        return;
+      }
       if (n_rows)
         hipLaunchKernelGGL(k_dense_count, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows,
                            hs.min_key, cnt.as<uint32_t>(), slot_of_row.as<uint32_t>());
@@ -2888,11 +2955,7 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       T->info.n_distinct = hs.n_distinct;
       T->info.max_dup = hs.max_dup;
       T->info.unique = hs.max_dup <= 1;
-      if (keys_only) {  // key presence (and multiplicity) only: no row lists
-        T->dense = std::move(cnt);
-        T->view.dense_u = T->dense.as<uint32_t>();
-        T->view.presence_only = 1;
-      } else if (T->info.unique) {
+      if (T->info.unique) {
         T->dense = std::move(cnt);  // reuse as the row array
         uint32_t* d = T->dense.as<uint32_t>();
         hipLaunchKernelGGL(k_fill_u32, grid_for(ctx, range), dim3(kBuildBlock), 0, st, d, range, kNone);
@@ -3485,15 +3548,12 @@ static void ja_pick_hot(prism_join_agg* J, hipStream_t st) {
     else
       hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_HASHED>, g, dim3(256), 0, st, t->view, unique, n_slots, min_bin,
                          list, n_list);
+    // most rows first, then the lower slot (the slot stays in the low word)
+    hipLaunchKernelGGL(k_ja_hot_sort, dim3(1), dim3(kJaHotSortThreads), 0, st, list, static_cast<uint32_t>(taken));
     PRISM_CHECK_LAUNCH();
     std::vector<unsigned long long> l(taken);
     PRISM_HIP(hipMemcpyAsync(l.data(), list, taken * 8, hipMemcpyDeviceToHost, st));
     PRISM_HIP(hipStreamSynchronize(st));
-    // most rows first, then the lower slot
-    std::sort(l.begin(), l.end(), [](unsigned long long a, unsigned long long b) {
-      const uint32_t ca = static_cast<uint32_t>(a >> 32), cb = static_cast<uint32_t>(b >> 32);
-      return ca != cb ? ca > cb : static_cast<uint32_t>(a) < static_cast<uint32_t>(b);
-    });
     uint32_t* keys = hot.data();
     for (size_t i = 0; i < l.size(); ++i) {
       const uint32_t sl = static_cast<uint32_t>(l[i]);
