@@ -136,9 +136,9 @@ __global__ void k_init_slots(Slot* s, uint64_t n) {
 // DENSE: count rows per key offset; remember each row's offset.  Counts
 // gather in a workgroup LDS table first (skewed keys would otherwise
 // serialise on one counter); offsets it cannot place are counted in HBM.
-// With `sample_shift` > 0 only rows i << sample_shift are counted (and no
-// row offsets are written).
-constexpr int kCountSlots = 4096, kCountProbes = 4;
+// With `sample_shift` > 0 only every 2^sample_shift-th run of kCountRun rows
+// is counted (and no row offsets are written).
+constexpr int kCountSlots = 4096, kCountProbes = 4, kCountRun = 256;
 __global__ __launch_bounds__(256) void k_dense_count(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key,
                                                      uint32_t* cnt, uint32_t* slot_of_row, int sample_shift = 0) {
   __shared__ uint32_t sk[kCountSlots], sc[kCountSlots];
@@ -147,9 +147,11 @@ __global__ __launch_bounds__(256) void k_dense_count(const int64_t* __restrict__
     sc[i] = 0;
   }
   __syncthreads();
-  const uint64_t ns = sample_shift ? ((n - 1) >> sample_shift) + 1 : n;
+  // sampled rows: every 2^sample_shift-th run of kCountRun rows
+  const uint64_t ns = sample_shift ? ceil_div(n, static_cast<uint64_t>(kCountRun) << sample_shift) * kCountRun : n;
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < ns; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = j << sample_shift;
+    const uint64_t i = ((j / kCountRun) << sample_shift) * kCountRun + j % kCountRun;
+    if (i >= n) continue;
     const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key));
     if (slot_of_row) slot_of_row[i] = d;
     uint32_t h = static_cast<uint32_t>((static_cast<uint64_t>(d) * 0x9E3779B97F4A7C15ULL) >> 52);  // 12 bits
@@ -177,12 +179,25 @@ __global__ __launch_bounds__(256) void k_dense_count(const int64_t* __restrict__
     if (sk[i] != kNone) atomicAdd(&cnt[sk[i]], sc[i]);
 }
 
-// DENSE key marks: cnt[d] = 1 for every key offset d of a build row (plain
-// stores: every writer stores the same value).
+// DENSE key marks: map[d] = 1 for every key offset d of a build row (plain
+// byte stores: every writer stores the same value).
 __global__ __launch_bounds__(256) void k_dense_mark(const int64_t* __restrict__ keys, uint64_t n, int64_t min_key,
-                                                    uint32_t* cnt) {
+                                                    uint8_t* map) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    cnt[static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key)] = 1u;
+    map[static_cast<uint64_t>(keys[i]) - static_cast<uint64_t>(min_key)] = 1;
+}
+
+// Byte marks (0 / 1, 32 per word) packed into bits: bit j of word w = map[32 w + j].
+__global__ __launch_bounds__(256) void k_pack_marks(const uint4* __restrict__ map, uint64_t n_words, uint32_t* bits) {
+  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < n_words; w += (uint64_t)gridDim.x * blockDim.x) {
+    const uint4 a = map[2 * w], b = map[2 * w + 1];
+    const uint32_t x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      r |= ((x[k] & 1u) | ((x[k] >> 7) & 2u) | ((x[k] >> 14) & 4u) | ((x[k] >> 21) & 8u)) << (4 * k);
+    bits[w] = r;
+  }
 }
 
 // Rows per partition of a partitioned HASHED table (partition = top hash bits).
@@ -1520,7 +1535,7 @@ __device__ __forceinline__ int64_t slot_of(const TableView& T, int64_t k, bool u
   if constexpr (KIND == PRISM_TABLE_DENSE) {
     const uint64_t d = static_cast<uint64_t>(k) - static_cast<uint64_t>(T.min_key);
     if (d >= T.range) return -1;
-    if (T.presence_only) return T.dense_u[d] != 0 ? static_cast<int64_t>(d) : -1;
+    if (T.presence_only) return ((T.present_bits[d >> 5] >> (d & 31)) & 1u) ? static_cast<int64_t>(d) : -1;
     if (unique) return T.dense_u[d] != kNone ? static_cast<int64_t>(d) : -1;
     return T.dense_nu[d].y != 0 ? static_cast<int64_t>(d) : -1;
   } else {
@@ -1552,19 +1567,6 @@ __device__ __forceinline__ int64_t probe_slot(const TableView& T, int64_t k, boo
     return ((present[d >> 5] >> (d & 31)) & 1u) ? static_cast<int64_t>(d) : -1;
   } else {
     return slot_of<KIND>(T, k, unique);
-  }
-}
-
-// Presence bitmap of a DENSE table: bit d = some build row has key min + d.
-__global__ void k_presence_bits(TableView T, uint32_t* bits, uint64_t n_words) {
-  for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < n_words; w += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t v = 0;
-    for (int b = 0; b < 32; ++b) {
-      const uint64_t d = w * 32 + b;
-      if (d < T.range && slot_of<PRISM_TABLE_DENSE>(T, static_cast<int64_t>(d + static_cast<uint64_t>(T.min_key)), true) >= 0)
-        v |= 1u << b;
-    }
-    bits[w] = v;
   }
 }
 
@@ -2681,6 +2683,22 @@ __global__ void k_minmax_i32_part(const int32_t* __restrict__ v, uint64_t n, int
   }
 }
 
+// Final min / max of k_minmax_i32_part's partials (one workgroup of 256).
+__global__ __launch_bounds__(256) void k_minmax_pairs(const int64_t* __restrict__ part, int n, int64_t* __restrict__ out) {
+  __shared__ int64_t lds[16];
+  int64_t mn = INT64_MAX, mx = INT64_MIN;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    mn = part[2 * i] < mn ? part[2 * i] : mn;
+    mx = part[2 * i + 1] > mx ? part[2 * i + 1] : mx;
+  }
+  mn = block_reduce(mn, MinOp(), lds);
+  mx = block_reduce(mx, MaxOp(), lds);
+  if (threadIdx.x == 0) {
+    out[0] = mn;
+    out[1] = mx;
+  }
+}
+
 }  // namespace join
 }  // namespace prism
 
@@ -2694,7 +2712,6 @@ struct prism_join_agg {
   prism::DeviceBuffer acc;     // cnt | sum (2 words) | first, per key slot
   prism::DeviceBuffer groups;  // cnt | sum | first | key, per group
   prism::DeviceBuffer counters;
-  prism::DeviceBuffer present;  // DENSE: presence bitmap of the key range
   int64_t attr_min = 0;
   uint64_t n_groups_cap = 0;
   bool dense_groups = true;
@@ -2920,19 +2937,29 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
       DeviceBuffer cnt(c, range * 4);
       hipLaunchKernelGGL(k_fill_u32, grid_for(ctx, range), dim3(kBuildBlock), 0, st, cnt.as<uint32_t>(), range, 0u);
       if (keys_only) {
-        // Key presence only (the join aggregate): dense_u[d] = 1 + the rows of
-        // key d among every 2^shift-th build row, 0 = absent.  Presence is
-        // exact; the multiplicity is an estimate, used to pick hot keys.
-        // Exact counts of a skewed key set cost device atomics for every key
-        // outside a workgroup's LDS table (~350 us for 16M Zipf keys).
+        // Key presence only (the join aggregate): a presence bitmap (exact)
+        // and dense_u[d] = the rows of key d among every 2^shift-th run of
+        // build rows (an estimate, used to pick hot keys).  Exact counts of a
+        // skewed key set cost device atomics for every key outside a
+        // workgroup's LDS table (~350 us for 16M Zipf keys).
         const int shift = n_rows >= (4ULL << 20) ? 3 : 0;
+        const uint64_t words = ceil_div(range, 32);
+        DeviceBuffer map(c, words * 32);
+        T->bits.alloc(c, words * 4);
+        hipLaunchKernelGGL(k_fill_u32, grid_for(ctx, words * 8), dim3(kBuildBlock), 0, st, map.as<uint32_t>(), words * 8,
+                           0u);
         if (n_rows) {
           hipLaunchKernelGGL(k_dense_mark, grid_for(ctx, n_rows), dim3(kBuildBlock), 0, st, d_keys, n_rows, hs.min_key,
-                             cnt.as<uint32_t>());
-          hipLaunchKernelGGL(k_dense_count, grid_for(ctx, n_rows >> shift), dim3(kBuildBlock), 0, st, d_keys, n_rows,
-                             hs.min_key, cnt.as<uint32_t>(), nullptr, shift);
+                             map.as<uint8_t>());
+          // one workgroup per CU: its LDS table sees enough sampled rows to
+          // merge the repeated keys before their counts go to HBM
+          hipLaunchKernelGGL(k_dense_count, dim3(static_cast<unsigned>(ctx->num_cus)), dim3(kBuildBlock), 0, st, d_keys,
+                             n_rows, hs.min_key, cnt.as<uint32_t>(), nullptr, shift);
         }
+        hipLaunchKernelGGL(k_pack_marks, grid_for(ctx, words), dim3(kBuildBlock), 0, st, map.as<uint4>(), words,
+                           T->bits.as<uint32_t>());
         PRISM_CHECK_LAUNCH();
+        T->view.present_bits = T->bits.as<uint32_t>();
         T->info.n_distinct = 0;  // not counted
         T->info.max_dup = 0;
         T->info.unique = false;
@@ -2941,7 +2968,8 @@ static prism_status build_impl(prism_ctx* ctx, const int64_t* d_keys, const int3
         T->view.presence_only = 1;
         T->view.min_key = T->info.min_key;
         T->view.range = range;
-        T->info.device_bytes = T->slots.bytes + T->dense.bytes + T->rows.bytes + T->payload.bytes;
+        T->info.device_bytes = T->slots.bytes + T->dense.bytes + T->rows.bytes + T->payload.bytes + T->bits.bytes;
+        PRISM_HIP(hipStreamSynchronize(st));  // the byte map goes back to the pool
         *out = guard.release();
         return;
       }
@@ -3595,33 +3623,29 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
     J->n_slots = t->info.kind == PRISM_TABLE_DENSE ? (t->info.slots ? t->info.slots : 1) : t->info.slots + 2;
     if (J->n_slots >= 0xFFFFFFFFULL) fail(PRISM_ERR_JOIN, "Join error: build side too large for the join aggregate");
     J->acc.alloc(c, J->n_slots * 4 * 8);
-    if (t->info.kind == PRISM_TABLE_DENSE && n_build) {
-      const uint64_t words = ceil_div(t->info.slots, 32);
-      J->present.alloc(c, words * 4);
-      hipLaunchKernelGGL(k_presence_bits, grid_for(c, words), dim3(kBuildBlock), 0, ctx->stream, t->view,
-                         J->present.as<uint32_t>(), words);
-      PRISM_CHECK_LAUNCH();
-    }
     J->counters.alloc(c, 64);
-    ja_pick_hot(J.get(), ctx->stream);
-    // attribute range: dense groups when it is small, else a hashed group table
     hipStream_t st = ctx->stream;
+    // attribute range (dense groups when it is small, else a hashed group
+    // table), read back with the hot-slot selection's first wait
+    int64_t* h_attr = reinterpret_cast<int64_t*>(static_cast<char*>(readback_block(ctx)) + 2048);
+    DeviceBuffer apart;
+    if (n_build) {
+      apart.alloc(c, kRedBlocks * 16 + 16);
+      int64_t* pa = apart.as<int64_t>();
+      hipLaunchKernelGGL(k_minmax_i32_part, dim3(kRedBlocks), dim3(kBuildBlock), 0, st, d_build_attr, n_build, pa);
+      hipLaunchKernelGGL(k_minmax_pairs, dim3(1), dim3(256), 0, st, pa, kRedBlocks, pa + 2 * kRedBlocks);
+      PRISM_CHECK_LAUNCH();
+      PRISM_HIP(hipMemcpyAsync(h_attr, pa + 2 * kRedBlocks, 16, hipMemcpyDeviceToHost, st));
+    }
+    ja_pick_hot(J.get(), st);
     int64_t lo = 0, hi = -1;
     if (n_build) {
-      DeviceBuffer part(c, kRedBlocks * 16);
-      hipLaunchKernelGGL(k_minmax_i32_part, dim3(kRedBlocks), dim3(kBuildBlock), 0, st, d_build_attr, n_build,
-                         part.as<int64_t>());
-      PRISM_CHECK_LAUNCH();
-      std::vector<int64_t> h(kRedBlocks * 2);
-      PRISM_HIP(hipMemcpyAsync(h.data(), part.ptr, kRedBlocks * 16, hipMemcpyDeviceToHost, st));
-      PRISM_HIP(hipStreamSynchronize(st));
-      lo = INT64_MAX;
-      hi = INT64_MIN;
-      for (int i = 0; i < kRedBlocks; ++i) {
-        lo = h[2 * i] < lo ? h[2 * i] : lo;
-        hi = h[2 * i + 1] > hi ? h[2 * i + 1] : hi;
-      }
+      PRISM_HIP(hipStreamSynchronize(st));  // idle by now: the selection waited on the stream
+      lo = h_attr[0];
+      hi = h_attr[1];
     }
+    // fold set-up buffers, released after the final wait below
+    DeviceBuffer fold_hist, fold_tmp;
     const uint64_t range = hi >= lo ? static_cast<uint64_t>(hi - lo) + 1 : 1;
     J->attr_min = lo;
     if (range <= (64ULL << 20) && range <= 4 * n_build + 1024) {
@@ -3636,15 +3660,16 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
         const uint32_t nbl = static_cast<uint32_t>(ceil_div(n_build, kFoldBlockRows));
         const uint64_t cells = static_cast<uint64_t>(nbk) * nbl;
         J->fold_offs.alloc(c, cells * 4 + 16);
-        DeviceBuffer hist(c, cells * 4 + 16);
+        fold_hist.alloc(c, cells * 4 + 16);
+        uint32_t* hist = fold_hist.as<uint32_t>();
         hipLaunchKernelGGL(k_fold_hist, dim3(nbl), dim3(1024), nbk * 4, st, d_build_attr, n_build, lo, nbk, nbl,
-                           hist.as<uint32_t>());
+                           hist);
         PRISM_CHECK_LAUNCH();
         size_t tmp = 0;
-        PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, hist.as<uint32_t>(), J->fold_offs.as<uint32_t>(),
+        PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, hist, J->fold_offs.as<uint32_t>(),
                                                    static_cast<int>(cells), st));
-        DeviceBuffer t2(c, tmp);
-        PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(t2.ptr, tmp, hist.as<uint32_t>(), J->fold_offs.as<uint32_t>(),
+        fold_tmp.alloc(c, tmp);
+        PRISM_HIP(hipcub::DeviceScan::ExclusiveSum(fold_tmp.ptr, tmp, hist, J->fold_offs.as<uint32_t>(),
                                                    static_cast<int>(cells), st));
         const size_t scatter_lds = ((nbk + 1) & ~1u) * 4 + static_cast<size_t>(kFoldBlockRows) * 16;
         if (t_kind_dense(J->table))
@@ -3660,7 +3685,6 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
         PRISM_CHECK_LAUNCH();
         J->fold_buckets = nbk;
         J->fold_blocks = nbl;
-        PRISM_HIP(hipStreamSynchronize(st));
       } else if (n_build) {
         // build rows sorted by attribute once, for the atomic-free fold
         J->fold.alloc(c, n_build * 12 + 16);
@@ -3737,7 +3761,7 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     const TableView& T = J->table->view;
     const bool unique = J->table->info.unique;
     const bool dense = J->table->info.kind == PRISM_TABLE_DENSE;
-    const uint32_t* present = dense ? J->present.as<uint32_t>() : nullptr;
+    const uint32_t* present = dense ? J->table->view.present_bits : nullptr;
     JaShape sh;
     // PRISM_JA_PART=0 keeps every cache miss on device atomics (the
     // unpartitioned pass); PRISM_JA_CHUNK sets the probe rows per partitioned

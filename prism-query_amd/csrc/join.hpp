@@ -38,7 +38,11 @@ struct TableView {
   const int32_t* row_payload;  // duplicates with payload: payload of rows[i]
   int has_payload;          // entries carry an int32 build payload
   int sentinel_present;     // HASHED: a build key equals INT64_MIN
-  int presence_only;        // DENSE: dense_u holds the key's build row count (0 = absent), no rows
+  // DENSE key presence only (the join aggregate): bit d of present_bits = a
+  // build row has key min_key + d; dense_u holds the rows of each key among
+  // a sample of the build rows (an estimate, to pick hot keys); no rows
+  int presence_only;
+  const uint32_t* present_bits;
 };
 
 }  // namespace join
@@ -51,6 +55,7 @@ struct prism_join_table {
   prism::DeviceBuffer dense;  // DENSE
   prism::DeviceBuffer rows;   // CSR rows
   prism::DeviceBuffer payload;  // DENSE unique {row, payload} entries / CSR row payloads
+  prism::DeviceBuffer bits;     // DENSE presence-only: key presence bitmap
   prism::join::TableView view{};
 };
 
