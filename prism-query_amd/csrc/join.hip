@@ -1719,9 +1719,15 @@ constexpr uint32_t kJaBlk = 256;          // level-1 records a K1 workgroup rese
 constexpr uint32_t kJaHole = 0xFFFFFFFFu; // slot of an unused reserved record
 constexpr int kJaPartBlock = 1024;
 constexpr int kJaWavesPB = kJaPartBlock / 64;
-constexpr uint32_t kJaPartRecs = 16384;   // level-1 records per workgroup in the repartition passes
-constexpr int kJaStage2 = 4096;           // repartition LDS staging records (all ranges together)
+constexpr uint32_t kJaPartRecs = 4096;    // level-1 records per block of the repartition
+constexpr int kJaPartPerThread = kJaPartRecs / kJaPartBlock;
 
+// Level 2 without a counting pass: a repartition block (kJaPartRecs level-1
+// records of one partition) orders its records by range in LDS, takes room
+// for all of them in its partition's l2 region with one device atomic, writes
+// them there contiguously, and appends one run {offset, records} per range it
+// holds to that range's run list.  A range's runs come from distinct blocks
+// of its partition, so a list holds at most nb runs.
 struct JaPart {
   unsigned long long* l1_sr;  // level-1 regions, partition p at p * cap1: slot | row << 32 (holes: slot kJaHole)
   long long* l1_pay;          //   and the row's payload
@@ -1730,11 +1736,14 @@ struct JaPart {
   int shift1;                 // slot >> shift1 = level-1 partition
   int fan;                    // aggregation ranges per level-1 partition (power of two)
   int n_p1;                   // level-1 partitions
-  uint32_t nb;                // repartition workgroups per level-1 partition
-  uint32_t* cnt2;             // per range: records
-  unsigned long long* off2;   // per range: first record in l2
-  unsigned long long* blk2;   // per (partition, block, range): records
-  uint4* l2;                  // {slot & (kJaSub - 1), row, payload lo, payload hi}, grouped by range
+  uint32_t nb;                // repartition blocks per level-1 partition, at most
+  unsigned long long* off2;   // per partition: its l2 region (records K1 reserved there, scanned)
+  unsigned long long* fill2;  // per partition: l2 records taken
+  uint32_t* blk0;             // per partition: its first repartition block; [n_p1] = all blocks
+  uint32_t* nruns;            // per range: runs
+  unsigned long long* run_off;  // per (range, run): first record in l2
+  uint32_t* run_cnt;            //   and its records
+  uint4* l2;                  // {slot & (kJaSub - 1), row, payload lo, payload hi}, in runs of one range
 };
 
 // Barrier for LDS traffic only: global stores and atomics in flight are not
@@ -2076,170 +2085,166 @@ __global__ __launch_bounds__(kJaHotSortThreads) void k_ja_hot_sort(unsigned long
   for (uint32_t i = threadIdx.x; i < n; i += kJaHotSortThreads) list[i] = v[i];
 }
 
-// Level-1 records of repartition workgroup b: partition b / nb, records [lo, hi).
-__device__ __forceinline__ bool ja_part_block(const JaPart& P, uint32_t& p, uint32_t& blk, uint64_t& lo, uint64_t& hi) {
-  p = blockIdx.x / P.nb;
-  blk = blockIdx.x % P.nb;
-  const uint64_t f = P.fill1[p];
-  const uint64_t fill = f < P.cap1 ? f : P.cap1;
-  lo = static_cast<uint64_t>(blk) * kJaPartRecs;
-  if (lo >= fill) return false;
-  hi = lo + kJaPartRecs < fill ? lo + kJaPartRecs : fill;
-  return true;
-}
-
-// K1b (count): records per (partition, block, range); holes are skipped.
-__global__ __launch_bounds__(kJaPartBlock) void k_ja_count2(JaPart P) {
-  __shared__ uint32_t hist[kJaFanMax];
-  uint32_t p, blk;
-  uint64_t lo, hi;
-  const bool any = ja_part_block(P, p, blk, lo, hi);
-  for (int i = threadIdx.x; i < P.fan; i += blockDim.x) hist[i] = 0;
-  __syncthreads();
-  if (any) {
-    const unsigned long long* sr = P.l1_sr + static_cast<uint64_t>(p) * P.cap1;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      const uint32_t s = static_cast<uint32_t>(sr[i]);
-      if (s != kJaHole) atomicAdd(&hist[(s >> kJaSubShift) & static_cast<uint32_t>(P.fan - 1)], 1u);
+// Per level-1 partition (one workgroup): its l2 region and first repartition
+// block (exclusive scans of the records K1 reserved, holes included, and of
+// their blocks), the l2 fill and the range run counts cleared.
+__global__ __launch_bounds__(kJaPartBlock) void k_ja_regions(JaPart P) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t n_ranges = static_cast<uint32_t>(P.n_p1) * static_cast<uint32_t>(P.fan);
+  for (uint32_t r = threadIdx.x; r < n_ranges; r += kJaPartBlock) P.nruns[r] = 0;
+  if (threadIdx.x >= 64) return;
+  static_assert(kJaP1Max <= 64, "one wave scans the partitions");
+  const uint64_t f0 = lane < P.n_p1 ? P.fill1[lane] : 0;
+  const uint64_t f = f0 < P.cap1 ? f0 : P.cap1;
+  const uint64_t nbk = ceil_div(f, kJaPartRecs);
+  uint64_t inc = f, incb = nbk;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t o = __shfl_up(inc, d, 64), ob = __shfl_up(incb, d, 64);
+    if (lane >= d) {
+      inc += o;
+      incb += ob;
     }
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < P.fan; i += blockDim.x)
-    P.blk2[static_cast<uint64_t>(blockIdx.x) * P.fan + i] = hist[i];
-}
-
-// Per level-1 partition (one workgroup): the blocks' record counts of each
-// range become exclusive offsets within the range (in place), and the range
-// totals go to cnt2.  Thread (chunk, range) sums a run of blocks, the runs are
-// scanned in LDS, then each thread rewrites its run as offsets.
-__global__ __launch_bounds__(kJaPartBlock) void k_ja_blk_scan(JaPart P) {
-  __shared__ unsigned long long part[kJaPartBlock];
-  const uint32_t fan = static_cast<uint32_t>(P.fan);  // a power of two <= kJaFanMax, so it divides the block
-  const uint32_t chunks = kJaPartBlock / fan;
-  const uint32_t g = threadIdx.x % fan, ch = threadIdx.x / fan;
-  const uint32_t per = (P.nb + chunks - 1) / chunks;
-  const uint32_t k0 = ch * per < P.nb ? ch * per : P.nb;
-  const uint32_t k1 = k0 + per < P.nb ? k0 + per : P.nb;
-  unsigned long long* b = P.blk2 + static_cast<uint64_t>(blockIdx.x) * P.nb * fan;
-  unsigned long long s = 0;
-  for (uint32_t k = k0; k < k1; ++k) s += b[static_cast<uint64_t>(k) * fan + g];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  unsigned long long run = 0;
-  for (uint32_t c = 0; c < ch; ++c) run += part[c * fan + g];
-  if (ch == chunks - 1) P.cnt2[static_cast<uint64_t>(blockIdx.x) * fan + g] = static_cast<uint32_t>(run + s);
-  for (uint32_t k = k0; k < k1; ++k) {
-    const unsigned long long v = b[static_cast<uint64_t>(k) * fan + g];
-    b[static_cast<uint64_t>(k) * fan + g] = run;
-    run += v;
+  if (lane < P.n_p1) {
+    P.off2[lane] = inc - f;
+    P.fill2[lane] = 0;
+    P.blk0[lane] = static_cast<uint32_t>(incb - nbk);
   }
+  if (lane == 63) P.blk0[P.n_p1] = static_cast<uint32_t>(incb);
 }
 
-// Exclusive scan of the range totals: each range's first place in l2 (one
-// workgroup; a few ranges per thread).
-__global__ __launch_bounds__(kJaPartBlock) void k_ja_scan2(JaPart P, uint32_t n_ranges) {
-  __shared__ unsigned long long part[kJaPartBlock];
-  const uint32_t per = (n_ranges + kJaPartBlock - 1) / kJaPartBlock;
-  const uint32_t b = threadIdx.x * per;
-  unsigned long long s = 0;
-  for (uint32_t i = 0; i < per; ++i)
-    if (b + i < n_ranges) s += P.cnt2[b + i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int d = 1; d < kJaPartBlock; d <<= 1) {
-    const unsigned long long v = threadIdx.x >= static_cast<uint32_t>(d) ? part[threadIdx.x - d] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  unsigned long long run = part[threadIdx.x] - s;
-  for (uint32_t i = 0; i < per; ++i)
-    if (b + i < n_ranges) {
-      P.off2[b + i] = run;
-      run += P.cnt2[b + i];
-    }
-}
-
-// K1b (scatter): level-1 records to their range's place in l2, staged in LDS;
-// each block writes after the records of the blocks before it (no device
-// atomics).
+// K1b: repartition blocks (persistent workgroups, blocks taken by stride).
+// Holes are skipped; a block's records are ranked by range with LDS counters,
+// placed range by range in LDS and written out in one coalesced run.
 __global__ __launch_bounds__(kJaPartBlock) void k_ja_scatter2(JaPart P) {
-  __shared__ uint4 st[kJaStage2];
-  __shared__ uint32_t st_n[kJaFanMax];
-  __shared__ unsigned long long cur[kJaFanMax];
-  uint32_t p, blk;
-  uint64_t lo, hi;
-  if (!ja_part_block(P, p, blk, lo, hi)) return;
-  const uint32_t fan = static_cast<uint32_t>(P.fan);
-  const uint32_t cap = kJaStage2 / fan;  // staging records per range
-  const uint32_t at = cap / 2;
+  __shared__ uint4 st[kJaPartRecs];
+  __shared__ uint32_t s_cnt[kJaFanMax], s_start[kJaFanMax], s_blk0[kJaP1Max + 1];
+  __shared__ unsigned long long s_base;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t i = threadIdx.x; i < fan; i += blockDim.x) {
-    // this block's first place in range i: after the earlier blocks' records
-    st_n[i] = 0;
-    cur[i] = P.off2[static_cast<uint64_t>(p) * fan + i] + P.blk2[(static_cast<uint64_t>(p) * P.nb + blk) * fan + i];
-  }
+  const uint32_t fan = static_cast<uint32_t>(P.fan);
+  for (int i = threadIdx.x; i <= P.n_p1; i += kJaPartBlock) s_blk0[i] = P.blk0[i];
   __syncthreads();
-  const uint64_t r0 = static_cast<uint64_t>(p) * P.cap1;
-  for (uint64_t base = lo;; base += kJaPartBlock) {
-    const uint64_t i = base + threadIdx.x;
-    if (i < hi) {
-      const unsigned long long sr = __builtin_nontemporal_load(P.l1_sr + r0 + i);  // last read of the record
-      const uint32_t s = static_cast<uint32_t>(sr);
-      if (s != kJaHole) {
-        const uint64_t x = static_cast<uint64_t>(__builtin_nontemporal_load(P.l1_pay + r0 + i));
-        const uint4 rec = make_uint4(s & (kJaSub - 1), static_cast<uint32_t>(sr >> 32), static_cast<uint32_t>(x),
-                                     static_cast<uint32_t>(x >> 32));
-        const uint32_t g = (s >> kJaSubShift) & (fan - 1);
-        const uint32_t k = atomicAdd(&st_n[g], 1u);
-        if (k < cap)
-          st[g * cap + k] = rec;
-        else
-          P.l2[atomicAdd(&cur[g], 1ULL)] = rec;
+  const uint32_t n_blocks = s_blk0[P.n_p1];
+  for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) {
+    uint32_t p = 0;  // the partition holding block b (uniform)
+    while (p + 1 < static_cast<uint32_t>(P.n_p1) && s_blk0[p + 1] <= b) ++p;
+    const uint64_t f0 = P.fill1[p];
+    const uint64_t fill = f0 < P.cap1 ? f0 : P.cap1;
+    const uint64_t lo = static_cast<uint64_t>(b - s_blk0[p]) * kJaPartRecs;
+    const uint64_t hi = lo + kJaPartRecs < fill ? lo + kJaPartRecs : fill;
+    if (threadIdx.x < fan) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t r0 = static_cast<uint64_t>(p) * P.cap1;
+    uint4 rec[kJaPartPerThread];
+    uint32_t grp[kJaPartPerThread], pos[kJaPartPerThread];
+#pragma unroll
+    for (int j = 0; j < kJaPartPerThread; ++j) {
+      const uint64_t i = lo + static_cast<uint64_t>(j) * kJaPartBlock + threadIdx.x;
+      grp[j] = 0xFFFFFFFFu;
+      if (i < hi) {
+        const unsigned long long sr = __builtin_nontemporal_load(P.l1_sr + r0 + i);  // last read of the record
+        const uint32_t sl = static_cast<uint32_t>(sr);
+        if (sl != kJaHole) {
+          const uint64_t x = static_cast<uint64_t>(__builtin_nontemporal_load(P.l1_pay + r0 + i));
+          rec[j] = make_uint4(sl & (kJaSub - 1), static_cast<uint32_t>(sr >> 32), static_cast<uint32_t>(x),
+                              static_cast<uint32_t>(x >> 32));
+          grp[j] = (sl >> kJaSubShift) & (fan - 1);
+          pos[j] = atomicAdd(&s_cnt[grp[j]], 1u);
+        }
       }
     }
-    const bool last = base + kJaPartBlock >= hi;
-    lds_barrier();
-    for (uint32_t g = wave; g < fan; g += kJaWavesPB) {
-      const uint32_t n0 = st_n[g];
-      const uint32_t m = n0 < cap ? n0 : cap;
-      if (m == 0 || (!last && m < at)) continue;
-      const unsigned long long pos = cur[g];
-      for (uint32_t j = lane; j < m; j += 64) P.l2[pos + j] = st[g * cap + j];
-      if (lane == 0) {
-        st_n[g] = 0;
-        cur[g] = pos + m;
+    __syncthreads();
+    if (wave == 0) {
+      const uint32_t c = static_cast<uint32_t>(lane) < fan ? s_cnt[lane] : 0u;
+      uint32_t inc = c;
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+      }
+      const uint32_t total = __shfl(inc, 63, 64);
+      unsigned long long base = 0;
+      if (lane == 0 && total) base = P.off2[p] + atomicAdd(&P.fill2[p], static_cast<unsigned long long>(total));
+      base = __shfl(base, 0, 64);
+      if (static_cast<uint32_t>(lane) < fan) s_start[lane] = inc - c;
+      if (lane == 0) s_base = base;
+      if (c) {
+        const uint32_t g = p * fan + static_cast<uint32_t>(lane);
+        const uint32_t k = atomicAdd(&P.nruns[g], 1u);
+        P.run_off[static_cast<uint64_t>(g) * P.nb + k] = base + (inc - c);
+        P.run_cnt[static_cast<uint64_t>(g) * P.nb + k] = c;
       }
     }
-    lds_barrier();
-    if (last) break;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kJaPartPerThread; ++j)
+      if (grp[j] != 0xFFFFFFFFu) st[s_start[grp[j]] + pos[j]] = rec[j];
+    __syncthreads();
+    const uint32_t total = s_start[fan - 1] + s_cnt[fan - 1];
+    const unsigned long long base = s_base;
+    for (uint32_t i = threadIdx.x; i < total; i += kJaPartBlock) P.l2[base + i] = st[i];
+    __syncthreads();  // st, s_cnt and s_start are rewritten by the next block
   }
 }
 
 // K2: one workgroup per range sums its records in LDS and adds the totals to
-// the slot arrays.
+// the slot arrays.  The run list is read into LDS in batches; a wave takes
+// kJaRunsAtOnce runs at a time, their first 64 records loaded together.
+constexpr uint32_t kJaRunBatch = 2048;
+constexpr int kJaRunsAtOnce = 4;
+__device__ __forceinline__ void ja_agg_rec(const v4u32& r, uint32_t* a_cnt, uint32_t* a_first, unsigned long long* a_lo,
+                                           long long* a_hi) {
+  atomicAdd(&a_cnt[r.x], 1u);
+  if (r.z) atomicAdd(&a_lo[r.x], static_cast<unsigned long long>(r.z));
+  if (r.w) atomicAdd(reinterpret_cast<unsigned long long*>(&a_hi[r.x]),
+                     static_cast<unsigned long long>(static_cast<long long>(static_cast<int32_t>(r.w))));
+  atomicMin(&a_first[r.x], r.y);  // no return: nothing waits on it
+}
+
 __global__ __launch_bounds__(kJaPartBlock) void k_ja_agg2(JaPart P, JaAcc A) {
   __shared__ uint32_t a_cnt[kJaSub], a_first[kJaSub];
   __shared__ unsigned long long a_lo[kJaSub];
   __shared__ long long a_hi[kJaSub];
+  __shared__ unsigned long long s_off[kJaRunBatch];
+  __shared__ uint32_t s_m[kJaRunBatch];
   const uint32_t g = blockIdx.x;
-  const uint32_t m = P.cnt2[g];
-  if (m == 0) return;
+  const uint32_t nr = P.nruns[g];
+  if (nr == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < kJaSub; i += blockDim.x) {
     a_cnt[i] = 0;
     a_first[i] = 0xFFFFFFFFu;
     a_lo[i] = 0;
     a_hi[i] = 0;
   }
-  __syncthreads();
-  const v4u32* rec = reinterpret_cast<const v4u32*>(P.l2 + P.off2[g]);
-  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-    const v4u32 r = __builtin_nontemporal_load(rec + i);  // the record's only read
-    atomicAdd(&a_cnt[r.x], 1u);
-    if (r.z) atomicAdd(&a_lo[r.x], static_cast<unsigned long long>(r.z));
-    if (r.w) atomicAdd(reinterpret_cast<unsigned long long*>(&a_hi[r.x]),
-                       static_cast<unsigned long long>(static_cast<long long>(static_cast<int32_t>(r.w))));
-    atomicMin(&a_first[r.x], r.y);  // no return: nothing waits on it
+  const v4u32* l2 = reinterpret_cast<const v4u32*>(P.l2);
+  for (uint32_t b0 = 0; b0 < nr; b0 += kJaRunBatch) {
+    const uint32_t nb = nr - b0 < kJaRunBatch ? nr - b0 : kJaRunBatch;
+    __syncthreads();  // the previous batch is done with s_off / s_m
+    for (uint32_t k = threadIdx.x; k < nb; k += kJaPartBlock) {
+      s_off[k] = P.run_off[static_cast<uint64_t>(g) * P.nb + b0 + k];
+      s_m[k] = P.run_cnt[static_cast<uint64_t>(g) * P.nb + b0 + k];
+    }
+    __syncthreads();
+    for (uint32_t k = static_cast<uint32_t>(wave) * kJaRunsAtOnce; k < nb; k += kJaWavesPB * kJaRunsAtOnce) {
+      uint32_t m[kJaRunsAtOnce];
+      unsigned long long off[kJaRunsAtOnce];
+      v4u32 r[kJaRunsAtOnce];
+#pragma unroll
+      for (int u = 0; u < kJaRunsAtOnce; ++u) {
+        m[u] = k + u < nb ? s_m[k + u] : 0u;
+        off[u] = k + u < nb ? s_off[k + u] : 0ULL;
+      }
+#pragma unroll
+      for (int u = 0; u < kJaRunsAtOnce; ++u)
+        if (static_cast<uint32_t>(lane) < m[u]) r[u] = __builtin_nontemporal_load(l2 + off[u] + lane);  // only read
+#pragma unroll
+      for (int u = 0; u < kJaRunsAtOnce; ++u)
+        if (static_cast<uint32_t>(lane) < m[u]) ja_agg_rec(r[u], a_cnt, a_first, a_lo, a_hi);
+#pragma unroll
+      for (int u = 0; u < kJaRunsAtOnce; ++u)
+        for (uint32_t i = 64 + static_cast<uint32_t>(lane); i < m[u]; i += 64)
+          ja_agg_rec(__builtin_nontemporal_load(l2 + off[u] + i), a_cnt, a_first, a_lo, a_hi);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kJaSub; i += blockDim.x) {
@@ -3765,7 +3770,7 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     JaShape sh;
     // PRISM_JA_PART=0 keeps every cache miss on device atomics (the
     // unpartitioned pass); PRISM_JA_CHUNK sets the probe rows per partitioned
-    // pass (scratch: 48 B per row of a pass).
+    // pass (scratch: about 52 B per row of a pass).
     const bool part = env_u64("PRISM_JA_PART", 1) != 0 && ja_shape(J->n_slots, sh);
     KernelTimer kt(c, "join_agg_probe");
     if (!part) {
@@ -3783,17 +3788,18 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
       PRISM_CHECK_LAUNCH();
       return;
     }
-    // Larger passes amortise each pass's fixed work (range totals, scans, the
-    // staging tails): for 256M probe rows, one pass of 256M rows (12 GB of
+    // Larger passes amortise each pass's fixed work (the staging tails, the
+    // region scan): for 256M probe rows, one pass of 256M rows (~13 GB of
     // scratch) measured 3.36 ms, passes of 128M 3.46 ms, of 64M 3.60 ms, of
-    // 32M 4.43 ms.  A device with less free memory gets passes whose scratch
-    // fits a quarter of it.
+    // 32M 4.43 ms (with the counted repartition before the run lists).  A
+    // device with less free memory gets passes whose scratch fits a quarter
+    // of it.
     uint64_t chunk = env_u64("PRISM_JA_CHUNK", 0);
     if (chunk == 0) {
       chunk = 256ULL << 20;
       size_t free_b = 0, total_b = 0;
       if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-        const uint64_t fit = static_cast<uint64_t>(free_b) / 4 / 48;
+        const uint64_t fit = static_cast<uint64_t>(free_b) / 4 / 52;
         if (fit < chunk) chunk = fit > (1ULL << 20) ? fit : (1ULL << 20);
       }
     }
@@ -3806,13 +3812,17 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     uint64_t cap1 = 2 * chunk / sh.n_p1 + 64;
     if (cap1 > chunk) cap1 = chunk;
     cap1 += g1max * kJaBlk;
-    if (chunk > J->part_chunk || J->part_l1.bytes < sh.n_p1 * cap1 * 16) {
+    // l2 holds what K1 reserved: the pass's rows plus the unused tails of the
+    // current and next blocks of every K1 workgroup and partition
+    const uint64_t l2_recs = chunk + g1max * sh.n_p1 * 2 * kJaBlk;
+    if (chunk > J->part_chunk || J->part_l1.bytes < sh.n_p1 * cap1 * 16 || J->part_l2.bytes < l2_recs * 16) {
       J->part_l1.reserve(c, sh.n_p1 * cap1 * 16);
-      J->part_l2.reserve(c, chunk * 16);
+      J->part_l2.reserve(c, l2_recs * 16);
       J->part_chunk = chunk;
     }
     const uint32_t nb = static_cast<uint32_t>(ceil_div(cap1, kJaPartRecs));
-    J->part_ctl.reserve(c, sh.n_p1 * 8 + n_ranges * (4 + 8) + static_cast<uint64_t>(sh.n_p1) * nb * sh.fan * 8 + 64);
+    const uint64_t n_runs = n_ranges * nb;
+    J->part_ctl.reserve(c, sh.n_p1 * 24 + (sh.n_p1 + 1) * 4 + n_ranges * 4 + n_runs * 12 + 256);
     JaPart P;
     char* l1 = J->part_l1.as<char>();
     P.l1_sr = reinterpret_cast<unsigned long long*>(l1);
@@ -3824,8 +3834,11 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     char* ctl = J->part_ctl.as<char>();
     P.fill1 = reinterpret_cast<unsigned long long*>(ctl);
     P.off2 = P.fill1 + sh.n_p1;
-    P.blk2 = P.off2 + n_ranges;
-    P.cnt2 = reinterpret_cast<uint32_t*>(P.blk2 + static_cast<uint64_t>(sh.n_p1) * nb * sh.fan);
+    P.fill2 = P.off2 + sh.n_p1;
+    P.run_off = P.fill2 + sh.n_p1;
+    P.run_cnt = reinterpret_cast<uint32_t*>(P.run_off + n_runs);
+    P.nruns = P.run_cnt + n_runs;
+    P.blk0 = P.nruns + n_ranges;
     P.nb = nb;
     P.l2 = J->part_l2.as<uint4>();
     const JaAcc A = J->accs();
@@ -3846,11 +3859,8 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
       else
         hipLaunchKernelGGL(k_ja_probe_part<PRISM_TABLE_HASHED>, dim3(static_cast<unsigned>(g1)), dim3(kJaBlock1), 0,
                            st, T, unique, kp, pp, m, probe_row_base + at, A, present, P, J->hot.as<uint32_t>(), cpart);
-      const dim3 gp(static_cast<unsigned>(sh.n_p1) * nb);
-      hipLaunchKernelGGL(k_ja_count2, gp, dim3(kJaPartBlock), 0, st, P);
-      hipLaunchKernelGGL(k_ja_blk_scan, dim3(static_cast<unsigned>(sh.n_p1)), dim3(kJaPartBlock), 0, st, P);
-      hipLaunchKernelGGL(k_ja_scan2, dim3(1), dim3(kJaPartBlock), 0, st, P, static_cast<uint32_t>(n_ranges));
-      hipLaunchKernelGGL(k_ja_scatter2, gp, dim3(kJaPartBlock), 0, st, P);
+      hipLaunchKernelGGL(k_ja_regions, dim3(1), dim3(kJaPartBlock), 0, st, P);
+      hipLaunchKernelGGL(k_ja_scatter2, dim3(static_cast<unsigned>(c->num_cus) * 2), dim3(kJaPartBlock), 0, st, P);
       hipLaunchKernelGGL(k_ja_agg2, dim3(static_cast<unsigned>(n_ranges)), dim3(kJaPartBlock), 0, st, P, A);
       PRISM_CHECK_LAUNCH();
     }
