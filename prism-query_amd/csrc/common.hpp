@@ -146,6 +146,11 @@ void* pinned_staging(prism_ctx* ctx, size_t n);
 
 // Pinned 4 KiB block for small synchronous device -> host reads (statistics,
 // counters); callers hold ctx->mu.
+// Pinned block per context for small readbacks and uploads, used by one call
+// at a time (under the context lock, drained before the call returns):
+// [0, 4 KiB) statistics, counters and column descriptors; from 4 KiB the join
+// aggregate's create (attribute range, hot-slot candidates and placement).
+constexpr size_t kReadbackBytes = 64 << 10;
 void* readback_block(prism_ctx* ctx);
 
 // Return every cached block to the device (hipFree).

@@ -185,7 +185,7 @@ void* pinned_staging(prism_ctx* ctx, size_t n) {
 }
 
 void* readback_block(prism_ctx* ctx) {
-  if (!ctx->readback) PRISM_HIP(hipHostMalloc(&ctx->readback, 4096, hipHostMallocDefault));
+  if (!ctx->readback) PRISM_HIP(hipHostMalloc(&ctx->readback, kReadbackBytes, hipHostMallocDefault));
   return ctx->readback;
 }
 

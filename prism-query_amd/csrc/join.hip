@@ -3549,46 +3549,79 @@ static bool t_kind_dense(const prism_join_table* t) { return t->info.kind == PRI
 
 // The slots with the most build rows go into the LDS cache's buckets (first
 // free way of either bucket, in rank order).  hot buffer: the kJaCache cache
-// keys (u32, kJaEmpty = free way).
-static void ja_pick_hot(prism_join_agg* J, hipStream_t st) {
-  Ctx* c = J->ctx;
+// keys (u32, kJaEmpty = free way).  Three steps, so that the create's other
+// device work runs while the host waits and places: a histogram of the slots'
+// row counts (read back with the create's first wait), the candidates at or
+// above the count bin that keeps at most kJaHotListMax of them (listed and
+// ordered on the device, read back behind an event), and their placement.
+constexpr size_t kRbJaAttr = 4096, kRbJaHist = 5120, kRbJaHot = 8192, kRbJaList = 16384;
+static_assert(kRbJaHist + kJaHotBins * 4 <= kRbJaHot && kRbJaHot + kJaCache * 4 <= kRbJaList &&
+                  kRbJaList + kJaHotListMax * 8 <= kReadbackBytes,
+              "join aggregate readbacks fit the readback block");
+
+struct JaHotSel {
+  DeviceBuffer scratch;
+  uint64_t taken = 0;
+  hipEvent_t listed = nullptr;
+  ~JaHotSel() {
+    if (listed) (void)hipEventDestroy(listed);
+  }
+};
+
+static void ja_hot_count(prism_join_agg* J, hipStream_t st, JaHotSel& H) {
   prism_join_table* t = J->table;
-  const bool dense = t->info.kind == PRISM_TABLE_DENSE;
-  const bool unique = t->info.unique;
   const uint64_t n_slots = t->info.slots;  // HASHED: the INT64_MIN sentinel slot is never hot
-  DeviceBuffer scratch(c, kJaHotBins * 4 + 16 + kJaHotListMax * 8);
-  unsigned int* hist = scratch.as<unsigned int>();
-  unsigned int* n_list = hist + kJaHotBins;
-  unsigned long long* list = reinterpret_cast<unsigned long long*>(scratch.as<char>() + kJaHotBins * 4 + 16);
+  H.scratch.alloc(J->ctx, kJaHotBins * 4 + 16 + kJaHotListMax * 8);
+  unsigned int* hist = H.scratch.as<unsigned int>();
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, hist, static_cast<uint64_t>(kJaHotBins + 4), 0u);
   const dim3 g(n_slots ? static_cast<unsigned>(std::min<uint64_t>(ceil_div(n_slots, 256), 1024)) : 1);
-  if (dense)
-    hipLaunchKernelGGL(k_ja_count_hist<PRISM_TABLE_DENSE>, g, dim3(256), 0, st, t->view, unique, n_slots, hist);
+  if (t->info.kind == PRISM_TABLE_DENSE)
+    hipLaunchKernelGGL(k_ja_count_hist<PRISM_TABLE_DENSE>, g, dim3(256), 0, st, t->view, t->info.unique, n_slots, hist);
   else
-    hipLaunchKernelGGL(k_ja_count_hist<PRISM_TABLE_HASHED>, g, dim3(256), 0, st, t->view, unique, n_slots, hist);
+    hipLaunchKernelGGL(k_ja_count_hist<PRISM_TABLE_HASHED>, g, dim3(256), 0, st, t->view, t->info.unique, n_slots, hist);
   PRISM_CHECK_LAUNCH();
-  std::vector<unsigned int> h(kJaHotBins);
-  PRISM_HIP(hipMemcpyAsync(h.data(), hist, kJaHotBins * 4, hipMemcpyDeviceToHost, st));
-  PRISM_HIP(hipStreamSynchronize(st));
+  char* rb = static_cast<char*>(readback_block(J->ctx));
+  PRISM_HIP(hipMemcpyAsync(rb + kRbJaHist, hist, kJaHotBins * 4, hipMemcpyDeviceToHost, st));
+}
+
+// After the wait that covers ja_hot_count's readback.
+static void ja_hot_list(prism_join_agg* J, hipStream_t st, JaHotSel& H) {
+  prism_join_table* t = J->table;
+  const uint64_t n_slots = t->info.slots;
+  char* rb = static_cast<char*>(readback_block(J->ctx));
+  const unsigned int* h = reinterpret_cast<const unsigned int*>(rb + kRbJaHist);
   int min_bin = kJaHotBins;
   uint64_t taken = 0;
   while (min_bin > 0 && taken + h[min_bin - 1] <= kJaHotListMax) taken += h[--min_bin];
-  std::vector<uint32_t> hot(kJaCache, kJaEmpty);
-  if (taken) {
-    if (dense)
-      hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_DENSE>, g, dim3(256), 0, st, t->view, unique, n_slots, min_bin, list,
-                         n_list);
-    else
-      hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_HASHED>, g, dim3(256), 0, st, t->view, unique, n_slots, min_bin,
-                         list, n_list);
-    // most rows first, then the lower slot (the slot stays in the low word)
-    hipLaunchKernelGGL(k_ja_hot_sort, dim3(1), dim3(kJaHotSortThreads), 0, st, list, static_cast<uint32_t>(taken));
-    PRISM_CHECK_LAUNCH();
-    std::vector<unsigned long long> l(taken);
-    PRISM_HIP(hipMemcpyAsync(l.data(), list, taken * 8, hipMemcpyDeviceToHost, st));
-    PRISM_HIP(hipStreamSynchronize(st));
-    uint32_t* keys = hot.data();
-    for (size_t i = 0; i < l.size(); ++i) {
+  H.taken = taken;
+  if (!taken) return;
+  unsigned int* n_list = H.scratch.as<unsigned int>() + kJaHotBins;
+  unsigned long long* list = reinterpret_cast<unsigned long long*>(H.scratch.as<char>() + kJaHotBins * 4 + 16);
+  const dim3 g(n_slots ? static_cast<unsigned>(std::min<uint64_t>(ceil_div(n_slots, 256), 1024)) : 1);
+  if (t->info.kind == PRISM_TABLE_DENSE)
+    hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_DENSE>, g, dim3(256), 0, st, t->view, t->info.unique, n_slots, min_bin,
+                       list, n_list);
+  else
+    hipLaunchKernelGGL(k_ja_hot_list<PRISM_TABLE_HASHED>, g, dim3(256), 0, st, t->view, t->info.unique, n_slots, min_bin,
+                       list, n_list);
+  // most rows first, then the lower slot (the slot stays in the low word)
+  hipLaunchKernelGGL(k_ja_hot_sort, dim3(1), dim3(kJaHotSortThreads), 0, st, list, static_cast<uint32_t>(taken));
+  PRISM_CHECK_LAUNCH();
+  PRISM_HIP(hipMemcpyAsync(rb + kRbJaList, list, taken * 8, hipMemcpyDeviceToHost, st));
+  PRISM_HIP(hipEventCreateWithFlags(&H.listed, hipEventDisableTiming));
+  PRISM_HIP(hipEventRecord(H.listed, st));
+}
+
+// Waits for the ordered candidates only (work queued after them keeps
+// running), places them and queues the upload of the cache keys.
+static void ja_hot_place(prism_join_agg* J, hipStream_t st, JaHotSel& H) {
+  char* rb = static_cast<char*>(readback_block(J->ctx));
+  uint32_t* keys = reinterpret_cast<uint32_t*>(rb + kRbJaHot);
+  for (int i = 0; i < kJaCache; ++i) keys[i] = kJaEmpty;
+  if (H.taken) {
+    PRISM_HIP(hipEventSynchronize(H.listed));
+    const unsigned long long* l = reinterpret_cast<const unsigned long long*>(rb + kRbJaList);
+    for (uint64_t i = 0; i < H.taken; ++i) {
       const uint32_t sl = static_cast<uint32_t>(l[i]);
       bool placed = false;
       for (int tb = 0; tb < 2 && !placed; ++tb) {
@@ -3601,9 +3634,8 @@ static void ja_pick_hot(prism_join_agg* J, hipStream_t st) {
       }
     }
   }
-  J->hot.alloc(c, hot.size() * 4);
-  PRISM_HIP(hipMemcpyAsync(J->hot.ptr, hot.data(), hot.size() * 4, hipMemcpyHostToDevice, st));
-  PRISM_HIP(hipStreamSynchronize(st));
+  J->hot.alloc(J->ctx, kJaCache * 4);
+  PRISM_HIP(hipMemcpyAsync(J->hot.ptr, keys, kJaCache * 4, hipMemcpyHostToDevice, st));
 }
 
 prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, const int32_t* d_build_attr,
@@ -3631,8 +3663,8 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
     J->counters.alloc(c, 64);
     hipStream_t st = ctx->stream;
     // attribute range (dense groups when it is small, else a hashed group
-    // table), read back with the hot-slot selection's first wait
-    int64_t* h_attr = reinterpret_cast<int64_t*>(static_cast<char*>(readback_block(ctx)) + 2048);
+    // table), read back with the hot-slot histogram
+    int64_t* h_attr = reinterpret_cast<int64_t*>(static_cast<char*>(readback_block(ctx)) + kRbJaAttr);
     DeviceBuffer apart;
     if (n_build) {
       apart.alloc(c, kRedBlocks * 16 + 16);
@@ -3642,13 +3674,15 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
       PRISM_CHECK_LAUNCH();
       PRISM_HIP(hipMemcpyAsync(h_attr, pa + 2 * kRedBlocks, 16, hipMemcpyDeviceToHost, st));
     }
-    ja_pick_hot(J.get(), st);
+    JaHotSel hot;
+    ja_hot_count(J.get(), st, hot);
+    PRISM_HIP(hipStreamSynchronize(st));
     int64_t lo = 0, hi = -1;
     if (n_build) {
-      PRISM_HIP(hipStreamSynchronize(st));  // idle by now: the selection waited on the stream
       lo = h_attr[0];
       hi = h_attr[1];
     }
+    ja_hot_list(J.get(), st, hot);
     // fold set-up buffers, released after the final wait below
     DeviceBuffer fold_hist, fold_tmp;
     const uint64_t range = hi >= lo ? static_cast<uint64_t>(hi - lo) + 1 : 1;
@@ -3721,6 +3755,7 @@ prism_status prism_join_agg_create(prism_ctx* ctx, const int64_t* d_build_keys, 
       J->groups.alloc(c, J->n_groups_cap * 5 * 8);
     }
     ja_clear(J.get());
+    ja_hot_place(J.get(), st, hot);
     PRISM_HIP(hipStreamSynchronize(st));
     *out = J.release();
   });
