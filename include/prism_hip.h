@@ -645,7 +645,10 @@ void prism_join_agg_destroy(prism_join_agg* j);
 /* build tables with HBM admission and LRU eviction, graph replay of         */
 /* recurring batch shapes, an online tile/occupancy tuner, p50/p99 latency.  */
 /* Request inputs are device-resident; they must stay valid until the        */
-/* request completes.                                                        */
+/* request completes.  The dispatcher thread polls for submissions for 5 ms  */
+/* after its last work before it sleeps (one busy host core while requests   */
+/* keep coming; PRISM_SVC_IDLE_SPIN_US sets the window in microseconds, 0    */
+/* sleeps at once).                                                          */
 /* ------------------------------------------------------------------------- */
 typedef struct prism_service prism_service;
 typedef struct {
