@@ -2604,8 +2604,11 @@ __global__ __launch_bounds__(256) void k_ja_collect_count(JaGroups G, unsigned i
   if (threadIdx.x == 0) block_count[blockIdx.x] = tot;
 }
 
+// first_out: the group's first pair as a sort key, probe row << bb | build row
+// (bb = 32: the pair as stored).
 __global__ __launch_bounds__(256) void k_ja_collect(JaGroups G, const unsigned int* __restrict__ block_count,
-                                                    unsigned long long* n_found, uint64_t* first_out, uint32_t* idx_out) {
+                                                    unsigned long long* n_found, uint64_t* first_out, uint32_t* idx_out,
+                                                    int bb) {
   __shared__ unsigned long long base;
   __shared__ unsigned int wave_n[4];
   if (threadIdx.x < 64) {  // this block's offset: the groups of the blocks before it
@@ -2629,7 +2632,8 @@ __global__ __launch_bounds__(256) void k_ja_collect(JaGroups G, const unsigned i
     for (int w = 0; w < wave; ++w) before += wave_n[w];
     if (found) {
       const uint64_t i = before + lanes_below(m);
-      first_out[i] = G.first[g];
+      const uint64_t f = G.first[g];
+      first_out[i] = ((f >> 32) << bb) | (f & 0xFFFFFFFFULL);
       idx_out[i] = static_cast<uint32_t>(g);
     }
     for (int w = 0; w < 4; ++w) at += wave_n[w];
@@ -2639,9 +2643,11 @@ __global__ __launch_bounds__(256) void k_ja_collect(JaGroups G, const unsigned i
 }
 
 // Output row i: attribute, count, sum lo, sum hi, first pair.
-__global__ void k_ja_gather(JaGroups G, const uint32_t* __restrict__ order, uint64_t n, int32_t* attr,
-                            unsigned long long* cnt, unsigned long long* sum_lo, long long* sum_hi,
-                            unsigned long long* first) {
+// The first n groups of `order` (n_dev: at most the count there).
+__global__ void k_ja_gather(JaGroups G, const uint32_t* __restrict__ order, uint64_t n,
+                            const unsigned long long* __restrict__ n_dev, int32_t* attr, unsigned long long* cnt,
+                            unsigned long long* sum_lo, long long* sum_hi, unsigned long long* first) {
+  if (n_dev && *n_dev < n) n = *n_dev;
   for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint32_t g = order[i];
@@ -2722,6 +2728,8 @@ struct prism_join_agg {
   bool dense_groups = true;
   prism::DeviceBuffer result;  // finished rows: attr(i32) cnt sum_lo sum_hi first
   uint64_t n_result = 0;
+  uint64_t result_rows = 0;    // rows the result's columns are laid out for (>= n_result)
+  uint64_t probe_end = 0;      // largest probe row + 1 since the last clear
   bool finished = false;
   // partitioned tail: level-1 regions (slot, row, payload arrays), level-2
   // records, per-partition and per-range counters
@@ -2770,6 +2778,7 @@ static void ja_clear(prism_join_agg* J) {
   PRISM_CHECK_LAUNCH();
   J->finished = false;
   J->n_result = 0;
+  J->probe_end = 0;
 }
 
 }  // namespace join
@@ -3796,6 +3805,7 @@ prism_status prism_join_agg_probe(prism_join_agg* J, const int64_t* d_probe_keys
     std::lock_guard<std::mutex> lock(J->ctx->mu);
     J->finished = false;
     if (n_probe == 0 || J->n_build == 0) return;
+    J->probe_end = std::max<uint64_t>(J->probe_end, probe_row_base + n_probe);
     Ctx* c = J->ctx;
     hipStream_t st = c->stream;
     const TableView& T = J->table->view;
@@ -3946,38 +3956,60 @@ prism_status prism_join_agg_finish(prism_join_agg* J, uint64_t* n_groups) {
     }
     DeviceBuffer first(c, G * 8), first_sorted(c, G * 8), idx(c, G * 4), idx_sorted(c, G * 4);
     unsigned long long* found_p = J->counters.as<unsigned long long>();
+    // Sort keys: the first pair as probe row << bb | build row, bb the bits of
+    // the largest build row, when both fit 63 bits (fewer radix passes).
+    auto bits_of = [](uint64_t v) { return v <= 1 ? 1 : 64 - __builtin_clzll(v - 1); };
+    const int bb0 = bits_of(J->n_build), pb = bits_of(J->probe_end);
+    const bool compact = bb0 < 32 && bb0 + pb <= 63;
+    const int bb = compact ? bb0 : 32, end_bit = compact ? bb0 + pb : 64;
+    // Dense groups of at most 4M: every group slot is sorted (the missing
+    // ones carry an all-ones key and sort last) and the gather reads the
+    // group count on the device, so the finish waits only once, at its end.
+    const bool one_wait = J->dense_groups && G <= (4ULL << 20);
+    if (one_wait) hipLaunchKernelGGL(k_fill_u64, grid_for(c, G), dim3(kBuildBlock), 0, st, first.as<unsigned long long>(), G, ~0ULL);
     DeviceBuffer bcount(c, kCollectBlocks * 4);
     hipLaunchKernelGGL(k_ja_collect_count, dim3(kCollectBlocks), dim3(256), 0, st, GV, bcount.as<unsigned int>());
     hipLaunchKernelGGL(k_ja_collect, dim3(kCollectBlocks), dim3(256), 0, st, GV, bcount.as<unsigned int>(), found_p,
-                       first.as<uint64_t>(), idx.as<uint32_t>());
+                       first.as<uint64_t>(), idx.as<uint32_t>(), bb);
     PRISM_CHECK_LAUNCH();
-    unsigned long long h[2];
-    PRISM_HIP(hipMemcpyAsync(h, J->counters.ptr, 16, hipMemcpyDeviceToHost, st));
-    PRISM_HIP(hipStreamSynchronize(st));
-    const uint64_t found = h[0];
-    if (static_cast<uint32_t>(h[1]) & 1u) fail(PRISM_ERR_INTERNAL, "Internal error: join aggregate group table full");
-    if (found > 1) {
+    unsigned long long* h = static_cast<unsigned long long*>(readback_block(c));
+    uint64_t found = 0, n_sort = G;
+    if (!one_wait) {
+      PRISM_HIP(hipMemcpyAsync(h, J->counters.ptr, 16, hipMemcpyDeviceToHost, st));
+      PRISM_HIP(hipStreamSynchronize(st));
+      found = h[0];
+      if (static_cast<uint32_t>(h[1]) & 1u) fail(PRISM_ERR_INTERNAL, "Internal error: join aggregate group table full");
+      n_sort = found;
+    }
+    if (n_sort > 1) {
       size_t tmp = 0;
       PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, first.as<uint64_t>(), first_sorted.as<uint64_t>(),
-                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(), static_cast<int>(found),
-                                                   0, 64, st));
+                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(), static_cast<int>(n_sort),
+                                                   0, end_bit, st));
       DeviceBuffer t(c, tmp);
       PRISM_HIP(hipcub::DeviceRadixSort::SortPairs(t.ptr, tmp, first.as<uint64_t>(), first_sorted.as<uint64_t>(),
-                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(), static_cast<int>(found),
-                                                   0, 64, st));
-    } else if (found == 1) {
+                                                   idx.as<uint32_t>(), idx_sorted.as<uint32_t>(), static_cast<int>(n_sort),
+                                                   0, end_bit, st));
+    } else if (n_sort == 1) {
       PRISM_HIP(hipMemcpyAsync(idx_sorted.ptr, idx.ptr, 4, hipMemcpyDeviceToDevice, st));
     }
-    const uint64_t nr = found ? found : 1;
+    const uint64_t nr = n_sort ? n_sort : 1;
     J->result.reserve(c, nr * (4 + 8 * 4) + 64);
     char* r = J->result.as<char>();
     int32_t* attr = reinterpret_cast<int32_t*>(r);
     unsigned long long* cnt = reinterpret_cast<unsigned long long*>(r + ((nr * 4 + 7) & ~7ULL));
-    if (found)
-      hipLaunchKernelGGL(k_ja_gather, grid_for(c, found), dim3(kBuildBlock), 0, st, GV, idx_sorted.as<uint32_t>(), found,
-                         attr, cnt, cnt + nr, reinterpret_cast<long long*>(cnt + 2 * nr), cnt + 3 * nr);
+    if (n_sort)
+      hipLaunchKernelGGL(k_ja_gather, grid_for(c, n_sort), dim3(kBuildBlock), 0, st, GV, idx_sorted.as<uint32_t>(),
+                         n_sort, one_wait ? found_p : nullptr, attr, cnt, cnt + nr,
+                         reinterpret_cast<long long*>(cnt + 2 * nr), cnt + 3 * nr);
     PRISM_CHECK_LAUNCH();
+    if (one_wait) PRISM_HIP(hipMemcpyAsync(h, J->counters.ptr, 16, hipMemcpyDeviceToHost, st));
     PRISM_HIP(hipStreamSynchronize(st));
+    if (one_wait) {
+      found = h[0];
+      if (static_cast<uint32_t>(h[1]) & 1u) fail(PRISM_ERR_INTERNAL, "Internal error: join aggregate group table full");
+    }
+    J->result_rows = nr;
     J->n_result = found;
     J->finished = true;
     *n_groups = found;
@@ -3992,7 +4024,7 @@ prism_status prism_join_agg_read(prism_join_agg* J, int32_t* attr, uint64_t* cou
     use_device(J->ctx);
     const uint64_t n = J->n_result;
     if (!n) return;
-    const uint64_t nr = n;
+    const uint64_t nr = J->result_rows;
     hipStream_t st = J->ctx->stream;
     const char* r = J->result.as<char>();
     const unsigned long long* cnt = reinterpret_cast<const unsigned long long*>(r + ((nr * 4 + 7) & ~7ULL));
