@@ -1589,7 +1589,8 @@ __host__ __device__ __forceinline__ uint32_t ja_bucket(uint32_t s, int t) {
 
 struct JaCache {
   uint32_t key[kJaCache];
-  unsigned long long cnt[kJaCache], lo[kJaCache], first[kJaCache];
+  uint32_t cnt[kJaCache];  // rows of one workgroup in one pass (a pass has fewer than 2^32 rows)
+  unsigned long long lo[kJaCache], first[kJaCache];
   long long hi[kJaCache];
 };
 
@@ -1640,7 +1641,7 @@ __device__ __forceinline__ int ja_cache_entry(JaCache& C, uint32_t s) {
 __device__ __forceinline__ bool ja_cache_add(JaCache& C, uint32_t s, int64_t x, unsigned long long row) {
   const int c = ja_cache_entry(C, s);
   if (c < 0) return false;
-  atomicAdd(&C.cnt[c], 1ULL);
+  atomicAdd(&C.cnt[c], 1u);
   const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(x));
   const long long hi = x >> 32;
   if (lo) atomicAdd(&C.lo[c], static_cast<unsigned long long>(lo));
@@ -1661,7 +1662,7 @@ __device__ __forceinline__ void ja_cache_flush(JaCache& C, const JaAcc& A) {
   for (int c = threadIdx.x; c < kJaCache; c += blockDim.x) {
     if (C.key[c] == kJaEmpty || C.cnt[c] == 0) continue;
     const uint64_t s = C.key[c];
-    atomicAdd(&A.w[4 * s], C.cnt[c]);
+    atomicAdd(&A.w[4 * s], static_cast<unsigned long long>(C.cnt[c]));
     Split128 sp;
     sp.lo32 = C.lo[c];
     sp.hi32 = C.hi[c];
